@@ -1,0 +1,133 @@
+/*
+ * kite_gpu.h -- C ABI of the MI355X (gfx950) exact k-mer engine.
+ *
+ * The reference (jiehua1995/Kite v0.7.5) is a monolithic Rust binary with no
+ * plugin or FFI interface.  These entry points sit at the seams its own code
+ * has between the count driver, the per-thread workers and the .kite writer,
+ * and between the .kite readers and the merge/analyze arithmetic.  Each
+ * declaration names the reference function(s) it replaces.
+ *
+ * Conventions: every function returns 0 on success and a negative code on
+ * failure; kite_gpu_last_error() then returns a message (thread-local).
+ * Keys cross the boundary as little-endian u64 words, word 0 least
+ * significant: 1 word for k <= 31, 2 for k <= 64, 4 for k <= 128
+ * (src/count/mod.rs:232-238).  A 4-word key is (lo = w0,w1; hi = w2,w3).
+ * No torch or HIP types appear in any signature.
+ */
+#ifndef KITE_GPU_H
+#define KITE_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KITE_GPU_ABI_VERSION 1
+
+enum {
+    KITE_OK = 0,
+    KITE_E_INVALID = -1,  /* bad argument (k, partitions, sizes, state) */
+    KITE_E_DEVICE = -2,   /* HIP runtime / device failure */
+    KITE_E_NOMEM = -3,    /* device budget exceeded */
+    KITE_E_CALLBACK = -4, /* the append callback returned non-zero */
+    KITE_E_OVERFLOW = -5  /* a merged count exceeds u32 */
+};
+
+/* Chunk codecs (src/format/run_block.rs:1-3). */
+#define KITE_CODEC_RAW 0x01u
+#define KITE_CODEC_DVLE 0x02u
+
+typedef struct kite_gpu_ctx kite_gpu_ctx;
+
+/* Receives one encoded run chunk, borrowed for the call.  Mirrors
+ * ConcurrentKiteWriter::append_chunk (src/count/writer.rs:285-348):
+ * key_min / key_max are the split keys (lo128, hi128) as 4 u64 words.
+ * Return non-zero to abort. */
+typedef int (*kite_append_chunk_fn)(void *user, uint32_t partition, uint32_t codec, uint64_t rows,
+                                    uint64_t raw_bytes, const uint64_t key_min[4],
+                                    const uint64_t key_max[4], const uint8_t *data, size_t len);
+
+typedef struct kite_count_stats {
+    uint64_t input_bytes;     /* FASTQ bytes counted (after the '@' chunk check) */
+    uint64_t kmers;           /* input k-mers (valid windows) */
+    uint64_t unique;          /* sum over batches of unique keys per batch */
+    uint64_t buckets;         /* LDS sort buckets of the last batch */
+    uint32_t batches;         /* device batches run */
+    uint32_t split_levels;    /* MSD split levels of the last batch */
+    uint32_t skipped_chunks;  /* fed chunks dropped for not starting with '@' */
+    uint32_t words;           /* key words (1, 2 or 4) */
+    double device_ms;         /* device time of all batches (events) */
+    double extract_ms, split_ms, sort_ms, compact_ms; /* last batch */
+} kite_count_stats;
+
+/* ---- context ------------------------------------------------------------ */
+
+/* Opens the engine on HIP device `device`.  hbm_budget_bytes bounds the
+ * device memory one batch may use (0: 90% of the device's free memory).
+ * Replaces the host memory planner's role for the hot path
+ * (src/count/mod.rs:624-864 derive_heuristic_plan, src/count/memory.rs). */
+int kite_gpu_open(int device, uint64_t hbm_budget_bytes, kite_gpu_ctx **out);
+void kite_gpu_close(kite_gpu_ctx *ctx);
+const char *kite_gpu_last_error(void);
+int kite_gpu_abi_version(void);
+
+/* ---- count ---------------------------------------------------------------
+ * begin + feed* + finish + emit replace the count driver's worker pool:
+ * run_count_typed's worker loop (src/count/mod.rs:512-540) over
+ * CountWorker::process_chunk (src/count/worker.rs:196-271) and
+ * CountWorker::flush_all / flush_bucket (worker.rs:273-449). */
+
+/* Starts a count session (validation as src/count/mod.rs:208-230). */
+int kite_gpu_count_begin(kite_gpu_ctx *ctx, uint32_t k, int canonical, uint32_t partitions);
+
+/* Feeds one record-aligned FASTQ chunk from host memory, as the reader
+ * thread hands chunks to workers (src/count/mod.rs:424-509).  A chunk that
+ * does not start with '@' yields no k-mers, as in the reference
+ * (src/count/parser.rs:24-27 with the error ignored at worker.rs:217). */
+int kite_gpu_count_feed(kite_gpu_ctx *ctx, const uint8_t *fastq, size_t len);
+
+/* Feeds FASTQ bytes already resident in device memory (one chunk).  The
+ * buffer must stay valid until kite_gpu_count_finish returns. */
+int kite_gpu_count_feed_device(kite_gpu_ctx *ctx, const void *d_fastq, size_t len);
+
+/* Runs the device pipeline on everything fed; results stay on the device. */
+int kite_gpu_count_finish(kite_gpu_ctx *ctx, kite_count_stats *stats);
+
+/* Encodes the results as DVLE (k <= 64) or RAW (k >= 65) run chunks, one
+ * per non-empty partition per batch, in partition order, and passes each to
+ * `fn` -- the job flush_bucket does before append_chunk
+ * (src/count/worker.rs:296-449, src/format/run_block.rs:70-95). */
+int kite_gpu_count_emit(kite_gpu_ctx *ctx, kite_append_chunk_fn fn, void *user);
+
+/* Size of the reduced result of the last batch. */
+int kite_gpu_count_result_size(kite_gpu_ctx *ctx, uint64_t *n_unique, uint32_t *words,
+                               uint32_t *partitions);
+
+/* Copies the last batch's result to host memory: keys (n_unique * words
+ * u64), counts (n_unique u32, saturated at u32::MAX) and part_off
+ * (partitions + 1 offsets).  Any pointer may be NULL. */
+int kite_gpu_count_download(kite_gpu_ctx *ctx, uint64_t *keys, uint32_t *counts,
+                            uint64_t *part_off);
+
+/* Device pointers of the last batch's result (valid until the next
+ * finish/close): keys, counts and the host copy of part_off. */
+int kite_gpu_count_device_result(kite_gpu_ctx *ctx, const uint64_t **d_keys,
+                                 const uint32_t **d_counts, const uint64_t **part_off);
+
+/* Turns per-kernel profiling on or off for later batches: HIP events around
+ * every launch on the engine's stream, plus each kernel's algorithmic HBM
+ * bytes.  Counterpart of the reference's --test-mode stage timers
+ * (src/stats.rs:8-273), at kernel granularity. */
+int kite_gpu_count_profile(kite_gpu_ctx *ctx, int enable);
+
+/* Writes the last batch's kernel profile as JSON into buf (cap bytes):
+ * {"<kernel>": {"ms": total, "launches": n, "bytes": algorithmic}, ...}.
+ * Returns the JSON length, or a negative code. */
+int kite_gpu_count_kernel_stats(kite_gpu_ctx *ctx, char *buf, size_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,12 @@
+"""kite_amd -- MI355X-native (gfx950) exact k-mer counting engine.
+
+Python mirror of the C ABI in include/kite_gpu.h, used by the tests and the
+benchmark.  The command-line drop-in for `kite count|dump|merge|histo|
+analyze|info` is the C++ host binary kite_amd/lib/kite.
+"""
+from ._native import (CountResult, CountStats, GpuContext, KiteError, SynthSpec, gpu_lib_path,
+                      load_gpu, synth_genome, synth_reads)
+
+__all__ = ["CountResult", "CountStats", "GpuContext", "KiteError", "SynthSpec", "gpu_lib_path",
+           "load_gpu", "synth_genome", "synth_reads"]
+__version__ = "0.1.0"

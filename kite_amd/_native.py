@@ -1,0 +1,265 @@
+"""ctypes bindings of libkite_gpu.so (include/kite_gpu.h) and libkite_synth.so.
+
+The GPU library is required: every entry point raises when it is missing or
+when no HIP device is present.  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+LIB_DIR = Path(__file__).resolve().parent / "lib"
+
+_gpu = None
+_synth = None
+
+
+class KiteError(RuntimeError):
+    """Error reported by the native engine (message from kite_gpu_last_error)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class CountStats(C.Structure):
+    _fields_ = [
+        ("input_bytes", C.c_uint64), ("kmers", C.c_uint64), ("unique", C.c_uint64),
+        ("buckets", C.c_uint64), ("batches", C.c_uint32), ("split_levels", C.c_uint32),
+        ("skipped_chunks", C.c_uint32), ("words", C.c_uint32), ("device_ms", C.c_double),
+        ("extract_ms", C.c_double), ("split_ms", C.c_double), ("sort_ms", C.c_double),
+        ("compact_ms", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+APPEND_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint64,
+                        C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint8),
+                        C.c_size_t)
+
+# name -> (restype, argtypes); every symbol include/kite_gpu.h declares
+GPU_SYMBOLS = {
+    "kite_gpu_abi_version": (C.c_int, []),
+    "kite_gpu_last_error": (C.c_char_p, []),
+    "kite_gpu_open": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "kite_gpu_close": (None, [C.c_void_p]),
+    "kite_gpu_count_begin": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_uint32]),
+    "kite_gpu_count_feed": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "kite_gpu_count_feed_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "kite_gpu_count_finish": (C.c_int, [C.c_void_p, C.POINTER(CountStats)]),
+    "kite_gpu_count_emit": (C.c_int, [C.c_void_p, APPEND_FN, C.c_void_p]),
+    "kite_gpu_count_result_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64),
+                                             C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "kite_gpu_count_download": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "kite_gpu_count_device_result": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p),
+                                               C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    "kite_gpu_count_profile": (C.c_int, [C.c_void_p, C.c_int]),
+    "kite_gpu_count_kernel_stats": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t]),
+}
+
+
+def gpu_lib_path() -> Path:
+    return LIB_DIR / "libkite_gpu.so"
+
+
+def load_gpu() -> C.CDLL:
+    """Load libkite_gpu.so (raises if it has not been built)."""
+    global _gpu
+    if _gpu is None:
+        path = gpu_lib_path()
+        if not path.exists():
+            raise RuntimeError(f"{path} is missing: build it with `python -m kite_amd._build`")
+        lib = C.CDLL(str(path), mode=C.RTLD_GLOBAL)
+        for name, (res, args) in GPU_SYMBOLS.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _gpu = lib
+    return _gpu
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        msg = load_gpu().kite_gpu_last_error()
+        raise KiteError(rc, msg.decode() if msg else "unknown error")
+
+
+def _ptr(buf) -> int:
+    if isinstance(buf, np.ndarray):
+        assert buf.flags["C_CONTIGUOUS"]
+        return buf.ctypes.data
+    if isinstance(buf, (bytes, bytearray)):
+        return C.cast(C.c_char_p(bytes(buf)), C.c_void_p).value  # pragma: no cover
+    raise TypeError(type(buf))
+
+
+class CountResult:
+    """Reduced count table: per-partition ascending unique keys and counts."""
+
+    def __init__(self, keys: np.ndarray, counts: np.ndarray, part_off: np.ndarray, words: int,
+                 k: int, canonical: bool):
+        self.keys = keys            # (n_unique, words) uint64, word 0 least significant
+        self.counts = counts        # (n_unique,) uint32
+        self.part_off = part_off    # (partitions + 1,) uint64
+        self.words = words
+        self.k = k
+        self.canonical = canonical
+
+    @property
+    def partitions(self) -> int:
+        return len(self.part_off) - 1
+
+    def __len__(self) -> int:
+        return len(self.counts)
+
+
+class GpuContext:
+    """One engine context on one HIP device (kite_gpu_open / kite_gpu_close)."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0):
+        lib = load_gpu()
+        h = C.c_void_p()
+        _check(lib.kite_gpu_open(device, hbm_budget_bytes, C.byref(h)))
+        self._h = h
+        self._lib = lib
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.kite_gpu_close(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- count session (kite_gpu_count_*) --------------------------------
+    def count_begin(self, k: int, canonical: bool, partitions: int) -> None:
+        _check(self._lib.kite_gpu_count_begin(self._h, k, int(bool(canonical)), partitions))
+        self._k, self._canonical = k, bool(canonical)
+
+    def count_feed(self, chunk: np.ndarray | bytes) -> None:
+        arr = np.frombuffer(chunk, dtype=np.uint8) if isinstance(chunk, (bytes, bytearray)) else chunk
+        arr = np.ascontiguousarray(arr, dtype=np.uint8)
+        _check(self._lib.kite_gpu_count_feed(self._h, arr.ctypes.data, arr.size))
+
+    def count_feed_device(self, dev_ptr: int, nbytes: int) -> None:
+        _check(self._lib.kite_gpu_count_feed_device(self._h, C.c_void_p(dev_ptr), nbytes))
+
+    def count_finish(self) -> CountStats:
+        st = CountStats()
+        _check(self._lib.kite_gpu_count_finish(self._h, C.byref(st)))
+        return st
+
+    def count_download(self) -> CountResult:
+        n = C.c_uint64()
+        w = C.c_uint32()
+        p = C.c_uint32()
+        _check(self._lib.kite_gpu_count_result_size(self._h, C.byref(n), C.byref(w), C.byref(p)))
+        keys = np.empty((n.value, w.value), dtype=np.uint64)
+        counts = np.empty(n.value, dtype=np.uint32)
+        part_off = np.empty(p.value + 1, dtype=np.uint64)
+        _check(self._lib.kite_gpu_count_download(self._h, keys.ctypes.data, counts.ctypes.data,
+                                                 part_off.ctypes.data))
+        return CountResult(keys, counts, part_off, w.value, self._k, self._canonical)
+
+    def count_emit(self) -> list[tuple]:
+        """Collect the encoded run chunks: (partition, codec, rows, raw_bytes,
+        key_min[4], key_max[4], payload bytes)."""
+        out: list[tuple] = []
+
+        def cb(user, part, codec, rows, raw, kmin, kmax, data, ln):
+            out.append((part, codec, rows, raw, tuple(kmin[i] for i in range(4)),
+                        tuple(kmax[i] for i in range(4)), C.string_at(data, ln)))
+            return 0
+
+        fn = APPEND_FN(cb)
+        _check(self._lib.kite_gpu_count_emit(self._h, fn, None))
+        return out
+
+    def count_profile(self, enable: bool) -> None:
+        _check(self._lib.kite_gpu_count_profile(self._h, int(enable)))
+
+    def count_kernel_stats(self) -> dict:
+        import json
+        buf = C.create_string_buffer(1 << 16)
+        n = self._lib.kite_gpu_count_kernel_stats(self._h, buf, len(buf))
+        if n < 0:
+            _check(n)
+        return json.loads(buf.value.decode())
+
+    def count(self, fastq: np.ndarray | bytes, k: int, canonical: bool, partitions: int,
+              chunks: list[tuple[int, int]] | None = None) -> tuple[CountResult, CountStats]:
+        """Count a FASTQ image held in host memory.  `chunks` lists
+        (offset, length) record-aligned pieces to feed; default: one chunk."""
+        arr = np.frombuffer(fastq, dtype=np.uint8) if isinstance(fastq, (bytes, bytearray)) else fastq
+        self.count_begin(k, canonical, partitions)
+        if chunks is None:
+            chunks = [(0, arr.size)]
+        for off, ln in chunks:
+            self.count_feed(arr[off:off + ln])
+        st = self.count_finish()
+        return self.count_download(), st
+
+
+# ---- synthetic data ---------------------------------------------------------
+class SynthSpec(C.Structure):
+    _fields_ = [
+        ("seed", C.c_uint64), ("genome_len", C.c_uint64), ("read_len", C.c_uint32),
+        ("both_strands", C.c_uint32), ("err_rate", C.c_double), ("n_rate", C.c_double),
+        ("repeat_segments", C.c_uint32), ("repeat_seg_len", C.c_uint32),
+        ("repeat_copies", C.c_uint32), ("_pad", C.c_uint32), ("repeat_div", C.c_double),
+        ("variant_seed", C.c_uint64), ("snv_rate", C.c_double),
+    ]
+
+
+def load_synth() -> C.CDLL:
+    global _synth
+    if _synth is None:
+        path = LIB_DIR / "libkite_synth.so"
+        if not path.exists():
+            raise RuntimeError(f"{path} is missing: build it with `python -m kite_amd._build`")
+        lib = C.CDLL(str(path))
+        lib.kite_synth_genome.restype = C.c_int
+        lib.kite_synth_genome.argtypes = [C.POINTER(SynthSpec), C.c_void_p]
+        lib.kite_synth_offset.restype = C.c_uint64
+        lib.kite_synth_offset.argtypes = [C.POINTER(SynthSpec), C.c_uint64]
+        lib.kite_synth_reads.restype = C.c_int64
+        lib.kite_synth_reads.argtypes = [C.POINTER(SynthSpec), C.c_void_p, C.c_uint64, C.c_uint64,
+                                         C.c_void_p, C.c_uint64, C.c_int]
+        _synth = lib
+    return _synth
+
+
+def synth_genome(spec: SynthSpec) -> np.ndarray:
+    g = np.empty(spec.genome_len, dtype=np.uint8)
+    if load_synth().kite_synth_genome(C.byref(spec), g.ctypes.data) != 0:
+        raise RuntimeError("genome generation failed")
+    return g
+
+
+def synth_reads(spec: SynthSpec, genome: np.ndarray, r0: int, r1: int,
+                threads: int | None = None, out: np.ndarray | None = None) -> np.ndarray:
+    lib = load_synth()
+    need = lib.kite_synth_reads(C.byref(spec), genome.ctypes.data, r0, r1, None, 0, 1)
+    if out is None:
+        out = np.empty(need, dtype=np.uint8)
+    assert out.size >= need
+    threads = threads or min(16, os.cpu_count() or 1)
+    n = lib.kite_synth_reads(C.byref(spec), genome.ctypes.data, r0, r1, out.ctypes.data, out.size,
+                             threads)
+    if n != need:
+        raise RuntimeError("read generation failed")
+    return out[:need]

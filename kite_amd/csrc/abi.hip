@@ -1,0 +1,374 @@
+// abi.hip -- the extern "C" boundary of libkite_gpu.so (declared in
+// include/kite_gpu.h): context, count session, run-chunk emission.
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/kite_gpu.h"
+#include "count.hpp"
+#include "dvle.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int set_error(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+template <class F>
+int guarded(F &&f) {
+    try {
+        return f();
+    } catch (const kite::Error &e) {
+        std::string m = e.what();
+        int code = (m.find("out of memory") != std::string::npos) ? KITE_E_NOMEM : KITE_E_DEVICE;
+        if (m.rfind("invalid", 0) == 0 || m.find("not supported") != std::string::npos)
+            code = KITE_E_INVALID;
+        return set_error(code, m);
+    } catch (const std::bad_alloc &) {
+        return set_error(KITE_E_NOMEM, "host allocation failed");
+    } catch (const std::exception &e) {
+        return set_error(KITE_E_DEVICE, e.what());
+    }
+}
+
+struct HostRun {
+    std::vector<uint64_t> keys;
+    std::vector<uint32_t> counts;
+    std::vector<uint64_t> part_off;
+};
+
+}  // namespace
+
+struct kite_gpu_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;
+    std::unique_ptr<kite::CountEngine> count;
+
+    // count session
+    bool session = false;
+    uint32_t k = 0, partitions = 0;
+    int canonical = 0, words = 1;
+    uint64_t batch_cap = 0;
+    kite::DevBuf staging;
+    uint64_t staged = 0;
+    const uint8_t *direct = nullptr;
+    uint64_t direct_len = 0;
+    bool device_result = false;
+    std::vector<HostRun> host_runs;
+    kite_count_stats stats{};
+};
+
+namespace {
+
+void bind(kite_gpu_ctx *c) { KITE_HIP(hipSetDevice(c->device)); }
+
+// Keep the staged FASTQ bytes when the staging buffer grows.
+void staging_reserve(kite_gpu_ctx *c, uint64_t need) {
+    if (need <= c->staging.bytes) return;
+    uint64_t want = std::max<uint64_t>(need, std::min<uint64_t>(c->batch_cap, 2 * c->staging.bytes));
+    want = std::max<uint64_t>(want, need);
+    void *np = nullptr;
+    KITE_HIP(hipMalloc(&np, want));
+    if (c->staged)
+        KITE_HIP(hipMemcpyAsync(np, c->staging.ptr, c->staged, hipMemcpyDeviceToDevice, c->stream));
+    KITE_HIP(hipStreamSynchronize(c->stream));
+    c->staging.release();
+    c->staging.ptr = np;
+    c->staging.bytes = want;
+}
+
+void stash_device_result(kite_gpu_ctx *c) {
+    if (!c->device_result) return;
+    HostRun r;
+    const uint64_t u = c->count->n_unique();
+    r.keys.resize(u * c->words);
+    r.counts.resize(u);
+    r.part_off = c->count->part_off();
+    if (u) {
+        KITE_HIP(hipMemcpy(r.keys.data(), c->count->keys(), u * 8 * c->words, hipMemcpyDeviceToHost));
+        KITE_HIP(hipMemcpy(r.counts.data(), c->count->counts(), u * 4, hipMemcpyDeviceToHost));
+    }
+    c->host_runs.push_back(std::move(r));
+    c->device_result = false;
+}
+
+void run_batch(kite_gpu_ctx *c, const uint8_t *ptr, uint64_t len) {
+    stash_device_result(c);
+    c->count->set_timing(true);
+    c->count->run(ptr, len, (int)c->k, c->canonical, c->partitions);
+    c->device_result = true;
+    const kite::StageTimes &t = c->count->times();
+    c->stats.kmers += c->count->n_kmers();
+    c->stats.unique += c->count->n_unique();
+    c->stats.buckets = c->count->buckets();
+    c->stats.split_levels = (uint32_t)c->count->levels();
+    c->stats.batches++;
+    c->stats.device_ms += t.total_ms;
+    c->stats.extract_ms = t.extract_ms;
+    c->stats.split_ms = t.split_ms;
+    c->stats.sort_ms = t.sort_ms;
+    c->stats.compact_ms = t.compact_ms;
+}
+
+void flush_pending(kite_gpu_ctx *c) {
+    if (c->direct) {
+        const uint8_t *p = c->direct;
+        uint64_t n = c->direct_len;
+        c->direct = nullptr;
+        c->direct_len = 0;
+        run_batch(c, p, n);
+    } else if (c->staged) {
+        run_batch(c, c->staging.as<uint8_t>(), c->staged);
+        c->staged = 0;
+    }
+}
+
+uint64_t count_newlines(const uint8_t *p, size_t n) {
+    uint64_t c = 0;
+    const uint8_t *e = p + n;
+    while (p < e) {
+        const void *q = memchr(p, '\n', (size_t)(e - p));
+        if (!q) break;
+        c++;
+        p = static_cast<const uint8_t *>(q) + 1;
+    }
+    return c;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *kite_gpu_last_error(void) { return g_last_error.c_str(); }
+
+int kite_gpu_abi_version(void) { return KITE_GPU_ABI_VERSION; }
+
+int kite_gpu_open(int device, uint64_t hbm_budget_bytes, kite_gpu_ctx **out) {
+    if (!out) return set_error(KITE_E_INVALID, "invalid argument: out is NULL");
+    *out = nullptr;
+    return guarded([&]() -> int {
+        int n = 0;
+        KITE_HIP(hipGetDeviceCount(&n));
+        if (device < 0 || device >= n)
+            return set_error(KITE_E_INVALID, "invalid device ordinal " + std::to_string(device));
+        auto c = std::make_unique<kite_gpu_ctx>();
+        c->device = device;
+        KITE_HIP(hipSetDevice(device));
+        KITE_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        size_t free_b = 0, total_b = 0;
+        KITE_HIP(hipMemGetInfo(&free_b, &total_b));
+        c->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)(free_b * 0.9);
+        c->count = std::make_unique<kite::CountEngine>(c->stream);
+        *out = c.release();
+        return KITE_OK;
+    });
+}
+
+void kite_gpu_close(kite_gpu_ctx *ctx) {
+    if (!ctx) return;
+    try {
+        bind(ctx);
+        (void)hipStreamSynchronize(ctx->stream);
+        ctx->count.reset();
+        ctx->staging.release();
+        (void)hipStreamDestroy(ctx->stream);
+    } catch (...) {
+    }
+    delete ctx;
+}
+
+int kite_gpu_count_begin(kite_gpu_ctx *ctx, uint32_t k, int canonical, uint32_t partitions) {
+    if (!ctx) return set_error(KITE_E_INVALID, "invalid argument: ctx is NULL");
+    if (k < 1 || k > 128)
+        return set_error(KITE_E_INVALID,
+                         "invalid k-mer length: --k must be between 1 and 128.\nGot: " + std::to_string(k));
+    if (partitions == 0 || (partitions & (partitions - 1)))
+        return set_error(KITE_E_INVALID,
+                         "invalid partition count: --partitions must be a power of 2.\nGot: " +
+                             std::to_string(partitions));
+    return guarded([&]() -> int {
+        bind(ctx);
+        ctx->session = true;
+        ctx->k = k;
+        ctx->canonical = canonical ? 1 : 0;
+        ctx->partitions = partitions;
+        ctx->words = k <= 31 ? 1 : (k <= 64 ? 2 : 4);
+        // device bytes per FASTQ byte, assuming at most one k-mer per two input bytes
+        uint64_t per_byte = (3ull * 8 * ctx->words + 8) / 2 + 2;
+        ctx->batch_cap = std::max<uint64_t>(1ull << 20, ctx->budget / per_byte);
+        ctx->staged = 0;
+        ctx->direct = nullptr;
+        ctx->direct_len = 0;
+        ctx->device_result = false;
+        ctx->host_runs.clear();
+        ctx->stats = kite_count_stats{};
+        ctx->stats.words = (uint32_t)ctx->words;
+        return KITE_OK;
+    });
+}
+
+int kite_gpu_count_feed(kite_gpu_ctx *ctx, const uint8_t *fastq, size_t len) {
+    if (!ctx || !ctx->session) return set_error(KITE_E_INVALID, "invalid state: no count session");
+    if (len == 0) return KITE_OK;
+    if (!fastq) return set_error(KITE_E_INVALID, "invalid argument: fastq is NULL");
+    if (fastq[0] != '@') {
+        ctx->stats.skipped_chunks++;
+        return KITE_OK;
+    }
+    return guarded([&]() -> int {
+        bind(ctx);
+        if (ctx->direct) flush_pending(ctx);
+        const uint64_t nl = count_newlines(fastq, len);
+        const bool ends_nl = fastq[len - 1] == '\n';
+        uint32_t pad = (uint32_t)((4 - nl % 4) % 4);
+        if (!ends_nl && pad == 0) pad = 4;
+        const uint64_t total = len + pad;
+        if (ctx->staged && ctx->staged + total > ctx->batch_cap) flush_pending(ctx);
+        staging_reserve(ctx, ctx->staged + total);
+        uint8_t *dst = ctx->staging.as<uint8_t>() + ctx->staged;
+        KITE_HIP(hipMemcpy(dst, fastq, len, hipMemcpyHostToDevice));
+        if (pad) {
+            static const uint8_t nls[4] = {'\n', '\n', '\n', '\n'};
+            KITE_HIP(hipMemcpy(dst + len, nls, pad, hipMemcpyHostToDevice));
+        }
+        ctx->staged += total;
+        ctx->stats.input_bytes += len;
+        return KITE_OK;
+    });
+}
+
+int kite_gpu_count_feed_device(kite_gpu_ctx *ctx, const void *d_fastq, size_t len) {
+    if (!ctx || !ctx->session) return set_error(KITE_E_INVALID, "invalid state: no count session");
+    if (len == 0) return KITE_OK;
+    if (!d_fastq) return set_error(KITE_E_INVALID, "invalid argument: d_fastq is NULL");
+    return guarded([&]() -> int {
+        bind(ctx);
+        flush_pending(ctx);
+        ctx->direct = static_cast<const uint8_t *>(d_fastq);
+        ctx->direct_len = len;
+        ctx->stats.input_bytes += len;
+        return KITE_OK;
+    });
+}
+
+int kite_gpu_count_finish(kite_gpu_ctx *ctx, kite_count_stats *stats) {
+    if (!ctx || !ctx->session) return set_error(KITE_E_INVALID, "invalid state: no count session");
+    return guarded([&]() -> int {
+        bind(ctx);
+        flush_pending(ctx);
+        if (stats) *stats = ctx->stats;
+        return KITE_OK;
+    });
+}
+
+int kite_gpu_count_result_size(kite_gpu_ctx *ctx, uint64_t *n_unique, uint32_t *words,
+                               uint32_t *partitions) {
+    if (!ctx || !ctx->session) return set_error(KITE_E_INVALID, "invalid state: no count session");
+    if (n_unique) *n_unique = ctx->device_result ? ctx->count->n_unique() : 0;
+    if (words) *words = (uint32_t)ctx->words;
+    if (partitions) *partitions = ctx->partitions;
+    return KITE_OK;
+}
+
+int kite_gpu_count_download(kite_gpu_ctx *ctx, uint64_t *keys, uint32_t *counts, uint64_t *part_off) {
+    if (!ctx || !ctx->session) return set_error(KITE_E_INVALID, "invalid state: no count session");
+    return guarded([&]() -> int {
+        bind(ctx);
+        const uint64_t u = ctx->device_result ? ctx->count->n_unique() : 0;
+        if (keys && u)
+            KITE_HIP(hipMemcpy(keys, ctx->count->keys(), u * 8 * ctx->words, hipMemcpyDeviceToHost));
+        if (counts && u)
+            KITE_HIP(hipMemcpy(counts, ctx->count->counts(), u * 4, hipMemcpyDeviceToHost));
+        if (part_off) {
+            if (ctx->device_result) {
+                const auto &po = ctx->count->part_off();
+                std::copy(po.begin(), po.end(), part_off);
+            } else {
+                std::fill(part_off, part_off + ctx->partitions + 1, 0);
+            }
+        }
+        return KITE_OK;
+    });
+}
+
+int kite_gpu_count_device_result(kite_gpu_ctx *ctx, const uint64_t **d_keys, const uint32_t **d_counts,
+                                 const uint64_t **part_off) {
+    if (!ctx || !ctx->session || !ctx->device_result)
+        return set_error(KITE_E_INVALID, "invalid state: no device result");
+    if (d_keys) *d_keys = ctx->count->keys();
+    if (d_counts) *d_counts = ctx->count->counts();
+    if (part_off) *part_off = ctx->count->part_off().data();
+    return KITE_OK;
+}
+
+int kite_gpu_count_emit(kite_gpu_ctx *ctx, kite_append_chunk_fn fn, void *user) {
+    if (!ctx || !ctx->session) return set_error(KITE_E_INVALID, "invalid state: no count session");
+    if (!fn) return set_error(KITE_E_INVALID, "invalid argument: callback is NULL");
+    return guarded([&]() -> int {
+        bind(ctx);
+        const int W = ctx->words;
+        const uint32_t P = ctx->partitions;
+        HostRun last;
+        if (ctx->device_result) {
+            const uint64_t u = ctx->count->n_unique();
+            last.keys.resize(u * W);
+            last.counts.resize(u);
+            last.part_off = ctx->count->part_off();
+            if (u) {
+                KITE_HIP(hipMemcpy(last.keys.data(), ctx->count->keys(), u * 8 * W, hipMemcpyDeviceToHost));
+                KITE_HIP(hipMemcpy(last.counts.data(), ctx->count->counts(), u * 4, hipMemcpyDeviceToHost));
+            }
+        }
+        std::vector<const HostRun *> runs;
+        for (const HostRun &r : ctx->host_runs) runs.push_back(&r);
+        if (ctx->device_result) runs.push_back(&last);
+        for (const HostRun *r : runs) {
+            std::vector<kite::EncodedChunk> cs =
+                kite::encode_partitions(r->keys.data(), r->counts.data(), r->part_off.data(), P, W);
+            for (uint32_t p = 0; p < P; p++) {
+                const kite::EncodedChunk &ch = cs[p];
+                if (!ch.rows) continue;
+                int rc = fn(user, p, ch.codec, ch.rows, ch.bytes.size(), ch.key_min, ch.key_max,
+                            ch.bytes.data(), ch.bytes.size());
+                if (rc)
+                    return set_error(KITE_E_CALLBACK,
+                                     "append callback failed for partition " + std::to_string(p));
+            }
+        }
+        return KITE_OK;
+    });
+}
+
+int kite_gpu_count_profile(kite_gpu_ctx *ctx, int enable) {
+    if (!ctx) return set_error(KITE_E_INVALID, "invalid argument: ctx is NULL");
+    ctx->count->set_kernel_profile(enable != 0);
+    return KITE_OK;
+}
+
+int kite_gpu_count_kernel_stats(kite_gpu_ctx *ctx, char *buf, size_t cap) {
+    if (!ctx) return set_error(KITE_E_INVALID, "invalid argument: ctx is NULL");
+    std::string js = "{";
+    bool first = true;
+    for (const auto &ks : ctx->count->kernel_stats()) {
+        char tmp[256];
+        snprintf(tmp, sizeof tmp, "%s\"%s\": {\"ms\": %.6f, \"launches\": %llu, \"bytes\": %llu}",
+                 first ? "" : ", ", ks.name.c_str(), ks.ms, (unsigned long long)ks.launches,
+                 (unsigned long long)ks.bytes);
+        js += tmp;
+        first = false;
+    }
+    js += "}";
+    if (!buf || cap <= js.size()) return set_error(KITE_E_INVALID, "buffer too small");
+    memcpy(buf, js.c_str(), js.size() + 1);
+    return (int)js.size();
+}
+
+}  // extern "C"

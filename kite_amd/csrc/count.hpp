@@ -1,0 +1,105 @@
+// count.hpp -- the gfx950 exact k-mer count pipeline (host-side interface).
+//
+// One batch of FASTQ bytes resident in HBM becomes per-partition sorted,
+// fully reduced (key, count) runs, also resident in HBM:
+//
+//   k_nl_count      newline count per 8 KiB sub-tile
+//   scan            line index at each sub-tile start
+//   k_extract<HIST> 2-bit pack, rolling fwd/rc k-mers, Fibonacci partition
+//                   histogram per (partition, workgroup)
+//   scan            partition-major destination of every workgroup's keys
+//   k_extract<SCAT> same k-mers, scattered into partition order
+//   k_seg<HIST/SCAT> MSD split of large partitions on the top key bits
+//   k_bucket_sort   per-bucket LSD radix sort in LDS + run-length counts
+//   scan + k_compact  contiguous per-partition (key, count) output
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace kite {
+
+struct Bucket {
+    uint64_t start;   // first key (index into the key array)
+    uint64_t size;    // keys in the bucket
+    uint32_t prefix;  // key bits (from the top of the 2k-bit key) shared by all keys
+    uint32_t array;   // 0: keys live in A, 1: in B
+};
+
+struct StageTimes {
+    float extract_ms = 0, split_ms = 0, sort_ms = 0, compact_ms = 0, total_ms = 0;
+};
+
+// Per-kernel device time (HIP events on the engine stream) and algorithmic
+// HBM bytes of the last run, filled when kernel profiling is on.
+struct KernelStat {
+    std::string name;
+    double ms = 0;
+    uint64_t launches = 0;
+    uint64_t bytes = 0;
+};
+
+struct CountTunables {
+    int split_target_div = 2;  // split partitions until max child <= cap / div
+    int max_digit_bits = 12;
+};
+
+class CountEngine {
+   public:
+    explicit CountEngine(hipStream_t s) : stream_(s) {}
+
+    // Count the FASTQ bytes d_fq[0, len).  Results stay on the device until
+    // the next run: keys() (n_unique * words u64), counts() (u32), and
+    // part_off() (host copy, partitions + 1 entries).
+    void run(const uint8_t *d_fq, uint64_t len, int k, int canonical, uint32_t partitions);
+
+    uint64_t n_kmers() const { return n_kmers_; }
+    uint64_t n_unique() const { return n_unique_; }
+    int words() const { return words_; }
+    const std::vector<uint64_t> &part_off() const { return part_off_h_; }
+    const uint64_t *keys() const { return out_keys_.as<uint64_t>(); }
+    const uint32_t *counts() const { return out_counts_.as<uint32_t>(); }
+    const StageTimes &times() const { return times_; }
+    uint64_t buckets() const { return n_buckets_; }
+    int levels() const { return levels_; }
+    void set_timing(bool on) { timing_ = on; }
+    void set_kernel_profile(bool on) { kprof_ = on; }
+    const std::vector<KernelStat> &kernel_stats() const { return kstats_; }
+    CountTunables tun;
+
+    // Bytes of device memory the pipeline needs for n k-mers of `words` words
+    // and `fastq` input bytes (used by the batch planner).
+    static uint64_t bytes_needed(uint64_t n, int words, uint64_t fastq, uint32_t partitions);
+
+   private:
+    template <int W>
+    void run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical, int pbits);
+
+    // records events around one launch when kernel profiling is on
+    template <class F>
+    void launch(const char *name, uint64_t bytes, F &&f);
+    void collect_kernel_stats();
+
+    hipStream_t stream_;
+    bool timing_ = false;
+    bool kprof_ = false;
+    std::vector<KernelStat> kstats_;
+    struct PendingEv {
+        int stat;
+        hipEvent_t a, b;
+    };
+    std::vector<PendingEv> pending_;
+    std::vector<hipEvent_t> ev_pool_;
+    DevBuf nl_, line0_, scan_tmp_, hist_, off_, part_off_d_;
+    DevBuf a_, b_, counts_tmp_, out_keys_, out_counts_;
+    DevBuf segs_, tile_pre_, child_hist_, child_cur_, buckets_, ucount_, uoff_, part_first_, part_u_;
+    std::vector<uint64_t> part_off_h_;
+    uint64_t n_kmers_ = 0, n_unique_ = 0, n_buckets_ = 0;
+    int words_ = 1, levels_ = 0;
+    StageTimes times_;
+};
+
+}  // namespace kite

