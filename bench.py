@@ -73,6 +73,9 @@ def main() -> int:
     ap.add_argument("--cpu-sample-reads", type=int, default=1_000_000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run one 1 GiB device-to-device copy (a known-size kernel that "
+                         "scales FETCH_SIZE / WRITE_SIZE counter readings)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -101,6 +104,14 @@ def main() -> int:
     dev = torch.from_numpy(fq).to("cuda")
     torch.cuda.synchronize()
     ptr, nbytes = dev.data_ptr(), dev.numel()
+
+    if args.calibrate_copy:
+        src = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log("calibration copy: 1 GiB read + 1 GiB written")
 
     ctx = GpuContext(0)
 

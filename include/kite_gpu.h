@@ -116,6 +116,28 @@ int kite_gpu_count_download(kite_gpu_ctx *ctx, uint64_t *keys, uint32_t *counts,
 int kite_gpu_count_device_result(kite_gpu_ctx *ctx, const uint64_t **d_keys,
                                  const uint32_t **d_counts, const uint64_t **part_off);
 
+/* ---- merge ---------------------------------------------------------------
+ * Sorted union with count sums of n_inputs tables that share one partition
+ * layout: merge_vectors_total (src/kiteops/mod.rs:587-724) as used per
+ * partition by build_merge_partition_chunk (src/merge/mod.rs:198-214).
+ * Table i: keys[i] (rows_i * words u64, ascending and unique within each
+ * partition), counts[i] (rows_i u64), part_off[i] (partitions + 1).
+ * out_keys / out_counts must hold sum(rows_i) rows; out_part_off receives
+ * partitions + 1 offsets.  Equal keys sum with u64 saturation. */
+int kite_gpu_merge_tables(kite_gpu_ctx *ctx, uint32_t words, uint32_t partitions, uint32_t n_inputs,
+                          const uint64_t *const *keys, const uint64_t *const *counts,
+                          const uint64_t *const *part_off, uint64_t *out_keys, uint64_t *out_counts,
+                          uint64_t *out_part_off);
+
+/* Union rows of n_inputs tables with one count column per table, zero
+ * filled: for_each_union_row (src/kiteops/mod.rs:625-802).  out_keys holds
+ * sum(rows_i) rows; out_matrix holds sum(rows_i) * n_inputs u32 (row major,
+ * saturated at u32::MAX); out_part_off receives partitions + 1 offsets. */
+int kite_gpu_union_tables(kite_gpu_ctx *ctx, uint32_t words, uint32_t partitions, uint32_t n_inputs,
+                          const uint64_t *const *keys, const uint64_t *const *counts,
+                          const uint64_t *const *part_off, uint64_t *out_keys, uint32_t *out_matrix,
+                          uint64_t *out_part_off);
+
 /* Turns per-kernel profiling on or off for later batches: HIP events around
  * every launch on the engine's stream, plus each kernel's algorithmic HBM
  * bytes.  Counterpart of the reference's --test-mode stage timers

@@ -58,6 +58,12 @@ GPU_SYMBOLS = {
     "kite_gpu_count_download": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "kite_gpu_count_device_result": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p),
                                                C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    "kite_gpu_merge_tables": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                        C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                        C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p, C.c_void_p]),
+    "kite_gpu_union_tables": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                        C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                        C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p, C.c_void_p]),
     "kite_gpu_count_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "kite_gpu_count_kernel_stats": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t]),
 }
@@ -188,6 +194,43 @@ class GpuContext:
         fn = APPEND_FN(cb)
         _check(self._lib.kite_gpu_count_emit(self._h, fn, None))
         return out
+
+    # -- merge / union (kite_gpu_merge_tables, kite_gpu_union_tables) ----
+    @staticmethod
+    def _tables(tables):
+        ks, cs, os_ = [], [], []
+        for keys, counts, off in tables:
+            ks.append(np.ascontiguousarray(keys, dtype=np.uint64))
+            cs.append(np.ascontiguousarray(counts, dtype=np.uint64))
+            os_.append(np.ascontiguousarray(off, dtype=np.uint64))
+        arr = lambda xs: (C.c_void_p * len(xs))(*[x.ctypes.data for x in xs])
+        return ks, cs, os_, arr(ks), arr(cs), arr(os_)
+
+    def merge_tables(self, tables, words: int):
+        """Sorted union with u64 count sums of [(keys, counts, part_off)]."""
+        ks, cs, os_, kp, cp, op = self._tables(tables)
+        P = len(os_[0]) - 1
+        n = sum(int(o[-1]) for o in os_)
+        ok = np.empty((max(n, 1), words), dtype=np.uint64)
+        oc = np.empty(max(n, 1), dtype=np.uint64)
+        oo = np.empty(P + 1, dtype=np.uint64)
+        _check(self._lib.kite_gpu_merge_tables(self._h, words, P, len(tables), kp, cp, op,
+                                               ok.ctypes.data, oc.ctypes.data, oo.ctypes.data))
+        u = int(oo[-1])
+        return ok[:u], oc[:u], oo
+
+    def union_tables(self, tables, words: int):
+        """Union keys, u32 count matrix (rows x tables) and offsets."""
+        ks, cs, os_, kp, cp, op = self._tables(tables)
+        P = len(os_[0]) - 1
+        n = sum(int(o[-1]) for o in os_)
+        ok = np.empty((max(n, 1), words), dtype=np.uint64)
+        om = np.empty((max(n, 1), len(tables)), dtype=np.uint32)
+        oo = np.empty(P + 1, dtype=np.uint64)
+        _check(self._lib.kite_gpu_union_tables(self._h, words, P, len(tables), kp, cp, op,
+                                               ok.ctypes.data, om.ctypes.data, oo.ctypes.data))
+        u = int(oo[-1])
+        return ok[:u], om[:u], oo
 
     def count_profile(self, enable: bool) -> None:
         _check(self._lib.kite_gpu_count_profile(self._h, int(enable)))

@@ -11,6 +11,7 @@
 #include "../../include/kite_gpu.h"
 #include "count.hpp"
 #include "dvle.hpp"
+#include "merge.hpp"
 
 namespace {
 
@@ -51,6 +52,7 @@ struct kite_gpu_ctx {
     hipStream_t stream = nullptr;
     uint64_t budget = 0;
     std::unique_ptr<kite::CountEngine> count;
+    std::unique_ptr<kite::MergeEngine> merge;
 
     // count session
     bool session = false;
@@ -167,6 +169,7 @@ int kite_gpu_open(int device, uint64_t hbm_budget_bytes, kite_gpu_ctx **out) {
         KITE_HIP(hipMemGetInfo(&free_b, &total_b));
         c->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)(free_b * 0.9);
         c->count = std::make_unique<kite::CountEngine>(c->stream);
+        c->merge = std::make_unique<kite::MergeEngine>(c->stream);
         *out = c.release();
         return KITE_OK;
     });
@@ -178,6 +181,7 @@ void kite_gpu_close(kite_gpu_ctx *ctx) {
         bind(ctx);
         (void)hipStreamSynchronize(ctx->stream);
         ctx->count.reset();
+        ctx->merge.reset();
         ctx->staging.release();
         (void)hipStreamDestroy(ctx->stream);
     } catch (...) {
@@ -343,6 +347,66 @@ int kite_gpu_count_emit(kite_gpu_ctx *ctx, kite_append_chunk_fn fn, void *user) 
                                      "append callback failed for partition " + std::to_string(p));
             }
         }
+        return KITE_OK;
+    });
+}
+
+namespace {
+int load_tables(kite_gpu_ctx *ctx, uint32_t words, uint32_t P, uint32_t n, const uint64_t *const *keys,
+                const uint64_t *const *counts, const uint64_t *const *part_off, std::vector<kite::DevTable> &tabs) {
+    if (words != 1 && words != 2 && words != 4) return set_error(KITE_E_INVALID, "invalid argument: words must be 1, 2 or 4");
+    if (P == 0 || n == 0 || !keys || !counts || !part_off)
+        return set_error(KITE_E_INVALID, "invalid argument: empty table list");
+    tabs = std::vector<kite::DevTable>(n);
+    for (uint32_t i = 0; i < n; i++) {
+        kite::HostTable ht;
+        ht.keys = keys[i];
+        ht.counts = counts[i];
+        ht.part_off.assign(part_off[i], part_off[i] + P + 1);
+        if (ht.part_off[0] != 0) return set_error(KITE_E_INVALID, "invalid argument: part_off must start at 0");
+        for (uint32_t p = 0; p < P; p++)
+            if (ht.part_off[p + 1] < ht.part_off[p])
+                return set_error(KITE_E_INVALID, "invalid argument: part_off must be non-decreasing");
+        ctx->merge->upload_table(ht, tabs[i], (int)words);
+    }
+    return KITE_OK;
+}
+}  // namespace
+
+int kite_gpu_merge_tables(kite_gpu_ctx *ctx, uint32_t words, uint32_t partitions, uint32_t n_inputs,
+                          const uint64_t *const *keys, const uint64_t *const *counts,
+                          const uint64_t *const *part_off, uint64_t *out_keys, uint64_t *out_counts,
+                          uint64_t *out_part_off) {
+    if (!ctx) return set_error(KITE_E_INVALID, "invalid argument: ctx is NULL");
+    return guarded([&]() -> int {
+        bind(ctx);
+        std::vector<kite::DevTable> tabs;
+        int rc = load_tables(ctx, words, partitions, n_inputs, keys, counts, part_off, tabs);
+        if (rc) return rc;
+        kite::DevTable u;
+        ctx->merge->merge_all(tabs, partitions, (int)words, u);
+        ctx->merge->download(u, (int)words, partitions, out_keys, out_counts, out_part_off);
+        return KITE_OK;
+    });
+}
+
+int kite_gpu_union_tables(kite_gpu_ctx *ctx, uint32_t words, uint32_t partitions, uint32_t n_inputs,
+                          const uint64_t *const *keys, const uint64_t *const *counts,
+                          const uint64_t *const *part_off, uint64_t *out_keys, uint32_t *out_matrix,
+                          uint64_t *out_part_off) {
+    if (!ctx) return set_error(KITE_E_INVALID, "invalid argument: ctx is NULL");
+    return guarded([&]() -> int {
+        bind(ctx);
+        std::vector<kite::DevTable> tabs;
+        int rc = load_tables(ctx, words, partitions, n_inputs, keys, counts, part_off, tabs);
+        if (rc) return rc;
+        kite::DevTable u;
+        ctx->merge->merge_all(tabs, partitions, (int)words, u);
+        kite::DevBuf m;
+        ctx->merge->union_matrix(tabs, u, partitions, (int)words, m);
+        ctx->merge->download(u, (int)words, partitions, out_keys, nullptr, out_part_off);
+        if (out_matrix && u.rows)
+            KITE_HIP(hipMemcpy(out_matrix, m.ptr, u.rows * n_inputs * 4, hipMemcpyDeviceToHost));
         return KITE_OK;
     });
 }

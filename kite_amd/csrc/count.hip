@@ -17,7 +17,9 @@
 // Device layout: keys are arrays of structures of W little-endian u64 words
 // (W = 1 for k <= 31, 2 for k <= 64, 4 above), w[0] least significant.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 #include "count.hpp"
 
@@ -37,9 +39,10 @@ constexpr uint32_t MAX_PARTITIONS = 8192;
 // ---- split / sort geometry --------------------------------------------------
 constexpr int ST = 256;                 // split threads
 template <int W> struct SplitCfg { static constexpr int IPT = 32 / W; static constexpr int TK = ST * IPT; };
-constexpr int BT = 512;                 // bucket-sort threads
+constexpr int BT = 256;                 // bucket-sort threads
 constexpr int BNW = BT / 64;
-template <int W> struct SortCfg { static constexpr int IPT = 16 / W; static constexpr int CAP = BT * IPT; };
+constexpr int SORT_CLASSES = 4;         // items per thread (4 << class) / W
+template <int W, int C> struct SortClass { static constexpr int IPT = (4 << C) / W; static constexpr int CAP = BT * IPT; };
 constexpr int MAX_DBITS = 12;
 
 template <int W>
@@ -371,36 +374,40 @@ __global__ __launch_bounds__(ST) void k_split(SplitArgs a) {
 }
 
 // ---- per-bucket LSD radix sort in LDS + run-length counts ------------------
+// One 256-thread workgroup per bucket; IPT (items per thread) sets the size
+// class, so small buckets take little LDS and many workgroups share a CU.
 // Keys are held warp-striped: wave w owns positions [w*64*ipt, (w+1)*64*ipt),
 // item `it` of lane l is position w*64*ipt + it*64 + l.  Ranking inside a
 // wave uses ballot-matched digits, processed item by item, so every pass is
 // stable in position order; waves are combined by a (digit, wave) scan.
-template <int W>
+template <int W, int IPT>
 __global__ __launch_bounds__(BT) void k_bucket_sort(uint64_t *__restrict__ A,
                                                      uint64_t *__restrict__ B,
                                                      uint32_t *__restrict__ counts_tmp,
                                                      const Bucket *__restrict__ buckets,
+                                                     const uint32_t *__restrict__ order,
                                                      uint32_t *__restrict__ ucount, int k) {
-    constexpr int IPT = SortCfg<W>::IPT, CAP = SortCfg<W>::CAP;
+    constexpr int CAP = BT * IPT;
     __shared__ uint64_t skeys[CAP * W];
     __shared__ uint32_t whist[256 * BNW];  // [digit][wave]
     __shared__ uint16_t upos[CAP + 1];
     __shared__ uint32_t ws[BNW + 1];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const Bucket bk = buckets[blockIdx.x];
+    const uint32_t bid = order[blockIdx.x];
+    const Bucket bk = buckets[bid];
     uint64_t *X = bk.array ? B : A;
     const uint64_t start = bk.start;
     const int bits = 2 * k - (int)bk.prefix;
     if (bk.size == 0) {
-        if (tid == 0) ucount[blockIdx.x] = 0;
+        if (tid == 0) ucount[bid] = 0;
         return;
     }
     if (bits <= 0 || bk.size == 1) {
         // every key in the bucket is identical: one run
         if (tid == 0) {
             counts_tmp[start] = bk.size > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)bk.size;
-            ucount[blockIdx.x] = 1;
+            ucount[bid] = 1;
         }
         return;
     }
@@ -426,19 +433,19 @@ __global__ __launch_bounds__(BT) void k_bucket_sort(uint64_t *__restrict__ A,
 #pragma unroll
         for (int it = 0; it < IPT; it++) {
             if (it < ipt) {
-            uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
-            bool valid = pos < m;
-            uint32_t d = valid ? key_bits<W>(key[it], shift, nb) : 0u;
-            uint64_t peers = __ballot(valid);
-            for (int bb = 0; bb < nb; bb++) {
-                uint64_t mb = __ballot(valid && ((d >> bb) & 1u));
-                peers &= ((d >> bb) & 1u) ? mb : ~mb;
-            }
-            uint32_t below = __popcll(peers & lanemask_lt());
-            uint32_t basev = whist[d * BNW + wave];
-            rank[it] = basev + below;
-            dig[it] = d;
-            if (valid && below == 0) whist[d * BNW + wave] = basev + (uint32_t)__popcll(peers);
+                uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
+                bool valid = pos < m;
+                uint32_t d = valid ? key_bits<W>(key[it], shift, nb) : 0u;
+                uint64_t peers = __ballot(valid);
+                for (int bb = 0; bb < nb; bb++) {
+                    uint64_t mb = __ballot(valid && ((d >> bb) & 1u));
+                    peers &= ((d >> bb) & 1u) ? mb : ~mb;
+                }
+                uint32_t below = __popcll(peers & lanemask_lt());
+                uint32_t basev = whist[d * BNW + wave];
+                rank[it] = basev + below;
+                dig[it] = d;
+                if (valid && below == 0) whist[d * BNW + wave] = basev + (uint32_t)__popcll(peers);
             }
         }
         __syncthreads();
@@ -509,7 +516,7 @@ __global__ __launch_bounds__(BT) void k_bucket_sort(uint64_t *__restrict__ A,
         for (int r = 0; r < W; r++) d[r] = skeys[p * W + r];
         counts_tmp[start + q] = c;
     }
-    if (tid == 0) ucount[blockIdx.x] = utotal;
+    if (tid == 0) ucount[bid] = utotal;
 }
 
 template <int W>
@@ -574,7 +581,8 @@ void upload(DevBuf &buf, const std::vector<T> &v, hipStream_t s) {
         KITE_HIP(hipMemcpyAsync(buf.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
 }
 
-int sort_cap(int W) { return W == 1 ? SortCfg<1>::CAP : (W == 2 ? SortCfg<2>::CAP : SortCfg<4>::CAP); }
+int class_cap(int W, int c) { return BT * ((4 << c) / W); }
+int sort_cap(int W) { return class_cap(W, SORT_CLASSES - 1); }
 int split_tk(int W) { return W == 1 ? SplitCfg<1>::TK : (W == 2 ? SplitCfg<2>::TK : SplitCfg<4>::TK); }
 }  // namespace
 
@@ -640,6 +648,8 @@ void CountEngine::run(const uint8_t *d_fq, uint64_t len, int k, int canonical, u
                     " are not supported by the GPU count engine");
     int pbits = __builtin_ctz(partitions);
     words_ = k <= 31 ? 1 : (k <= 64 ? 2 : 4);
+    if (const char *e = getenv("KITE_SPLIT_DIV")) tun.split_target_div = std::max(1, atoi(e));
+    if (const char *e = getenv("KITE_MAX_DBITS")) tun.max_digit_bits = std::min(MAX_DBITS, std::max(1, atoi(e)));
     if (words_ == 1) run_w<1>(d_fq, len, k, canonical, pbits);
     else if (words_ == 2) run_w<2>(d_fq, len, k, canonical, pbits);
     else run_w<4>(d_fq, len, k, canonical, pbits);
@@ -840,10 +850,36 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
     uint32_t *ucount = static_cast<uint32_t *>(ucount_.ensure(std::max<uint64_t>(nb, 1) * 4));
     if (nb > 0) {
         upload(buckets_, nodes, s);
-        launch("k_bucket_sort", 0, [&] {
-            k_bucket_sort<W><<<dim3((unsigned)nb), dim3(BT), 0, s>>>(A, B, ctmp, buckets_.as<Bucket>(),
-                                                                  ucount, k);
-        });
+        // size classes: the smallest LDS footprint that holds each bucket
+        std::vector<uint32_t> order;
+        order.reserve(nb);
+        uint32_t cls_off[SORT_CLASSES + 1] = {0};
+        for (int c = 0; c < SORT_CLASSES; c++) {
+            cls_off[c] = (uint32_t)order.size();
+            for (uint32_t b = 0; b < nb; b++) {
+                const Bucket &bk = nodes[b];
+                int cb = 0;
+                if ((int)bk.prefix < kbits && bk.size > 1)
+                    while (cb < SORT_CLASSES - 1 && bk.size > (uint64_t)class_cap(W, cb)) cb++;
+                if (cb == c) order.push_back(b);
+            }
+        }
+        cls_off[SORT_CLASSES] = (uint32_t)order.size();
+        upload(order_, order, s);
+        const uint32_t *ord = order_.as<uint32_t>();
+        auto go = [&](auto cls, uint32_t first, uint32_t cnt) {
+            constexpr int C = decltype(cls)::value;
+            if (!cnt) return;
+            launch("k_bucket_sort", 0, [&] {
+                k_bucket_sort<W, SortClass<W, C>::IPT><<<dim3(cnt), dim3(BT), 0, s>>>(
+                    A, B, ctmp, buckets_.as<Bucket>(), ord + first, ucount, k);
+            });
+        };
+        go(std::integral_constant<int, 0>{}, cls_off[0], cls_off[1] - cls_off[0]);
+        go(std::integral_constant<int, 1>{}, cls_off[1], cls_off[2] - cls_off[1]);
+        go(std::integral_constant<int, 2>{}, cls_off[2], cls_off[3] - cls_off[2]);
+        go(std::integral_constant<int, 3>{}, cls_off[3], cls_off[4] - cls_off[3]);
+        KITE_HIP(hipStreamSynchronize(s));  // `order` is a local vector
     }
     tm.mark();
     uint64_t *stmp2 = static_cast<uint64_t *>(scan_tmp_.ensure(

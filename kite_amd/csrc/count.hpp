@@ -43,7 +43,7 @@ struct KernelStat {
 };
 
 struct CountTunables {
-    int split_target_div = 2;  // split partitions until max child <= cap / div
+    int split_target_div = 4;  // split partitions until max child <= largest cap / div
     int max_digit_bits = 12;
 };
 
@@ -96,6 +96,7 @@ class CountEngine {
     DevBuf nl_, line0_, scan_tmp_, hist_, off_, part_off_d_;
     DevBuf a_, b_, counts_tmp_, out_keys_, out_counts_;
     DevBuf segs_, tile_pre_, child_hist_, child_cur_, buckets_, ucount_, uoff_, part_first_, part_u_;
+    DevBuf order_;
     std::vector<uint64_t> part_off_h_;
     uint64_t n_kmers_ = 0, n_unique_ = 0, n_buckets_ = 0;
     int words_ = 1, levels_ = 0;

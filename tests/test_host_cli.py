@@ -1,0 +1,154 @@
+"""The C++ host (`kite_amd/lib/kite`) on the CPU: dump, histo and info
+against the oracle and the golden fixtures, on files the oracle and an
+independent Python writer produce (no GPU involved)."""
+import json
+import os
+import random
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import brute
+import kite_file as kf
+import oracle_lib as ol
+
+ROOT = Path(__file__).resolve().parents[1]
+KITE = ROOT / "kite_amd" / "lib" / "kite"
+GOLD = ROOT / "tests" / "golden"
+
+
+def kite(*args, check=True):
+    r = subprocess.run([str(KITE), *map(str, args)], capture_output=True, text=True, timeout=300)
+    if check and r.returncode != 0:
+        raise AssertionError(f"kite {args} failed ({r.returncode}):\n{r.stderr}")
+    return r
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_binary():
+    if not KITE.exists():
+        pytest.skip("kite host binary not built")
+
+
+@pytest.mark.parametrize("name", list(json.loads((GOLD / "small.json").read_text())))
+def test_dump_matches_golden(name, tmp_path):
+    m = json.loads((GOLD / "small.json").read_text())[name]
+    fq = (GOLD / "small.fastq").read_bytes()
+    t = ol.count_fastq(fq, m["k"], m["canonical"], m["partitions"], chunk_size=700, threads=3)
+    ol.write_table(tmp_path / "a.kite", t)
+    kite("dump", "-i", tmp_path / "a.kite", "-o", tmp_path / "a.tsv")
+    assert (tmp_path / "a.tsv").read_bytes() == (GOLD / f"small_{name}.tsv").read_bytes()
+    out = kite("dump", "-i", tmp_path / "a.kite", "-o", "-").stdout.encode()
+    assert out == (GOLD / f"small_{name}.tsv").read_bytes()
+
+
+def test_dump_multirun_saturation_and_filters(tmp_path):
+    rng = random.Random(4)
+    runs = []
+    for _ in range(30):
+        p = rng.randrange(4)
+        keys = sorted(rng.sample(range(5000), 40))
+        runs.append((p, [(k, rng.choice([1, 3, 7, 0xFFFFFFF0])) for k in keys]))
+    for mn, mx in [(None, None), (2, None), (None, 100), (3, 0xFFFFFFFF)]:
+        f = tmp_path / "m.kite"
+        kf.write(f, 7, False, 4, 64, runs, min_count=mn, max_count=mx)
+        ol.kite_dump(f, tmp_path / "o.tsv")
+        kite("dump", "-i", f, "-o", tmp_path / "h.tsv", "-t", 3)
+        assert (tmp_path / "h.tsv").read_bytes() == (tmp_path / "o.tsv").read_bytes()
+
+
+@pytest.mark.parametrize("kw,k", [(128, 45), (256, 70), (256, 128)])
+def test_dump_wide_keys(kw, k, tmp_path):
+    rng = random.Random(kw + k)
+    runs = []
+    for _ in range(6):
+        keys = sorted(rng.sample(range(1 << 40), 50))
+        keys = [x * (1 << (2 * k - 45)) + rng.randrange(7) for x in keys]
+        runs.append((rng.randrange(2), sorted(set(keys)) and [(v, rng.randrange(1, 9)) for v in sorted(set(keys))]))
+    f = tmp_path / "w.kite"
+    kf.write(f, k, True, 2, kw, runs)
+    ol.kite_dump(f, tmp_path / "o.tsv")
+    kite("dump", "-i", f, "-o", tmp_path / "h.tsv")
+    assert (tmp_path / "h.tsv").read_bytes() == (tmp_path / "o.tsv").read_bytes()
+
+
+def test_histo_matches_oracle(tmp_path):
+    fq = (GOLD / "small.fastq").read_bytes()
+    paths = []
+    for i, (k, P) in enumerate([(5, 4), (5, 4), (5, 4)]):
+        t = ol.count_fastq(fq[: len(fq) // (i + 1)], k, False, P)
+        p = tmp_path / f"s{i}.kite"
+        ol.write_table(p, t, min_count=2 if i == 1 else None)
+        paths.append(p)
+    ol.kite_histo(paths, tmp_path / "o.tsv")
+    kite("histo", "-i", *paths, "-o", tmp_path / "h.tsv")
+    assert (tmp_path / "h.tsv").read_bytes() == (tmp_path / "o.tsv").read_bytes()
+    kite("histo", "-i", *paths, "-o", tmp_path / "l.tsv", "--label", "a", "b", "c")
+    assert (tmp_path / "l.tsv").read_text().splitlines()[0] == "count\ta\tb\tc"
+
+
+def test_info_marker_edits_rewrite_only_the_header(tmp_path):
+    f = tmp_path / "i.kite"
+    kf.write(f, 5, True, 2, 64, [(0, [(1, 2), (9, 5)]), (1, [(4, 1)])], min_count=1, max_count=10)
+    before = f.read_bytes()
+    kite("info", "-i", f, "--set-min", 3)
+    after = f.read_bytes()
+    assert after[128:] == before[128:]
+    h = kf.read(f)
+    assert (h.min_count, h.max_count) == (3, 10)
+    kite("info", "-i", f, "--clear-max")
+    assert (kf.read(f).min_count, kf.read(f).max_count) == (3, None)
+    kite("info", "-i", f, "--clear-min", "--set-max", 7)
+    assert (kf.read(f).min_count, kf.read(f).max_count) == (None, 7)
+    out = kite("info", "-i", f).stdout
+    assert "k\t5" in out and "partitions\t2" in out
+
+
+@pytest.mark.parametrize("bad,msg", [
+    (b"NOPE" + bytes(200), "invalid magic"),
+    (b"KX2!\x02\x00" + bytes(200), "unsupported version"),
+])
+def test_reader_rejects_bad_files(bad, msg, tmp_path):
+    f = tmp_path / "bad.kite"
+    f.write_bytes(bad)
+    r = kite("dump", "-i", f, "-o", tmp_path / "x.tsv", check=False)
+    assert r.returncode == 1 and msg in r.stderr
+
+
+def test_reader_rejects_truncated_footer(tmp_path):
+    f = tmp_path / "t.kite"
+    kf.write(f, 5, True, 2, 64, [(0, [(1, 2)])])
+    data = f.read_bytes()
+    f.write_bytes(data[:-3])
+    r = kite("dump", "-i", f, "-o", tmp_path / "x.tsv", check=False)
+    assert r.returncode == 1 and "footer" in r.stderr
+
+
+@pytest.mark.parametrize("args,msg", [
+    (["-k", "0"], "invalid k-mer length"),
+    (["-k", "129"], "invalid k-mer length"),
+    (["--partitions", "300"], "must be a power of 2"),
+    (["--min", "5", "--max", "2"], "--min must be less than or equal to --max"),
+])
+def test_count_validation_errors(args, msg, tmp_path):
+    fq = tmp_path / "x.fq"
+    fq.write_bytes(b"@r\nACGT\n+\nIIII\n")
+    r = kite("count", "-i", fq, "-o", tmp_path / "x.kite", *args, check=False)
+    assert r.returncode == 1 and msg in r.stderr
+
+
+def test_count_output_must_be_kite(tmp_path):
+    fq = tmp_path / "x.fq"
+    fq.write_bytes(b"@r\nACGT\n+\nIIII\n")
+    r = kite("count", "-i", fq, "-o", tmp_path / "x.out", check=False)
+    assert r.returncode == 1 and "output must end with .kite" in r.stderr
+
+
+def test_memory_spec_errors(tmp_path):
+    fq = tmp_path / "x.fq"
+    fq.write_bytes(b"@r\nACGT\n+\nIIII\n")
+    for spec, msg in [("12X", "suffix must be M, G, or T"), ("0", "greater than zero"), ("G", "expected formats")]:
+        r = kite("count", "-i", fq, "-o", tmp_path / "x.kite", "--memory", spec, check=False)
+        assert r.returncode == 1 and msg in r.stderr, r.stderr
