@@ -138,6 +138,48 @@ int kite_gpu_union_tables(kite_gpu_ctx *ctx, uint32_t words, uint32_t partitions
                           const uint64_t *const *part_off, uint64_t *out_keys, uint32_t *out_matrix,
                           uint64_t *out_part_off);
 
+/* ---- analyze ---------------------------------------------------------------
+ * Correlation of the columns of a union count matrix (rows = union k-mers,
+ * columns = samples): Pearson on ln(1 + count) (PearsonAccumulator,
+ * src/analyze/mod.rs:194-270, values :780-784) and Spearman as Pearson of
+ * 1-based average-tie ranks over all rows (average_ranks + pearson_slice,
+ * :406-438, :559-578).  The Gram runs on bf16 MFMA with a hi/lo split (FP32
+ * within a K-tile, FP64 across tiles); column means and variances, the
+ * reduction and the normalisation are FP64.  Any output pointer may be NULL. */
+#define KITE_CORR_PEARSON 0x1u
+#define KITE_CORR_SPEARMAN 0x2u
+#define KITE_CORR_FP64_BASELINE 0x1u /* flags: also run the FP64 MFMA Gram */
+#define KITE_CORR_LOLO 0x2u          /* flags: add the lo*lo product */
+
+typedef struct kite_corr_stats {
+    double gram_ms_pearson, gram_ms_spearman; /* split-bf16 Gram + FP64 reduction */
+    double fp64_ms_pearson, fp64_ms_spearman; /* FP64 MFMA Gram + reduction */
+    uint64_t rows;
+    uint32_t cols, code_bytes;                /* bytes per matrix cell read by the Gram */
+} kite_corr_stats;
+
+/* counts: N x n row-major u32 (host).  pearson / spearman / cov: n x n
+ * (cov of ln(1+count), divisor N-1: covariance_matrix :238-252). */
+int kite_gpu_corr_counts(kite_gpu_ctx *ctx, const uint32_t *counts, uint64_t N, uint32_t n, uint32_t methods,
+                         uint32_t flags, double *pearson, double *spearman, double *cov, double *pearson_fp64,
+                         double *spearman_fp64, kite_corr_stats *stats);
+
+/* The same on the union of n_inputs tables (see kite_gpu_union_tables),
+ * built and consumed on the device; out_keys / out_matrix / out_part_off
+ * optionally return the union (for exact_count_matrix.tsv). */
+int kite_gpu_analyze_tables(kite_gpu_ctx *ctx, uint32_t words, uint32_t partitions, uint32_t n_inputs,
+                            const uint64_t *const *keys, const uint64_t *const *counts,
+                            const uint64_t *const *part_off, uint32_t methods, uint32_t flags,
+                            uint64_t *out_keys, uint32_t *out_matrix, uint64_t *out_part_off, double *pearson,
+                            double *spearman, double *cov, double *pearson_fp64, double *spearman_fp64,
+                            kite_corr_stats *stats);
+
+/* PCA of an n x n covariance (pca_scores, src/analyze/mod.rs:1605-1625):
+ * FP64 cyclic Jacobi on the device, eigenvalues descending; scores
+ * (n x dims) = eigenvector * sqrt(max(eigenvalue, 0)).  n <= 64. */
+int kite_gpu_pca(kite_gpu_ctx *ctx, const double *cov, uint32_t n, uint32_t dims, double *scores,
+                 double *eigenvalues);
+
 /* Turns per-kernel profiling on or off for later batches: HIP events around
  * every launch on the engine's stream, plus each kernel's algorithmic HBM
  * bytes.  Counterpart of the reference's --test-mode stage timers

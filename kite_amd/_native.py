@@ -38,6 +38,20 @@ class CountStats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class CorrStats(C.Structure):
+    _fields_ = [
+        ("gram_ms_pearson", C.c_double), ("gram_ms_spearman", C.c_double),
+        ("fp64_ms_pearson", C.c_double), ("fp64_ms_spearman", C.c_double),
+        ("rows", C.c_uint64), ("cols", C.c_uint32), ("code_bytes", C.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+CORR_PEARSON, CORR_SPEARMAN = 1, 2
+CORR_FP64_BASELINE, CORR_LOLO = 1, 2
+
 APPEND_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint64,
                         C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint8),
                         C.c_size_t)
@@ -64,6 +78,16 @@ GPU_SYMBOLS = {
     "kite_gpu_union_tables": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
                                         C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                         C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p, C.c_void_p]),
+    "kite_gpu_corr_counts": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32,
+                                       C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.POINTER(CorrStats)]),
+    "kite_gpu_analyze_tables": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                          C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                          C.POINTER(C.c_void_p), C.c_uint32, C.c_uint32, C.c_void_p,
+                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_void_p, C.POINTER(CorrStats)]),
+    "kite_gpu_pca": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
+                               C.c_void_p]),
     "kite_gpu_count_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "kite_gpu_count_kernel_stats": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t]),
 }
@@ -231,6 +255,55 @@ class GpuContext:
                                                ok.ctypes.data, om.ctypes.data, oo.ctypes.data))
         u = int(oo[-1])
         return ok[:u], om[:u], oo
+
+    # -- analyze (kite_gpu_corr_counts, kite_gpu_analyze_tables, kite_gpu_pca)
+    def corr_counts(self, counts: np.ndarray, methods: int = 3, flags: int = 0) -> dict:
+        """Correlation of the columns of an N x n u32 count matrix."""
+        m = np.ascontiguousarray(counts, dtype=np.uint32)
+        N, n = m.shape
+        outs = {k: np.zeros((n, n)) for k in ("pearson", "spearman", "cov", "pearson_fp64",
+                                               "spearman_fp64")}
+        st = CorrStats()
+        _check(self._lib.kite_gpu_corr_counts(
+            self._h, m.ctypes.data if N else None, N, n, methods, flags,
+            *[outs[k].ctypes.data for k in ("pearson", "spearman", "cov", "pearson_fp64",
+                                             "spearman_fp64")], C.byref(st)))
+        outs["stats"] = st.as_dict()
+        return outs
+
+    def analyze_tables(self, tables, words: int, methods: int = 3, flags: int = 0,
+                       want_union: bool = False) -> dict:
+        ks, cs, os_, kp, cp, op = self._tables(tables)
+        P = len(os_[0]) - 1
+        n = len(tables)
+        rows = sum(int(o[-1]) for o in os_)
+        uk = np.empty((max(rows, 1), words), dtype=np.uint64) if want_union else None
+        um = np.empty((max(rows, 1), n), dtype=np.uint32) if want_union else None
+        uo = np.empty(P + 1, dtype=np.uint64)
+        outs = {k: np.zeros((n, n)) for k in ("pearson", "spearman", "cov", "pearson_fp64",
+                                               "spearman_fp64")}
+        st = CorrStats()
+        _check(self._lib.kite_gpu_analyze_tables(
+            self._h, words, P, n, kp, cp, op, methods, flags,
+            uk.ctypes.data if want_union else None, um.ctypes.data if want_union else None,
+            uo.ctypes.data,
+            *[outs[k].ctypes.data for k in ("pearson", "spearman", "cov", "pearson_fp64",
+                                             "spearman_fp64")], C.byref(st)))
+        outs["stats"] = st.as_dict()
+        outs["part_off"] = uo
+        if want_union:
+            u = int(uo[-1])
+            outs["keys"], outs["matrix"] = uk[:u], um[:u]
+        return outs
+
+    def pca(self, cov: np.ndarray, dims: int) -> tuple[np.ndarray, np.ndarray]:
+        c = np.ascontiguousarray(cov, dtype=np.float64)
+        n = c.shape[0]
+        scores = np.zeros((n, dims))
+        ev = np.zeros(n)
+        _check(self._lib.kite_gpu_pca(self._h, c.ctypes.data, n, dims, scores.ctypes.data,
+                                      ev.ctypes.data))
+        return scores, ev
 
     def count_profile(self, enable: bool) -> None:
         _check(self._lib.kite_gpu_count_profile(self._h, int(enable)))

@@ -2,6 +2,7 @@
 // include/kite_gpu.h): context, count session, run-chunk emission.
 #include <algorithm>
 #include <atomic>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -11,6 +12,7 @@
 #include "../../include/kite_gpu.h"
 #include "count.hpp"
 #include "dvle.hpp"
+#include "corr.hpp"
 #include "merge.hpp"
 
 namespace {
@@ -53,6 +55,7 @@ struct kite_gpu_ctx {
     uint64_t budget = 0;
     std::unique_ptr<kite::CountEngine> count;
     std::unique_ptr<kite::MergeEngine> merge;
+    std::unique_ptr<kite::CorrEngine> corr;
 
     // count session
     bool session = false;
@@ -170,6 +173,7 @@ int kite_gpu_open(int device, uint64_t hbm_budget_bytes, kite_gpu_ctx **out) {
         c->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)(free_b * 0.9);
         c->count = std::make_unique<kite::CountEngine>(c->stream);
         c->merge = std::make_unique<kite::MergeEngine>(c->stream);
+        c->corr = std::make_unique<kite::CorrEngine>(c->stream);
         *out = c.release();
         return KITE_OK;
     });
@@ -182,6 +186,7 @@ void kite_gpu_close(kite_gpu_ctx *ctx) {
         (void)hipStreamSynchronize(ctx->stream);
         ctx->count.reset();
         ctx->merge.reset();
+        ctx->corr.reset();
         ctx->staging.release();
         (void)hipStreamDestroy(ctx->stream);
     } catch (...) {
@@ -407,6 +412,112 @@ int kite_gpu_union_tables(kite_gpu_ctx *ctx, uint32_t words, uint32_t partitions
         ctx->merge->download(u, (int)words, partitions, out_keys, nullptr, out_part_off);
         if (out_matrix && u.rows)
             KITE_HIP(hipMemcpy(out_matrix, m.ptr, u.rows * n_inputs * 4, hipMemcpyDeviceToHost));
+        return KITE_OK;
+    });
+}
+
+namespace {
+void copy_out(const std::vector<double> &v, double *dst) {
+    if (dst && !v.empty()) std::copy(v.begin(), v.end(), dst);
+}
+
+int corr_device(kite_gpu_ctx *ctx, const uint32_t *d_counts, uint64_t N, uint32_t n, uint32_t methods,
+                uint32_t flags, double *pearson, double *spearman, double *cov, double *pearson_fp64,
+                double *spearman_fp64, kite_corr_stats *stats) {
+    if (n < 1 || n > 32) return set_error(KITE_E_INVALID, "invalid argument: analyze supports 1..32 samples on the GPU");
+    // maximum count decides the code width the Gram reads
+    const uint32_t maxc = ctx->corr->max_count(d_counts, N * n);
+    if (maxc > (64u << 20))
+        return set_error(KITE_E_INVALID, "counts above 2^26 are not supported by the correlation engine");
+    const int cb = maxc < 65535 ? 2 : 4;
+    kite::DevBuf codes;
+    const void *dc = d_counts;
+    if (cb == 2) {
+        ctx->corr->to_codes(d_counts, N * n, codes, 2);
+        dc = codes.ptr;
+    }
+    kite::CorrRequest req;
+    req.methods = methods;
+    req.fp64_baseline = (flags & KITE_CORR_FP64_BASELINE) != 0;
+    req.with_lolo = (flags & KITE_CORR_LOLO) != 0;
+    kite::CorrOutput out;
+    ctx->corr->run(dc, cb, N, (int)n, maxc, req, out);
+    copy_out(out.pearson, pearson);
+    copy_out(out.spearman, spearman);
+    copy_out(out.cov, cov);
+    copy_out(out.pearson_fp64, pearson_fp64);
+    copy_out(out.spearman_fp64, spearman_fp64);
+    if (stats) {
+        stats->gram_ms_pearson = out.gram_ms_pearson;
+        stats->gram_ms_spearman = out.gram_ms_spearman;
+        stats->fp64_ms_pearson = out.fp64_ms_pearson;
+        stats->fp64_ms_spearman = out.fp64_ms_spearman;
+        stats->rows = N;
+        stats->cols = n;
+        stats->code_bytes = (uint32_t)cb;
+    }
+    return KITE_OK;
+}
+}  // namespace
+
+int kite_gpu_corr_counts(kite_gpu_ctx *ctx, const uint32_t *counts, uint64_t N, uint32_t n, uint32_t methods,
+                         uint32_t flags, double *pearson, double *spearman, double *cov, double *pearson_fp64,
+                         double *spearman_fp64, kite_corr_stats *stats) {
+    if (!ctx) return set_error(KITE_E_INVALID, "invalid argument: ctx is NULL");
+    if (!counts && N) return set_error(KITE_E_INVALID, "invalid argument: counts is NULL");
+    return guarded([&]() -> int {
+        bind(ctx);
+        kite::DevBuf d;
+        d.ensure(std::max<uint64_t>(N * n, 1) * 4);
+        if (N) KITE_HIP(hipMemcpy(d.ptr, counts, N * n * 4, hipMemcpyHostToDevice));
+        return corr_device(ctx, d.as<uint32_t>(), N, n, methods, flags, pearson, spearman, cov, pearson_fp64,
+                           spearman_fp64, stats);
+    });
+}
+
+int kite_gpu_analyze_tables(kite_gpu_ctx *ctx, uint32_t words, uint32_t partitions, uint32_t n_inputs,
+                            const uint64_t *const *keys, const uint64_t *const *counts,
+                            const uint64_t *const *part_off, uint32_t methods, uint32_t flags,
+                            uint64_t *out_keys, uint32_t *out_matrix, uint64_t *out_part_off, double *pearson,
+                            double *spearman, double *cov, double *pearson_fp64, double *spearman_fp64,
+                            kite_corr_stats *stats) {
+    if (!ctx) return set_error(KITE_E_INVALID, "invalid argument: ctx is NULL");
+    return guarded([&]() -> int {
+        bind(ctx);
+        std::vector<kite::DevTable> tabs;
+        int rc = load_tables(ctx, words, partitions, n_inputs, keys, counts, part_off, tabs);
+        if (rc) return rc;
+        kite::DevTable u;
+        ctx->merge->merge_all(tabs, partitions, (int)words, u);
+        kite::DevBuf m;
+        ctx->merge->union_matrix(tabs, u, partitions, (int)words, m);
+        tabs.clear();
+        ctx->merge->download(u, (int)words, partitions, out_keys, nullptr, out_part_off);
+        if (out_matrix && u.rows) KITE_HIP(hipMemcpy(out_matrix, m.ptr, u.rows * n_inputs * 4, hipMemcpyDeviceToHost));
+        return corr_device(ctx, m.as<uint32_t>(), u.rows, n_inputs, methods, flags, pearson, spearman, cov,
+                           pearson_fp64, spearman_fp64, stats);
+    });
+}
+
+int kite_gpu_pca(kite_gpu_ctx *ctx, const double *cov, uint32_t n, uint32_t dims, double *scores,
+                 double *eigenvalues) {
+    if (!ctx || !cov) return set_error(KITE_E_INVALID, "invalid argument: NULL pointer");
+    if (n < 1 || n > 64) return set_error(KITE_E_INVALID, "invalid argument: PCA supports 1..64 samples");
+    return guarded([&]() -> int {
+        bind(ctx);
+        std::vector<double> c(cov, cov + (size_t)n * n), ev, V;
+        ctx->corr->pca(c, (int)n, ev, V);
+        std::vector<int> order(n);
+        for (uint32_t i = 0; i < n; i++) order[i] = (int)i;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ev[a] > ev[b]; });
+        if (eigenvalues)
+            for (uint32_t d = 0; d < n; d++) eigenvalues[d] = ev[order[d]];
+        if (scores)
+            for (uint32_t d = 0; d < dims; d++) {
+                const bool have = d < n;
+                const double sc = have ? std::sqrt(std::max(0.0, ev[order[d]])) : 0.0;
+                for (uint32_t i = 0; i < n; i++) scores[i * dims + d] = have ? V[i * n + order[d]] * sc : 0.0;
+            }
         return KITE_OK;
     });
 }

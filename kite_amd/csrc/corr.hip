@@ -1,0 +1,468 @@
+// corr.hip -- Pearson (log1p counts) and Spearman (average-tie ranks)
+// correlation of union count columns, and the FP64 Jacobi PCA.
+//
+// Reference semantics (src/analyze/mod.rs): values are ln(1 + count)
+// (:780-784); Pearson r = centered cross / sqrt(centered sumsq products),
+// 0 when the denominator is <= 0 (:222-236); Spearman = Pearson of 1-based
+// average-tie ranks over all union rows, zeros included (:406-438,
+// :559-578); covariance divides by N-1 (:238-252); PCA scores are
+// v * sqrt(max(lambda, 0)) with eigenvalues descending (:1605-1625).
+//
+// Every value depends only on (column, count), so the column statistics
+// come exactly from per-column count histograms: means and variances in
+// FP64, ranks as cum(<c) + (h(c) + 1) / 2.  The only per-row work is the
+// centered Gram G = Z^T Z of standardized values z = (v - mean) / sd, done
+// with bf16 MFMA on a hi/lo split of z (hi*hi + hi*lo + lo*hi, FP32 inside
+// a K-tile, FP64 across tiles), or with FP64 MFMA as the baseline.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "corr.hpp"
+
+namespace kite {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+constexpr int HIST_SMALL = 256;   // LDS-aggregated count values per column
+constexpr int MAX_COLS = 32;      // one 32x32 MFMA tile of columns
+constexpr int GT = 256;           // Gram workgroup threads (4 waves)
+constexpr int FLUSH_STEPS = 32;   // bf16 steps (16 rows each) per FP32 flush: 512 rows
+
+template <class T>
+__global__ __launch_bounds__(256) void k_col_hist(const T *__restrict__ codes, uint64_t N, int n,
+                                                  uint32_t stride, uint32_t *__restrict__ hist) {
+    __shared__ uint32_t sm[MAX_COLS * HIST_SMALL];
+    for (int i = threadIdx.x; i < n * HIST_SMALL; i += 256) sm[i] = 0;
+    __syncthreads();
+    const uint64_t total = N * (uint64_t)n;
+    for (uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (uint64_t)gridDim.x * 256) {
+        uint32_t c = (uint32_t)codes[e];
+        uint32_t j = (uint32_t)(e % (uint64_t)n);
+        if (c < HIST_SMALL) atomicAdd(&sm[j * HIST_SMALL + c], 1u);
+        else atomicAdd(&hist[(uint64_t)j * stride + c], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n * HIST_SMALL; i += 256) {
+        uint32_t v = sm[i];
+        if (v) atomicAdd(&hist[(uint64_t)(i / HIST_SMALL) * stride + (i % HIST_SMALL)], v);
+    }
+}
+
+__device__ __forceinline__ __bf16 bf16_from_bits(uint16_t b) {
+    __bf16 r;
+    memcpy(&r, &b, 2);
+    return r;
+}
+
+// lut: per column `stride` entries of packed (hi | lo << 16) bf16 bits.
+// partial: [wave][16][64] FP64 accumulators in MFMA output layout.
+template <class T>
+__global__ __launch_bounds__(GT) void k_gram_bf16(const T *__restrict__ codes, uint64_t N, int n,
+                                                   const uint32_t *__restrict__ lut, uint32_t stride,
+                                                   uint64_t rows_per_wave, int with_lolo,
+                                                   double *__restrict__ partial) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t wave = (uint64_t)blockIdx.x * (GT / 64) + (threadIdx.x >> 6);
+    const uint64_t r0 = wave * rows_per_wave;
+    const uint64_t r1 = std::min<uint64_t>(N, r0 + rows_per_wave);
+    const int col = lane & 31;
+    const int kb = 8 * (lane >> 5);
+    const bool colok = col < n;
+    const uint32_t *lcol = lut + (uint64_t)(colok ? col : 0) * stride;
+    floatx16 acc;
+    double facc[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        acc[i] = 0.f;
+        facc[i] = 0.0;
+    }
+    int steps = 0;
+    for (uint64_t r = r0; r < r1; r += 16) {
+        bf16x8 hi, lo;
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            uint64_t row = r + kb + t;
+            uint32_t packed = 0;
+            if (colok && row < r1) packed = lcol[(uint32_t)codes[row * n + col]];
+            hi[t] = bf16_from_bits((uint16_t)(packed & 0xFFFF));
+            lo[t] = bf16_from_bits((uint16_t)(packed >> 16));
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi, hi, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi, lo, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lo, hi, acc, 0, 0, 0);
+        if (with_lolo) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lo, lo, acc, 0, 0, 0);
+        if (++steps == FLUSH_STEPS) {
+            steps = 0;
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                facc[i] += (double)acc[i];
+                acc[i] = 0.f;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 16; i++) facc[i] += (double)acc[i];
+#pragma unroll
+    for (int i = 0; i < 16; i++) partial[(wave * 16 + i) * 64 + lane] = facc[i];
+}
+
+// FP64 baseline: v_mfma_f64_16x16x4 on the same codes, FP64 values.
+// partial: [wave][3 blocks (00, 01, 11)][4][64].
+template <class T>
+__global__ __launch_bounds__(GT) void k_gram_f64(const T *__restrict__ codes, uint64_t N, int n,
+                                                  const double *__restrict__ lut, uint32_t stride,
+                                                  uint64_t rows_per_wave, double *__restrict__ partial) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t wave = (uint64_t)blockIdx.x * (GT / 64) + (threadIdx.x >> 6);
+    const uint64_t r0 = wave * rows_per_wave;
+    const uint64_t r1 = std::min<uint64_t>(N, r0 + rows_per_wave);
+    const int c0 = lane & 15, c1 = 16 + (lane & 15);
+    const int kk = lane >> 4;
+    const bool ok0 = c0 < n, ok1 = c1 < n;
+    doublex4 d00 = {0, 0, 0, 0}, d01 = {0, 0, 0, 0}, d11 = {0, 0, 0, 0};
+    for (uint64_t r = r0; r < r1; r += 4) {
+        uint64_t row = r + kk;
+        double a0 = 0, a1 = 0;
+        if (row < r1) {
+            if (ok0) a0 = lut[(uint64_t)c0 * stride + (uint32_t)codes[row * n + c0]];
+            if (ok1) a1 = lut[(uint64_t)c1 * stride + (uint32_t)codes[row * n + c1]];
+        }
+        d00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, a0, d00, 0, 0, 0);
+        d01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, a1, d01, 0, 0, 0);
+        d11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, a1, d11, 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        partial[((wave * 3 + 0) * 4 + i) * 64 + lane] = d00[i];
+        partial[((wave * 3 + 1) * 4 + i) * 64 + lane] = d01[i];
+        partial[((wave * 3 + 2) * 4 + i) * 64 + lane] = d11[i];
+    }
+}
+
+// Sum partials over waves in a fixed order (deterministic).
+__global__ void k_sum_partials(const double *__restrict__ partial, uint64_t waves, uint32_t per_wave,
+                               double *__restrict__ out) {
+    uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= per_wave) return;
+    double s = 0;
+    for (uint64_t w = 0; w < waves; w++) s += partial[w * per_wave + e];
+    out[e] = s;
+}
+
+// Cyclic Jacobi on a symmetric n x n FP64 matrix in one wave (n <= 64):
+// lane k owns row k of A and of V; each rotation reads the old rows p, q.
+__global__ __launch_bounds__(64) void k_jacobi(double *__restrict__ A, double *__restrict__ V, int n,
+                                               int max_sweeps, double *__restrict__ evals) {
+    __shared__ double a[64][65];
+    __shared__ double v[64][65];
+    const int k = threadIdx.x;
+    for (int j = 0; j < n; j++)
+        if (k < n) {
+            a[k][j] = A[k * n + j];
+            v[k][j] = (k == j) ? 1.0 : 0.0;
+        }
+    __syncthreads();
+    for (int sweep = 0; sweep < max_sweeps; sweep++) {
+        double off = 0, tot = 0;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                tot += a[i][j] * a[i][j];
+                if (i != j) off += a[i][j] * a[i][j];
+            }
+        if (off <= 1e-30 * tot || off == 0.0) break;
+        for (int p = 0; p < n - 1; p++)
+            for (int q = p + 1; q < n; q++) {
+                const double apq = a[p][q];
+                if (apq == 0.0) continue;
+                const double app = a[p][p], aqq = a[q][q];
+                const double theta = (aqq - app) / (2.0 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                double akp = 0, akq = 0, vkp = 0, vkq = 0;
+                if (k < n) {
+                    akp = a[k][p];
+                    akq = a[k][q];
+                    vkp = v[k][p];
+                    vkq = v[k][q];
+                }
+                __syncthreads();
+                if (k < n) {
+                    double nkp = c * akp - s * akq, nkq = s * akp + c * akq;
+                    a[k][p] = nkp;
+                    a[k][q] = nkq;
+                    a[p][k] = nkp;
+                    a[q][k] = nkq;
+                    v[k][p] = c * vkp - s * vkq;
+                    v[k][q] = s * vkp + c * vkq;
+                }
+                __syncthreads();
+                if (k == 0) {
+                    a[p][p] = app - t * apq;
+                    a[q][q] = aqq + t * apq;
+                    a[p][q] = 0.0;
+                    a[q][p] = 0.0;
+                }
+                __syncthreads();
+            }
+    }
+    if (k < n) {
+        evals[k] = a[k][k];
+        for (int j = 0; j < n; j++) V[k * n + j] = v[k][j];
+    }
+}
+
+__global__ void k_max_u32(const uint32_t *__restrict__ p, uint64_t n, uint32_t *__restrict__ out) {
+    uint32_t m = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        m = std::max(m, p[i]);
+    for (int d = 32; d > 0; d >>= 1) m = std::max(m, (uint32_t)__shfl_xor((int)m, d, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+__global__ void k_zero_cells(uint32_t *p, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+template <class T>
+__global__ void k_u32_to_codes(const uint32_t *__restrict__ in, uint64_t n, T *__restrict__ out) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (T)in[i];
+}
+
+// ---- host -----------------------------------------------------------------
+namespace {
+uint16_t bf16_bits_rne(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    uint32_t r = u + 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(r >> 16);
+}
+float bf16_to_float(uint16_t b) {
+    uint32_t u = (uint32_t)b << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+}  // namespace
+
+void CorrEngine::column_stats(const void *d_codes, int code_bytes, uint64_t N, int n, uint32_t maxc) {
+    const uint32_t stride = std::max<uint32_t>(maxc + 1, HIST_SMALL);
+    stride_ = stride;
+    hist_.ensure((uint64_t)n * stride * 4);
+    {
+        const uint64_t cells = (uint64_t)n * stride;
+        k_zero_cells<<<(uint32_t)((cells + 255) / 256), 256, 0, stream_>>>(hist_.as<uint32_t>(), cells);
+        KITE_CHECK_LAUNCH();
+    }
+    const uint64_t total = N * (uint64_t)n;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(4096, std::max<uint64_t>(1, (total + 255) / 256));
+    if (total) {
+        if (code_bytes == 2)
+            k_col_hist<uint16_t><<<grid, 256, 0, stream_>>>((const uint16_t *)d_codes, N, n, stride, hist_.as<uint32_t>());
+        else
+            k_col_hist<uint32_t><<<grid, 256, 0, stream_>>>((const uint32_t *)d_codes, N, n, stride, hist_.as<uint32_t>());
+        KITE_CHECK_LAUNCH();
+    }
+    hist_h_.resize((uint64_t)n * stride);
+    KITE_HIP(hipMemcpyAsync(hist_h_.data(), hist_.ptr, hist_h_.size() * 4, hipMemcpyDeviceToHost, stream_));
+    KITE_HIP(hipStreamSynchronize(stream_));
+}
+
+// Standardized value per (column, count): z = (v - mean) / sd, sd over N.
+void CorrEngine::make_lut(int method, uint64_t N, int n, std::vector<double> &z, std::vector<double> &sd) {
+    const uint32_t stride = stride_;
+    z.assign((uint64_t)n * stride, 0.0);
+    sd.assign(n, 0.0);
+    for (int j = 0; j < n; j++) {
+        const uint32_t *h = hist_h_.data() + (uint64_t)j * stride;
+        std::vector<double> val(stride, 0.0);
+        double cum = 0;
+        for (uint32_t c = 0; c < stride; c++) {
+            if (method == 0) val[c] = std::log1p((double)c);
+            else val[c] = cum + ((double)h[c] + 1.0) / 2.0;  // average tie rank, 1-based
+            cum += h[c];
+        }
+        double mean = 0;
+        for (uint32_t c = 0; c < stride; c++) mean += (double)h[c] * val[c];
+        mean /= (double)N;
+        if (method == 1) mean = ((double)N + 1.0) / 2.0;  // exact mean of 1..N ranks
+        double var = 0;
+        for (uint32_t c = 0; c < stride; c++) {
+            double d = val[c] - mean;
+            var += (double)h[c] * d * d;
+        }
+        double s = var > 0 ? std::sqrt(var / (double)N) : 0.0;
+        sd[j] = s;
+        for (uint32_t c = 0; c < stride; c++) z[(uint64_t)j * stride + c] = s > 0 ? (val[c] - mean) / s : 0.0;
+    }
+}
+
+void CorrEngine::gram(const void *d_codes, int code_bytes, uint64_t N, int n, const std::vector<double> &z,
+                      bool fp64, bool with_lolo, std::vector<double> &G, float *ms) {
+    const uint32_t stride = stride_;
+    const uint64_t rpw = std::max<uint64_t>(512, ((N + 4095) / 4096 + 511) / 512 * 512);  // multiple of 16
+    const uint64_t waves = std::max<uint64_t>(1, (N + rpw - 1) / rpw);
+    const uint64_t wgs = (waves + 3) / 4;
+    const uint64_t waves_launched = wgs * 4;
+    hipEvent_t e0, e1;
+    KITE_HIP(hipEventCreate(&e0));
+    KITE_HIP(hipEventCreate(&e1));
+    std::vector<double> out;
+    if (!fp64) {
+        std::vector<uint32_t> lut(z.size());
+        for (size_t i = 0; i < z.size(); i++) {
+            float zf = (float)z[i];
+            uint16_t hi = bf16_bits_rne(zf);
+            uint16_t lo = bf16_bits_rne((float)(z[i] - (double)bf16_to_float(hi)));
+            lut[i] = (uint32_t)hi | ((uint32_t)lo << 16);
+        }
+        lut_.ensure(lut.size() * 4);
+        KITE_HIP(hipMemcpyAsync(lut_.ptr, lut.data(), lut.size() * 4, hipMemcpyHostToDevice, stream_));
+        const uint32_t per_wave = 16 * 64;
+        partial_.ensure(waves_launched * per_wave * 8);
+        sum_.ensure(per_wave * 8);
+        KITE_HIP(hipEventRecord(e0, stream_));
+        if (code_bytes == 2)
+            k_gram_bf16<uint16_t><<<(uint32_t)wgs, GT, 0, stream_>>>((const uint16_t *)d_codes, N, n, lut_.as<uint32_t>(),
+                                                                     stride, rpw, with_lolo, partial_.as<double>());
+        else
+            k_gram_bf16<uint32_t><<<(uint32_t)wgs, GT, 0, stream_>>>((const uint32_t *)d_codes, N, n, lut_.as<uint32_t>(),
+                                                                     stride, rpw, with_lolo, partial_.as<double>());
+        KITE_CHECK_LAUNCH();
+        k_sum_partials<<<(per_wave + 255) / 256, 256, 0, stream_>>>(partial_.as<double>(), waves_launched, per_wave,
+                                                                    sum_.as<double>());
+        KITE_CHECK_LAUNCH();
+        KITE_HIP(hipEventRecord(e1, stream_));
+        out.resize(per_wave);
+        KITE_HIP(hipMemcpyAsync(out.data(), sum_.ptr, per_wave * 8, hipMemcpyDeviceToHost, stream_));
+        KITE_HIP(hipStreamSynchronize(stream_));
+        // lane l, element i -> (m, n) = (8*(i/4) + 4*(l/32) + i%4, l%32)
+        G.assign(MAX_COLS * MAX_COLS, 0.0);
+        for (int i = 0; i < 16; i++)
+            for (int l = 0; l < 64; l++) {
+                int m = 8 * (i / 4) + 4 * (l / 32) + (i % 4), nn = l % 32;
+                G[m * MAX_COLS + nn] = out[i * 64 + l];
+            }
+    } else {
+        lut_.ensure(z.size() * 8);
+        KITE_HIP(hipMemcpyAsync(lut_.ptr, z.data(), z.size() * 8, hipMemcpyHostToDevice, stream_));
+        const uint64_t rpw4 = (rpw + 3) / 4 * 4;
+        const uint64_t waves4 = std::max<uint64_t>(1, (N + rpw4 - 1) / rpw4);
+        const uint64_t wgs4 = (waves4 + 3) / 4;
+        const uint32_t per_wave = 3 * 4 * 64;
+        partial_.ensure(wgs4 * 4 * per_wave * 8);
+        sum_.ensure(per_wave * 8);
+        KITE_HIP(hipEventRecord(e0, stream_));
+        if (code_bytes == 2)
+            k_gram_f64<uint16_t><<<(uint32_t)wgs4, GT, 0, stream_>>>((const uint16_t *)d_codes, N, n, lut_.as<double>(),
+                                                                     stride, rpw4, partial_.as<double>());
+        else
+            k_gram_f64<uint32_t><<<(uint32_t)wgs4, GT, 0, stream_>>>((const uint32_t *)d_codes, N, n, lut_.as<double>(),
+                                                                     stride, rpw4, partial_.as<double>());
+        KITE_CHECK_LAUNCH();
+        k_sum_partials<<<(per_wave + 255) / 256, 256, 0, stream_>>>(partial_.as<double>(), wgs4 * 4, per_wave,
+                                                                    sum_.as<double>());
+        KITE_CHECK_LAUNCH();
+        KITE_HIP(hipEventRecord(e1, stream_));
+        out.resize(per_wave);
+        KITE_HIP(hipMemcpyAsync(out.data(), sum_.ptr, per_wave * 8, hipMemcpyDeviceToHost, stream_));
+        KITE_HIP(hipStreamSynchronize(stream_));
+        // block b in (00, 01, 11); lane l, element i -> (m, n) = (4*(l/16) + i, l%16)
+        G.assign(MAX_COLS * MAX_COLS, 0.0);
+        for (int b = 0; b < 3; b++)
+            for (int i = 0; i < 4; i++)
+                for (int l = 0; l < 64; l++) {
+                    int m = 4 * (l / 16) + i, nn = l % 16;
+                    int bm = b == 2 ? 16 : 0, bn = b == 0 ? 0 : 16;
+                    double v = out[(b * 4 + i) * 64 + l];
+                    G[(bm + m) * MAX_COLS + bn + nn] = v;
+                    if (b == 1) G[(bn + nn) * MAX_COLS + bm + m] = v;
+                }
+    }
+    KITE_HIP(hipEventElapsedTime(ms, e0, e1));
+    KITE_HIP(hipEventDestroy(e0));
+    KITE_HIP(hipEventDestroy(e1));
+}
+
+void CorrEngine::run(const void *d_codes, int code_bytes, uint64_t N, int n, uint32_t maxc, const CorrRequest &req,
+                     CorrOutput &out) {
+    if (n < 1 || n > MAX_COLS) throw Error("invalid argument: sample count must be 1.." + std::to_string(MAX_COLS));
+    column_stats(d_codes, code_bytes, N, n, maxc);
+    for (int method = 0; method < 2; method++) {
+        if (!(req.methods & (1u << method))) continue;
+        std::vector<double> z, sd, G, G64;
+        make_lut(method, N, n, z, sd);
+        float ms = 0, ms64 = 0;
+        gram(d_codes, code_bytes, N, n, z, false, req.with_lolo, G, &ms);
+        if (req.fp64_baseline) gram(d_codes, code_bytes, N, n, z, true, false, G64, &ms64);
+        auto corr_of = [&](const std::vector<double> &g, std::vector<double> &r) {
+            r.assign((size_t)n * n, 0.0);
+            for (int i = 0; i < n; i++)
+                for (int j = 0; j < n; j++) {
+                    double den = std::sqrt(std::max(0.0, g[i * MAX_COLS + i]) * std::max(0.0, g[j * MAX_COLS + j]));
+                    r[i * n + j] = den > 0 ? g[i * MAX_COLS + j] / den : 0.0;
+                }
+        };
+        std::vector<double> &r = method == 0 ? out.pearson : out.spearman;
+        corr_of(G, r);
+        if (req.fp64_baseline) corr_of(G64, method == 0 ? out.pearson_fp64 : out.spearman_fp64);
+        (method == 0 ? out.gram_ms_pearson : out.gram_ms_spearman) = ms;
+        (method == 0 ? out.fp64_ms_pearson : out.fp64_ms_spearman) = ms64;
+        if (method == 0) {
+            // covariance of log1p values (N - 1 divisor) from the standardized Gram
+            out.cov.assign((size_t)n * n, 0.0);
+            if (N > 1)
+                for (int i = 0; i < n; i++)
+                    for (int j = 0; j < n; j++)
+                        out.cov[i * n + j] = G[i * MAX_COLS + j] * sd[i] * sd[j] / (double)(N - 1);
+        }
+    }
+}
+
+void CorrEngine::pca(const std::vector<double> &cov, int n, std::vector<double> &evals, std::vector<double> &evecs) {
+    if (n < 1 || n > 64) throw Error("invalid argument: PCA supports 1..64 samples");
+    DevBuf a, v, e;
+    a.ensure((size_t)n * n * 8);
+    v.ensure((size_t)n * n * 8);
+    e.ensure((size_t)n * 8);
+    KITE_HIP(hipMemcpyAsync(a.ptr, cov.data(), (size_t)n * n * 8, hipMemcpyHostToDevice, stream_));
+    k_jacobi<<<1, 64, 0, stream_>>>(a.as<double>(), v.as<double>(), n, 60, e.as<double>());
+    KITE_CHECK_LAUNCH();
+    evals.resize(n);
+    evecs.resize((size_t)n * n);
+    KITE_HIP(hipMemcpyAsync(evals.data(), e.ptr, (size_t)n * 8, hipMemcpyDeviceToHost, stream_));
+    KITE_HIP(hipMemcpyAsync(evecs.data(), v.ptr, (size_t)n * n * 8, hipMemcpyDeviceToHost, stream_));
+    KITE_HIP(hipStreamSynchronize(stream_));
+}
+
+uint32_t CorrEngine::max_count(const uint32_t *d_counts, uint64_t cells) {
+    DevBuf m;
+    m.ensure(4);
+    k_zero_cells<<<1, 64, 0, stream_>>>(m.as<uint32_t>(), 1);
+    KITE_CHECK_LAUNCH();
+    if (cells) {
+        uint32_t g = (uint32_t)std::min<uint64_t>(2048, (cells + 255) / 256);
+        k_max_u32<<<g, 256, 0, stream_>>>(d_counts, cells, m.as<uint32_t>());
+        KITE_CHECK_LAUNCH();
+    }
+    uint32_t h = 0;
+    KITE_HIP(hipMemcpyAsync(&h, m.ptr, 4, hipMemcpyDeviceToHost, stream_));
+    KITE_HIP(hipStreamSynchronize(stream_));
+    return h;
+}
+
+void CorrEngine::to_codes(const uint32_t *d_counts, uint64_t cells, DevBuf &codes, int code_bytes) {
+    codes.ensure(std::max<uint64_t>(cells, 1) * code_bytes);
+    if (!cells) return;
+    const uint32_t g = (uint32_t)((cells + 255) / 256);
+    if (code_bytes == 2) k_u32_to_codes<uint16_t><<<g, 256, 0, stream_>>>(d_counts, cells, codes.as<uint16_t>());
+    else k_u32_to_codes<uint32_t><<<g, 256, 0, stream_>>>(d_counts, cells, codes.as<uint32_t>());
+    KITE_CHECK_LAUNCH();
+}
+
+}  // namespace kite

@@ -122,3 +122,42 @@ def test_cli_merge_bytes_match_oracle(tmp_path, k, canon, P):
     ol.kite_merge(paths, tmp_path / "o.kite", lanes=lanes())
     assert (tmp_path / "g.kite").read_bytes() == (tmp_path / "o.kite").read_bytes()
     assert (tmp_path / "g.merge_report.md").exists()
+
+
+def test_cli_analyze_tables(tmp_path):
+    import sys
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import corr_oracle as co
+    from kite_amd.workloads import K31_CANON_50MBP
+    paths = []
+    for i in range(4):
+        w = K31_CANON_50MBP.scaled(4000 + 2000 * i, genome_len=400_000)
+        t = ol.count_fastq(w.fastq(), 21, True, 16)
+        p = tmp_path / f"sample{i}.kite"
+        ol.write_table(p, t, min_count=2 if i == 3 else None)
+        paths.append(p)
+    out = tmp_path / "an"
+    kite("analyze", "-i", *paths, "-o", out, "--correlation", "all", "--group", "a", "a", "b", "b")
+    keys, matrix = ol.kite_union(paths)
+    rows = (out / "exact_count_matrix.tsv").read_text().splitlines()
+    assert rows[0] == "kmer\tsample0\tsample1\tsample2\tsample3"
+    assert len(rows) == 1 + len(keys)
+    for r in range(0, len(keys), max(1, len(keys) // 500)):
+        f = rows[1 + r].split("\t")
+        assert f[0] == ol.decode_key(ol.words_to_int(keys[r]), 1, 21)
+        assert [int(x) for x in f[1:]] == matrix[r].tolist()
+    V = co.transform(matrix)
+
+    def table(name):
+        lines = (out / name).read_text().splitlines()
+        assert lines[0].split("\t")[1:] == [f"sample{i}" for i in range(4)]
+        return np.array([[float(x) for x in l.split("\t")[1:]] for l in lines[1:]])
+
+    assert np.max(np.abs(table("correlation_pearson.tsv") - co.pearson(V))) <= 1e-5 + 1e-8
+    assert np.max(np.abs(table("correlation_spearman.tsv") - co.spearman(V))) <= 1e-5 + 1e-8
+    emb = [l.split("\t") for l in (out / "embedding_pca.tsv").read_text().splitlines()]
+    assert emb[0] == ["sample", "group", "method", "dim1", "dim2"]
+    assert [e[1] for e in emb[1:]] == ["a", "a", "b", "b"] and {e[2] for e in emb[1:]} == {"pca"}
+    S, _ = co.pca_scores(co.covariance(V), 2)
+    got = np.array([[float(e[3]), float(e[4])] for e in emb[1:]])
+    assert np.max(np.abs(co.sign_normalize(got) - co.sign_normalize(S))) <= 1e-4 * max(1, np.abs(S).max())
