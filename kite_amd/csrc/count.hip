@@ -519,6 +519,190 @@ __global__ __launch_bounds__(BT) void k_bucket_sort(uint64_t *__restrict__ A,
     if (tid == 0) ucount[bid] = utotal;
 }
 
+// Dedup-first variant: every key is inserted into an LDS open-addressing
+// table (a slot holds the first position of its key, claimed by CAS) and
+// duplicates add to that position's count; only the unique keys, carrying
+// their counts, then go through the LSD passes.  Output equals
+// k_bucket_sort's: ascending unique keys with their run lengths.
+template <int W, int IPT>
+__global__ __launch_bounds__(BT) void k_bucket_dedup_sort(uint64_t *__restrict__ A,
+                                                           uint64_t *__restrict__ B,
+                                                           uint32_t *__restrict__ counts_tmp,
+                                                           const Bucket *__restrict__ buckets,
+                                                           const uint32_t *__restrict__ order,
+                                                           uint32_t *__restrict__ ucount, int k) {
+    constexpr int CAP = BT * IPT;
+    constexpr int HT = 2 * CAP;  // load factor <= 1/2
+    __shared__ uint64_t skeys[CAP * W];
+    __shared__ uint32_t scnt[CAP];
+    __shared__ uint32_t ht[HT];
+    __shared__ uint32_t whist[256 * BNW];
+    __shared__ uint32_t ws[BNW + 1];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const uint32_t bid = order[blockIdx.x];
+    const Bucket bk = buckets[bid];
+    uint64_t *X = bk.array ? B : A;
+    const uint64_t start = bk.start;
+    const int bits = 2 * k - (int)bk.prefix;
+    if (bk.size == 0) {
+        if (tid == 0) ucount[bid] = 0;
+        return;
+    }
+    if (bits <= 0 || bk.size == 1) {
+        if (tid == 0) {
+            counts_tmp[start] = bk.size > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)bk.size;
+            ucount[bid] = 1;
+        }
+        return;
+    }
+    const uint32_t m = (uint32_t)bk.size;
+    int ipt = (int)((m + BT - 1) / BT);
+    uint32_t wbase = (uint32_t)wave * 64u * (uint32_t)ipt;
+    for (int j = tid; j < HT; j += BT) ht[j] = 0;
+    for (int j = tid; j < CAP; j += BT) scnt[j] = 0;
+    Key<W> key[IPT];
+#pragma unroll
+    for (int it = 0; it < IPT; it++) {
+        uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
+        if (it < ipt && pos < m) {
+            const uint64_t *p = X + (start + pos) * W;
+#pragma unroll
+            for (int r = 0; r < W; r++) {
+                key[it].w[r] = p[r];
+                skeys[pos * W + r] = key[it].w[r];
+            }
+        }
+    }
+    __syncthreads();
+    uint32_t is_rep = 0;  // bit it: item it holds the first position of its key
+#pragma unroll
+    for (int it = 0; it < IPT; it++) {
+        uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
+        if (it < ipt && pos < m) {
+            uint64_t f = key[it].w[0];
+#pragma unroll
+            for (int r = 1; r < W; r++) f ^= key[it].w[r] * 0xC2B2AE3D27D4EB4Full;
+            uint32_t h = (uint32_t)((f * 0xD6E8FEB86659FD93ull) >> 40) & (HT - 1);
+            uint32_t rep;
+            for (;;) {
+                uint32_t old = atomicCAS(&ht[h], 0u, pos + 1);
+                if (old == 0) {
+                    rep = pos;
+                    break;
+                }
+                bool eq = true;
+#pragma unroll
+                for (int r = 0; r < W; r++) eq = eq && skeys[(old - 1) * W + r] == key[it].w[r];
+                if (eq) {
+                    rep = old - 1;
+                    break;
+                }
+                h = (h + 1) & (HT - 1);
+            }
+            atomicAdd(&scnt[rep], 1u);
+            if (rep == pos) is_rep |= 1u << it;
+        }
+    }
+    __syncthreads();
+    // compact the representatives with their counts
+    uint32_t mine = __popc(is_rep), U;
+    uint32_t q = block_excl_scan<BT, uint32_t>(mine, ws, &U);
+    uint32_t cnt[IPT];
+#pragma unroll
+    for (int it = 0; it < IPT; it++) {
+        uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
+        cnt[it] = ((is_rep >> it) & 1u) ? scnt[pos] : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IPT; it++) {
+        if ((is_rep >> it) & 1u) {
+#pragma unroll
+            for (int r = 0; r < W; r++) skeys[q * W + r] = key[it].w[r];
+            scnt[q] = cnt[it];
+            q++;
+        }
+    }
+    __syncthreads();
+    ipt = (int)((U + BT - 1) / BT);
+    wbase = (uint32_t)wave * 64u * (uint32_t)ipt;
+#pragma unroll
+    for (int it = 0; it < IPT; it++) {
+        uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
+        if (it < ipt && pos < U) {
+#pragma unroll
+            for (int r = 0; r < W; r++) key[it].w[r] = skeys[pos * W + r];
+            cnt[it] = scnt[pos];
+        }
+    }
+    __syncthreads();
+    for (int shift = 0; shift < bits && U > 1; shift += 8) {
+        const int nb = std::min(8, bits - shift);
+        for (int j = tid; j < 256 * BNW; j += BT) whist[j] = 0;
+        __syncthreads();
+        uint32_t rank[IPT], dig[IPT];
+#pragma unroll
+        for (int it = 0; it < IPT; it++) {
+            if (it < ipt) {
+                uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
+                bool valid = pos < U;
+                uint32_t d = valid ? key_bits<W>(key[it], shift, nb) : 0u;
+                uint64_t peers = __ballot(valid);
+                for (int bb = 0; bb < nb; bb++) {
+                    uint64_t mb = __ballot(valid && ((d >> bb) & 1u));
+                    peers &= ((d >> bb) & 1u) ? mb : ~mb;
+                }
+                uint32_t below = __popcll(peers & lanemask_lt());
+                uint32_t basev = whist[d * BNW + wave];
+                rank[it] = basev + below;
+                dig[it] = d;
+                if (valid && below == 0) whist[d * BNW + wave] = basev + (uint32_t)__popcll(peers);
+            }
+        }
+        __syncthreads();
+        {
+            uint32_t v0 = whist[tid * 4 + 0], v1 = whist[tid * 4 + 1];
+            uint32_t v2 = whist[tid * 4 + 2], v3 = whist[tid * 4 + 3];
+            uint32_t total;
+            uint32_t ex = block_excl_scan<BT, uint32_t>(v0 + v1 + v2 + v3, ws, &total);
+            whist[tid * 4 + 0] = ex;
+            whist[tid * 4 + 1] = ex + v0;
+            whist[tid * 4 + 2] = ex + v0 + v1;
+            whist[tid * 4 + 3] = ex + v0 + v1 + v2;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < IPT; it++) {
+            uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
+            if (it < ipt && pos < U) {
+                uint32_t dst = whist[dig[it] * BNW + wave] + rank[it];
+#pragma unroll
+                for (int r = 0; r < W; r++) skeys[dst * W + r] = key[it].w[r];
+                scnt[dst] = cnt[it];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < IPT; it++) {
+            uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
+            if (it < ipt && pos < U) {
+#pragma unroll
+                for (int r = 0; r < W; r++) key[it].w[r] = skeys[pos * W + r];
+                cnt[it] = scnt[pos];
+            }
+        }
+        __syncthreads();
+    }
+    for (uint32_t q2 = tid; q2 < U; q2 += BT) {
+        uint64_t *d = X + (start + q2) * W;
+#pragma unroll
+        for (int r = 0; r < W; r++) d[r] = skeys[q2 * W + r];
+        counts_tmp[start + q2] = scnt[q2];
+    }
+    if (tid == 0) ucount[bid] = U;
+}
+
 template <int W>
 __global__ __launch_bounds__(256) void k_compact(const uint64_t *__restrict__ A,
                                                  const uint64_t *__restrict__ B,
@@ -649,6 +833,7 @@ void CountEngine::run(const uint8_t *d_fq, uint64_t len, int k, int canonical, u
     int pbits = __builtin_ctz(partitions);
     words_ = k <= 31 ? 1 : (k <= 64 ? 2 : 4);
     if (const char *e = getenv("KITE_SPLIT_DIV")) tun.split_target_div = std::max(1, atoi(e));
+    if (const char *e = getenv("KITE_SORT_DEDUP")) tun.dedup = atoi(e) != 0;
     if (const char *e = getenv("KITE_MAX_DBITS")) tun.max_digit_bits = std::min(MAX_DBITS, std::max(1, atoi(e)));
     if (words_ == 1) run_w<1>(d_fq, len, k, canonical, pbits);
     else if (words_ == 2) run_w<2>(d_fq, len, k, canonical, pbits);
@@ -870,6 +1055,15 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
         auto go = [&](auto cls, uint32_t first, uint32_t cnt) {
             constexpr int C = decltype(cls)::value;
             if (!cnt) return;
+            if constexpr (C < SORT_CLASSES - 1) {
+                if (tun.dedup) {
+                    launch("k_bucket_sort", 0, [&] {
+                        k_bucket_dedup_sort<W, SortClass<W, C>::IPT><<<dim3(cnt), dim3(BT), 0, s>>>(
+                            A, B, ctmp, buckets_.as<Bucket>(), ord + first, ucount, k);
+                    });
+                    return;
+                }
+            }
             launch("k_bucket_sort", 0, [&] {
                 k_bucket_sort<W, SortClass<W, C>::IPT><<<dim3(cnt), dim3(BT), 0, s>>>(
                     A, B, ctmp, buckets_.as<Bucket>(), ord + first, ucount, k);

@@ -353,13 +353,20 @@ int run_info(int argc, char **argv) {
     Args a({{"input", 'i'}, {"set-min", 0}, {"set-max", 0}, {"clear-min", 0, false}, {"clear-max", 0, false},
             {"test-mode", 0, false}},
            argc, argv, 2);
+    print_command_line("info", argc, argv);
     if (!a.has("input")) throw UsageError("the following required arguments were not provided:\n  --input <FILE>");
     const std::string path = a.get("input");
     const auto smin = a.get_u32("set-min"), smax = a.get_u32("set-max");
     const bool cmin = a.flag("clear-min"), cmax = a.flag("clear-max");
     if (smin && cmin) throw Error("--set-min and --clear-min cannot be used together");
     if (smax && cmax) throw Error("--set-max and --clear-max cannot be used together");
+    fprintf(stderr, "[kite info] opening: %s\n", path.c_str());
     if (smin || smax || cmin || cmax) {
+        Header cur = Reader(path).header();
+        auto nmin = cmin ? std::nullopt : (smin ? smin : cur.min_count);
+        auto nmax = cmax ? std::nullopt : (smax ? smax : cur.max_count);
+        if (nmin && nmax && *nmin > *nmax)
+            throw Error("header min (" + std::to_string(*nmin) + ") must be <= header max (" + std::to_string(*nmax) + ")");
         // update_count_filters_in_file (src/format/header.rs:166-183): only the 128-byte header changes
         FILE *f = fopen(path.c_str(), "r+b");
         if (!f) throw Error("failed to open " + path + ": " + strerror(errno));
@@ -369,12 +376,6 @@ int run_info(int argc, char **argv) {
             throw Error("failed to read header");
         }
         Header h = Header::parse(hb);
-        auto nmin = cmin ? std::nullopt : (smin ? smin : h.min_count);
-        auto nmax = cmax ? std::nullopt : (smax ? smax : h.max_count);
-        if (nmin && nmax && *nmin > *nmax) {
-            fclose(f);
-            throw Error("invalid count filter range: min must be less than or equal to max");
-        }
         h.min_count = nmin;
         h.max_count = nmax;
         h.write(hb);
@@ -384,14 +385,51 @@ int run_info(int argc, char **argv) {
             throw Error("failed to write header");
         }
         fclose(f);
+        fprintf(stderr, "[kite info] updated header count filters: min=%s, max=%s\n", fmt_opt(nmin).c_str(),
+                fmt_opt(nmax).c_str());
     }
+    // report layout of run_info (src/info/mod.rs:108-157)
     Reader r(path);
     const Header &h = r.header();
-    printf("file\t%s\nformat\tKX2 v1\nk\t%u\ncanonical\t%s\npartitions\t%u\nkey_width\t%u\ncodec_mask\t%u\n", path.c_str(),
-           h.k, h.canonical ? "true" : "false", h.partitions, h.key_width, h.codec_mask);
-    printf("total_chunks\t%llu\ntotal_rows\t%llu\nmin\t%s\nmax\t%s\nfile_bytes\t%llu\n",
-           (unsigned long long)h.total_chunks, (unsigned long long)h.total_rows, fmt_opt(h.min_count).c_str(),
-           fmt_opt(h.max_count).c_str(), (unsigned long long)r.file_size());
+    uint64_t pbytes = 0, prows = 0, hot = 0, cold = UINT64_MAX, maxc = 0, minc = UINT64_MAX;
+    for (uint32_t p = 0; p < h.partitions; p++) {
+        auto [first, cnt] = r.partition_chunks(p);
+        uint64_t rows = 0, bytes = 0;
+        for (uint64_t c = first; c < first + cnt; c++) {
+            rows += r.chunks()[c].rows;
+            bytes += r.chunks()[c].encoded;
+        }
+        prows += rows;
+        pbytes += bytes;
+        hot = std::max(hot, rows);
+        cold = std::min(cold, rows);
+    }
+    if (!h.partitions) cold = 0;
+    uint64_t raw = 0, dvle = 0, zstd = 0;
+    for (auto &c : r.chunks()) {
+        maxc = std::max(maxc, c.encoded);
+        minc = std::min(minc, c.encoded);
+        raw += (c.codec & CODEC_RAW) != 0;
+        dvle += (c.codec & CODEC_DVLE) != 0;
+        zstd += (c.codec & CODEC_ZSTD) != 0;
+    }
+    if (r.chunks().empty()) minc = 0;
+    auto U = [](uint64_t v) { return (unsigned long long)v; };
+    printf("Kite Database\n  Path:               %s\n  File Size:          %llu bytes\n\n", path.c_str(), U(r.file_size()));
+    printf("Format\n  Version:            1\n  K:                  %u\n  Canonical:          %s\n", h.k,
+           h.canonical ? "true" : "false");
+    printf("  Min Count:          %s\n  Max Count:          %s\n", fmt_opt(h.min_count).c_str(), fmt_opt(h.max_count).c_str());
+    printf("  Filter Semantics:   downstream commands exclude rows outside Min/Max\n");
+    printf("  Partitions:         %u\n  Key Width:          %u\n\n", h.partitions, h.key_width);
+    printf("Content\n  Total Chunks:       %llu\n  Total Rows:         %llu\n  Partition Rows:     %llu\n"
+           "  Partition Bytes:    %llu\n\n",
+           U(r.chunk_count()), U(h.total_rows), U(prows), U(pbytes));
+    printf("Codec Mix\n  RAW Chunks:         %llu\n  DVLE Chunks:        %llu\n  ZSTD Chunks:        %llu\n\n", U(raw),
+           U(dvle), U(zstd));
+    printf("Distribution\n  Largest Chunk:      %llu bytes\n  Smallest Chunk:     %llu bytes\n"
+           "  Hottest Partition:  %llu rows\n  Coldest Partition:  %llu rows\n",
+           U(maxc), U(minc), U(hot), U(cold));
+    fflush(stdout);
     return 0;
 }
 

@@ -158,3 +158,23 @@ def test_full_size_properties(gpu_ctx):
     for s in ol.parse_sequences(fq[:2_000_000].tobytes())[:1000]:
         exp += len(brute.kmers(s.decode(), 31, True))
     assert exp > 0
+
+
+@pytest.mark.parametrize("dedup", ["0", "1"])
+@pytest.mark.parametrize("split_div", ["1", "4", "16"])
+def test_sort_variants_agree(gpu_ctx, monkeypatch, dedup, split_div):
+    """Both LDS bucket sorts (plain LSD, hash-dedup first) and several split
+    widths give the oracle's table."""
+    monkeypatch.setenv("KITE_SORT_DEDUP", dedup)
+    monkeypatch.setenv("KITE_SPLIT_DIV", split_div)
+    from kite_amd.workloads import K31_CANON_50MBP, K63_CANON_100MBP
+    for w, n in ((K31_CANON_50MBP, 150000), (K63_CANON_100MBP, 40000)):
+        fq = w.fastq(n_reads=n)
+        res, tab, _ = run_both(gpu_ctx, fq, w.k, w.canonical, 64)
+        check_same(res, tab, f"{w.name} dedup={dedup} div={split_div}")
+    rng = random.Random(11)
+    s = "".join(rng.choice("ACGT") for _ in range(150))
+    fq = ("".join(f"@r{i}\n{s}\n+\n{'I' * 150}\n" for i in range(5000))).encode()
+    for k in (15, 40, 90):
+        res, tab, _ = run_both(gpu_ctx, fq, k, True, 2)
+        check_same(res, tab, f"repeats k={k} dedup={dedup}")

@@ -103,7 +103,11 @@ def test_info_marker_edits_rewrite_only_the_header(tmp_path):
     kite("info", "-i", f, "--clear-min", "--set-max", 7)
     assert (kf.read(f).min_count, kf.read(f).max_count) == (None, 7)
     out = kite("info", "-i", f).stdout
-    assert "k\t5" in out and "partitions\t2" in out
+    assert "  K:                  5\n" in out and "  Partitions:         2\n" in out
+    assert "  Min Count:          none\n  Max Count:          7\n" in out
+    assert "  DVLE Chunks:        2\n" in out and "  Hottest Partition:  2 rows\n" in out
+    r = kite("info", "-i", f, "--set-min", 9, check=False)
+    assert r.returncode == 1 and "header min (9) must be <= header max (7)" in r.stderr
 
 
 @pytest.mark.parametrize("bad,msg", [
