@@ -7,6 +7,18 @@ runs resident in HBM.  The default workload is the single-sample k=31
 canonical count (50 Mbp genome with 10% diverged repeats, 5,000,000 reads
 of 150 bp, 1% substitution errors, 0.01% N, 256 partitions).
 
+Two more measurements ride on the same line:
+
+* ``batch24``: 24 independent samples (20 Mbp base genome + 2% SNVs per
+  sample, log-uniform 2e5..8e6 reads) counted from pinned host memory, once
+  one at a time ("plain": per-sample device allocation, H2D, kernels, D2H,
+  synchronise) and once through the pipelined batch (persistent buffers,
+  copies overlapped with kernels).  Rate = total input k-mers / wall time,
+  both including the H2D of the FASTQ and the D2H of the reduced tables.
+* ``analyze``: Pearson + Spearman of the union count matrix of those 24
+  tables (--min 2), split-bf16 MFMA Gram against the FP64 MFMA Gram, and
+  the FP64 oracle on a row sample.
+
 Prints one JSON line on stdout; progress goes to stderr.
 """
 from __future__ import annotations
@@ -63,6 +75,144 @@ def cpu_baseline(w, n_reads: int, threads: int) -> dict:
         path.unlink(missing_ok=True)
 
 
+def traffic_from_profile(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` on `workload`, from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE, copy-calibrated)."""
+    path = ROOT / "profiles" / f"hbm_traffic_{workload}.md"
+    if not path.exists():
+        return None, None
+    for line in path.read_text().splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 2 and cells[0] == f"traffic_bytes_per_launch:{kernel}":
+            return float(cells[1]), str(path.relative_to(ROOT))
+    return None, None
+
+
+def bench_batch24(ctx, scale: float, reps: int) -> tuple[dict, list]:
+    """Plain vs pipelined count of the 24-sample cohort from pinned memory."""
+    import ctypes as C
+
+    from kite_amd import BATCH_PIPELINED, BATCH_PLAIN, PinnedArray, synth_reads
+    from kite_amd._native import load_synth
+    from kite_amd.workloads import batch24
+
+    ws = batch24(24, scale=scale)
+    t0 = time.time()
+    pins = []
+    lib = load_synth()
+    for w in ws:
+        g = w.genome()
+        spec = w.spec()
+        need = lib.kite_synth_reads(C.byref(spec), g.ctypes.data, 0, w.n_reads, None, 0, 1)
+        pa = PinnedArray(need)
+        synth_reads(spec, g, 0, w.n_reads, out=pa.array)
+        pins.append(pa)
+    arrs = [p.array for p in pins]
+    total_bytes = sum(a.size for a in arrs)
+    log(f"batch24: {len(arrs)} samples, {sum(w.n_reads for w in ws)} reads, {total_bytes / 1e9:.2f} GB "
+        f"pinned FASTQ (generated in {time.time() - t0:.1f}s)")
+    k, canon, P = ws[0].k, ws[0].canonical, ws[0].partitions
+
+    # untimed: size the persistent buffers, keep the tables, and check that
+    # both modes give identical tables (checksums per sample)
+    def digest(i, st, keys, counts, off):
+        return (st["kmers"], st["unique"], int(np.bitwise_xor.reduce(keys.ravel())) if keys.size else 0,
+                int(counts.astype(np.uint64).sum()), int(off[-1]))
+
+    tables = []
+
+    def keep(i, st, keys, counts, off):
+        tables.append((keys[:, 0].copy(), counts.copy(), off.copy()))
+        return digest(i, st, keys, counts, off)
+
+    d_pipe = ctx.count_batch(arrs, k, canon, P, mode=BATCH_PIPELINED, on_result=keep)
+    d_plain = ctx.count_batch(arrs, k, canon, P, mode=BATCH_PLAIN, on_result=digest)
+    assert d_pipe == d_plain, "plain and pipelined batch tables differ"
+    kmers = sum(d[0] for d in d_pipe)
+
+    def nothing(i, st, keys, counts, off):
+        return 0
+
+    res = {}
+    for name, mode in (("plain", BATCH_PLAIN), ("pipelined", BATCH_PIPELINED)):
+        best = None
+        for _ in range(reps):
+            t = time.perf_counter()
+            ctx.count_batch(arrs, k, canon, P, mode=mode, on_result=nothing)
+            dt = time.perf_counter() - t
+            best = dt if best is None or dt < best else best
+        res[name] = {"wall_s": best, "kmers_per_s": kmers / best}
+        log(f"batch24 {name}: {best * 1e3:.1f} ms, {kmers / best / 1e9:.2f} G k-mers/s (best of {reps})")
+    out = {
+        "samples": len(arrs), "reads": int(sum(w.n_reads for w in ws)), "fastq_bytes": int(total_bytes),
+        "input_kmers": int(kmers), "unique_kmers_total": int(sum(d[1] for d in d_pipe)),
+        "plain": res["plain"], "pipelined": res["pipelined"],
+        "speedup": res["plain"]["wall_s"] / res["pipelined"]["wall_s"],
+        "includes": "H2D from pinned host memory, kernels, D2H of the reduced tables",
+        "bit_exact_plain_vs_pipelined": True, "scale": scale,
+    }
+    for p in pins:
+        p.free()
+    return out, tables
+
+
+def bench_analyze(ctx, tables: list, min_count: int, cpu_rows_frac_parts: int) -> dict:
+    """Correlation of the union of the cohort tables (counts >= min_count)."""
+    from kite_amd._native import CORR_FP64_BASELINE
+    filt = []
+    for keys, counts, off in tables:
+        m = counts >= min_count
+        cs = np.concatenate([np.zeros(1, dtype=np.uint64), np.cumsum(m, dtype=np.uint64)])
+        filt.append((keys[m], counts[m].astype(np.uint64), cs[off.astype(np.int64)]))
+    t = time.perf_counter()
+    r = ctx.analyze_tables(filt, 1, methods=3, flags=CORR_FP64_BASELINE)
+    wall = time.perf_counter() - t
+    st = r["stats"]
+    err_p = float(np.max(np.abs(r["pearson"] - r["pearson_fp64"])))
+    err_s = float(np.max(np.abs(r["spearman"] - r["spearman_fp64"])))
+    log(f"analyze: {st['rows']} union rows x {st['cols']}; split-bf16 Gram {st['gram_ms_pearson']:.2f} + "
+        f"{st['gram_ms_spearman']:.2f} ms, FP64 Gram {st['fp64_ms_pearson']:.2f} + {st['fp64_ms_spearman']:.2f} ms; "
+        f"max |bf16 - fp64| pearson {err_p:.2e} spearman {err_s:.2e}")
+    out = {
+        "union_rows": int(st["rows"]), "samples": int(st["cols"]), "code_bytes": int(st["code_bytes"]),
+        "corr_ms_split_bf16": st["gram_ms_pearson"] + st["gram_ms_spearman"],
+        "corr_ms_fp64_gpu": st["fp64_ms_pearson"] + st["fp64_ms_spearman"],
+        "speedup_vs_fp64_gpu": (st["fp64_ms_pearson"] + st["fp64_ms_spearman"]) /
+                               max(1e-9, st["gram_ms_pearson"] + st["gram_ms_spearman"]),
+        "max_abs_err_vs_fp64_gpu": {"pearson": err_p, "spearman": err_s},
+        "analyze_wall_s": wall,
+    }
+    # FP64 oracle on a row sample: the union rows of the first partitions
+    P = len(filt[0][2]) - 1
+    sub = []
+    for keys, counts, off in filt:
+        e = int(off[cpu_rows_frac_parts])
+        o = off[:cpu_rows_frac_parts + 1].copy()
+        sub.append((keys[:e], counts[:e],
+                    np.concatenate([o, np.full(P - cpu_rows_frac_parts, o[-1], dtype=np.uint64)])))
+    uk, um, uo = ctx.union_tables(sub, 1)
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import corr_oracle as co
+    t = time.perf_counter()
+    x = co.transform(um)
+    p_cpu = co.pearson(x)
+    s_cpu = co.spearman(um.astype(np.float64))
+    cpu_s = time.perf_counter() - t
+    g = ctx.corr_counts(um, methods=3, flags=CORR_FP64_BASELINE)
+    out["cpu_fp64_oracle"] = {
+        "rows": int(um.shape[0]), "seconds": cpu_s, "rows_per_s": um.shape[0] / cpu_s, "cores": 1,
+        "note": "numpy FP64 (one thread except the BLAS Gram)",
+        "sample": f"union rows of partitions [0, {cpu_rows_frac_parts}) of {P}",
+        "max_abs_err_split_bf16": {"pearson": float(np.max(np.abs(g["pearson"] - p_cpu))),
+                                   "spearman": float(np.max(np.abs(g["spearman"] - s_cpu)))},
+        "max_abs_err_fp64_gpu": {"pearson": float(np.max(np.abs(g["pearson_fp64"] - p_cpu))),
+                                 "spearman": float(np.max(np.abs(g["spearman_fp64"] - s_cpu)))},
+    }
+    log(f"analyze oracle sample: {um.shape[0]} rows in {cpu_s:.1f}s; max err bf16 vs oracle "
+        f"{out['cpu_fp64_oracle']['max_abs_err_split_bf16']}")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +223,12 @@ def main() -> int:
     ap.add_argument("--cpu-sample-reads", type=int, default=1_000_000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 24-sample batch and analyze legs")
+    ap.add_argument("--no-analyze", action="store_true")
+    ap.add_argument("--batch-scale", type=float, default=1.0, help="scale of the 24 read counts")
+    ap.add_argument("--batch-reps", type=int, default=2)
+    ap.add_argument("--analyze-cpu-parts", type=int, default=8,
+                    help="partitions (of 256) whose union rows the CPU oracle sample covers")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one 1 GiB device-to-device copy (a known-size kernel that "
                          "scales FETCH_SIZE / WRITE_SIZE counter readings)")
@@ -157,6 +313,17 @@ def main() -> int:
                for n, v in sorted(kern.items(), key=lambda kv: -kv[1]["ms"])}
     log("kernels: " + json.dumps(kernels))
 
+    traffic, traffic_src = traffic_from_profile(dom_name, w.name)
+
+    batch = analyze = None
+    if not args.no_batch:
+        del dev
+        torch.cuda.empty_cache()
+        batch, tables = bench_batch24(ctx, args.batch_scale, args.batch_reps)
+        if not args.no_analyze:
+            analyze = bench_analyze(ctx, tables, 2, args.analyze_cpu_parts)
+        del tables
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -183,9 +350,12 @@ def main() -> int:
                    "unique_kmers": int(stats[-1].unique), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes_per_launch": per_launch_bytes,
+                     "traffic": traffic, "traffic_source": traffic_src,
+                     "algorithmic_bytes_per_launch": per_launch_bytes,
                      "avg_launch_ms": per_launch_s * 1e3},
         "kernels": kernels,
+        "batch24": batch,
+        "analyze": analyze,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -116,6 +116,45 @@ int kite_gpu_count_download(kite_gpu_ctx *ctx, uint64_t *keys, uint32_t *counts,
 int kite_gpu_count_device_result(kite_gpu_ctx *ctx, const uint64_t **d_keys,
                                  const uint32_t **d_counts, const uint64_t **part_off);
 
+/* ---- batched count of many samples ---------------------------------------
+ * Counts n_samples independent FASTQ buffers (host memory; pinned memory
+ * from kite_gpu_host_alloc lets the copies overlap the kernels).  Each
+ * sample is one record-aligned chunk and is reduced completely, as one
+ * `kite count` run per sample would (run_count_typed, src/count/mod.rs:
+ * 512-540, once per input file).  The reference has no multi-sample entry
+ * point; this replaces a loop of count runs over the inputs of a cohort.
+ *
+ * mode KITE_BATCH_PLAIN: one sample at a time -- fresh device allocations,
+ * H2D copy, kernels, D2H copy, synchronise, free.
+ * mode KITE_BATCH_PIPELINED: persistent device buffers; the H2D copy of
+ * sample i+1 and the D2H copy of sample i-1 run on their own streams while
+ * sample i is counted, and results are handed to `fn` on a host thread of
+ * the engine's own while later samples run.
+ *
+ * `fn` receives sample i's result, borrowed for the call: keys
+ * (stats->unique * words u64), counts (u32, saturated at u32::MAX) and
+ * part_off (partitions + 1).  Calls come in sample order.  Return non-zero
+ * to abort the batch (KITE_E_CALLBACK). */
+#define KITE_BATCH_PLAIN 0
+#define KITE_BATCH_PIPELINED 1
+
+typedef struct kite_batch_sample {
+    const uint8_t *fastq;
+    size_t len;
+} kite_batch_sample;
+
+typedef int (*kite_batch_result_fn)(void *user, uint32_t sample, const kite_count_stats *stats,
+                                    const uint64_t *keys, const uint32_t *counts,
+                                    const uint64_t *part_off);
+
+int kite_gpu_count_batch(kite_gpu_ctx *ctx, uint32_t k, int canonical, uint32_t partitions,
+                         const kite_batch_sample *samples, uint32_t n_samples, int mode,
+                         kite_batch_result_fn fn, void *user);
+
+/* Page-locked host memory for FASTQ buffers and results (hipHostMalloc). */
+int kite_gpu_host_alloc(uint64_t bytes, void **out);
+void kite_gpu_host_free(void *p);
+
 /* ---- merge ---------------------------------------------------------------
  * Sorted union with count sums of n_inputs tables that share one partition
  * layout: merge_vectors_total (src/kiteops/mod.rs:587-724) as used per

@@ -56,6 +56,14 @@ APPEND_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint64,
                         C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint8),
                         C.c_size_t)
 
+class BatchSample(C.Structure):
+    _fields_ = [("fastq", C.c_void_p), ("len", C.c_size_t)]
+
+
+BATCH_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint32, C.POINTER(CountStats), C.POINTER(C.c_uint64),
+                       C.POINTER(C.c_uint32), C.POINTER(C.c_uint64))
+BATCH_PLAIN, BATCH_PIPELINED = 0, 1
+
 # name -> (restype, argtypes); every symbol include/kite_gpu.h declares
 GPU_SYMBOLS = {
     "kite_gpu_abi_version": (C.c_int, []),
@@ -88,6 +96,11 @@ GPU_SYMBOLS = {
                                           C.c_void_p, C.c_void_p, C.POINTER(CorrStats)]),
     "kite_gpu_pca": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
                                C.c_void_p]),
+    "kite_gpu_count_batch": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_uint32,
+                                       C.POINTER(BatchSample), C.c_uint32, C.c_int, BATCH_FN,
+                                       C.c_void_p]),
+    "kite_gpu_host_alloc": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
+    "kite_gpu_host_free": (None, [C.c_void_p]),
     "kite_gpu_count_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "kite_gpu_count_kernel_stats": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t]),
 }
@@ -126,6 +139,29 @@ def _ptr(buf) -> int:
     if isinstance(buf, (bytes, bytearray)):
         return C.cast(C.c_char_p(bytes(buf)), C.c_void_p).value  # pragma: no cover
     raise TypeError(type(buf))
+
+
+class PinnedArray:
+    """Page-locked host bytes (kite_gpu_host_alloc); `.array` is a uint8 view."""
+
+    def __init__(self, nbytes: int):
+        lib = load_gpu()
+        p = C.c_void_p()
+        _check(lib.kite_gpu_host_alloc(nbytes, C.byref(p)))
+        self._lib, self.ptr, self.nbytes = lib, p.value, nbytes
+        self.array = np.ctypeslib.as_array((C.c_uint8 * max(nbytes, 1)).from_address(p.value))[:nbytes]
+
+    def free(self) -> None:
+        if self.ptr:
+            self.array = None
+            self._lib.kite_gpu_host_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class CountResult:
@@ -304,6 +340,48 @@ class GpuContext:
         _check(self._lib.kite_gpu_pca(self._h, c.ctypes.data, n, dims, scores.ctypes.data,
                                       ev.ctypes.data))
         return scores, ev
+
+    # -- batched count of many samples (kite_gpu_count_batch) ------------
+    def count_batch(self, samples, k: int, canonical: bool, partitions: int,
+                    mode: int = BATCH_PIPELINED, on_result=None) -> list:
+        """Count each FASTQ buffer in `samples` (uint8 arrays, ideally from
+        PinnedArray) as its own table.  With on_result=None the tables are
+        returned as [(CountResult, stats dict)]; otherwise
+        on_result(i, stats dict, keys, counts, part_off) receives views that
+        are only valid during the call, and the list holds its return values."""
+        arrs = [np.ascontiguousarray(a, dtype=np.uint8) for a in samples]
+        descr = (BatchSample * max(len(arrs), 1))()
+        for i, a in enumerate(arrs):
+            descr[i].fastq = a.ctypes.data if a.size else None
+            descr[i].len = a.size
+        words = 1 if k <= 31 else (2 if k <= 64 else 4)
+        P = partitions
+        out: list = [None] * len(arrs)
+        errors: list = []
+
+        def cb(user, i, st, kp, cp, op):
+            try:
+                d = {f: getattr(st.contents, f) for f, _ in CountStats._fields_}
+                u = d["unique"]
+                keys = np.ctypeslib.as_array(kp, shape=(max(u, 1) * words,))[:u * words].reshape(u, words)
+                counts = np.ctypeslib.as_array(cp, shape=(max(u, 1),))[:u]
+                off = np.ctypeslib.as_array(op, shape=(P + 1,))
+                if on_result is None:
+                    out[i] = (CountResult(keys.copy(), counts.copy(), off.copy(), words, k, bool(canonical)), d)
+                else:
+                    out[i] = on_result(i, d, keys, counts, off)
+                return 0
+            except Exception as e:  # pragma: no cover - surfaced after the call
+                errors.append(e)
+                return 1
+
+        fn = BATCH_FN(cb)
+        rc = self._lib.kite_gpu_count_batch(self._h, k, int(bool(canonical)), P, descr, len(arrs), mode,
+                                            fn, None)
+        if errors:
+            raise errors[0]
+        _check(rc)
+        return out
 
     def count_profile(self, enable: bool) -> None:
         _check(self._lib.kite_gpu_count_profile(self._h, int(enable)))

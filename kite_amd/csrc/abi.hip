@@ -3,8 +3,11 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -47,6 +50,45 @@ struct HostRun {
     std::vector<uint64_t> part_off;
 };
 
+// Page-locked host buffer that grows on demand.
+struct PinnedBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf &) = delete;
+    PinnedBuf &operator=(const PinnedBuf &) = delete;
+    ~PinnedBuf() { release(); }
+    void release() {
+        if (ptr) (void)hipHostFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    void *ensure(size_t need) {
+        if (need == 0) need = 256;
+        if (need > bytes) {
+            release();
+            size_t want = need + need / 8;
+            KITE_HIP(hipHostMalloc(&ptr, want, hipHostMallocDefault));
+            bytes = want;
+        }
+        return ptr;
+    }
+    template <class T>
+    T *as() const { return static_cast<T *>(ptr); }
+};
+
+// One result slot of the batched count: the host copy of a sample's keys,
+// counts and partition offsets, and the event that ends its D2H copy.
+struct BatchSlot {
+    PinnedBuf keys, counts;
+    std::vector<uint64_t> part_off;
+    kite_count_stats st{};
+    uint32_t sample = 0;
+    hipEvent_t d2h_done = nullptr;
+    hipEvent_t run_done = nullptr;  // the run that read input buffer [slot] ended
+    hipEvent_t in_ready = nullptr;  // input buffer [slot] holds its sample
+};
+
 }  // namespace
 
 struct kite_gpu_ctx {
@@ -69,6 +111,11 @@ struct kite_gpu_ctx {
     bool device_result = false;
     std::vector<HostRun> host_runs;
     kite_count_stats stats{};
+
+    // batched count: copy streams, double-buffered input and result slots
+    hipStream_t h2d = nullptr, d2h = nullptr;
+    kite::DevBuf batch_in[2];
+    BatchSlot slots[2];
 };
 
 namespace {
@@ -188,6 +235,15 @@ void kite_gpu_close(kite_gpu_ctx *ctx) {
         ctx->merge.reset();
         ctx->corr.reset();
         ctx->staging.release();
+        for (int b = 0; b < 2; b++) {
+            ctx->batch_in[b].release();
+            ctx->slots[b].keys.release();
+            ctx->slots[b].counts.release();
+            for (hipEvent_t e : {ctx->slots[b].d2h_done, ctx->slots[b].run_done, ctx->slots[b].in_ready})
+                if (e) (void)hipEventDestroy(e);
+        }
+        if (ctx->h2d) (void)hipStreamDestroy(ctx->h2d);
+        if (ctx->d2h) (void)hipStreamDestroy(ctx->d2h);
         (void)hipStreamDestroy(ctx->stream);
     } catch (...) {
     }
@@ -544,6 +600,250 @@ int kite_gpu_count_kernel_stats(kite_gpu_ctx *ctx, char *buf, size_t cap) {
     if (!buf || cap <= js.size()) return set_error(KITE_E_INVALID, "buffer too small");
     memcpy(buf, js.c_str(), js.size() + 1);
     return (int)js.size();
+}
+
+}  // extern "C"
+
+// ---- batched count -----------------------------------------------------------
+namespace {
+
+void batch_stats(kite_gpu_ctx *c, const kite_batch_sample &smp, kite_count_stats &st) {
+    const kite::StageTimes &t = c->count->times();
+    st = kite_count_stats{};
+    st.input_bytes = smp.len;
+    st.kmers = c->count->n_kmers();
+    st.unique = c->count->n_unique();
+    st.buckets = c->count->buckets();
+    st.batches = 1;
+    st.split_levels = (uint32_t)c->count->levels();
+    st.skipped_chunks = (smp.len > 0 && smp.fastq[0] != '@') ? 1u : 0u;
+    st.words = (uint32_t)c->words;
+    st.device_ms = t.total_ms;
+    st.extract_ms = t.extract_ms;
+    st.split_ms = t.split_ms;
+    st.sort_ms = t.sort_ms;
+    st.compact_ms = t.compact_ms;
+}
+
+void batch_plain(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, kite_batch_result_fn fn,
+                 void *user) {
+    BatchSlot &sl = c->slots[0];
+    const int W = c->words;
+    for (uint32_t i = 0; i < n; i++) {
+        c->count->release();
+        void *d = nullptr;
+        if (smp[i].len) {
+            KITE_HIP(hipMalloc(&d, smp[i].len));
+            KITE_HIP(hipMemcpy(d, smp[i].fastq, smp[i].len, hipMemcpyHostToDevice));
+        }
+        c->count->set_slot(0);
+        c->count->set_timing(true);
+        try {
+            c->count->run(static_cast<const uint8_t *>(d), smp[i].len, (int)c->k, c->canonical, c->partitions);
+        } catch (...) {
+            if (d) (void)hipFree(d);
+            throw;
+        }
+        const uint64_t u = c->count->n_unique();
+        uint64_t *keys = static_cast<uint64_t *>(sl.keys.ensure(u * 8 * W));
+        uint32_t *counts = static_cast<uint32_t *>(sl.counts.ensure(u * 4));
+        if (u) {
+            KITE_HIP(hipMemcpy(keys, c->count->keys(), u * 8 * W, hipMemcpyDeviceToHost));
+            KITE_HIP(hipMemcpy(counts, c->count->counts(), u * 4, hipMemcpyDeviceToHost));
+        }
+        if (d) KITE_HIP(hipFree(d));
+        batch_stats(c, smp[i], sl.st);
+        sl.part_off = c->count->part_off();
+        if (fn(user, i, &sl.st, keys, counts, sl.part_off.data()) != 0)
+            throw kite::Error("callback aborted the batch");
+    }
+}
+
+// Hands finished result slots to the caller's callback on its own thread,
+// in sample order, and frees each slot for reuse afterwards.
+struct Consumer {
+    kite_gpu_ctx *c;
+    kite_batch_result_fn fn;
+    void *user;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<int> queue;
+    bool free_slot[2] = {true, true};
+    bool done = false;
+    std::string error;
+    std::thread th;
+
+    Consumer(kite_gpu_ctx *cc, kite_batch_result_fn f, void *u) : c(cc), fn(f), user(u) {
+        th = std::thread([this] { loop(); });
+    }
+    ~Consumer() { stop(); }
+    void stop() {
+        {
+            std::lock_guard<std::mutex> g(mu);
+            done = true;
+        }
+        cv.notify_all();
+        if (th.joinable()) th.join();
+    }
+    void loop() {
+        for (;;) {
+            int s;
+            {
+                std::unique_lock<std::mutex> g(mu);
+                cv.wait(g, [&] { return done || !queue.empty(); });
+                if (queue.empty()) return;
+                s = queue.front();
+                queue.pop_front();
+            }
+            BatchSlot &sl = c->slots[s];
+            std::string err;
+            hipError_t e = hipEventSynchronize(sl.d2h_done);
+            if (e != hipSuccess) err = std::string("HIP error: ") + hipGetErrorString(e);
+            else if (fn(user, sl.sample, &sl.st, sl.keys.as<uint64_t>(), sl.counts.as<uint32_t>(),
+                        sl.part_off.data()) != 0)
+                err = "callback aborted the batch";
+            {
+                std::lock_guard<std::mutex> g(mu);
+                if (!err.empty() && error.empty()) error = err;
+                free_slot[s] = true;
+            }
+            cv.notify_all();
+        }
+    }
+    // Blocks until slot s is free; returns false once an error was recorded.
+    bool acquire(int s) {
+        std::unique_lock<std::mutex> g(mu);
+        cv.wait(g, [&] { return free_slot[s] || !error.empty(); });
+        if (!error.empty()) return false;
+        free_slot[s] = false;
+        return true;
+    }
+    void push(int s) {
+        {
+            std::lock_guard<std::mutex> g(mu);
+            queue.push_back(s);
+        }
+        cv.notify_all();
+    }
+    // Waits for every queued slot to be consumed.
+    void drain() {
+        std::unique_lock<std::mutex> g(mu);
+        cv.wait(g, [&] { return (queue.empty() && free_slot[0] && free_slot[1]) || !error.empty(); });
+    }
+    std::string err() {
+        std::lock_guard<std::mutex> g(mu);
+        return error;
+    }
+};
+
+void batch_pipelined(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, kite_batch_result_fn fn,
+                     void *user) {
+    if (!c->h2d) KITE_HIP(hipStreamCreateWithFlags(&c->h2d, hipStreamNonBlocking));
+    if (!c->d2h) KITE_HIP(hipStreamCreateWithFlags(&c->d2h, hipStreamNonBlocking));
+    for (int b = 0; b < 2; b++) {
+        BatchSlot &sl = c->slots[b];
+        for (hipEvent_t *e : {&sl.d2h_done, &sl.run_done, &sl.in_ready})
+            if (!*e) KITE_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    }
+    uint64_t maxlen = 0;
+    for (uint32_t i = 0; i < n; i++) maxlen = std::max<uint64_t>(maxlen, smp[i].len);
+    // both input buffers are sized before any copy is queued into them
+    for (int b = 0; b < (n > 1 ? 2 : 1); b++) c->batch_in[b].ensure(maxlen);
+    const int W = c->words;
+
+    auto h2d = [&](uint32_t j) {
+        const int b = (int)(j & 1);
+        if (j >= 2) KITE_HIP(hipStreamWaitEvent(c->h2d, c->slots[b].run_done, 0));
+        if (smp[j].len)
+            KITE_HIP(hipMemcpyAsync(c->batch_in[b].ptr, smp[j].fastq, smp[j].len, hipMemcpyHostToDevice, c->h2d));
+        KITE_HIP(hipEventRecord(c->slots[b].in_ready, c->h2d));
+    };
+
+    Consumer cons(c, fn, user);
+    std::string failure;
+    try {
+        if (n) h2d(0);
+        for (uint32_t i = 0; i < n; i++) {
+            const int s = (int)(i & 1);
+            BatchSlot &sl = c->slots[s];
+            if (i + 1 < n) h2d(i + 1);
+            KITE_HIP(hipStreamWaitEvent(c->stream, sl.in_ready, 0));
+            // the D2H copy of sample i-2 reads the engine's result slot s
+            if (i >= 2) KITE_HIP(hipStreamWaitEvent(c->stream, sl.d2h_done, 0));
+            c->count->set_slot(s);
+            c->count->set_timing(true);
+            c->count->run(c->batch_in[s].as<uint8_t>(), smp[i].len, (int)c->k, c->canonical, c->partitions);
+            KITE_HIP(hipEventRecord(sl.run_done, c->stream));
+            if (!cons.acquire(s)) break;
+            const uint64_t u = c->count->n_unique();
+            uint64_t *keys = static_cast<uint64_t *>(sl.keys.ensure(u * 8 * W));
+            uint32_t *counts = static_cast<uint32_t *>(sl.counts.ensure(u * 4));
+            batch_stats(c, smp[i], sl.st);
+            sl.part_off = c->count->part_off();
+            sl.sample = i;
+            KITE_HIP(hipStreamWaitEvent(c->d2h, sl.run_done, 0));
+            if (u) {
+                KITE_HIP(hipMemcpyAsync(keys, c->count->keys(), u * 8 * W, hipMemcpyDeviceToHost, c->d2h));
+                KITE_HIP(hipMemcpyAsync(counts, c->count->counts(), u * 4, hipMemcpyDeviceToHost, c->d2h));
+            }
+            KITE_HIP(hipEventRecord(sl.d2h_done, c->d2h));
+            cons.push(s);
+        }
+        cons.drain();
+    } catch (const std::exception &e) {
+        failure = e.what();
+    }
+    cons.stop();
+    (void)hipStreamSynchronize(c->h2d);
+    (void)hipStreamSynchronize(c->d2h);
+    if (!failure.empty()) throw kite::Error(failure);
+    std::string ce = cons.err();
+    if (!ce.empty()) throw kite::Error(ce);
+}
+
+}  // namespace
+
+extern "C" {
+
+int kite_gpu_count_batch(kite_gpu_ctx *ctx, uint32_t k, int canonical, uint32_t partitions,
+                         const kite_batch_sample *samples, uint32_t n_samples, int mode,
+                         kite_batch_result_fn fn, void *user) {
+    if (!ctx) return set_error(KITE_E_INVALID, "invalid argument: ctx is NULL");
+    if (!fn) return set_error(KITE_E_INVALID, "invalid argument: callback is NULL");
+    if (n_samples && !samples) return set_error(KITE_E_INVALID, "invalid argument: samples is NULL");
+    if (mode != KITE_BATCH_PLAIN && mode != KITE_BATCH_PIPELINED)
+        return set_error(KITE_E_INVALID, "invalid argument: unknown batch mode");
+    for (uint32_t i = 0; i < n_samples; i++)
+        if (samples[i].len && !samples[i].fastq)
+            return set_error(KITE_E_INVALID, "invalid argument: sample " + std::to_string(i) + " has no data");
+    int rc = kite_gpu_count_begin(ctx, k, canonical, partitions);
+    if (rc) return rc;
+    ctx->session = false;  // the batch does not leave a count session behind
+    rc = guarded([&]() -> int {
+        bind(ctx);
+        for (uint32_t i = 0; i < n_samples; i++)
+            if (samples[i].len > ctx->batch_cap)
+                return set_error(KITE_E_NOMEM, "sample " + std::to_string(i) +
+                                                   " exceeds the device budget of one batch");
+        if (mode == KITE_BATCH_PLAIN) batch_plain(ctx, samples, n_samples, fn, user);
+        else batch_pipelined(ctx, samples, n_samples, fn, user);
+        return KITE_OK;
+    });
+    if (rc == KITE_E_DEVICE && g_last_error.find("callback aborted") != std::string::npos) rc = KITE_E_CALLBACK;
+    return rc;
+}
+
+int kite_gpu_host_alloc(uint64_t bytes, void **out) {
+    if (!out) return set_error(KITE_E_INVALID, "invalid argument: out is NULL");
+    *out = nullptr;
+    return guarded([&]() -> int {
+        KITE_HIP(hipHostMalloc(out, std::max<uint64_t>(bytes, 1), hipHostMallocDefault));
+        return KITE_OK;
+    });
+}
+
+void kite_gpu_host_free(void *p) {
+    if (p) (void)hipHostFree(p);
 }
 
 }  // extern "C"

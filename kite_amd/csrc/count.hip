@@ -815,6 +815,14 @@ void CountEngine::collect_kernel_stats() {
     pending_.clear();
 }
 
+void CountEngine::release() {
+    KITE_HIP(hipStreamSynchronize(stream_));
+    for (DevBuf *b : {&nl_, &line0_, &scan_tmp_, &hist_, &off_, &part_off_d_, &a_, &b_, &counts_tmp_, &out_keys_[0],
+                      &out_keys_[1], &out_counts_[0], &out_counts_[1], &segs_, &tile_pre_, &child_hist_, &child_cur_,
+                      &buckets_, &ucount_, &uoff_, &part_first_, &part_u_, &order_})
+        b->release();
+}
+
 uint64_t CountEngine::bytes_needed(uint64_t n, int words, uint64_t fastq, uint32_t partitions) {
     uint64_t kb = 8ull * (uint64_t)words;
     uint64_t n_sub = ceil_div(fastq, SUB);
@@ -1083,8 +1091,8 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
     scan_to_u64(ucount, uoff, nb, stmp2, s);
     KITE_HIP(hipMemcpyAsync(&n_unique_, uoff + nb, 8, hipMemcpyDeviceToHost, s));
     KITE_HIP(hipStreamSynchronize(s));
-    uint64_t *ok = static_cast<uint64_t *>(out_keys_.ensure(std::max<uint64_t>(n_unique_, 1) * 8 * W));
-    uint32_t *oc = static_cast<uint32_t *>(out_counts_.ensure(std::max<uint64_t>(n_unique_, 1) * 4));
+    uint64_t *ok = static_cast<uint64_t *>(out_keys_[slot_].ensure(std::max<uint64_t>(n_unique_, 1) * 8 * W));
+    uint32_t *oc = static_cast<uint32_t *>(out_counts_[slot_].ensure(std::max<uint64_t>(n_unique_, 1) * 4));
     if (nb > 0) {
         launch("k_compact", n_unique_ * (8 * W + 4) * 2, [&] {
             k_compact<W><<<dim3((unsigned)nb), dim3(256), 0, s>>>(A, B, ctmp, buckets_.as<Bucket>(),

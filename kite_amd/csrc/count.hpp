@@ -61,8 +61,14 @@ class CountEngine {
     uint64_t n_unique() const { return n_unique_; }
     int words() const { return words_; }
     const std::vector<uint64_t> &part_off() const { return part_off_h_; }
-    const uint64_t *keys() const { return out_keys_.as<uint64_t>(); }
-    const uint32_t *counts() const { return out_counts_.as<uint32_t>(); }
+    const uint64_t *keys() const { return out_keys_[slot_].as<uint64_t>(); }
+    const uint32_t *counts() const { return out_counts_[slot_].as<uint32_t>(); }
+    // Two output slots let a copy stream drain one result while the next
+    // run writes the other.
+    void set_slot(int s) { slot_ = s & 1; }
+    int slot() const { return slot_; }
+    // Frees every device buffer (the next run allocates afresh).
+    void release();
     const StageTimes &times() const { return times_; }
     uint64_t buckets() const { return n_buckets_; }
     int levels() const { return levels_; }
@@ -95,7 +101,8 @@ class CountEngine {
     std::vector<PendingEv> pending_;
     std::vector<hipEvent_t> ev_pool_;
     DevBuf nl_, line0_, scan_tmp_, hist_, off_, part_off_d_;
-    DevBuf a_, b_, counts_tmp_, out_keys_, out_counts_;
+    DevBuf a_, b_, counts_tmp_, out_keys_[2], out_counts_[2];
+    int slot_ = 0;
     DevBuf segs_, tile_pre_, child_hist_, child_cur_, buckets_, ucount_, uoff_, part_first_, part_u_;
     DevBuf order_;
     std::vector<uint64_t> part_off_h_;

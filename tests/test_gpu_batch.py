@@ -1,0 +1,98 @@
+"""Batched multi-sample count (kite_gpu_count_batch) against single-sample
+counts and the oracle: both modes must return every sample's table
+bit-exactly, in sample order."""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from kite_amd import BATCH_PIPELINED, BATCH_PLAIN, KiteError, PinnedArray
+from kite_amd.workloads import batch24
+
+pytestmark = pytest.mark.gpu
+
+
+def _same(a, b, label):
+    assert np.array_equal(a.part_off, b.part_off), label
+    assert np.array_equal(a.keys, b.keys), label
+    assert np.array_equal(a.counts, b.counts), label
+
+
+def _samples(n_reads_scale=0.004):
+    ws = batch24(8, scale=n_reads_scale)
+    g = ws[0].genome()
+    out = []
+    for w in ws:
+        fq = w.fastq(genome=g)
+        pa = PinnedArray(fq.size)
+        pa.array[:] = fq
+        out.append(pa)
+    return ws, out
+
+
+def test_batch_modes_match_single_counts(gpu_ctx):
+    ws, pins = _samples()
+    arrs = [p.array for p in pins]
+    # edge cases: an empty sample and one that does not start with '@'
+    bad = np.frombuffer(b"xr\nACGTACGTACGTACGTACGTACGTACGTACGTAA\n+\n" + b"I" * 34 + b"\n", dtype=np.uint8)
+    arrs = arrs[:3] + [np.zeros(0, dtype=np.uint8), bad] + arrs[3:]
+    single = [gpu_ctx.count(a, 31, True, 256)[0] for a in arrs]
+    for mode in (BATCH_PLAIN, BATCH_PIPELINED, BATCH_PIPELINED):
+        res = gpu_ctx.count_batch(arrs, 31, True, 256, mode=mode)
+        assert len(res) == len(arrs)
+        for i, (r, st) in enumerate(res):
+            _same(r, single[i], f"mode={mode} sample={i}")
+            assert st["unique"] == len(single[i].counts)
+            assert st["input_bytes"] == arrs[i].size
+        assert res[4][1]["skipped_chunks"] == 1 and len(res[4][0].counts) == 0
+        assert res[3][1]["kmers"] == 0
+    for p in pins:
+        p.free()
+
+
+def test_batch_against_oracle(gpu_ctx):
+    ws, pins = _samples(0.002)
+    arrs = [p.array for p in pins[:4]]
+    res = gpu_ctx.count_batch(arrs, 31, True, 256, mode=BATCH_PIPELINED)
+    for i, a in enumerate(arrs):
+        tab = ol.count_fastq(a, 31, True, 256, chunk_size=1 << 40, threads=8)
+        r, st = res[i]
+        assert st["kmers"] == tab.n_kmers
+        assert np.array_equal(r.keys, tab.keys)
+        assert np.array_equal(r.counts, np.minimum(tab.counts, 0xFFFFFFFF).astype(np.uint32))
+    for p in pins:
+        p.free()
+
+
+def test_batch_growing_samples_and_other_widths(gpu_ctx):
+    """Samples of increasing size force the persistent buffers to grow
+    inside the pipeline; k = 63 exercises 2-word keys."""
+    from kite_amd.workloads import K63_CANON_100MBP as w
+    g = w.genome()
+    arrs = [w.fastq(n_reads=n, genome=g, first_read=7 * n) for n in (2000, 9000, 30000, 500)]
+    single = [gpu_ctx.count(a, 63, True, 64)[0] for a in arrs]
+    res = gpu_ctx.count_batch(arrs, 63, True, 64, mode=BATCH_PIPELINED)
+    for i in range(len(arrs)):
+        _same(res[i][0], single[i], f"k63 sample={i}")
+
+
+def test_batch_callback_abort(gpu_ctx):
+    ws, pins = _samples(0.001)
+    arrs = [p.array for p in pins[:3]]
+    seen = []
+
+    def cb(i, st, keys, counts, off):
+        seen.append(i)
+        if i == 1:
+            raise RuntimeError("stop here")
+        return int(counts.sum())
+
+    with pytest.raises(RuntimeError, match="stop here"):
+        gpu_ctx.count_batch(arrs, 31, True, 256, mode=BATCH_PIPELINED, on_result=cb)
+    assert seen[:2] == [0, 1]
+    # the context stays usable
+    res = gpu_ctx.count_batch(arrs, 31, True, 256, mode=BATCH_PIPELINED)
+    assert len(res) == 3
+    with pytest.raises(KiteError):
+        gpu_ctx.count_batch(arrs, 31, True, 300, mode=BATCH_PIPELINED)
+    for p in pins:
+        p.free()
