@@ -372,12 +372,13 @@ void CorrEngine::gram(const void *d_codes, int code_bytes, uint64_t N, int n, co
         out.resize(per_wave);
         KITE_HIP(hipMemcpyAsync(out.data(), sum_.ptr, per_wave * 8, hipMemcpyDeviceToHost, stream_));
         KITE_HIP(hipStreamSynchronize(stream_));
-        // block b in (00, 01, 11); lane l, element i -> (m, n) = (4*(l/16) + i, l%16)
+        // block b in (00, 01, 11); lane l, element i -> (m, n) = (4*i + l/16, l%16)
+        // (the FP64 16x16x4 output layout, measured on gfx950)
         G.assign(MAX_COLS * MAX_COLS, 0.0);
         for (int b = 0; b < 3; b++)
             for (int i = 0; i < 4; i++)
                 for (int l = 0; l < 64; l++) {
-                    int m = 4 * (l / 16) + i, nn = l % 16;
+                    int m = 4 * i + l / 16, nn = l % 16;
                     int bm = b == 2 ? 16 : 0, bn = b == 0 ? 0 : 16;
                     double v = out[(b * 4 + i) * 64 + l];
                     G[(bm + m) * MAX_COLS + bn + nn] = v;

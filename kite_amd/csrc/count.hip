@@ -106,6 +106,17 @@ __device__ __forceinline__ uint32_t key_bits(const Key<W> &a, int shift, int nb)
     return (uint32_t)(v & ((1ull << nb) - 1));
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  This bijection on [0, n) gives XCD x a contiguous range of logical
+// tiles, so tiles that write neighbouring destination lines share an L2 and
+// their partial-line writes merge there instead of being written back twice.
+constexpr uint32_t N_XCD = 8;
+__device__ __forceinline__ uint64_t xcd_tile(uint64_t b, uint64_t n) {
+    const uint64_t x = b % N_XCD, i = b / N_XCD;
+    const uint64_t q = n / N_XCD, r = n % N_XCD;
+    return x * q + (x < r ? x : r) + i;
+}
+
 // ---- newline count per sub-tile -------------------------------------------
 __global__ __launch_bounds__(XT) void k_nl_count(const uint8_t *__restrict__ fq, uint64_t len,
                                                   uint32_t *__restrict__ out) {
@@ -152,7 +163,7 @@ __global__ __launch_bounds__(XT) void k_extract(ExtractArgs a) {
     uint32_t *cnt = dyn;
     uint64_t *pbase = reinterpret_cast<uint64_t *>(dyn + ((P + 1) & ~1u));
     const int tid = threadIdx.x;
-    const uint32_t wg = blockIdx.x;
+    const uint32_t wg = (uint32_t)xcd_tile(blockIdx.x, gridDim.x);
     const uint8_t *sbytes = reinterpret_cast<const uint8_t *>(sb4);
     const uint32_t *sb = reinterpret_cast<const uint32_t *>(sb4);
     const bool live = a.len > 0 && a.fq[0] == '@';
@@ -321,7 +332,7 @@ __global__ __launch_bounds__(ST) void k_split(SplitArgs a) {
     __shared__ uint32_t hist[1 << MAX_DBITS];
     __shared__ uint64_t cbase[SCATTER ? (1 << MAX_DBITS) : 1];
     const int tid = threadIdx.x;
-    const uint64_t b = blockIdx.x;
+    const uint64_t b = xcd_tile(blockIdx.x, gridDim.x);
     uint32_t lo = 0, hi = a.nseg;  // largest s with tile_pre[s] <= b
     while (hi - lo > 1) {
         uint32_t mid = (lo + hi) / 2;
@@ -519,25 +530,37 @@ __global__ __launch_bounds__(BT) void k_bucket_sort(uint64_t *__restrict__ A,
     if (tid == 0) ucount[bid] = utotal;
 }
 
-// Dedup-first variant: every key is inserted into an LDS open-addressing
+constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v / 2); }
+
+// Hash-then-bin variant: every key is inserted into an LDS open-addressing
 // table (a slot holds the first position of its key, claimed by CAS) and
-// duplicates add to that position's count; only the unique keys, carrying
-// their counts, then go through the LSD passes.  Output equals
+// duplicates add to that position's count.  The unique keys, carrying their
+// counts, are then distributed by one counting pass over 2*CAP bins on
+// their top remaining key bits (order inside a bin does not matter: keys
+// are distinct), and each thread insertion-sorts the few keys of its own
+// contiguous bins.  A bin above LOCAL_MAX keys (skewed key bits) sends the
+// whole bucket through the LSD passes instead.  Output equals
 // k_bucket_sort's: ascending unique keys with their run lengths.
 template <int W, int IPT>
-__global__ __launch_bounds__(BT) void k_bucket_dedup_sort(uint64_t *__restrict__ A,
-                                                           uint64_t *__restrict__ B,
-                                                           uint32_t *__restrict__ counts_tmp,
-                                                           const Bucket *__restrict__ buckets,
-                                                           const uint32_t *__restrict__ order,
-                                                           uint32_t *__restrict__ ucount, int k) {
+__global__ __launch_bounds__(BT) void k_bucket_hash_sort(uint64_t *__restrict__ A,
+                                                          uint64_t *__restrict__ B,
+                                                          uint32_t *__restrict__ counts_tmp,
+                                                          const Bucket *__restrict__ buckets,
+                                                          const uint32_t *__restrict__ order,
+                                                          uint32_t *__restrict__ ucount, int k) {
     constexpr int CAP = BT * IPT;
-    constexpr int HT = 2 * CAP;  // load factor <= 1/2
+    constexpr int HT = 2 * CAP;  // hash slots (load factor <= 1/2), then bins
+    constexpr int BPT = HT / BT; // bins per thread
+    constexpr int DB = ilog2c(HT);
+    constexpr int LOCAL_MAX = 16;
+    constexpr bool WH_IN_HT = HT >= 256 * BNW;  // the LSD histogram reuses the table when it fits
     __shared__ uint64_t skeys[CAP * W];
     __shared__ uint32_t scnt[CAP];
     __shared__ uint32_t ht[HT];
-    __shared__ uint32_t whist[256 * BNW];
+    __shared__ uint32_t whist_own[WH_IN_HT ? 1 : 256 * BNW];
     __shared__ uint32_t ws[BNW + 1];
+    __shared__ uint32_t s_fallback;
+    uint32_t *whist = WH_IN_HT ? ht : whist_own;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const uint32_t bid = order[blockIdx.x];
@@ -575,6 +598,7 @@ __global__ __launch_bounds__(BT) void k_bucket_dedup_sort(uint64_t *__restrict__
         }
     }
     __syncthreads();
+    // 1. hash dedup: the first position of every key collects its count
     uint32_t is_rep = 0;  // bit it: item it holds the first position of its key
 #pragma unroll
     for (int it = 0; it < IPT; it++) {
@@ -636,63 +660,138 @@ __global__ __launch_bounds__(BT) void k_bucket_dedup_sort(uint64_t *__restrict__
             cnt[it] = scnt[pos];
         }
     }
+    // 2. one counting pass over HT bins on the top remaining key bits
+    const int db = bits < DB ? bits : DB;
+    const int dshift = bits - db;
+    for (int j = tid; j < HT; j += BT) ht[j] = 0;
+    if (tid == 0) s_fallback = 0;
     __syncthreads();
-    for (int shift = 0; shift < bits && U > 1; shift += 8) {
-        const int nb = std::min(8, bits - shift);
-        for (int j = tid; j < 256 * BNW; j += BT) whist[j] = 0;
-        __syncthreads();
-        uint32_t rank[IPT], dig[IPT];
+    uint32_t slot[IPT], dig[IPT];
 #pragma unroll
-        for (int it = 0; it < IPT; it++) {
-            if (it < ipt) {
-                uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
-                bool valid = pos < U;
-                uint32_t d = valid ? key_bits<W>(key[it], shift, nb) : 0u;
-                uint64_t peers = __ballot(valid);
-                for (int bb = 0; bb < nb; bb++) {
-                    uint64_t mb = __ballot(valid && ((d >> bb) & 1u));
-                    peers &= ((d >> bb) & 1u) ? mb : ~mb;
+    for (int it = 0; it < IPT; it++) {
+        uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
+        if (it < ipt && pos < U) {
+            dig[it] = key_bits<W>(key[it], dshift, db);
+            slot[it] = atomicAdd(&ht[dig[it]], 1u);
+        }
+    }
+    __syncthreads();
+    {
+        uint32_t v[BPT], sum = 0, big = 0;
+#pragma unroll
+        for (int b = 0; b < BPT; b++) {
+            v[b] = ht[tid * BPT + b];
+            sum += v[b];
+            big |= v[b] > (uint32_t)LOCAL_MAX;
+        }
+        uint32_t total;
+        uint32_t run = block_excl_scan<BT, uint32_t>(sum, ws, &total);
+#pragma unroll
+        for (int b = 0; b < BPT; b++) {
+            ht[tid * BPT + b] = run;
+            run += v[b];
+        }
+        if (big) s_fallback = 1;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IPT; it++) {
+        uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
+        if (it < ipt && pos < U) {
+            uint32_t dst = ht[dig[it]] + slot[it];
+#pragma unroll
+            for (int r = 0; r < W; r++) skeys[dst * W + r] = key[it].w[r];
+            scnt[dst] = cnt[it];
+        }
+    }
+    __syncthreads();
+    if (!s_fallback) {
+        // 3. every thread sorts the keys of its own bins (at most LOCAL_MAX each)
+        uint32_t lo = ht[tid * BPT];
+        for (int b = 0; b < BPT; b++) {
+            const uint32_t hi = b + 1 < BPT ? ht[tid * BPT + b + 1] : (tid + 1 < BT ? ht[(tid + 1) * BPT] : U);
+            for (uint32_t i = lo + 1; i < hi; i++) {
+                Key<W> kk;
+#pragma unroll
+                for (int r = 0; r < W; r++) kk.w[r] = skeys[i * W + r];
+                const uint32_t c = scnt[i];
+                uint32_t j = i;
+                while (j > lo) {
+                    Key<W> pk;
+#pragma unroll
+                    for (int r = 0; r < W; r++) pk.w[r] = skeys[(j - 1) * W + r];
+                    if (!key_less<W>(kk, pk)) break;
+#pragma unroll
+                    for (int r = 0; r < W; r++) skeys[j * W + r] = pk.w[r];
+                    scnt[j] = scnt[j - 1];
+                    j--;
                 }
-                uint32_t below = __popcll(peers & lanemask_lt());
-                uint32_t basev = whist[d * BNW + wave];
-                rank[it] = basev + below;
-                dig[it] = d;
-                if (valid && below == 0) whist[d * BNW + wave] = basev + (uint32_t)__popcll(peers);
+#pragma unroll
+                for (int r = 0; r < W; r++) skeys[j * W + r] = kk.w[r];
+                scnt[j] = c;
             }
+            lo = hi;
         }
         __syncthreads();
-        {
-            uint32_t v0 = whist[tid * 4 + 0], v1 = whist[tid * 4 + 1];
-            uint32_t v2 = whist[tid * 4 + 2], v3 = whist[tid * 4 + 3];
-            uint32_t total;
-            uint32_t ex = block_excl_scan<BT, uint32_t>(v0 + v1 + v2 + v3, ws, &total);
-            whist[tid * 4 + 0] = ex;
-            whist[tid * 4 + 1] = ex + v0;
-            whist[tid * 4 + 2] = ex + v0 + v1;
-            whist[tid * 4 + 3] = ex + v0 + v1 + v2;
-        }
-        __syncthreads();
+    } else {
+        // skewed key bits: stable LSD passes over the unique keys
+        for (int shift = 0; shift < bits && U > 1; shift += 8) {
+            const int nb = std::min(8, bits - shift);
+            for (int j = tid; j < 256 * BNW; j += BT) whist[j] = 0;
+            __syncthreads();
+            uint32_t rank[IPT];
 #pragma unroll
-        for (int it = 0; it < IPT; it++) {
-            uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
-            if (it < ipt && pos < U) {
-                uint32_t dst = whist[dig[it] * BNW + wave] + rank[it];
-#pragma unroll
-                for (int r = 0; r < W; r++) skeys[dst * W + r] = key[it].w[r];
-                scnt[dst] = cnt[it];
+            for (int it = 0; it < IPT; it++) {
+                if (it < ipt) {
+                    uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
+                    bool valid = pos < U;
+                    uint32_t d = valid ? key_bits<W>(key[it], shift, nb) : 0u;
+                    uint64_t peers = __ballot(valid);
+                    for (int bb = 0; bb < nb; bb++) {
+                        uint64_t mb = __ballot(valid && ((d >> bb) & 1u));
+                        peers &= ((d >> bb) & 1u) ? mb : ~mb;
+                    }
+                    uint32_t below = __popcll(peers & lanemask_lt());
+                    uint32_t basev = whist[d * BNW + wave];
+                    rank[it] = basev + below;
+                    dig[it] = d;
+                    if (valid && below == 0) whist[d * BNW + wave] = basev + (uint32_t)__popcll(peers);
+                }
             }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int it = 0; it < IPT; it++) {
-            uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
-            if (it < ipt && pos < U) {
-#pragma unroll
-                for (int r = 0; r < W; r++) key[it].w[r] = skeys[pos * W + r];
-                cnt[it] = scnt[pos];
+            __syncthreads();
+            {
+                uint32_t v0 = whist[tid * 4 + 0], v1 = whist[tid * 4 + 1];
+                uint32_t v2 = whist[tid * 4 + 2], v3 = whist[tid * 4 + 3];
+                uint32_t total;
+                uint32_t ex = block_excl_scan<BT, uint32_t>(v0 + v1 + v2 + v3, ws, &total);
+                whist[tid * 4 + 0] = ex;
+                whist[tid * 4 + 1] = ex + v0;
+                whist[tid * 4 + 2] = ex + v0 + v1;
+                whist[tid * 4 + 3] = ex + v0 + v1 + v2;
             }
+            __syncthreads();
+#pragma unroll
+            for (int it = 0; it < IPT; it++) {
+                uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
+                if (it < ipt && pos < U) {
+                    uint32_t dst = whist[dig[it] * BNW + wave] + rank[it];
+#pragma unroll
+                    for (int r = 0; r < W; r++) skeys[dst * W + r] = key[it].w[r];
+                    scnt[dst] = cnt[it];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int it = 0; it < IPT; it++) {
+                uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
+                if (it < ipt && pos < U) {
+#pragma unroll
+                    for (int r = 0; r < W; r++) key[it].w[r] = skeys[pos * W + r];
+                    cnt[it] = scnt[pos];
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
     for (uint32_t q2 = tid; q2 < U; q2 += BT) {
         uint64_t *d = X + (start + q2) * W;
@@ -1065,10 +1164,12 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
         auto go = [&](auto cls, uint32_t first, uint32_t cnt) {
             constexpr int C = decltype(cls)::value;
             if (!cnt) return;
-            if constexpr (C < SORT_CLASSES - 1) {
+            // the hash kernel's table doubles the LDS footprint: above class 1
+            // it would run one workgroup per CU, where the LSD kernel is faster
+            if constexpr (C < 2) {
                 if (tun.dedup) {
                     launch("k_bucket_sort", 0, [&] {
-                        k_bucket_dedup_sort<W, SortClass<W, C>::IPT><<<dim3(cnt), dim3(BT), 0, s>>>(
+                        k_bucket_hash_sort<W, SortClass<W, C>::IPT><<<dim3(cnt), dim3(BT), 0, s>>>(
                             A, B, ctmp, buckets_.as<Bucket>(), ord + first, ucount, k);
                     });
                     return;

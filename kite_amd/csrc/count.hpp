@@ -45,7 +45,7 @@ struct KernelStat {
 struct CountTunables {
     int split_target_div = 4;  // split partitions until max child <= largest cap / div
     int max_digit_bits = 12;
-    bool dedup = false;  // hash-dedup each bucket in LDS before sorting its unique keys
+    bool dedup = true;  // small buckets: hash dedup + one bin pass instead of LSD passes
 };
 
 class CountEngine {

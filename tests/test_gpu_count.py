@@ -178,3 +178,21 @@ def test_sort_variants_agree(gpu_ctx, monkeypatch, dedup, split_div):
     for k in (15, 40, 90):
         res, tab, _ = run_both(gpu_ctx, fq, k, True, 2)
         check_same(res, tab, f"repeats k={k} dedup={dedup}")
+
+
+@pytest.mark.parametrize("dedup", ["0", "1"])
+def test_low_complexity_keys(gpu_ctx, monkeypatch, dedup):
+    """Reads of a long A run followed by random bases: most distinct keys
+    share their top key bits, so whole partitions land in one LDS bin (the
+    hash-then-bin sort then takes its LSD fallback)."""
+    monkeypatch.setenv("KITE_SORT_DEDUP", dedup)
+    rng = random.Random(21)
+    recs = []
+    for i in range(400):
+        tail = "".join(rng.choice("ACGT") for _ in range(40))
+        s = "A" * 100 + tail
+        recs.append(f"@lc{i}\n{s}\n+\n{'I' * len(s)}\n")
+    fq = "".join(recs).encode()
+    for k, canon, P in ((31, False, 8), (31, True, 4), (45, False, 8), (21, False, 1)):
+        res, tab, _ = run_both(gpu_ctx, fq, k, canon, P)
+        check_same(res, tab, f"low complexity k={k} canon={canon} P={P} dedup={dedup}")
