@@ -58,6 +58,59 @@ __device__ __forceinline__ __bf16 bf16_from_bits(uint16_t b) {
     return r;
 }
 
+// ---- Gram kernels --------------------------------------------------------
+// Each wave owns a contiguous range of rows and walks it in blocks of RB
+// rows.  The codes of the next block are loaded into registers (coalesced
+// 32-bit words) while the current block is multiplied; a block passes
+// through the wave's own LDS area so that every lane can read the (row,
+// column) cells its MFMA operands need.  The value table of the first LSM
+// counts of every column lives in LDS; larger counts read the global table.
+constexpr int RB = 64;    // rows per wave block
+constexpr int LSM = 256;  // table entries per column held in LDS (<= stride)
+constexpr int LSM64 = 128;  // the same for the FP64 table (8-byte entries)
+// Column strides of the LDS tables, one entry longer than the table: lanes
+// that read different columns at the same (mostly small) count then fall
+// into different LDS banks.
+constexpr int LSP = LSM + 1;
+constexpr int LSP64 = LSM64 + 1;
+
+template <class T>
+struct CodeBlock {
+    static constexpr int WORDS = RB * MAX_COLS * (int)sizeof(T) / 4;  // LDS words per wave
+    static constexpr int MAXW = WORDS / 64;                          // words per lane
+};
+
+// Loads the cnt codes starting at element e0 (e0 * sizeof(T) is 4-byte
+// aligned) as 32-bit words; words past the block read as zero.
+template <class T>
+__device__ __forceinline__ void fetch_codes(const T *__restrict__ codes, uint64_t e0, uint32_t cnt,
+                                            uint32_t (&w)[CodeBlock<T>::MAXW], int lane) {
+    const uint32_t nbytes = cnt * (uint32_t)sizeof(T);
+    const uint32_t nfull = nbytes / 4;
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(codes + e0);
+#pragma unroll
+    for (int j = 0; j < CodeBlock<T>::MAXW; j++) {
+        const uint32_t idx = (uint32_t)j * 64u + (uint32_t)lane;
+        uint32_t v = 0;
+        if (idx < nfull) v = src[idx];
+        else if (idx == nfull && (nbytes & 3u)) v = (uint32_t)codes[e0 + cnt - 1];  // odd trailing u16
+        w[j] = v;
+    }
+}
+
+template <class T>
+__device__ __forceinline__ void stage_codes(uint32_t *__restrict__ sw, const uint32_t (&w)[CodeBlock<T>::MAXW],
+                                            int lane) {
+#pragma unroll
+    for (int j = 0; j < CodeBlock<T>::MAXW; j++) sw[j * 64 + lane] = w[j];
+}
+
+template <class T>
+__device__ __forceinline__ uint32_t staged_code(const uint32_t *__restrict__ sw, uint32_t e) {
+    if constexpr (sizeof(T) == 2) return reinterpret_cast<const uint16_t *>(sw)[e];
+    else return sw[e];
+}
+
 // lut: per column `stride` entries of packed (hi | lo << 16) bf16 bits.
 // partial: [wave][16][64] FP64 accumulators in MFMA output layout.
 template <class T>
@@ -65,14 +118,22 @@ __global__ __launch_bounds__(GT) void k_gram_bf16(const T *__restrict__ codes, u
                                                    const uint32_t *__restrict__ lut, uint32_t stride,
                                                    uint64_t rows_per_wave, int with_lolo,
                                                    double *__restrict__ partial) {
+    __shared__ uint32_t slut[MAX_COLS * LSP];
+    __shared__ uint32_t sblk[GT / 64][CodeBlock<T>::WORDS];
     const int lane = threadIdx.x & 63;
-    const uint64_t wave = (uint64_t)blockIdx.x * (GT / 64) + (threadIdx.x >> 6);
-    const uint64_t r0 = wave * rows_per_wave;
+    const int wv = threadIdx.x >> 6;
+    for (int i = threadIdx.x; i < n * LSM; i += GT)
+        slut[(i / LSM) * LSP + (i % LSM)] = lut[(uint64_t)(i / LSM) * stride + (i % LSM)];
+    __syncthreads();
+    const uint64_t wave = (uint64_t)blockIdx.x * (GT / 64) + wv;
+    const uint64_t r0 = std::min<uint64_t>(N, wave * rows_per_wave);
     const uint64_t r1 = std::min<uint64_t>(N, r0 + rows_per_wave);
     const int col = lane & 31;
     const int kb = 8 * (lane >> 5);
     const bool colok = col < n;
     const uint32_t *lcol = lut + (uint64_t)(colok ? col : 0) * stride;
+    const uint32_t *scol = slut + (colok ? col : 0) * LSP;
+    uint32_t *sw = sblk[wv];
     floatx16 acc;
     double facc[16];
 #pragma unroll
@@ -81,28 +142,42 @@ __global__ __launch_bounds__(GT) void k_gram_bf16(const T *__restrict__ codes, u
         facc[i] = 0.0;
     }
     int steps = 0;
-    for (uint64_t r = r0; r < r1; r += 16) {
-        bf16x8 hi, lo;
+    uint32_t w[CodeBlock<T>::MAXW];
+    if (r0 < r1) fetch_codes<T>(codes, r0 * (uint64_t)n, (uint32_t)(std::min<uint64_t>(RB, r1 - r0) * n), w, lane);
+    for (uint64_t rb = r0; rb < r1; rb += RB) {
+        const uint32_t rows = (uint32_t)std::min<uint64_t>(RB, r1 - rb);
+        stage_codes<T>(sw, w, lane);
+        __builtin_amdgcn_wave_barrier();
+        if (rb + RB < r1)
+            fetch_codes<T>(codes, (rb + RB) * (uint64_t)n, (uint32_t)(std::min<uint64_t>(RB, r1 - rb - RB) * n), w,
+                           lane);
+        for (uint32_t s0 = 0; s0 < rows; s0 += 16) {
+            bf16x8 hi, lo;
 #pragma unroll
-        for (int t = 0; t < 8; t++) {
-            uint64_t row = r + kb + t;
-            uint32_t packed = 0;
-            if (colok && row < r1) packed = lcol[(uint32_t)codes[row * n + col]];
-            hi[t] = bf16_from_bits((uint16_t)(packed & 0xFFFF));
-            lo[t] = bf16_from_bits((uint16_t)(packed >> 16));
-        }
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi, hi, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi, lo, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lo, hi, acc, 0, 0, 0);
-        if (with_lolo) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lo, lo, acc, 0, 0, 0);
-        if (++steps == FLUSH_STEPS) {
-            steps = 0;
+            for (int t = 0; t < 8; t++) {
+                const uint32_t rl = s0 + (uint32_t)kb + (uint32_t)t;
+                uint32_t packed = 0;
+                if (colok && rl < rows) {
+                    const uint32_t c = staged_code<T>(sw, rl * (uint32_t)n + (uint32_t)col);
+                    packed = c < (uint32_t)LSM ? scol[c] : lcol[c];
+                }
+                hi[t] = bf16_from_bits((uint16_t)(packed & 0xFFFF));
+                lo[t] = bf16_from_bits((uint16_t)(packed >> 16));
+            }
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi, hi, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi, lo, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lo, hi, acc, 0, 0, 0);
+            if (with_lolo) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lo, lo, acc, 0, 0, 0);
+            if (++steps == FLUSH_STEPS) {
+                steps = 0;
 #pragma unroll
-            for (int i = 0; i < 16; i++) {
-                facc[i] += (double)acc[i];
-                acc[i] = 0.f;
+                for (int i = 0; i < 16; i++) {
+                    facc[i] += (double)acc[i];
+                    acc[i] = 0.f;
+                }
             }
         }
+        __builtin_amdgcn_wave_barrier();
     }
 #pragma unroll
     for (int i = 0; i < 16; i++) facc[i] += (double)acc[i];
@@ -110,30 +185,56 @@ __global__ __launch_bounds__(GT) void k_gram_bf16(const T *__restrict__ codes, u
     for (int i = 0; i < 16; i++) partial[(wave * 16 + i) * 64 + lane] = facc[i];
 }
 
-// FP64 baseline: v_mfma_f64_16x16x4 on the same codes, FP64 values.
-// partial: [wave][3 blocks (00, 01, 11)][4][64].
+// FP64 baseline: v_mfma_f64_16x16x4 on the same codes and the same staging,
+// FP64 values.  partial: [wave][3 blocks (00, 01, 11)][4][64].
 template <class T>
 __global__ __launch_bounds__(GT) void k_gram_f64(const T *__restrict__ codes, uint64_t N, int n,
                                                   const double *__restrict__ lut, uint32_t stride,
                                                   uint64_t rows_per_wave, double *__restrict__ partial) {
+    __shared__ double slut[MAX_COLS * LSP64];
+    __shared__ uint32_t sblk[GT / 64][CodeBlock<T>::WORDS];
     const int lane = threadIdx.x & 63;
-    const uint64_t wave = (uint64_t)blockIdx.x * (GT / 64) + (threadIdx.x >> 6);
-    const uint64_t r0 = wave * rows_per_wave;
+    const int wv = threadIdx.x >> 6;
+    for (int i = threadIdx.x; i < n * LSM64; i += GT)
+        slut[(i / LSM64) * LSP64 + (i % LSM64)] = lut[(uint64_t)(i / LSM64) * stride + (i % LSM64)];
+    __syncthreads();
+    const uint64_t wave = (uint64_t)blockIdx.x * (GT / 64) + wv;
+    const uint64_t r0 = std::min<uint64_t>(N, wave * rows_per_wave);
     const uint64_t r1 = std::min<uint64_t>(N, r0 + rows_per_wave);
     const int c0 = lane & 15, c1 = 16 + (lane & 15);
     const int kk = lane >> 4;
     const bool ok0 = c0 < n, ok1 = c1 < n;
+    const double *l0 = lut + (uint64_t)(ok0 ? c0 : 0) * stride, *l1 = lut + (uint64_t)(ok1 ? c1 : 0) * stride;
+    const double *s0l = slut + (ok0 ? c0 : 0) * LSP64, *s1l = slut + (ok1 ? c1 : 0) * LSP64;
+    uint32_t *sw = sblk[wv];
     doublex4 d00 = {0, 0, 0, 0}, d01 = {0, 0, 0, 0}, d11 = {0, 0, 0, 0};
-    for (uint64_t r = r0; r < r1; r += 4) {
-        uint64_t row = r + kk;
-        double a0 = 0, a1 = 0;
-        if (row < r1) {
-            if (ok0) a0 = lut[(uint64_t)c0 * stride + (uint32_t)codes[row * n + c0]];
-            if (ok1) a1 = lut[(uint64_t)c1 * stride + (uint32_t)codes[row * n + c1]];
+    uint32_t w[CodeBlock<T>::MAXW];
+    if (r0 < r1) fetch_codes<T>(codes, r0 * (uint64_t)n, (uint32_t)(std::min<uint64_t>(RB, r1 - r0) * n), w, lane);
+    for (uint64_t rb = r0; rb < r1; rb += RB) {
+        const uint32_t rows = (uint32_t)std::min<uint64_t>(RB, r1 - rb);
+        stage_codes<T>(sw, w, lane);
+        __builtin_amdgcn_wave_barrier();
+        if (rb + RB < r1)
+            fetch_codes<T>(codes, (rb + RB) * (uint64_t)n, (uint32_t)(std::min<uint64_t>(RB, r1 - rb - RB) * n), w,
+                           lane);
+        for (uint32_t s4 = 0; s4 < rows; s4 += 4) {
+            const uint32_t rl = s4 + (uint32_t)kk;
+            double a0 = 0, a1 = 0;
+            if (rl < rows) {
+                if (ok0) {
+                    const uint32_t c = staged_code<T>(sw, rl * (uint32_t)n + (uint32_t)c0);
+                    a0 = c < (uint32_t)LSM64 ? s0l[c] : l0[c];
+                }
+                if (ok1) {
+                    const uint32_t c = staged_code<T>(sw, rl * (uint32_t)n + (uint32_t)c1);
+                    a1 = c < (uint32_t)LSM64 ? s1l[c] : l1[c];
+                }
+            }
+            d00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, a0, d00, 0, 0, 0);
+            d01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, a1, d01, 0, 0, 0);
+            d11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, a1, d11, 0, 0, 0);
         }
-        d00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, a0, d00, 0, 0, 0);
-        d01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, a1, d01, 0, 0, 0);
-        d11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, a1, d11, 0, 0, 0);
+        __builtin_amdgcn_wave_barrier();
     }
 #pragma unroll
     for (int i = 0; i < 4; i++) {
@@ -143,7 +244,20 @@ __global__ __launch_bounds__(GT) void k_gram_f64(const T *__restrict__ codes, ui
     }
 }
 
-// Sum partials over waves in a fixed order (deterministic).
+// Sum partials over waves in a fixed order (deterministic), in two levels:
+// stage r sums waves [r*chunk, (r+1)*chunk) of every element, then the
+// stage sums are added in order.
+__global__ void k_sum_partials_stage(const double *__restrict__ partial, uint64_t waves, uint32_t per_wave,
+                                     uint64_t chunk, double *__restrict__ stage) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t r = blockIdx.y;
+    if (e >= per_wave) return;
+    const uint64_t w0 = r * chunk, w1 = std::min<uint64_t>(waves, w0 + chunk);
+    double s = 0;
+    for (uint64_t w = w0; w < w1; w++) s += partial[w * per_wave + e];
+    stage[r * per_wave + e] = s;
+}
+
 __global__ void k_sum_partials(const double *__restrict__ partial, uint64_t waves, uint32_t per_wave,
                                double *__restrict__ out) {
     uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -302,10 +416,23 @@ void CorrEngine::make_lut(int method, uint64_t N, int n, std::vector<double> &z,
     }
 }
 
+void CorrEngine::reduce_partials(uint64_t waves, uint32_t per_wave) {
+    const uint64_t R = std::min<uint64_t>(256, std::max<uint64_t>(1, waves));
+    const uint64_t chunk = (waves + R - 1) / R;
+    stage_.ensure(R * per_wave * 8);
+    k_sum_partials_stage<<<dim3((per_wave + 255) / 256, (unsigned)R), 256, 0, stream_>>>(
+        partial_.as<double>(), waves, per_wave, chunk, stage_.as<double>());
+    KITE_CHECK_LAUNCH();
+    k_sum_partials<<<(per_wave + 255) / 256, 256, 0, stream_>>>(stage_.as<double>(), R, per_wave, sum_.as<double>());
+    KITE_CHECK_LAUNCH();
+}
+
 void CorrEngine::gram(const void *d_codes, int code_bytes, uint64_t N, int n, const std::vector<double> &z,
                       bool fp64, bool with_lolo, std::vector<double> &G, float *ms) {
     const uint32_t stride = stride_;
-    const uint64_t rpw = std::max<uint64_t>(512, ((N + 4095) / 4096 + 511) / 512 * 512);  // multiple of 16
+    // about 12K waves (several rounds of 256 CUs x 12 resident waves, so the
+    // last round is a small fraction), each a multiple of 512 rows
+    const uint64_t rpw = std::max<uint64_t>(512, ((N + 12287) / 12288 + 511) / 512 * 512);
     const uint64_t waves = std::max<uint64_t>(1, (N + rpw - 1) / rpw);
     const uint64_t wgs = (waves + 3) / 4;
     const uint64_t waves_launched = wgs * 4;
@@ -334,9 +461,7 @@ void CorrEngine::gram(const void *d_codes, int code_bytes, uint64_t N, int n, co
             k_gram_bf16<uint32_t><<<(uint32_t)wgs, GT, 0, stream_>>>((const uint32_t *)d_codes, N, n, lut_.as<uint32_t>(),
                                                                      stride, rpw, with_lolo, partial_.as<double>());
         KITE_CHECK_LAUNCH();
-        k_sum_partials<<<(per_wave + 255) / 256, 256, 0, stream_>>>(partial_.as<double>(), waves_launched, per_wave,
-                                                                    sum_.as<double>());
-        KITE_CHECK_LAUNCH();
+        reduce_partials(waves_launched, per_wave);
         KITE_HIP(hipEventRecord(e1, stream_));
         out.resize(per_wave);
         KITE_HIP(hipMemcpyAsync(out.data(), sum_.ptr, per_wave * 8, hipMemcpyDeviceToHost, stream_));
@@ -365,9 +490,7 @@ void CorrEngine::gram(const void *d_codes, int code_bytes, uint64_t N, int n, co
             k_gram_f64<uint32_t><<<(uint32_t)wgs4, GT, 0, stream_>>>((const uint32_t *)d_codes, N, n, lut_.as<double>(),
                                                                      stride, rpw4, partial_.as<double>());
         KITE_CHECK_LAUNCH();
-        k_sum_partials<<<(per_wave + 255) / 256, 256, 0, stream_>>>(partial_.as<double>(), wgs4 * 4, per_wave,
-                                                                    sum_.as<double>());
-        KITE_CHECK_LAUNCH();
+        reduce_partials(wgs4 * 4, per_wave);
         KITE_HIP(hipEventRecord(e1, stream_));
         out.resize(per_wave);
         KITE_HIP(hipMemcpyAsync(out.data(), sum_.ptr, per_wave * 8, hipMemcpyDeviceToHost, stream_));

@@ -38,9 +38,11 @@ class CorrEngine {
     void gram(const void *d_codes, int code_bytes, uint64_t N, int n, const std::vector<double> &z, bool fp64,
               bool with_lolo, std::vector<double> &G, float *ms);
 
+    void reduce_partials(uint64_t waves, uint32_t per_wave);
+
     hipStream_t stream_;
     uint32_t stride_ = 0;
-    DevBuf hist_, lut_, partial_, sum_;
+    DevBuf hist_, lut_, partial_, sum_, stage_;
     std::vector<uint32_t> hist_h_;
 };
 

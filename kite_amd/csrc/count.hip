@@ -32,7 +32,6 @@ constexpr int SEGB = 32;                // bytes per segment (one thread)
 constexpr int HALO = 128;               // >= k_max - 1 bytes before each sub-tile
 constexpr int HSEG = HALO / SEGB;       // 4 halo segments
 constexpr int NSEG = SUB / SEGB + HSEG; // 260 segments per sub-tile
-constexpr int SUBS = 4;                 // sub-tiles per extract workgroup
 constexpr int EXT_U4 = (SUB + HALO) / 16;
 constexpr uint32_t MAX_PARTITIONS = 8192;
 
@@ -142,6 +141,7 @@ struct ExtractArgs {
     const uint64_t *sub_line0;  // lines before each sub-tile
     uint64_t n_sub;
     uint32_t nwg;               // extract workgroups = matrix columns
+    uint32_t subs;              // sub-tiles per workgroup
     int k, canonical, pbits;
     uint32_t *hist;             // [P][nwg] counts (histogram pass)
     const uint64_t *off;        // scanned [P][nwg] (scatter pass)
@@ -183,8 +183,8 @@ __global__ __launch_bounds__(XT) void k_extract(ExtractArgs a) {
     const int rs = 64 * W - 2 * a.k;
     const int rq = rs >> 6, rb = rs & 63;
 
-    for (int si = 0; si < SUBS; si++) {
-        uint64_t st = (uint64_t)wg * SUBS + si;
+    for (uint32_t si = 0; si < a.subs; si++) {
+        uint64_t st = (uint64_t)wg * a.subs + si;
         if (st >= a.n_sub) break;
         const int64_t base = (int64_t)(st * SUB);
         __syncthreads();
@@ -320,15 +320,23 @@ struct SplitArgs {
     uint64_t *dst;
     const Seg *segs;
     uint32_t nseg;
-    const uint64_t *tile_pre;  // nseg + 1 tile prefix
+    const uint64_t *tile_pre;  // nseg + 1 prefix of workgroup chunks
     int shift, dbits;
-    uint32_t *child_hist;      // histogram pass
+    uint32_t *child_hist;      // histogram pass: global child totals
+    uint32_t *wg_hist;         // [workgroup][2^dbits] child counts of each chunk
     uint64_t *child_cur;       // scatter pass: running destination per child
+    uint32_t chunk;            // tiles of TK keys per workgroup
 };
 
+// One workgroup handles a chunk of `chunk` tiles (TK keys each) of one segment,
+// so that each child receives a run of about TK*chunk/2^dbits keys from it.
+// The histogram pass keeps every chunk's child counts; the scatter pass
+// reserves one range per (chunk, child) with a single atomic and fills it
+// through an LDS cursor.
 template <int W, bool SCATTER>
 __global__ __launch_bounds__(ST) void k_split(SplitArgs a) {
     constexpr int IPT = SplitCfg<W>::IPT, TK = SplitCfg<W>::TK;
+    const uint64_t TKL = (uint64_t)TK * a.chunk;
     __shared__ uint32_t hist[1 << MAX_DBITS];
     __shared__ uint64_t cbase[SCATTER ? (1 << MAX_DBITS) : 1];
     const int tid = threadIdx.x;
@@ -341,44 +349,62 @@ __global__ __launch_bounds__(ST) void k_split(SplitArgs a) {
     }
     const Seg sg = a.segs[lo];
     const uint64_t t = b - a.tile_pre[lo];
-    const uint64_t first = sg.start + t * TK;
-    const uint64_t n = std::min<uint64_t>((uint64_t)TK, sg.size - t * TK);
+    const uint64_t first = sg.start + t * TKL;
+    const uint64_t n = std::min<uint64_t>(TKL, sg.size - t * TKL);
     const uint32_t nd = 1u << a.dbits;
-    for (uint32_t g = tid; g < nd; g += ST) hist[g] = 0;
-    __syncthreads();
-    Key<W> key[IPT];
-    uint32_t dig[IPT];
-#pragma unroll
-    for (int it = 0; it < IPT; it++) {
-        uint64_t i = (uint64_t)it * ST + tid;
-        if (i < n) {
-            const uint64_t *p = a.src + (first + i) * W;
-#pragma unroll
-            for (int r = 0; r < W; r++) key[it].w[r] = p[r];
-            dig[it] = key_bits<W>(key[it], a.shift, a.dbits);
-            atomicAdd(&hist[dig[it]], 1u);
-        }
-    }
-    __syncthreads();
+    uint32_t *row = a.wg_hist + b * nd;
     if constexpr (!SCATTER) {
-        for (uint32_t g = tid; g < nd; g += ST)
-            if (hist[g]) atomicAdd(&a.child_hist[sg.child_base + g], hist[g]);
+        for (uint32_t g = tid; g < nd; g += ST) hist[g] = 0;
+        __syncthreads();
+        for (uint64_t c0 = 0; c0 < n; c0 += TK) {
+#pragma unroll
+            for (int it = 0; it < IPT; it++) {
+                uint64_t i = c0 + (uint64_t)it * ST + tid;
+                if (i < n) {
+                    Key<W> key;
+                    const uint64_t *p = a.src + (first + i) * W;
+#pragma unroll
+                    for (int r = 0; r < W; r++) key.w[r] = p[r];
+                    atomicAdd(&hist[key_bits<W>(key, a.shift, a.dbits)], 1u);
+                }
+            }
+        }
+        __syncthreads();
+        for (uint32_t g = tid; g < nd; g += ST) {
+            const uint32_t c = hist[g];
+            row[g] = c;
+            if (c) atomicAdd(&a.child_hist[sg.child_base + g], c);
+        }
     } else {
         for (uint32_t g = tid; g < nd; g += ST) {
-            uint32_t c = hist[g];
+            const uint32_t c = row[g];
             if (c) cbase[g] = atomicAdd((unsigned long long *)&a.child_cur[sg.child_base + g],
                                         (unsigned long long)c);
             hist[g] = 0;
         }
         __syncthreads();
+        for (uint64_t c0 = 0; c0 < n; c0 += TK) {
+            Key<W> key[IPT];
+            uint32_t dig[IPT];
 #pragma unroll
-        for (int it = 0; it < IPT; it++) {
-            uint64_t i = (uint64_t)it * ST + tid;
-            if (i < n) {
-                uint64_t slot = cbase[dig[it]] + atomicAdd(&hist[dig[it]], 1u);
-                uint64_t *d = a.dst + slot * W;
+            for (int it = 0; it < IPT; it++) {
+                uint64_t i = c0 + (uint64_t)it * ST + tid;
+                if (i < n) {
+                    const uint64_t *p = a.src + (first + i) * W;
 #pragma unroll
-                for (int r = 0; r < W; r++) d[r] = key[it].w[r];
+                    for (int r = 0; r < W; r++) key[it].w[r] = p[r];
+                    dig[it] = key_bits<W>(key[it], a.shift, a.dbits);
+                }
+            }
+#pragma unroll
+            for (int it = 0; it < IPT; it++) {
+                uint64_t i = c0 + (uint64_t)it * ST + tid;
+                if (i < n) {
+                    uint64_t slot = cbase[dig[it]] + atomicAdd(&hist[dig[it]], 1u);
+                    uint64_t *d = a.dst + slot * W;
+#pragma unroll
+                    for (int r = 0; r < W; r++) d[r] = key[it].w[r];
+                }
             }
         }
     }
@@ -541,24 +567,25 @@ constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v / 2); }
 // contiguous bins.  A bin above LOCAL_MAX keys (skewed key bits) sends the
 // whole bucket through the LSD passes instead.  Output equals
 // k_bucket_sort's: ascending unique keys with their run lengths.
-template <int W, int IPT>
-__global__ __launch_bounds__(BT) void k_bucket_hash_sort(uint64_t *__restrict__ A,
+template <int W, int IPT, int NT>
+__global__ __launch_bounds__(NT) void k_bucket_hash_sort(uint64_t *__restrict__ A,
                                                           uint64_t *__restrict__ B,
                                                           uint32_t *__restrict__ counts_tmp,
                                                           const Bucket *__restrict__ buckets,
                                                           const uint32_t *__restrict__ order,
                                                           uint32_t *__restrict__ ucount, int k) {
-    constexpr int CAP = BT * IPT;
+    constexpr int NW = NT / 64;
+    constexpr int CAP = NT * IPT;
     constexpr int HT = 2 * CAP;  // hash slots (load factor <= 1/2), then bins
-    constexpr int BPT = HT / BT; // bins per thread
+    constexpr int BPT = HT / NT; // bins per thread
     constexpr int DB = ilog2c(HT);
     constexpr int LOCAL_MAX = 16;
-    constexpr bool WH_IN_HT = HT >= 256 * BNW;  // the LSD histogram reuses the table when it fits
+    constexpr bool WH_IN_HT = HT >= 256 * NW;  // the LSD histogram reuses the table when it fits
     __shared__ uint64_t skeys[CAP * W];
     __shared__ uint32_t scnt[CAP];
     __shared__ uint32_t ht[HT];
-    __shared__ uint32_t whist_own[WH_IN_HT ? 1 : 256 * BNW];
-    __shared__ uint32_t ws[BNW + 1];
+    __shared__ uint32_t whist_own[WH_IN_HT ? 1 : 256 * NW];
+    __shared__ uint32_t ws[NW + 1];
     __shared__ uint32_t s_fallback;
     uint32_t *whist = WH_IN_HT ? ht : whist_own;
     const int tid = threadIdx.x;
@@ -580,10 +607,10 @@ __global__ __launch_bounds__(BT) void k_bucket_hash_sort(uint64_t *__restrict__ 
         return;
     }
     const uint32_t m = (uint32_t)bk.size;
-    int ipt = (int)((m + BT - 1) / BT);
+    int ipt = (int)((m + NT - 1) / NT);
     uint32_t wbase = (uint32_t)wave * 64u * (uint32_t)ipt;
-    for (int j = tid; j < HT; j += BT) ht[j] = 0;
-    for (int j = tid; j < CAP; j += BT) scnt[j] = 0;
+    for (int j = tid; j < HT; j += NT) ht[j] = 0;
+    for (int j = tid; j < CAP; j += NT) scnt[j] = 0;
     Key<W> key[IPT];
 #pragma unroll
     for (int it = 0; it < IPT; it++) {
@@ -598,8 +625,11 @@ __global__ __launch_bounds__(BT) void k_bucket_hash_sort(uint64_t *__restrict__ 
         }
     }
     __syncthreads();
-    // 1. hash dedup: the first position of every key collects its count
-    uint32_t is_rep = 0;  // bit it: item it holds the first position of its key
+    // 1. hash dedup: the first position of every key collects its count.
+    // Probes go in rounds over all of a thread's items, so the independent
+    // LDS compare-and-swaps of one round are in flight together.
+    uint32_t h[IPT], rep[IPT];
+    uint32_t pending = 0;
 #pragma unroll
     for (int it = 0; it < IPT; it++) {
         uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
@@ -607,75 +637,67 @@ __global__ __launch_bounds__(BT) void k_bucket_hash_sort(uint64_t *__restrict__ 
             uint64_t f = key[it].w[0];
 #pragma unroll
             for (int r = 1; r < W; r++) f ^= key[it].w[r] * 0xC2B2AE3D27D4EB4Full;
-            uint32_t h = (uint32_t)((f * 0xD6E8FEB86659FD93ull) >> 40) & (HT - 1);
-            uint32_t rep;
-            for (;;) {
-                uint32_t old = atomicCAS(&ht[h], 0u, pos + 1);
-                if (old == 0) {
-                    rep = pos;
-                    break;
-                }
-                bool eq = true;
+            h[it] = (uint32_t)((f * 0xD6E8FEB86659FD93ull) >> 40) & (HT - 1);
+            pending |= 1u << it;
+        }
+        rep[it] = 0;
+    }
+    while (pending) {
+        uint32_t old[IPT];
 #pragma unroll
-                for (int r = 0; r < W; r++) eq = eq && skeys[(old - 1) * W + r] == key[it].w[r];
-                if (eq) {
-                    rep = old - 1;
-                    break;
-                }
-                h = (h + 1) & (HT - 1);
+        for (int it = 0; it < IPT; it++) {
+            const uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
+            old[it] = ((pending >> it) & 1u) ? atomicCAS(&ht[h[it]], 0u, pos + 1) : 0u;
+        }
+#pragma unroll
+        for (int it = 0; it < IPT; it++) {
+            if (!((pending >> it) & 1u)) continue;
+            const uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
+            if (old[it] == 0) {
+                rep[it] = pos;
+                pending &= ~(1u << it);
+                continue;
             }
-            atomicAdd(&scnt[rep], 1u);
-            if (rep == pos) is_rep |= 1u << it;
+            bool eq = true;
+#pragma unroll
+            for (int r = 0; r < W; r++) eq = eq && skeys[(old[it] - 1) * W + r] == key[it].w[r];
+            if (eq) {
+                rep[it] = old[it] - 1;
+                pending &= ~(1u << it);
+            } else {
+                h[it] = (h[it] + 1) & (HT - 1);
+            }
         }
     }
-    __syncthreads();
-    // compact the representatives with their counts
-    uint32_t mine = __popc(is_rep), U;
-    uint32_t q = block_excl_scan<BT, uint32_t>(mine, ws, &U);
-    uint32_t cnt[IPT];
+    uint32_t is_rep = 0;  // bit it: item it holds the first position of its key
 #pragma unroll
     for (int it = 0; it < IPT; it++) {
         uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
-        cnt[it] = ((is_rep >> it) & 1u) ? scnt[pos] : 0u;
+        if (it < ipt && pos < m) {
+            atomicAdd(&scnt[rep[it]], 1u);
+            if (rep[it] == pos) is_rep |= 1u << it;
+        }
     }
     __syncthreads();
+    // 2. one counting pass of the representatives over HT bins on their top
+    // remaining key bits
+    const int db = bits < DB ? bits : DB;
+    const int dshift = bits - db;
+    for (int j = tid; j < HT; j += NT) ht[j] = 0;
+    if (tid == 0) s_fallback = 0;
+    __syncthreads();
+    uint32_t cnt[IPT], slot[IPT], dig[IPT];
 #pragma unroll
     for (int it = 0; it < IPT; it++) {
         if ((is_rep >> it) & 1u) {
-#pragma unroll
-            for (int r = 0; r < W; r++) skeys[q * W + r] = key[it].w[r];
-            scnt[q] = cnt[it];
-            q++;
-        }
-    }
-    __syncthreads();
-    ipt = (int)((U + BT - 1) / BT);
-    wbase = (uint32_t)wave * 64u * (uint32_t)ipt;
-#pragma unroll
-    for (int it = 0; it < IPT; it++) {
-        uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
-        if (it < ipt && pos < U) {
-#pragma unroll
-            for (int r = 0; r < W; r++) key[it].w[r] = skeys[pos * W + r];
+            const uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
             cnt[it] = scnt[pos];
-        }
-    }
-    // 2. one counting pass over HT bins on the top remaining key bits
-    const int db = bits < DB ? bits : DB;
-    const int dshift = bits - db;
-    for (int j = tid; j < HT; j += BT) ht[j] = 0;
-    if (tid == 0) s_fallback = 0;
-    __syncthreads();
-    uint32_t slot[IPT], dig[IPT];
-#pragma unroll
-    for (int it = 0; it < IPT; it++) {
-        uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
-        if (it < ipt && pos < U) {
             dig[it] = key_bits<W>(key[it], dshift, db);
             slot[it] = atomicAdd(&ht[dig[it]], 1u);
         }
     }
     __syncthreads();
+    uint32_t boff[BPT + 1], U;
     {
         uint32_t v[BPT], sum = 0, big = 0;
 #pragma unroll
@@ -684,21 +706,21 @@ __global__ __launch_bounds__(BT) void k_bucket_hash_sort(uint64_t *__restrict__ 
             sum += v[b];
             big |= v[b] > (uint32_t)LOCAL_MAX;
         }
-        uint32_t total;
-        uint32_t run = block_excl_scan<BT, uint32_t>(sum, ws, &total);
+        uint32_t run = block_excl_scan<NT, uint32_t>(sum, ws, &U);
 #pragma unroll
         for (int b = 0; b < BPT; b++) {
+            boff[b] = run;
             ht[tid * BPT + b] = run;
             run += v[b];
         }
+        boff[BPT] = run;
         if (big) s_fallback = 1;
     }
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < IPT; it++) {
-        uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
-        if (it < ipt && pos < U) {
-            uint32_t dst = ht[dig[it]] + slot[it];
+        if ((is_rep >> it) & 1u) {
+            const uint32_t dst = ht[dig[it]] + slot[it];
 #pragma unroll
             for (int r = 0; r < W; r++) skeys[dst * W + r] = key[it].w[r];
             scnt[dst] = cnt[it];
@@ -707,9 +729,9 @@ __global__ __launch_bounds__(BT) void k_bucket_hash_sort(uint64_t *__restrict__ 
     __syncthreads();
     if (!s_fallback) {
         // 3. every thread sorts the keys of its own bins (at most LOCAL_MAX each)
-        uint32_t lo = ht[tid * BPT];
+#pragma unroll
         for (int b = 0; b < BPT; b++) {
-            const uint32_t hi = b + 1 < BPT ? ht[tid * BPT + b + 1] : (tid + 1 < BT ? ht[(tid + 1) * BPT] : U);
+            const uint32_t lo = boff[b], hi = boff[b + 1];
             for (uint32_t i = lo + 1; i < hi; i++) {
                 Key<W> kk;
 #pragma unroll
@@ -730,14 +752,26 @@ __global__ __launch_bounds__(BT) void k_bucket_hash_sort(uint64_t *__restrict__ 
                 for (int r = 0; r < W; r++) skeys[j * W + r] = kk.w[r];
                 scnt[j] = c;
             }
-            lo = hi;
         }
         __syncthreads();
     } else {
-        // skewed key bits: stable LSD passes over the unique keys
+        // skewed key bits: stable LSD passes over the unique keys, reloaded
+        // from skeys[0, U) in the wave-striped layout
+        ipt = (int)((U + NT - 1) / NT);
+        wbase = (uint32_t)wave * 64u * (uint32_t)ipt;
+#pragma unroll
+        for (int it = 0; it < IPT; it++) {
+            uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
+            if (it < ipt && pos < U) {
+#pragma unroll
+                for (int r = 0; r < W; r++) key[it].w[r] = skeys[pos * W + r];
+                cnt[it] = scnt[pos];
+            }
+        }
+        __syncthreads();
         for (int shift = 0; shift < bits && U > 1; shift += 8) {
             const int nb = std::min(8, bits - shift);
-            for (int j = tid; j < 256 * BNW; j += BT) whist[j] = 0;
+            for (int j = tid; j < 256 * NW; j += NT) whist[j] = 0;
             __syncthreads();
             uint32_t rank[IPT];
 #pragma unroll
@@ -752,10 +786,10 @@ __global__ __launch_bounds__(BT) void k_bucket_hash_sort(uint64_t *__restrict__ 
                         peers &= ((d >> bb) & 1u) ? mb : ~mb;
                     }
                     uint32_t below = __popcll(peers & lanemask_lt());
-                    uint32_t basev = whist[d * BNW + wave];
+                    uint32_t basev = whist[d * NW + wave];
                     rank[it] = basev + below;
                     dig[it] = d;
-                    if (valid && below == 0) whist[d * BNW + wave] = basev + (uint32_t)__popcll(peers);
+                    if (valid && below == 0) whist[d * NW + wave] = basev + (uint32_t)__popcll(peers);
                 }
             }
             __syncthreads();
@@ -763,7 +797,7 @@ __global__ __launch_bounds__(BT) void k_bucket_hash_sort(uint64_t *__restrict__ 
                 uint32_t v0 = whist[tid * 4 + 0], v1 = whist[tid * 4 + 1];
                 uint32_t v2 = whist[tid * 4 + 2], v3 = whist[tid * 4 + 3];
                 uint32_t total;
-                uint32_t ex = block_excl_scan<BT, uint32_t>(v0 + v1 + v2 + v3, ws, &total);
+                uint32_t ex = block_excl_scan<NT, uint32_t>(v0 + v1 + v2 + v3, ws, &total);
                 whist[tid * 4 + 0] = ex;
                 whist[tid * 4 + 1] = ex + v0;
                 whist[tid * 4 + 2] = ex + v0 + v1;
@@ -774,7 +808,7 @@ __global__ __launch_bounds__(BT) void k_bucket_hash_sort(uint64_t *__restrict__ 
             for (int it = 0; it < IPT; it++) {
                 uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
                 if (it < ipt && pos < U) {
-                    uint32_t dst = whist[dig[it] * BNW + wave] + rank[it];
+                    uint32_t dst = whist[dig[it] * NW + wave] + rank[it];
 #pragma unroll
                     for (int r = 0; r < W; r++) skeys[dst * W + r] = key[it].w[r];
                     scnt[dst] = cnt[it];
@@ -793,7 +827,7 @@ __global__ __launch_bounds__(BT) void k_bucket_hash_sort(uint64_t *__restrict__ 
             __syncthreads();
         }
     }
-    for (uint32_t q2 = tid; q2 < U; q2 += BT) {
+    for (uint32_t q2 = tid; q2 < U; q2 += NT) {
         uint64_t *d = X + (start + q2) * W;
 #pragma unroll
         for (int r = 0; r < W; r++) d[r] = skeys[q2 * W + r];
@@ -832,6 +866,206 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
     if (i < n) p[i] = v;
 }
 
+// ---- bucket plan on the device ------------------------------------------------
+// The node list (Bucket + partition per node) starts as one node per
+// partition.  Every MSD level replaces the nodes above the largest LDS
+// capacity by their non-empty children (in child order, so the list stays
+// in key order within each partition); the host only reads back a few
+// counts per level.
+
+__global__ void k_init_nodes(const uint64_t *__restrict__ part_off, uint32_t P, Bucket *__restrict__ nodes,
+                             uint32_t *__restrict__ npart) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    nodes[p] = Bucket{part_off[p], part_off[p + 1] - part_off[p], 0u, 0u};
+    npart[p] = p;
+}
+
+// stats[0] = largest active size, stats[1] = keys in active nodes
+__global__ void k_level_flags(const Bucket *__restrict__ nodes, uint32_t nn, uint64_t cap, int kbits, uint64_t tkl,
+                              uint32_t *__restrict__ act, uint64_t *__restrict__ tiles,
+                              unsigned long long *__restrict__ stats) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    const Bucket b = nodes[i];
+    const bool a = b.size > cap && (int)b.prefix < kbits;
+    act[i] = a ? 1u : 0u;
+    tiles[i] = a ? (b.size + tkl - 1) / tkl : 0;
+    if (a) {
+        atomicMax(&stats[0], (unsigned long long)b.size);
+        atomicAdd(&stats[1], (unsigned long long)b.size);
+    }
+}
+
+__global__ void k_level_segs(const Bucket *__restrict__ nodes, uint32_t nn, const uint32_t *__restrict__ act,
+                             const uint64_t *__restrict__ act_rank, const uint64_t *__restrict__ tile_off, uint32_t nd,
+                             Seg *__restrict__ segs, uint64_t *__restrict__ tile_pre, uint32_t *__restrict__ seg_node) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) tile_pre[act_rank[nn]] = tile_off[nn];
+    if (i >= nn || !act[i]) return;
+    const uint64_t r = act_rank[i];
+    const Bucket b = nodes[i];
+    segs[r] = Seg{b.start, b.size, r * nd};
+    tile_pre[r] = tile_off[i];
+    seg_node[r] = i;
+}
+
+// One workgroup per split segment: child start offsets (exclusive scan of
+// the child counts from the segment start), each child's rank among the
+// non-empty children and their number.
+__global__ __launch_bounds__(256) void k_child_scan(const Seg *__restrict__ segs, const uint32_t *__restrict__ child_hist,
+                                                    uint32_t nd, uint64_t *__restrict__ child_cur,
+                                                    uint32_t *__restrict__ nz_rank, uint32_t *__restrict__ nz_count) {
+    __shared__ uint64_t ws64[256 / 64 + 1];
+    __shared__ uint32_t ws32[256 / 64 + 1];
+    const uint32_t r = blockIdx.x;
+    const uint64_t base = (uint64_t)r * nd;
+    const uint32_t per = (nd + 255) / 256;
+    const uint32_t g0 = threadIdx.x * per;
+    uint64_t csum = 0;
+    uint32_t zsum = 0;
+    for (uint32_t j = 0; j < per; j++) {
+        const uint32_t g = g0 + j;
+        if (g < nd) {
+            const uint32_t c = child_hist[base + g];
+            csum += c;
+            zsum += c ? 1u : 0u;
+        }
+    }
+    uint64_t ctot;
+    uint32_t ztot;
+    uint64_t cex = block_excl_scan<256, uint64_t>(csum, ws64, &ctot);
+    uint32_t zex = block_excl_scan<256, uint32_t>(zsum, ws32, &ztot);
+    const uint64_t start = segs[r].start;
+    for (uint32_t j = 0; j < per; j++) {
+        const uint32_t g = g0 + j;
+        if (g < nd) {
+            const uint32_t c = child_hist[base + g];
+            child_cur[base + g] = start + cex;
+            nz_rank[base + g] = zex;
+            cex += c;
+            zex += c ? 1u : 0u;
+        }
+    }
+    if (threadIdx.x == 0) nz_count[r] = ztot;
+}
+
+__global__ void k_level_counts(uint32_t nn, const uint32_t *__restrict__ act, const uint64_t *__restrict__ act_rank,
+                               const uint32_t *__restrict__ nz_count, uint32_t *__restrict__ out_cnt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nn) out_cnt[i] = act[i] ? nz_count[act_rank[i]] : 1u;
+}
+
+__global__ void k_level_emit_nodes(const Bucket *__restrict__ nodes, const uint32_t *__restrict__ npart, uint32_t nn,
+                                   const uint32_t *__restrict__ act, const uint64_t *__restrict__ new_base,
+                                   Bucket *__restrict__ out, uint32_t *__restrict__ out_part) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn || act[i]) return;
+    const uint64_t o = new_base[i];
+    out[o] = nodes[i];
+    out_part[o] = npart[i];
+}
+
+// Children of split node seg_node[r]: runs at the child start offsets (read
+// before the scatter pass advances them), one level deeper, in the other
+// key array.
+__global__ void k_level_emit_children(const uint32_t *__restrict__ seg_node, const uint64_t *__restrict__ new_base,
+                                      const Bucket *__restrict__ nodes, const uint32_t *__restrict__ npart,
+                                      const uint32_t *__restrict__ child_hist, const uint64_t *__restrict__ child_cur,
+                                      const uint32_t *__restrict__ nz_rank, uint64_t nchild, uint32_t nd, int d,
+                                      Bucket *__restrict__ out, uint32_t *__restrict__ out_part) {
+    const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= nchild) return;
+    const uint32_t c = child_hist[id];
+    if (!c) return;
+    const uint32_t i = seg_node[id / nd];
+    const Bucket b = nodes[i];
+    const uint64_t o = new_base[i] + nz_rank[id];
+    out[o] = Bucket{child_cur[id], c, b.prefix + (uint32_t)d, b.array ^ 1u};
+    out_part[o] = npart[i];
+}
+
+struct ClassCaps {
+    uint64_t cap[SORT_CLASSES];
+};
+
+// Size class of every bucket: the smallest LDS footprint that holds it.
+// Counts are aggregated per wave (one atomic per class per wave): the four
+// counters would otherwise take one contended atomic per bucket.
+__global__ __launch_bounds__(256) void k_classify(const Bucket *__restrict__ nodes, uint32_t nb, int kbits,
+                                                  ClassCaps caps, uint32_t *__restrict__ cls,
+                                                  uint32_t *__restrict__ cls_count) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = -1;
+    if (b < nb) {
+        const Bucket bk = nodes[b];
+        c = 0;
+        if ((int)bk.prefix < kbits && bk.size > 1)
+            while (c < SORT_CLASSES - 1 && bk.size > caps.cap[c]) c++;
+        cls[b] = (uint32_t)c;
+    }
+#pragma unroll
+    for (int q = 0; q < SORT_CLASSES; q++) {
+        const uint64_t m = __ballot(c == q);
+        if (m && __lane_id() == (unsigned)__ffsll((unsigned long long)m) - 1u)
+            atomicAdd(&cls_count[q], (uint32_t)__popcll(m));
+    }
+}
+
+// order[cls_off[c] ...]: the buckets of class c (any order inside a class),
+// one cursor atomic per class per wave.
+__global__ __launch_bounds__(256) void k_order_fill(const uint32_t *__restrict__ cls, uint32_t nb,
+                                                    const uint32_t *__restrict__ cls_off,
+                                                    uint32_t *__restrict__ cursor, uint32_t *__restrict__ order) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = b < nb ? (int)cls[b] : -1;
+    const uint32_t lane = __lane_id();
+#pragma unroll
+    for (int q = 0; q < SORT_CLASSES; q++) {
+        const uint64_t m = __ballot(c == q);
+        if (!m) continue;
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&cursor[q], (uint32_t)__popcll(m));
+        base = __shfl(base, (int)leader);
+        if (c == q) order[cls_off[q] + base + (uint32_t)__popcll(m & lanemask_lt())] = b;
+    }
+}
+
+// First node of every partition p in [0, P] (node partitions ascend).
+__global__ void k_part_first(const uint32_t *__restrict__ npart, uint32_t nb, uint32_t P, uint32_t *__restrict__ pfirst) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > P) return;
+    uint32_t lo = 0, hi = nb;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (npart[mid] < p) lo = mid + 1;
+        else hi = mid;
+    }
+    pfirst[p] = lo;
+}
+
+// Small results go to the host through page-locked memory written by a
+// kernel, not through the copy engines, so they never wait behind large
+// transfers that other streams keep in flight.
+template <class T>
+__global__ void k_put_host(const T *__restrict__ src, uint32_t n, T *__restrict__ dst) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+// Class offsets from the class counts; the counters become zeroed cursors.
+__global__ void k_class_offsets(uint32_t *__restrict__ cls_count, uint32_t *__restrict__ cls_off) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint32_t o = 0;
+    for (int c = 0; c < SORT_CLASSES; c++) {
+        cls_off[c] = o;
+        o += cls_count[c];
+        cls_count[c] = 0;
+    }
+    cls_off[SORT_CLASSES] = o;
+}
+
 // ---- host orchestration -----------------------------------------------------
 namespace {
 struct EventTimer {
@@ -856,13 +1090,6 @@ struct EventTimer {
         return t;
     }
 };
-
-template <class T>
-void upload(DevBuf &buf, const std::vector<T> &v, hipStream_t s) {
-    buf.ensure(v.size() * sizeof(T));
-    if (!v.empty())
-        KITE_HIP(hipMemcpyAsync(buf.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
-}
 
 int class_cap(int W, int c) { return BT * ((4 << c) / W); }
 int sort_cap(int W) { return class_cap(W, SORT_CLASSES - 1); }
@@ -918,14 +1145,33 @@ void CountEngine::release() {
     KITE_HIP(hipStreamSynchronize(stream_));
     for (DevBuf *b : {&nl_, &line0_, &scan_tmp_, &hist_, &off_, &part_off_d_, &a_, &b_, &counts_tmp_, &out_keys_[0],
                       &out_keys_[1], &out_counts_[0], &out_counts_[1], &segs_, &tile_pre_, &child_hist_, &child_cur_,
-                      &buckets_, &ucount_, &uoff_, &part_first_, &part_u_, &order_})
+                      &ucount_, &uoff_, &part_first_, &part_u_, &order_, &wg_hist_, &nodes_[0], &nodes_[1],
+                      &npart_[0], &npart_[1], &act_, &tiles_, &act_rank_, &tile_off_, &lstats_, &seg_node_,
+                      &nz_rank_, &nz_count_, &new_cnt_, &new_base_, &cls_, &cls_count_})
         b->release();
+}
+
+uint64_t *CountEngine::pinned_host(size_t bytes) {
+    if (bytes > pin_bytes_) {
+        if (pin_) {
+            KITE_HIP(hipStreamSynchronize(stream_));
+            KITE_HIP(hipHostFree(pin_));
+            pin_ = nullptr;
+        }
+        KITE_HIP(hipHostMalloc(&pin_, bytes, hipHostMallocDefault));
+        pin_bytes_ = bytes;
+    }
+    return static_cast<uint64_t *>(pin_);
+}
+
+CountEngine::~CountEngine() {
+    if (pin_) (void)hipHostFree(pin_);
 }
 
 uint64_t CountEngine::bytes_needed(uint64_t n, int words, uint64_t fastq, uint32_t partitions) {
     uint64_t kb = 8ull * (uint64_t)words;
     uint64_t n_sub = ceil_div(fastq, SUB);
-    uint64_t nwg = ceil_div(n_sub, SUBS);
+    uint64_t nwg = n_sub;  // one sub-tile per workgroup: the largest matrix
     uint64_t mat = (uint64_t)partitions * nwg;
     return n * (3 * kb + 8) + mat * 12 + n_sub * 12 + (n / 256 + 1) * 64 + (64ull << 20);
 }
@@ -944,6 +1190,9 @@ void CountEngine::run(const uint8_t *d_fq, uint64_t len, int k, int canonical, u
     if (const char *e = getenv("KITE_SPLIT_DIV")) tun.split_target_div = std::max(1, atoi(e));
     if (const char *e = getenv("KITE_SORT_DEDUP")) tun.dedup = atoi(e) != 0;
     if (const char *e = getenv("KITE_MAX_DBITS")) tun.max_digit_bits = std::min(MAX_DBITS, std::max(1, atoi(e)));
+    if (const char *e = getenv("KITE_HASH_CLASSES")) tun.hash_classes = std::min(3, std::max(0, atoi(e)));
+    if (const char *e = getenv("KITE_SPLIT_CHUNK")) tun.split_chunk = std::min(64, std::max(1, atoi(e)));
+    if (const char *e = getenv("KITE_EXTRACT_SUBS")) tun.extract_subs = std::min(64, std::max(1, atoi(e)));
     if (words_ == 1) run_w<1>(d_fq, len, k, canonical, pbits);
     else if (words_ == 2) run_w<2>(d_fq, len, k, canonical, pbits);
     else run_w<4>(d_fq, len, k, canonical, pbits);
@@ -954,7 +1203,8 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
     hipStream_t s = stream_;
     const uint32_t P = 1u << pbits;
     const int CAP = sort_cap(W);
-    const uint64_t TK = (uint64_t)split_tk(W);
+    const uint32_t spl = (uint32_t)std::max(1, tun.split_chunk);
+    const uint64_t TK = (uint64_t)split_tk(W) * spl;  // keys per split workgroup
     EventTimer tm(timing_, s);
     tm.mark();
     part_off_h_.assign(P + 1, 0);
@@ -964,8 +1214,17 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
     levels_ = 0;
 
     const uint64_t n_sub = std::max<uint64_t>(1, ceil_div(len, SUB));
-    const uint32_t nwg = (uint32_t)ceil_div(n_sub, SUBS);
+    const uint32_t subs = (uint32_t)std::max(1, tun.extract_subs);
+    const uint32_t nwg = (uint32_t)ceil_div(n_sub, subs);
     const uint64_t mat = (uint64_t)P * nwg;
+
+    // page-locked host scratch for small read-backs
+    uint64_t *pin = pinned_host(((uint64_t)P + 1) * 8 + 256);
+    auto to_host = [&](auto *dev, uint32_t n, auto *host) {
+        using T = std::remove_const_t<std::remove_pointer_t<decltype(dev)>>;
+        k_put_host<T><<<dim3((n + 255) / 256), dim3(256), 0, s>>>(dev, n, reinterpret_cast<T *>(host));
+        KITE_CHECK_LAUNCH();
+    };
 
     // 1. line index at each sub-tile start
     uint32_t *nl = static_cast<uint32_t *>(nl_.ensure(n_sub * 4));
@@ -987,6 +1246,7 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
     ea.sub_line0 = line0;
     ea.n_sub = n_sub;
     ea.nwg = nwg;
+    ea.subs = subs;
     ea.k = k;
     ea.canonical = canonical;
     ea.pbits = pbits;
@@ -1012,8 +1272,9 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
     uint64_t *part_off_d = static_cast<uint64_t *>(part_off_d_.ensure((P + 1) * 8));
     k_part_gather<<<dim3((P + 1 + 255) / 256), dim3(256), 0, s>>>(off, nwg, P, part_off_d);
     KITE_CHECK_LAUNCH();
-    KITE_HIP(hipMemcpyAsync(part_off_h_.data(), part_off_d, (P + 1) * 8, hipMemcpyDeviceToHost, s));
+    to_host(part_off_d, P + 1, pin);
     KITE_HIP(hipStreamSynchronize(s));
+    std::copy(pin, pin + P + 1, part_off_h_.begin());
     const uint64_t n = part_off_h_[P];
     n_kmers_ = n;
 
@@ -1029,186 +1290,182 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
     }
     tm.mark();
 
-    // 4. MSD splits until every bucket fits one LDS sort (or is exhausted)
-    std::vector<Bucket> nodes;
-    std::vector<uint32_t> node_part;
-    nodes.reserve(P);
-    for (uint32_t p = 0; p < P; p++) {
-        uint64_t sz = part_off_h_[p + 1] - part_off_h_[p];
-        if (sz == 0) continue;
-        nodes.push_back(Bucket{part_off_h_[p], sz, 0u, 0u});
-        node_part.push_back(p);
-    }
+    // 4. MSD splits until every bucket fits one LDS sort (or is exhausted);
+    // the node list is built on the device
     const int kbits = 2 * k;
+    auto grid1 = [](uint64_t n) { return dim3((unsigned)std::max<uint64_t>(1, ceil_div(n, 256))); };
+    uint32_t nn = P;
+    int cur_list = 0;
+    {
+        Bucket *nodes = static_cast<Bucket *>(nodes_[0].ensure((uint64_t)P * sizeof(Bucket)));
+        uint32_t *npart = static_cast<uint32_t *>(npart_[0].ensure((uint64_t)P * 4));
+        k_init_nodes<<<grid1(P), 256, 0, s>>>(part_off_d, P, nodes, npart);
+        KITE_CHECK_LAUNCH();
+    }
+    int prefix = 0;
+    uint32_t src_arr = 0;
+    uint64_t *hv = pin + P + 1;  // small read-backs (page-locked host memory)
     for (;;) {
-        std::vector<uint32_t> act;
-        uint64_t maxsz = 0;
-        for (uint32_t i = 0; i < nodes.size(); i++)
-            if (nodes[i].size > (uint64_t)CAP && (int)nodes[i].prefix < kbits) {
-                act.push_back(i);
-                maxsz = std::max(maxsz, nodes[i].size);
-            }
-        if (act.empty()) break;
-        const int prefix = (int)nodes[act[0]].prefix;  // uniform within a level
-        const uint32_t src_arr = nodes[act[0]].array;
+        Bucket *nodes = nodes_[cur_list].as<Bucket>();
+        uint32_t *npart = npart_[cur_list].as<uint32_t>();
+        uint32_t *act = static_cast<uint32_t *>(act_.ensure((uint64_t)nn * 4));
+        uint64_t *tiles = static_cast<uint64_t *>(tiles_.ensure((uint64_t)nn * 8));
+        uint64_t *act_rank = static_cast<uint64_t *>(act_rank_.ensure(((uint64_t)nn + 1) * 8));
+        uint64_t *tile_off = static_cast<uint64_t *>(tile_off_.ensure(((uint64_t)nn + 1) * 8));
+        unsigned long long *lstats = static_cast<unsigned long long *>(lstats_.ensure(16));
+        uint64_t *stmp = static_cast<uint64_t *>(scan_tmp_.ensure(
+            std::max<uint64_t>(scan_elems, scan_temp_elems((uint64_t)nn + 1)) * 8));
+        k_fill_u32<<<1, 64, 0, s>>>(reinterpret_cast<uint32_t *>(lstats), 4, 0);
+        KITE_CHECK_LAUNCH();
+        k_level_flags<<<grid1(nn), 256, 0, s>>>(nodes, nn, (uint64_t)CAP, kbits, TK, act, tiles, lstats);
+        KITE_CHECK_LAUNCH();
+        scan_to_u64(act, act_rank, nn, stmp, s);
+        scan_to_u64(tiles, tile_off, nn, stmp, s);
+        to_host(act_rank + nn, 1, hv + 0);
+        to_host(tile_off + nn, 1, hv + 1);
+        to_host(lstats, 2, reinterpret_cast<unsigned long long *>(hv + 2));
+        KITE_HIP(hipStreamSynchronize(s));
+        const uint64_t nact = hv[0], tiles_total = hv[1], maxsz = hv[2], split_keys = hv[3];
+        if (nact == 0) break;
         int d = 1;
-        while (d < tun.max_digit_bits &&
-               (maxsz >> d) > (uint64_t)(CAP / std::max(1, tun.split_target_div)))
-            d++;
+        while (d < tun.max_digit_bits && (maxsz >> d) > (uint64_t)(CAP / std::max(1, tun.split_target_div))) d++;
         d = std::min(d, kbits - prefix);
         const uint32_t nd = 1u << d;
-        std::vector<Seg> segs(act.size());
-        std::vector<uint64_t> tpre(act.size() + 1, 0);
-        for (size_t i = 0; i < act.size(); i++) {
-            const Bucket &nb = nodes[act[i]];
-            segs[i] = Seg{nb.start, nb.size, (uint64_t)i * nd};
-            tpre[i + 1] = tpre[i] + ceil_div(nb.size, TK);
-        }
-        const uint64_t tiles = tpre.back();
-        const uint64_t nchild = (uint64_t)act.size() * nd;
-        upload(segs_, segs, s);
-        upload(tile_pre_, tpre, s);
+        const uint64_t nchild = nact * nd;
+        Seg *segs = static_cast<Seg *>(segs_.ensure(nact * sizeof(Seg)));
+        uint64_t *tpre = static_cast<uint64_t *>(tile_pre_.ensure((nact + 1) * 8));
+        uint32_t *seg_node = static_cast<uint32_t *>(seg_node_.ensure(nact * 4));
+        k_level_segs<<<grid1(nn), 256, 0, s>>>(nodes, nn, act, act_rank, tile_off, nd, segs, tpre, seg_node);
+        KITE_CHECK_LAUNCH();
         uint32_t *ch = static_cast<uint32_t *>(child_hist_.ensure(nchild * 4));
-        k_fill_u32<<<dim3((unsigned)ceil_div(nchild, 256)), dim3(256), 0, s>>>(ch, nchild, 0);
+        k_fill_u32<<<grid1(nchild), 256, 0, s>>>(ch, nchild, 0);
         KITE_CHECK_LAUNCH();
         SplitArgs sa{};
         sa.src = src_arr ? B : A;
         sa.dst = src_arr ? A : B;
-        sa.segs = segs_.as<Seg>();
-        sa.nseg = (uint32_t)act.size();
-        sa.tile_pre = tile_pre_.as<uint64_t>();
+        sa.segs = segs;
+        sa.nseg = (uint32_t)nact;
+        sa.tile_pre = tpre;
         sa.shift = kbits - prefix - d;
         sa.dbits = d;
+        sa.chunk = spl;
         sa.child_hist = ch;
-        uint64_t split_keys = 0;
-        for (const Seg &sg : segs) split_keys += sg.size;
+        sa.wg_hist = static_cast<uint32_t *>(wg_hist_.ensure(tiles_total * nd * 4));
         launch("k_split_hist", split_keys * 8 * W,
-               [&] { k_split<W, false><<<dim3((unsigned)tiles), dim3(ST), 0, s>>>(sa); });
-        std::vector<uint32_t> chh(nchild);
-        KITE_HIP(hipMemcpyAsync(chh.data(), ch, nchild * 4, hipMemcpyDeviceToHost, s));
+               [&] { k_split<W, false><<<dim3((unsigned)tiles_total), dim3(ST), 0, s>>>(sa); });
+        uint64_t *ccur = static_cast<uint64_t *>(child_cur_.ensure(nchild * 8));
+        uint32_t *nzr = static_cast<uint32_t *>(nz_rank_.ensure(nchild * 4));
+        uint32_t *nzc = static_cast<uint32_t *>(nz_count_.ensure(nact * 4));
+        k_child_scan<<<dim3((unsigned)nact), 256, 0, s>>>(segs, ch, nd, ccur, nzr, nzc);
+        KITE_CHECK_LAUNCH();
+        uint32_t *ncnt = static_cast<uint32_t *>(new_cnt_.ensure((uint64_t)nn * 4));
+        uint64_t *nbase = static_cast<uint64_t *>(new_base_.ensure(((uint64_t)nn + 1) * 8));
+        k_level_counts<<<grid1(nn), 256, 0, s>>>(nn, act, act_rank, nzc, ncnt);
+        KITE_CHECK_LAUNCH();
+        scan_to_u64(ncnt, nbase, nn, stmp, s);
+        to_host(nbase + nn, 1, hv + 4);
         KITE_HIP(hipStreamSynchronize(s));
-        std::vector<uint64_t> cur(nchild);
-        for (size_t i = 0; i < act.size(); i++) {
-            uint64_t o = segs[i].start;
-            for (uint32_t g = 0; g < nd; g++) {
-                cur[i * nd + g] = o;
-                o += chh[i * nd + g];
-            }
-        }
-        upload(child_cur_, cur, s);
-        sa.child_cur = child_cur_.as<uint64_t>();
+        const uint64_t nn_next = hv[4];
+        if (nn_next > 0xFFFFFFFFull) throw Error("internal: bucket list above 2^32 entries");
+        const int nxt = cur_list ^ 1;
+        Bucket *out_nodes = static_cast<Bucket *>(nodes_[nxt].ensure(std::max<uint64_t>(nn_next, 1) * sizeof(Bucket)));
+        uint32_t *out_part = static_cast<uint32_t *>(npart_[nxt].ensure(std::max<uint64_t>(nn_next, 1) * 4));
+        k_level_emit_nodes<<<grid1(nn), 256, 0, s>>>(nodes, npart, nn, act, nbase, out_nodes, out_part);
+        KITE_CHECK_LAUNCH();
+        k_level_emit_children<<<grid1(nchild), 256, 0, s>>>(seg_node, nbase, nodes, npart, ch, ccur, nzr, nchild, nd, d,
+                                                            out_nodes, out_part);
+        KITE_CHECK_LAUNCH();
+        sa.child_cur = ccur;
         launch("k_split_scatter", split_keys * 16 * W,
-               [&] { k_split<W, true><<<dim3((unsigned)tiles), dim3(ST), 0, s>>>(sa); });
+               [&] { k_split<W, true><<<dim3((unsigned)tiles_total), dim3(ST), 0, s>>>(sa); });
         levels_++;
-        // expand the node list in order
-        std::vector<Bucket> nn;
-        std::vector<uint32_t> np;
-        nn.reserve(nodes.size() + nchild);
-        size_t ai = 0;
-        for (uint32_t i = 0; i < nodes.size(); i++) {
-            if (ai < act.size() && act[ai] == i) {
-                for (uint32_t g = 0; g < nd; g++) {
-                    uint64_t sz = chh[ai * nd + g];
-                    if (!sz) continue;
-                    nn.push_back(Bucket{cur[ai * nd + g], sz, (uint32_t)(prefix + d), src_arr ^ 1u});
-                    np.push_back(node_part[i]);
-                }
-                ai++;
-            } else {
-                nn.push_back(nodes[i]);
-                np.push_back(node_part[i]);
-            }
-        }
-        nodes.swap(nn);
-        node_part.swap(np);
-        KITE_HIP(hipStreamSynchronize(s));  // host vectors of this level go out of scope
-        // the next level only splits nodes produced by this one
-        for (auto &nd2 : nodes)
-            if (nd2.size > (uint64_t)CAP && (int)nd2.prefix < kbits && (int)nd2.prefix != prefix + d)
-                throw Error("internal: mixed split levels");
+        nn = (uint32_t)nn_next;
+        cur_list = nxt;
+        prefix += d;
+        src_arr ^= 1u;
     }
     tm.mark();
 
     // 5. sort + run-length count every bucket
-    const uint64_t nb = nodes.size();
+    const uint64_t nb = nn;
     n_buckets_ = nb;
-    std::vector<uint32_t> pfirst(P + 1, (uint32_t)nb);
-    {
-        uint32_t j = 0;
-        for (uint32_t p = 0; p <= P; p++) {
-            while (j < nb && node_part[j] < p) j++;
-            pfirst[p] = j;
-        }
-    }
+    Bucket *bkts = nodes_[cur_list].as<Bucket>();
+    uint32_t *npart_fin = npart_[cur_list].as<uint32_t>();
     uint64_t *uoff = static_cast<uint64_t *>(uoff_.ensure((nb + 1) * 8));
     uint32_t *ucount = static_cast<uint32_t *>(ucount_.ensure(std::max<uint64_t>(nb, 1) * 4));
+    uint32_t *pfirst = static_cast<uint32_t *>(part_first_.ensure((uint64_t)(P + 1) * 4));
+    k_part_first<<<grid1(P + 1), 256, 0, s>>>(npart_fin, (uint32_t)nb, P, pfirst);
+    KITE_CHECK_LAUNCH();
     if (nb > 0) {
-        upload(buckets_, nodes, s);
         // size classes: the smallest LDS footprint that holds each bucket
-        std::vector<uint32_t> order;
-        order.reserve(nb);
+        uint32_t *cls = static_cast<uint32_t *>(cls_.ensure(nb * 4));
+        uint32_t *cls_count = static_cast<uint32_t *>(cls_count_.ensure(4 * (SORT_CLASSES + 1) * 2));
+        k_fill_u32<<<1, 64, 0, s>>>(cls_count, 2 * (SORT_CLASSES + 1), 0);
+        KITE_CHECK_LAUNCH();
+        ClassCaps caps;
+        for (int c = 0; c < SORT_CLASSES; c++) caps.cap[c] = (uint64_t)class_cap(W, c);
+        k_classify<<<grid1(nb), 256, 0, s>>>(bkts, (uint32_t)nb, kbits, caps, cls, cls_count);
+        KITE_CHECK_LAUNCH();
+        uint32_t *hcls = reinterpret_cast<uint32_t *>(hv + 5);
+        to_host(cls_count, SORT_CLASSES, hcls);
+        uint32_t *dcls_off = cls_count + (SORT_CLASSES + 1);
+        k_class_offsets<<<1, 1, 0, s>>>(cls_count, dcls_off);
+        KITE_CHECK_LAUNCH();
+        KITE_HIP(hipStreamSynchronize(s));
         uint32_t cls_off[SORT_CLASSES + 1] = {0};
-        for (int c = 0; c < SORT_CLASSES; c++) {
-            cls_off[c] = (uint32_t)order.size();
-            for (uint32_t b = 0; b < nb; b++) {
-                const Bucket &bk = nodes[b];
-                int cb = 0;
-                if ((int)bk.prefix < kbits && bk.size > 1)
-                    while (cb < SORT_CLASSES - 1 && bk.size > (uint64_t)class_cap(W, cb)) cb++;
-                if (cb == c) order.push_back(b);
-            }
-        }
-        cls_off[SORT_CLASSES] = (uint32_t)order.size();
-        upload(order_, order, s);
+        for (int c = 0; c < SORT_CLASSES; c++) cls_off[c + 1] = cls_off[c] + hcls[c];
+        uint32_t *ordp = static_cast<uint32_t *>(order_.ensure(nb * 4));
+        k_order_fill<<<grid1(nb), 256, 0, s>>>(cls, (uint32_t)nb, dcls_off, cls_count, ordp);
+        KITE_CHECK_LAUNCH();
         const uint32_t *ord = order_.as<uint32_t>();
         auto go = [&](auto cls, uint32_t first, uint32_t cnt) {
             constexpr int C = decltype(cls)::value;
             if (!cnt) return;
-            // the hash kernel's table doubles the LDS footprint: above class 1
-            // it would run one workgroup per CU, where the LSD kernel is faster
-            if constexpr (C < 2) {
-                if (tun.dedup) {
+            // hash kernel: 4 / W items per thread, so a bigger class gets more
+            // threads (up to 1024) rather than more items per thread
+            if constexpr (C < 3) {
+                if (tun.dedup && C < tun.hash_classes) {
+                    constexpr int IPTH = W == 1 ? 4 : (W == 2 ? 2 : 1);
+                    constexpr int NTH = SortClass<W, C>::CAP / IPTH;
                     launch("k_bucket_sort", 0, [&] {
-                        k_bucket_hash_sort<W, SortClass<W, C>::IPT><<<dim3(cnt), dim3(BT), 0, s>>>(
-                            A, B, ctmp, buckets_.as<Bucket>(), ord + first, ucount, k);
+                        k_bucket_hash_sort<W, IPTH, NTH><<<dim3(cnt), dim3(NTH), 0, s>>>(A, B, ctmp, bkts, ord + first,
+                                                                                        ucount, k);
                     });
                     return;
                 }
             }
             launch("k_bucket_sort", 0, [&] {
                 k_bucket_sort<W, SortClass<W, C>::IPT><<<dim3(cnt), dim3(BT), 0, s>>>(
-                    A, B, ctmp, buckets_.as<Bucket>(), ord + first, ucount, k);
+                    A, B, ctmp, bkts, ord + first, ucount, k);
             });
         };
         go(std::integral_constant<int, 0>{}, cls_off[0], cls_off[1] - cls_off[0]);
         go(std::integral_constant<int, 1>{}, cls_off[1], cls_off[2] - cls_off[1]);
         go(std::integral_constant<int, 2>{}, cls_off[2], cls_off[3] - cls_off[2]);
         go(std::integral_constant<int, 3>{}, cls_off[3], cls_off[4] - cls_off[3]);
-        KITE_HIP(hipStreamSynchronize(s));  // `order` is a local vector
     }
     tm.mark();
     uint64_t *stmp2 = static_cast<uint64_t *>(scan_tmp_.ensure(
         std::max<uint64_t>(scan_elems, scan_temp_elems(nb + 1)) * 8));
     scan_to_u64(ucount, uoff, nb, stmp2, s);
-    KITE_HIP(hipMemcpyAsync(&n_unique_, uoff + nb, 8, hipMemcpyDeviceToHost, s));
+    to_host(uoff + nb, 1, hv + 8);
     KITE_HIP(hipStreamSynchronize(s));
+    n_unique_ = hv[8];
     uint64_t *ok = static_cast<uint64_t *>(out_keys_[slot_].ensure(std::max<uint64_t>(n_unique_, 1) * 8 * W));
     uint32_t *oc = static_cast<uint32_t *>(out_counts_[slot_].ensure(std::max<uint64_t>(n_unique_, 1) * 4));
     if (nb > 0) {
         launch("k_compact", n_unique_ * (8 * W + 4) * 2, [&] {
-            k_compact<W><<<dim3((unsigned)nb), dim3(256), 0, s>>>(A, B, ctmp, buckets_.as<Bucket>(),
-                                                               uoff, ok, oc);
+            k_compact<W><<<dim3((unsigned)nb), dim3(256), 0, s>>>(A, B, ctmp, bkts, uoff, ok, oc);
         });
     }
     // per-partition output offsets
-    upload(part_first_, pfirst, s);
     uint64_t *pu = static_cast<uint64_t *>(part_u_.ensure((P + 1) * 8));
-    k_gather_u64<<<dim3((P + 1 + 255) / 256), dim3(256), 0, s>>>(uoff, part_first_.as<uint32_t>(),
-                                                               P + 1, pu);
+    k_gather_u64<<<dim3((P + 1 + 255) / 256), dim3(256), 0, s>>>(uoff, pfirst, P + 1, pu);
     KITE_CHECK_LAUNCH();
-    KITE_HIP(hipMemcpyAsync(part_off_h_.data(), pu, (P + 1) * 8, hipMemcpyDeviceToHost, s));
+    to_host(pu, P + 1, pin);
     tm.mark();
     KITE_HIP(hipStreamSynchronize(s));
+    std::copy(pin, pin + P + 1, part_off_h_.begin());
     if (kprof_) {
         collect_kernel_stats();
         for (auto &ks : kstats_)  // sort reads every key once, writes the unique runs

@@ -9,8 +9,10 @@
 //                   histogram per (partition, workgroup)
 //   scan            partition-major destination of every workgroup's keys
 //   k_extract<SCAT> same k-mers, scattered into partition order
-//   k_seg<HIST/SCAT> MSD split of large partitions on the top key bits
-//   k_bucket_sort   per-bucket LSD radix sort in LDS + run-length counts
+//   k_split<HIST/SCAT> MSD split of large partitions on the top key bits
+//                   (the bucket list itself is built on the device)
+//   k_bucket_hash_sort / k_bucket_sort
+//                   per-bucket sort in LDS + run-length counts
 //   scan + k_compact  contiguous per-partition (key, count) output
 #pragma once
 
@@ -46,11 +48,17 @@ struct CountTunables {
     int split_target_div = 4;  // split partitions until max child <= largest cap / div
     int max_digit_bits = 12;
     bool dedup = true;  // small buckets: hash dedup + one bin pass instead of LSD passes
+    int hash_classes = 3;  // size classes [0, hash_classes) use the hash kernel
+    int split_chunk = 1;   // split tiles per workgroup (longer runs per child, fewer atomics)
+    int extract_subs = 4;  // 8 KiB sub-tiles per extract workgroup
 };
 
 class CountEngine {
    public:
     explicit CountEngine(hipStream_t s) : stream_(s) {}
+    ~CountEngine();
+    CountEngine(const CountEngine &) = delete;
+    CountEngine &operator=(const CountEngine &) = delete;
 
     // Count the FASTQ bytes d_fq[0, len).  Results stay on the device until
     // the next run: keys() (n_unique * words u64), counts() (u32), and
@@ -103,8 +111,14 @@ class CountEngine {
     DevBuf nl_, line0_, scan_tmp_, hist_, off_, part_off_d_;
     DevBuf a_, b_, counts_tmp_, out_keys_[2], out_counts_[2];
     int slot_ = 0;
-    DevBuf segs_, tile_pre_, child_hist_, child_cur_, buckets_, ucount_, uoff_, part_first_, part_u_;
-    DevBuf order_;
+    DevBuf segs_, tile_pre_, child_hist_, child_cur_, ucount_, uoff_, part_first_, part_u_;
+    DevBuf order_, wg_hist_;
+    // device bucket plan: node lists (ping-pong), per-level scratch, classes
+    DevBuf nodes_[2], npart_[2], act_, tiles_, act_rank_, tile_off_, lstats_, seg_node_;
+    DevBuf nz_rank_, nz_count_, new_cnt_, new_base_, cls_, cls_count_;
+    void *pin_ = nullptr;  // page-locked scratch for small read-backs
+    size_t pin_bytes_ = 0;
+    uint64_t *pinned_host(size_t bytes);
     std::vector<uint64_t> part_off_h_;
     uint64_t n_kmers_ = 0, n_unique_ = 0, n_buckets_ = 0;
     int words_ = 1, levels_ = 0;

@@ -196,3 +196,27 @@ def test_low_complexity_keys(gpu_ctx, monkeypatch, dedup):
     for k, canon, P in ((31, False, 8), (31, True, 4), (45, False, 8), (21, False, 1)):
         res, tab, _ = run_both(gpu_ctx, fq, k, canon, P)
         check_same(res, tab, f"low complexity k={k} canon={canon} P={P} dedup={dedup}")
+
+
+@pytest.mark.parametrize("chunk,subs", [("3", "3"), ("1", "7"), ("8", "16")])
+def test_workgroup_chunking_tunables(gpu_ctx, monkeypatch, chunk, subs):
+    """Split chunks of several tiles and extract workgroups of several
+    sub-tiles (including counts that do not divide the work) give the
+    oracle's table."""
+    monkeypatch.setenv("KITE_SPLIT_CHUNK", chunk)
+    monkeypatch.setenv("KITE_EXTRACT_SUBS", subs)
+    from kite_amd.workloads import K31_CANON_50MBP, K63_CANON_100MBP
+    for w, n, P in ((K31_CANON_50MBP, 200000, 16), (K63_CANON_100MBP, 30000, 64)):
+        fq = w.fastq(n_reads=n)
+        res, tab, _ = run_both(gpu_ctx, fq, w.k, w.canonical, P)
+        check_same(res, tab, f"{w.name} chunk={chunk} subs={subs}")
+
+
+def test_hash_kernel_on_class_two(gpu_ctx, monkeypatch):
+    """The hash kernel with 1024 threads on the 4096-key class."""
+    monkeypatch.setenv("KITE_HASH_CLASSES", "3")
+    from kite_amd.workloads import K31_CANON_50MBP, K63_CANON_100MBP
+    for w, n, P in ((K31_CANON_50MBP, 300000, 8), (K63_CANON_100MBP, 40000, 4)):
+        fq = w.fastq(n_reads=n)
+        res, tab, _ = run_both(gpu_ctx, fq, w.k, w.canonical, P)
+        check_same(res, tab, f"{w.name} hash classes 3")
