@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define KITE_GPU_ABI_VERSION 1
+#define KITE_GPU_ABI_VERSION 2
 
 enum {
     KITE_OK = 0,
@@ -191,10 +191,15 @@ int kite_gpu_union_tables(kite_gpu_ctx *ctx, uint32_t words, uint32_t partitions
 #define KITE_CORR_LOLO 0x2u          /* flags: add the lo*lo product */
 
 typedef struct kite_corr_stats {
-    double gram_ms_pearson, gram_ms_spearman; /* split-bf16 Gram + FP64 reduction */
-    double fp64_ms_pearson, fp64_ms_spearman; /* FP64 MFMA Gram + reduction */
+    double gram_ms_pearson, gram_ms_spearman; /* split-bf16 Gram + FP64 reduction (share of the pass) */
+    double fp64_ms_pearson, fp64_ms_spearman; /* FP64 MFMA Gram + reduction (share of the pass) */
     uint64_t rows;
-    uint32_t cols, code_bytes;                /* bytes per matrix cell read by the Gram */
+    uint32_t cols, code_bytes;                /* bytes per matrix cell read by the Gram: 1 = exact
+                                                 8-bit rank of the count within its column */
+    double gram_ms_total, fp64_ms_total;      /* all requested methods: the split-bf16 Gram computes
+                                                 them in one pass over the matrix, the FP64 Gram in
+                                                 one pass per method (its faster form) */
+    uint32_t passes, fp64_passes;             /* passes over the matrix of each Gram */
 } kite_corr_stats;
 
 /* counts: N x n row-major u32 (host).  pearson / spearman / cov: n x n

@@ -43,6 +43,8 @@ class CorrStats(C.Structure):
         ("gram_ms_pearson", C.c_double), ("gram_ms_spearman", C.c_double),
         ("fp64_ms_pearson", C.c_double), ("fp64_ms_spearman", C.c_double),
         ("rows", C.c_uint64), ("cols", C.c_uint32), ("code_bytes", C.c_uint32),
+        ("gram_ms_total", C.c_double), ("fp64_ms_total", C.c_double), ("passes", C.c_uint32),
+        ("fp64_passes", C.c_uint32),
     ]
 
     def as_dict(self) -> dict:

@@ -14,8 +14,8 @@
 
 #include "../../include/kite_gpu.h"
 #include "count.hpp"
-#include "dvle.hpp"
 #include "corr.hpp"
+#include "encode.hpp"
 #include "merge.hpp"
 
 namespace {
@@ -43,12 +43,6 @@ int guarded(F &&f) {
         return set_error(KITE_E_DEVICE, e.what());
     }
 }
-
-struct HostRun {
-    std::vector<uint64_t> keys;
-    std::vector<uint32_t> counts;
-    std::vector<uint64_t> part_off;
-};
 
 // Page-locked host buffer that grows on demand.
 struct PinnedBuf {
@@ -109,7 +103,8 @@ struct kite_gpu_ctx {
     const uint8_t *direct = nullptr;
     uint64_t direct_len = 0;
     bool device_result = false;
-    std::vector<HostRun> host_runs;
+    std::vector<kite::EncodedRuns> host_runs;  // earlier batches, encoded on the device
+    kite::EncodeBufs enc_bufs;
     kite_count_stats stats{};
 
     // batched count: copy streams, double-buffered input and result slots
@@ -137,17 +132,16 @@ void staging_reserve(kite_gpu_ctx *c, uint64_t need) {
     c->staging.bytes = want;
 }
 
+// Encodes the device result into chunk payloads (only those cross PCIe).
+void encode_device_result(kite_gpu_ctx *c, kite::EncodedRuns &out) {
+    kite::encode_runs_device(c->count->keys(), c->count->counts(), c->count->d_part_off(), c->count->part_off(),
+                             c->partitions, c->words, c->stream, c->enc_bufs, out);
+}
+
 void stash_device_result(kite_gpu_ctx *c) {
     if (!c->device_result) return;
-    HostRun r;
-    const uint64_t u = c->count->n_unique();
-    r.keys.resize(u * c->words);
-    r.counts.resize(u);
-    r.part_off = c->count->part_off();
-    if (u) {
-        KITE_HIP(hipMemcpy(r.keys.data(), c->count->keys(), u * 8 * c->words, hipMemcpyDeviceToHost));
-        KITE_HIP(hipMemcpy(r.counts.data(), c->count->counts(), u * 4, hipMemcpyDeviceToHost));
-    }
+    kite::EncodedRuns r;
+    encode_device_result(c, r);
     c->host_runs.push_back(std::move(r));
     c->device_result = false;
 }
@@ -379,30 +373,19 @@ int kite_gpu_count_emit(kite_gpu_ctx *ctx, kite_append_chunk_fn fn, void *user) 
     if (!fn) return set_error(KITE_E_INVALID, "invalid argument: callback is NULL");
     return guarded([&]() -> int {
         bind(ctx);
-        const int W = ctx->words;
         const uint32_t P = ctx->partitions;
-        HostRun last;
-        if (ctx->device_result) {
-            const uint64_t u = ctx->count->n_unique();
-            last.keys.resize(u * W);
-            last.counts.resize(u);
-            last.part_off = ctx->count->part_off();
-            if (u) {
-                KITE_HIP(hipMemcpy(last.keys.data(), ctx->count->keys(), u * 8 * W, hipMemcpyDeviceToHost));
-                KITE_HIP(hipMemcpy(last.counts.data(), ctx->count->counts(), u * 4, hipMemcpyDeviceToHost));
-            }
-        }
-        std::vector<const HostRun *> runs;
-        for (const HostRun &r : ctx->host_runs) runs.push_back(&r);
+        kite::EncodedRuns last;
+        if (ctx->device_result) encode_device_result(ctx, last);
+        std::vector<const kite::EncodedRuns *> runs;
+        for (const kite::EncodedRuns &r : ctx->host_runs) runs.push_back(&r);
         if (ctx->device_result) runs.push_back(&last);
-        for (const HostRun *r : runs) {
-            std::vector<kite::EncodedChunk> cs =
-                kite::encode_partitions(r->keys.data(), r->counts.data(), r->part_off.data(), P, W);
+        for (const kite::EncodedRuns *r : runs) {
             for (uint32_t p = 0; p < P; p++) {
-                const kite::EncodedChunk &ch = cs[p];
-                if (!ch.rows) continue;
-                int rc = fn(user, p, ch.codec, ch.rows, ch.bytes.size(), ch.key_min, ch.key_max,
-                            ch.bytes.data(), ch.bytes.size());
+                const uint64_t rows = r->rows_off[p + 1] - r->rows_off[p];
+                if (!rows) continue;
+                const uint64_t b0 = r->byte_off[p], len = r->byte_off[p + 1] - b0;
+                int rc = fn(user, p, r->codec, rows, len, &r->minmax[(size_t)p * 8], &r->minmax[(size_t)p * 8 + 4],
+                            r->bytes.data() + b0, len);
                 if (rc)
                     return set_error(KITE_E_CALLBACK,
                                      "append callback failed for partition " + std::to_string(p));
@@ -485,10 +468,13 @@ int corr_device(kite_gpu_ctx *ctx, const uint32_t *d_counts, uint64_t N, uint32_
     const uint32_t maxc = ctx->corr->max_count(d_counts, N * n);
     if (maxc > (64u << 20))
         return set_error(KITE_E_INVALID, "counts above 2^26 are not supported by the correlation engine");
-    const int cb = maxc < 65535 ? 2 : 4;
+    int cb = maxc < 65535 ? 2 : 4;
     kite::DevBuf codes;
     const void *dc = d_counts;
-    if (cb == 2) {
+    if (ctx->corr->dense_codes(d_counts, N, (int)n, maxc, codes)) {
+        cb = 1;  // every column has at most 256 distinct counts
+        dc = codes.ptr;
+    } else if (cb == 2) {
         ctx->corr->to_codes(d_counts, N * n, codes, 2);
         dc = codes.ptr;
     }
@@ -511,6 +497,10 @@ int corr_device(kite_gpu_ctx *ctx, const uint32_t *d_counts, uint64_t N, uint32_
         stats->rows = N;
         stats->cols = n;
         stats->code_bytes = (uint32_t)cb;
+        stats->gram_ms_total = out.gram_ms_total;
+        stats->fp64_ms_total = out.fp64_ms_total;
+        stats->passes = (uint32_t)out.passes;
+        stats->fp64_passes = (uint32_t)out.fp64_passes;
     }
     return KITE_OK;
 }

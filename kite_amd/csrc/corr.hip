@@ -72,7 +72,6 @@ constexpr int LSM64 = 128;  // the same for the FP64 table (8-byte entries)
 // that read different columns at the same (mostly small) count then fall
 // into different LDS banks.
 constexpr int LSP = LSM + 1;
-constexpr int LSP64 = LSM64 + 1;
 
 template <class T>
 struct CodeBlock {
@@ -92,8 +91,13 @@ __device__ __forceinline__ void fetch_codes(const T *__restrict__ codes, uint64_
     for (int j = 0; j < CodeBlock<T>::MAXW; j++) {
         const uint32_t idx = (uint32_t)j * 64u + (uint32_t)lane;
         uint32_t v = 0;
-        if (idx < nfull) v = src[idx];
-        else if (idx == nfull && (nbytes & 3u)) v = (uint32_t)codes[e0 + cnt - 1];  // odd trailing u16
+        if (idx < nfull) {
+            v = src[idx];
+        } else if (idx == nfull && (nbytes & 3u)) {  // trailing partial word: element by element
+            const uint32_t first = nfull * 4u / (uint32_t)sizeof(T);
+            for (uint32_t q = first; q < cnt; q++)
+                v |= (uint32_t)codes[e0 + q] << (8u * (uint32_t)sizeof(T) * (q - first));
+        }
         w[j] = v;
     }
 }
@@ -107,23 +111,27 @@ __device__ __forceinline__ void stage_codes(uint32_t *__restrict__ sw, const uin
 
 template <class T>
 __device__ __forceinline__ uint32_t staged_code(const uint32_t *__restrict__ sw, uint32_t e) {
-    if constexpr (sizeof(T) == 2) return reinterpret_cast<const uint16_t *>(sw)[e];
+    if constexpr (sizeof(T) == 1) return reinterpret_cast<const uint8_t *>(sw)[e];
+    else if constexpr (sizeof(T) == 2) return reinterpret_cast<const uint16_t *>(sw)[e];
     else return sw[e];
 }
 
-// lut: per column `stride` entries of packed (hi | lo << 16) bf16 bits.
-// partial: [wave][16][64] FP64 accumulators in MFMA output layout.
-template <class T>
+// lut: NM tables (one per method) of `stride` entries per column, each entry
+// packed (hi | lo << 16) bf16 bits; table m starts at m * n * stride.
+// partial: [wave][NM][16][64] FP64 accumulators in MFMA output layout.
+template <class T, int NM>
 __global__ __launch_bounds__(GT) void k_gram_bf16(const T *__restrict__ codes, uint64_t N, int n,
                                                    const uint32_t *__restrict__ lut, uint32_t stride,
                                                    uint64_t rows_per_wave, int with_lolo,
                                                    double *__restrict__ partial) {
-    __shared__ uint32_t slut[MAX_COLS * LSP];
+    __shared__ uint32_t slut[NM][MAX_COLS * LSP];
     __shared__ uint32_t sblk[GT / 64][CodeBlock<T>::WORDS];
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
-    for (int i = threadIdx.x; i < n * LSM; i += GT)
-        slut[(i / LSM) * LSP + (i % LSM)] = lut[(uint64_t)(i / LSM) * stride + (i % LSM)];
+    for (int m = 0; m < NM; m++)
+        for (int i = threadIdx.x; i < n * LSM; i += GT)
+            slut[m][(i / LSM) * LSP + (i % LSM)] =
+                lut[(uint64_t)m * n * stride + (uint64_t)(i / LSM) * stride + (i % LSM)];
     __syncthreads();
     const uint64_t wave = (uint64_t)blockIdx.x * (GT / 64) + wv;
     const uint64_t r0 = std::min<uint64_t>(N, wave * rows_per_wave);
@@ -131,17 +139,54 @@ __global__ __launch_bounds__(GT) void k_gram_bf16(const T *__restrict__ codes, u
     const int col = lane & 31;
     const int kb = 8 * (lane >> 5);
     const bool colok = col < n;
-    const uint32_t *lcol = lut + (uint64_t)(colok ? col : 0) * stride;
-    const uint32_t *scol = slut + (colok ? col : 0) * LSP;
+    const int ccol = colok ? col : 0;
     uint32_t *sw = sblk[wv];
-    floatx16 acc;
-    double facc[16];
+    floatx16 acc[NM];
+    double facc[NM][16];
 #pragma unroll
-    for (int i = 0; i < 16; i++) {
-        acc[i] = 0.f;
-        facc[i] = 0.0;
-    }
+    for (int m = 0; m < NM; m++)
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            acc[m][i] = 0.f;
+            facc[m][i] = 0.0;
+        }
     int steps = 0;
+    // one 16-row step: codes of rows s0 + kb + t, their table values, the MFMAs
+    auto step = [&](uint32_t s0, uint32_t rows) {
+        uint32_t c[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            const uint32_t rl = s0 + (uint32_t)kb + (uint32_t)t;
+            c[t] = (colok && rl < rows) ? staged_code<T>(sw, rl * (uint32_t)n + (uint32_t)col) : 0xFFFFFFFFu;
+        }
+#pragma unroll
+        for (int m = 0; m < NM; m++) {
+            bf16x8 hi, lo;
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+                uint32_t packed = 0;
+                if (c[t] != 0xFFFFFFFFu)
+                    packed = c[t] < (uint32_t)LSM ? slut[m][ccol * LSP + c[t]]
+                                                  : lut[(uint64_t)m * n * stride + (uint64_t)ccol * stride + c[t]];
+                hi[t] = bf16_from_bits((uint16_t)(packed & 0xFFFF));
+                lo[t] = bf16_from_bits((uint16_t)(packed >> 16));
+            }
+            acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi, hi, acc[m], 0, 0, 0);
+            acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi, lo, acc[m], 0, 0, 0);
+            acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lo, hi, acc[m], 0, 0, 0);
+            if (with_lolo) acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lo, lo, acc[m], 0, 0, 0);
+        }
+        if (++steps == FLUSH_STEPS) {
+            steps = 0;
+#pragma unroll
+            for (int m = 0; m < NM; m++)
+#pragma unroll
+                for (int i = 0; i < 16; i++) {
+                    facc[m][i] += (double)acc[m][i];
+                    acc[m][i] = 0.f;
+                }
+        }
+    };
     uint32_t w[CodeBlock<T>::MAXW];
     if (r0 < r1) fetch_codes<T>(codes, r0 * (uint64_t)n, (uint32_t)(std::min<uint64_t>(RB, r1 - r0) * n), w, lane);
     for (uint64_t rb = r0; rb < r1; rb += RB) {
@@ -151,52 +196,41 @@ __global__ __launch_bounds__(GT) void k_gram_bf16(const T *__restrict__ codes, u
         if (rb + RB < r1)
             fetch_codes<T>(codes, (rb + RB) * (uint64_t)n, (uint32_t)(std::min<uint64_t>(RB, r1 - rb - RB) * n), w,
                            lane);
-        for (uint32_t s0 = 0; s0 < rows; s0 += 16) {
-            bf16x8 hi, lo;
+        if (rows == RB) {
 #pragma unroll
-            for (int t = 0; t < 8; t++) {
-                const uint32_t rl = s0 + (uint32_t)kb + (uint32_t)t;
-                uint32_t packed = 0;
-                if (colok && rl < rows) {
-                    const uint32_t c = staged_code<T>(sw, rl * (uint32_t)n + (uint32_t)col);
-                    packed = c < (uint32_t)LSM ? scol[c] : lcol[c];
-                }
-                hi[t] = bf16_from_bits((uint16_t)(packed & 0xFFFF));
-                lo[t] = bf16_from_bits((uint16_t)(packed >> 16));
-            }
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi, hi, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi, lo, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lo, hi, acc, 0, 0, 0);
-            if (with_lolo) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lo, lo, acc, 0, 0, 0);
-            if (++steps == FLUSH_STEPS) {
-                steps = 0;
-#pragma unroll
-                for (int i = 0; i < 16; i++) {
-                    facc[i] += (double)acc[i];
-                    acc[i] = 0.f;
-                }
-            }
+            for (int q = 0; q < RB / 16; q++) step((uint32_t)q * 16u, RB);
+        } else {
+            for (uint32_t s0 = 0; s0 < rows; s0 += 16) step(s0, rows);
         }
         __builtin_amdgcn_wave_barrier();
     }
 #pragma unroll
-    for (int i = 0; i < 16; i++) facc[i] += (double)acc[i];
+    for (int m = 0; m < NM; m++)
 #pragma unroll
-    for (int i = 0; i < 16; i++) partial[(wave * 16 + i) * 64 + lane] = facc[i];
+        for (int i = 0; i < 16; i++) facc[m][i] += (double)acc[m][i];
+#pragma unroll
+    for (int m = 0; m < NM; m++)
+#pragma unroll
+        for (int i = 0; i < 16; i++) partial[((wave * NM + m) * 16 + i) * 64 + lane] = facc[m][i];
 }
 
 // FP64 baseline: v_mfma_f64_16x16x4 on the same codes and the same staging,
-// FP64 values.  partial: [wave][3 blocks (00, 01, 11)][4][64].
-template <class T>
+// FP64 values, NM tables per pass.  partial: [wave][NM][3 blocks (00, 01,
+// 11)][4][64].
+template <class T, int NM>
 __global__ __launch_bounds__(GT) void k_gram_f64(const T *__restrict__ codes, uint64_t N, int n,
                                                   const double *__restrict__ lut, uint32_t stride,
                                                   uint64_t rows_per_wave, double *__restrict__ partial) {
-    __shared__ double slut[MAX_COLS * LSP64];
+    constexpr int L64 = sizeof(T) == 1 && NM == 1 ? 256 : LSM64;  // 8-bit codes: the whole table
+    constexpr int P64 = L64 + 1;
+    __shared__ double slut[NM][MAX_COLS * P64];
     __shared__ uint32_t sblk[GT / 64][CodeBlock<T>::WORDS];
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
-    for (int i = threadIdx.x; i < n * LSM64; i += GT)
-        slut[(i / LSM64) * LSP64 + (i % LSM64)] = lut[(uint64_t)(i / LSM64) * stride + (i % LSM64)];
+    for (int m = 0; m < NM; m++)
+        for (int i = threadIdx.x; i < n * L64; i += GT)
+            slut[m][(i / L64) * P64 + (i % L64)] =
+                lut[(uint64_t)m * n * stride + (uint64_t)(i / L64) * stride + (i % L64)];
     __syncthreads();
     const uint64_t wave = (uint64_t)blockIdx.x * (GT / 64) + wv;
     const uint64_t r0 = std::min<uint64_t>(N, wave * rows_per_wave);
@@ -204,10 +238,18 @@ __global__ __launch_bounds__(GT) void k_gram_f64(const T *__restrict__ codes, ui
     const int c0 = lane & 15, c1 = 16 + (lane & 15);
     const int kk = lane >> 4;
     const bool ok0 = c0 < n, ok1 = c1 < n;
-    const double *l0 = lut + (uint64_t)(ok0 ? c0 : 0) * stride, *l1 = lut + (uint64_t)(ok1 ? c1 : 0) * stride;
-    const double *s0l = slut + (ok0 ? c0 : 0) * LSP64, *s1l = slut + (ok1 ? c1 : 0) * LSP64;
+    const int k0 = ok0 ? c0 : 0, k1 = ok1 ? c1 : 0;
     uint32_t *sw = sblk[wv];
-    doublex4 d00 = {0, 0, 0, 0}, d01 = {0, 0, 0, 0}, d11 = {0, 0, 0, 0};
+    doublex4 d00[NM], d01[NM], d11[NM];
+#pragma unroll
+    for (int m = 0; m < NM; m++) {
+        d00[m] = doublex4{0, 0, 0, 0};
+        d01[m] = doublex4{0, 0, 0, 0};
+        d11[m] = doublex4{0, 0, 0, 0};
+    }
+    auto value = [&](int m, int cc, uint32_t c) -> double {
+        return c < (uint32_t)L64 ? slut[m][cc * P64 + c] : lut[(uint64_t)m * n * stride + (uint64_t)cc * stride + c];
+    };
     uint32_t w[CodeBlock<T>::MAXW];
     if (r0 < r1) fetch_codes<T>(codes, r0 * (uint64_t)n, (uint32_t)(std::min<uint64_t>(RB, r1 - r0) * n), w, lane);
     for (uint64_t rb = r0; rb < r1; rb += RB) {
@@ -219,29 +261,28 @@ __global__ __launch_bounds__(GT) void k_gram_f64(const T *__restrict__ codes, ui
                            lane);
         for (uint32_t s4 = 0; s4 < rows; s4 += 4) {
             const uint32_t rl = s4 + (uint32_t)kk;
-            double a0 = 0, a1 = 0;
-            if (rl < rows) {
-                if (ok0) {
-                    const uint32_t c = staged_code<T>(sw, rl * (uint32_t)n + (uint32_t)c0);
-                    a0 = c < (uint32_t)LSM64 ? s0l[c] : l0[c];
-                }
-                if (ok1) {
-                    const uint32_t c = staged_code<T>(sw, rl * (uint32_t)n + (uint32_t)c1);
-                    a1 = c < (uint32_t)LSM64 ? s1l[c] : l1[c];
-                }
+            const bool rv = rl < rows;
+            const uint32_t ca = (rv && ok0) ? staged_code<T>(sw, rl * (uint32_t)n + (uint32_t)c0) : 0xFFFFFFFFu;
+            const uint32_t cb = (rv && ok1) ? staged_code<T>(sw, rl * (uint32_t)n + (uint32_t)c1) : 0xFFFFFFFFu;
+#pragma unroll
+            for (int m = 0; m < NM; m++) {
+                const double a0 = ca != 0xFFFFFFFFu ? value(m, k0, ca) : 0.0;
+                const double a1 = cb != 0xFFFFFFFFu ? value(m, k1, cb) : 0.0;
+                d00[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, a0, d00[m], 0, 0, 0);
+                d01[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, a1, d01[m], 0, 0, 0);
+                d11[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, a1, d11[m], 0, 0, 0);
             }
-            d00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, a0, d00, 0, 0, 0);
-            d01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, a1, d01, 0, 0, 0);
-            d11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, a1, d11, 0, 0, 0);
         }
         __builtin_amdgcn_wave_barrier();
     }
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        partial[((wave * 3 + 0) * 4 + i) * 64 + lane] = d00[i];
-        partial[((wave * 3 + 1) * 4 + i) * 64 + lane] = d01[i];
-        partial[((wave * 3 + 2) * 4 + i) * 64 + lane] = d11[i];
-    }
+    for (int m = 0; m < NM; m++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            partial[(((wave * NM + m) * 3 + 0) * 4 + i) * 64 + lane] = d00[m][i];
+            partial[(((wave * NM + m) * 3 + 1) * 4 + i) * 64 + lane] = d01[m][i];
+            partial[(((wave * NM + m) * 3 + 2) * 4 + i) * 64 + lane] = d11[m][i];
+        }
 }
 
 // Sum partials over waves in a fixed order (deterministic), in two levels:
@@ -342,6 +383,14 @@ __global__ void k_zero_cells(uint32_t *p, uint64_t n) {
     if (i < n) p[i] = 0;
 }
 
+// 8-bit codes: each count becomes the rank of its value among the distinct
+// values of its column (map: [column][stride] u8).
+__global__ void k_dense_codes(const uint32_t *__restrict__ in, uint64_t cells, uint32_t n,
+                              const uint8_t *__restrict__ map, uint32_t stride, uint8_t *__restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cells) out[i] = map[(uint64_t)(i % n) * stride + in[i]];
+}
+
 template <class T>
 __global__ void k_u32_to_codes(const uint32_t *__restrict__ in, uint64_t n, T *__restrict__ out) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -397,7 +446,7 @@ void CorrEngine::make_lut(int method, uint64_t N, int n, std::vector<double> &z,
         std::vector<double> val(stride, 0.0);
         double cum = 0;
         for (uint32_t c = 0; c < stride; c++) {
-            if (method == 0) val[c] = std::log1p((double)c);
+            if (method == 0) val[c] = std::log1p((double)(dense_ ? dense_vals_[(uint64_t)j * stride + c] : c));
             else val[c] = cum + ((double)h[c] + 1.0) / 2.0;  // average tie rank, 1-based
             cum += h[c];
         }
@@ -427,39 +476,66 @@ void CorrEngine::reduce_partials(uint64_t waves, uint32_t per_wave) {
     KITE_CHECK_LAUNCH();
 }
 
-void CorrEngine::gram(const void *d_codes, int code_bytes, uint64_t N, int n, const std::vector<double> &z,
-                      bool fp64, bool with_lolo, std::vector<double> &G, float *ms) {
+void CorrEngine::gram(const void *d_codes, int code_bytes, uint64_t N, int n,
+                      const std::vector<const std::vector<double> *> &zs, bool fp64, bool with_lolo,
+                      std::vector<std::vector<double>> &Gs, float *ms) {
     const uint32_t stride = stride_;
+    const int NM = (int)zs.size();
+    if (NM < 1 || NM > 2) throw Error("internal: one or two tables per Gram pass");
     // about 12K waves (several rounds of 256 CUs x 12 resident waves, so the
     // last round is a small fraction), each a multiple of 512 rows
     const uint64_t rpw = std::max<uint64_t>(512, ((N + 12287) / 12288 + 511) / 512 * 512);
     const uint64_t waves = std::max<uint64_t>(1, (N + rpw - 1) / rpw);
     const uint64_t wgs = (waves + 3) / 4;
     const uint64_t waves_launched = wgs * 4;
+    const size_t tsz = (size_t)n * stride;
     hipEvent_t e0, e1;
     KITE_HIP(hipEventCreate(&e0));
     KITE_HIP(hipEventCreate(&e1));
     std::vector<double> out;
+    Gs.assign(NM, std::vector<double>(MAX_COLS * MAX_COLS, 0.0));
     if (!fp64) {
-        std::vector<uint32_t> lut(z.size());
-        for (size_t i = 0; i < z.size(); i++) {
-            float zf = (float)z[i];
-            uint16_t hi = bf16_bits_rne(zf);
-            uint16_t lo = bf16_bits_rne((float)(z[i] - (double)bf16_to_float(hi)));
-            lut[i] = (uint32_t)hi | ((uint32_t)lo << 16);
-        }
+        std::vector<uint32_t> lut(NM * tsz);
+        for (int m = 0; m < NM; m++)
+            for (size_t i = 0; i < tsz; i++) {
+                const double zv = (*zs[m])[i];
+                uint16_t hi = bf16_bits_rne((float)zv);
+                uint16_t lo = bf16_bits_rne((float)(zv - (double)bf16_to_float(hi)));
+                lut[m * tsz + i] = (uint32_t)hi | ((uint32_t)lo << 16);
+            }
         lut_.ensure(lut.size() * 4);
         KITE_HIP(hipMemcpyAsync(lut_.ptr, lut.data(), lut.size() * 4, hipMemcpyHostToDevice, stream_));
-        const uint32_t per_wave = 16 * 64;
+        const uint32_t per_wave = NM * 16 * 64;
         partial_.ensure(waves_launched * per_wave * 8);
         sum_.ensure(per_wave * 8);
         KITE_HIP(hipEventRecord(e0, stream_));
-        if (code_bytes == 2)
-            k_gram_bf16<uint16_t><<<(uint32_t)wgs, GT, 0, stream_>>>((const uint16_t *)d_codes, N, n, lut_.as<uint32_t>(),
-                                                                     stride, rpw, with_lolo, partial_.as<double>());
-        else
-            k_gram_bf16<uint32_t><<<(uint32_t)wgs, GT, 0, stream_>>>((const uint32_t *)d_codes, N, n, lut_.as<uint32_t>(),
-                                                                     stride, rpw, with_lolo, partial_.as<double>());
+        if (NM == 1) {
+            if (code_bytes == 1)
+                k_gram_bf16<uint8_t, 1><<<(uint32_t)wgs, GT, 0, stream_>>>((const uint8_t *)d_codes, N, n,
+                                                                           lut_.as<uint32_t>(), stride, rpw, with_lolo,
+                                                                           partial_.as<double>());
+            else if (code_bytes == 2)
+                k_gram_bf16<uint16_t, 1><<<(uint32_t)wgs, GT, 0, stream_>>>((const uint16_t *)d_codes, N, n,
+                                                                            lut_.as<uint32_t>(), stride, rpw, with_lolo,
+                                                                            partial_.as<double>());
+            else
+                k_gram_bf16<uint32_t, 1><<<(uint32_t)wgs, GT, 0, stream_>>>((const uint32_t *)d_codes, N, n,
+                                                                            lut_.as<uint32_t>(), stride, rpw, with_lolo,
+                                                                            partial_.as<double>());
+        } else {
+            if (code_bytes == 1)
+                k_gram_bf16<uint8_t, 2><<<(uint32_t)wgs, GT, 0, stream_>>>((const uint8_t *)d_codes, N, n,
+                                                                           lut_.as<uint32_t>(), stride, rpw, with_lolo,
+                                                                           partial_.as<double>());
+            else if (code_bytes == 2)
+                k_gram_bf16<uint16_t, 2><<<(uint32_t)wgs, GT, 0, stream_>>>((const uint16_t *)d_codes, N, n,
+                                                                            lut_.as<uint32_t>(), stride, rpw, with_lolo,
+                                                                            partial_.as<double>());
+            else
+                k_gram_bf16<uint32_t, 2><<<(uint32_t)wgs, GT, 0, stream_>>>((const uint32_t *)d_codes, N, n,
+                                                                            lut_.as<uint32_t>(), stride, rpw, with_lolo,
+                                                                            partial_.as<double>());
+        }
         KITE_CHECK_LAUNCH();
         reduce_partials(waves_launched, per_wave);
         KITE_HIP(hipEventRecord(e1, stream_));
@@ -467,28 +543,51 @@ void CorrEngine::gram(const void *d_codes, int code_bytes, uint64_t N, int n, co
         KITE_HIP(hipMemcpyAsync(out.data(), sum_.ptr, per_wave * 8, hipMemcpyDeviceToHost, stream_));
         KITE_HIP(hipStreamSynchronize(stream_));
         // lane l, element i -> (m, n) = (8*(i/4) + 4*(l/32) + i%4, l%32)
-        G.assign(MAX_COLS * MAX_COLS, 0.0);
-        for (int i = 0; i < 16; i++)
-            for (int l = 0; l < 64; l++) {
-                int m = 8 * (i / 4) + 4 * (l / 32) + (i % 4), nn = l % 32;
-                G[m * MAX_COLS + nn] = out[i * 64 + l];
-            }
+        for (int mt = 0; mt < NM; mt++)
+            for (int i = 0; i < 16; i++)
+                for (int l = 0; l < 64; l++) {
+                    int m = 8 * (i / 4) + 4 * (l / 32) + (i % 4), nn = l % 32;
+                    Gs[mt][m * MAX_COLS + nn] = out[(mt * 16 + i) * 64 + l];
+                }
     } else {
-        lut_.ensure(z.size() * 8);
-        KITE_HIP(hipMemcpyAsync(lut_.ptr, z.data(), z.size() * 8, hipMemcpyHostToDevice, stream_));
+        std::vector<double> lut(NM * tsz);
+        for (int m = 0; m < NM; m++) std::copy(zs[m]->begin(), zs[m]->end(), lut.begin() + m * tsz);
+        lut_.ensure(lut.size() * 8);
+        KITE_HIP(hipMemcpyAsync(lut_.ptr, lut.data(), lut.size() * 8, hipMemcpyHostToDevice, stream_));
         const uint64_t rpw4 = (rpw + 3) / 4 * 4;
         const uint64_t waves4 = std::max<uint64_t>(1, (N + rpw4 - 1) / rpw4);
         const uint64_t wgs4 = (waves4 + 3) / 4;
-        const uint32_t per_wave = 3 * 4 * 64;
+        const uint32_t per_wave = NM * 3 * 4 * 64;
         partial_.ensure(wgs4 * 4 * per_wave * 8);
         sum_.ensure(per_wave * 8);
         KITE_HIP(hipEventRecord(e0, stream_));
-        if (code_bytes == 2)
-            k_gram_f64<uint16_t><<<(uint32_t)wgs4, GT, 0, stream_>>>((const uint16_t *)d_codes, N, n, lut_.as<double>(),
-                                                                     stride, rpw4, partial_.as<double>());
-        else
-            k_gram_f64<uint32_t><<<(uint32_t)wgs4, GT, 0, stream_>>>((const uint32_t *)d_codes, N, n, lut_.as<double>(),
-                                                                     stride, rpw4, partial_.as<double>());
+        if (NM == 1) {
+            if (code_bytes == 1)
+                k_gram_f64<uint8_t, 1><<<(uint32_t)wgs4, GT, 0, stream_>>>((const uint8_t *)d_codes, N, n,
+                                                                           lut_.as<double>(), stride, rpw4,
+                                                                           partial_.as<double>());
+            else if (code_bytes == 2)
+                k_gram_f64<uint16_t, 1><<<(uint32_t)wgs4, GT, 0, stream_>>>((const uint16_t *)d_codes, N, n,
+                                                                            lut_.as<double>(), stride, rpw4,
+                                                                            partial_.as<double>());
+            else
+                k_gram_f64<uint32_t, 1><<<(uint32_t)wgs4, GT, 0, stream_>>>((const uint32_t *)d_codes, N, n,
+                                                                            lut_.as<double>(), stride, rpw4,
+                                                                            partial_.as<double>());
+        } else {
+            if (code_bytes == 1)
+                k_gram_f64<uint8_t, 2><<<(uint32_t)wgs4, GT, 0, stream_>>>((const uint8_t *)d_codes, N, n,
+                                                                           lut_.as<double>(), stride, rpw4,
+                                                                           partial_.as<double>());
+            else if (code_bytes == 2)
+                k_gram_f64<uint16_t, 2><<<(uint32_t)wgs4, GT, 0, stream_>>>((const uint16_t *)d_codes, N, n,
+                                                                            lut_.as<double>(), stride, rpw4,
+                                                                            partial_.as<double>());
+            else
+                k_gram_f64<uint32_t, 2><<<(uint32_t)wgs4, GT, 0, stream_>>>((const uint32_t *)d_codes, N, n,
+                                                                            lut_.as<double>(), stride, rpw4,
+                                                                            partial_.as<double>());
+        }
         KITE_CHECK_LAUNCH();
         reduce_partials(wgs4 * 4, per_wave);
         KITE_HIP(hipEventRecord(e1, stream_));
@@ -497,16 +596,16 @@ void CorrEngine::gram(const void *d_codes, int code_bytes, uint64_t N, int n, co
         KITE_HIP(hipStreamSynchronize(stream_));
         // block b in (00, 01, 11); lane l, element i -> (m, n) = (4*i + l/16, l%16)
         // (the FP64 16x16x4 output layout, measured on gfx950)
-        G.assign(MAX_COLS * MAX_COLS, 0.0);
-        for (int b = 0; b < 3; b++)
-            for (int i = 0; i < 4; i++)
-                for (int l = 0; l < 64; l++) {
-                    int m = 4 * i + l / 16, nn = l % 16;
-                    int bm = b == 2 ? 16 : 0, bn = b == 0 ? 0 : 16;
-                    double v = out[(b * 4 + i) * 64 + l];
-                    G[(bm + m) * MAX_COLS + bn + nn] = v;
-                    if (b == 1) G[(bn + nn) * MAX_COLS + bm + m] = v;
-                }
+        for (int mt = 0; mt < NM; mt++)
+            for (int b = 0; b < 3; b++)
+                for (int i = 0; i < 4; i++)
+                    for (int l = 0; l < 64; l++) {
+                        int m = 4 * i + l / 16, nn = l % 16;
+                        int bm = b == 2 ? 16 : 0, bn = b == 0 ? 0 : 16;
+                        double v = out[((mt * 3 + b) * 4 + i) * 64 + l];
+                        Gs[mt][(bm + m) * MAX_COLS + bn + nn] = v;
+                        if (b == 1) Gs[mt][(bn + nn) * MAX_COLS + bm + m] = v;
+                    }
     }
     KITE_HIP(hipEventElapsedTime(ms, e0, e1));
     KITE_HIP(hipEventDestroy(e0));
@@ -516,34 +615,65 @@ void CorrEngine::gram(const void *d_codes, int code_bytes, uint64_t N, int n, co
 void CorrEngine::run(const void *d_codes, int code_bytes, uint64_t N, int n, uint32_t maxc, const CorrRequest &req,
                      CorrOutput &out) {
     if (n < 1 || n > MAX_COLS) throw Error("invalid argument: sample count must be 1.." + std::to_string(MAX_COLS));
-    column_stats(d_codes, code_bytes, N, n, maxc);
-    for (int method = 0; method < 2; method++) {
-        if (!(req.methods & (1u << method))) continue;
-        std::vector<double> z, sd, G, G64;
-        make_lut(method, N, n, z, sd);
-        float ms = 0, ms64 = 0;
-        gram(d_codes, code_bytes, N, n, z, false, req.with_lolo, G, &ms);
-        if (req.fp64_baseline) gram(d_codes, code_bytes, N, n, z, true, false, G64, &ms64);
-        auto corr_of = [&](const std::vector<double> &g, std::vector<double> &r) {
-            r.assign((size_t)n * n, 0.0);
-            for (int i = 0; i < n; i++)
-                for (int j = 0; j < n; j++) {
-                    double den = std::sqrt(std::max(0.0, g[i * MAX_COLS + i]) * std::max(0.0, g[j * MAX_COLS + j]));
-                    r[i * n + j] = den > 0 ? g[i * MAX_COLS + j] / den : 0.0;
-                }
-        };
-        std::vector<double> &r = method == 0 ? out.pearson : out.spearman;
-        corr_of(G, r);
-        if (req.fp64_baseline) corr_of(G64, method == 0 ? out.pearson_fp64 : out.spearman_fp64);
-        (method == 0 ? out.gram_ms_pearson : out.gram_ms_spearman) = ms;
-        (method == 0 ? out.fp64_ms_pearson : out.fp64_ms_spearman) = ms64;
+    if (code_bytes == 1) {
+        if (!dense_) throw Error("internal: 8-bit codes without their value table");
+    } else {
+        dense_ = false;
+        column_stats(d_codes, code_bytes, N, n, maxc);
+    }
+    // the requested methods share one pass over the codes
+    std::vector<int> methods;
+    for (int method = 0; method < 2; method++)
+        if (req.methods & (1u << method)) methods.push_back(method);
+    if (methods.empty()) return;
+    std::vector<std::vector<double>> z(methods.size()), sd(methods.size());
+    std::vector<const std::vector<double> *> zp;
+    for (size_t q = 0; q < methods.size(); q++) {
+        make_lut(methods[q], N, n, z[q], sd[q]);
+        zp.push_back(&z[q]);
+    }
+    std::vector<std::vector<double>> G, G64;
+    float ms = 0, ms64 = 0;
+    gram(d_codes, code_bytes, N, n, zp, false, req.with_lolo, G, &ms);
+    std::vector<float> ms64_each(methods.size(), 0.f);
+    if (req.fp64_baseline) {
+        // the FP64 Gram is bound by FP64 MFMA throughput, not by reading the
+        // codes: one pass per table is its faster form (measured), so the
+        // baseline takes that form
+        G64.resize(methods.size());
+        for (size_t q = 0; q < methods.size(); q++) {
+            std::vector<std::vector<double>> g1;
+            gram(d_codes, code_bytes, N, n, {zp[q]}, true, false, g1, &ms64_each[q]);
+            G64[q] = std::move(g1[0]);
+            ms64 += ms64_each[q];
+        }
+    }
+    out.gram_ms_total = ms;
+    out.fp64_ms_total = ms64;
+    out.passes = 1;
+    out.fp64_passes = req.fp64_baseline ? (int)methods.size() : 0;
+    auto corr_of = [&](const std::vector<double> &g, std::vector<double> &r) {
+        r.assign((size_t)n * n, 0.0);
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                double den = std::sqrt(std::max(0.0, g[i * MAX_COLS + i]) * std::max(0.0, g[j * MAX_COLS + j]));
+                r[i * n + j] = den > 0 ? g[i * MAX_COLS + j] / den : 0.0;
+            }
+    };
+    for (size_t q = 0; q < methods.size(); q++) {
+        const int method = methods[q];
+        corr_of(G[q], method == 0 ? out.pearson : out.spearman);
+        if (req.fp64_baseline) corr_of(G64[q], method == 0 ? out.pearson_fp64 : out.spearman_fp64);
+        // one pass: its time is shared by the methods it computed
+        (method == 0 ? out.gram_ms_pearson : out.gram_ms_spearman) = ms / (float)methods.size();
+        (method == 0 ? out.fp64_ms_pearson : out.fp64_ms_spearman) = ms64_each[q];
         if (method == 0) {
             // covariance of log1p values (N - 1 divisor) from the standardized Gram
             out.cov.assign((size_t)n * n, 0.0);
             if (N > 1)
                 for (int i = 0; i < n; i++)
                     for (int j = 0; j < n; j++)
-                        out.cov[i * n + j] = G[i * MAX_COLS + j] * sd[i] * sd[j] / (double)(N - 1);
+                        out.cov[i * n + j] = G[q][i * MAX_COLS + j] * sd[q][i] * sd[q][j] / (double)(N - 1);
         }
     }
 }
@@ -578,6 +708,46 @@ uint32_t CorrEngine::max_count(const uint32_t *d_counts, uint64_t cells) {
     KITE_HIP(hipMemcpyAsync(&h, m.ptr, 4, hipMemcpyDeviceToHost, stream_));
     KITE_HIP(hipStreamSynchronize(stream_));
     return h;
+}
+
+bool CorrEngine::dense_codes(const uint32_t *d_counts, uint64_t N, int n, uint32_t maxc, DevBuf &codes) {
+    dense_ = false;
+    if (maxc >= (1u << 20) || n < 1 || n > MAX_COLS) return false;
+    const uint64_t cells = N * (uint64_t)n;
+    // histogram of the raw counts (the value table needs it anyway)
+    column_stats(d_counts, 4, N, n, maxc);
+    const uint32_t stride0 = stride_;
+    std::vector<uint8_t> map((uint64_t)n * stride0, 0);
+    std::vector<uint32_t> vals((uint64_t)n * 256, 0), hcode((uint64_t)n * 256, 0);
+    for (int j = 0; j < n; j++) {
+        const uint32_t *h = hist_h_.data() + (uint64_t)j * stride0;
+        uint32_t code = 0;
+        for (uint32_t v = 0; v < stride0; v++) {
+            if (!h[v]) continue;
+            if (code == 256) return false;  // more than 256 distinct counts in this column
+            map[(uint64_t)j * stride0 + v] = (uint8_t)code;
+            vals[(uint64_t)j * 256 + code] = v;
+            hcode[(uint64_t)j * 256 + code] = h[v];
+            code++;
+        }
+    }
+    DevBuf dmap;
+    dmap.ensure(map.size());
+    KITE_HIP(hipMemcpyAsync(dmap.ptr, map.data(), map.size(), hipMemcpyHostToDevice, stream_));
+    codes.ensure(std::max<uint64_t>(cells, 1));
+    if (cells) {
+        k_dense_codes<<<(uint32_t)((cells + 255) / 256), 256, 0, stream_>>>(d_counts, cells, (uint32_t)n,
+                                                                            dmap.as<uint8_t>(), stride0,
+                                                                            codes.as<uint8_t>());
+        KITE_CHECK_LAUNCH();
+    }
+    KITE_HIP(hipStreamSynchronize(stream_));  // `map` is a local vector
+    // from here on the value table is indexed by code
+    hist_h_.swap(hcode);
+    dense_vals_.swap(vals);
+    stride_ = 256;
+    dense_ = true;
+    return true;
 }
 
 void CorrEngine::to_codes(const uint32_t *d_counts, uint64_t cells, DevBuf &codes, int code_bytes) {

@@ -18,25 +18,34 @@ struct CorrRequest {
 struct CorrOutput {
     std::vector<double> pearson, spearman, cov;        // n x n
     std::vector<double> pearson_fp64, spearman_fp64;   // FP64 Gram results
-    float gram_ms_pearson = 0, gram_ms_spearman = 0;   // split-bf16 Gram (+ reduce)
-    float fp64_ms_pearson = 0, fp64_ms_spearman = 0;   // FP64 Gram (+ reduce)
+    float gram_ms_pearson = 0, gram_ms_spearman = 0;   // split-bf16 Gram (+ reduce), share of the pass
+    float fp64_ms_pearson = 0, fp64_ms_spearman = 0;   // FP64 Gram (+ reduce), share of the pass
+    float gram_ms_total = 0, fp64_ms_total = 0;        // the whole pass (all requested methods)
+    int passes = 0;                                    // passes over the codes of the split-bf16 Gram
+    int fp64_passes = 0;                               // passes of the FP64 Gram (one per table)
 };
 
 class CorrEngine {
    public:
     explicit CorrEngine(hipStream_t s) : stream_(s) {}
-    // d_codes: N x n row-major count values (u16 or u32), all <= maxc.
+    // d_codes: N x n row-major count values (u16 or u32), all <= maxc, or
+    // 8-bit codes from dense_codes() (code_bytes 1).
     void run(const void *d_codes, int code_bytes, uint64_t N, int n, uint32_t maxc, const CorrRequest &req,
              CorrOutput &out);
     void pca(const std::vector<double> &cov, int n, std::vector<double> &evals, std::vector<double> &evecs);
     void to_codes(const uint32_t *d_counts, uint64_t cells, DevBuf &codes, int code_bytes);
+    // 8-bit codes when every column has at most 256 distinct counts: code =
+    // rank of the count among its column's values (exact).  Returns false
+    // (and leaves codes alone) otherwise; run() then takes code_bytes 1.
+    bool dense_codes(const uint32_t *d_counts, uint64_t N, int n, uint32_t maxc, DevBuf &codes);
     uint32_t max_count(const uint32_t *d_counts, uint64_t cells);
 
    private:
     void column_stats(const void *d_codes, int code_bytes, uint64_t N, int n, uint32_t maxc);
     void make_lut(int method, uint64_t N, int n, std::vector<double> &z, std::vector<double> &sd);
-    void gram(const void *d_codes, int code_bytes, uint64_t N, int n, const std::vector<double> &z, bool fp64,
-              bool with_lolo, std::vector<double> &G, float *ms);
+    // One pass over the codes computes the Gram of every table in zs (1 or 2).
+    void gram(const void *d_codes, int code_bytes, uint64_t N, int n, const std::vector<const std::vector<double> *> &zs,
+              bool fp64, bool with_lolo, std::vector<std::vector<double>> &Gs, float *ms);
 
     void reduce_partials(uint64_t waves, uint32_t per_wave);
 
@@ -44,6 +53,8 @@ class CorrEngine {
     uint32_t stride_ = 0;
     DevBuf hist_, lut_, partial_, sum_, stage_;
     std::vector<uint32_t> hist_h_;
+    bool dense_ = false;                 // tables are indexed by 8-bit codes
+    std::vector<uint32_t> dense_vals_;   // [column][code] -> count value
 };
 
 }  // namespace kite

@@ -352,7 +352,7 @@ __global__ __launch_bounds__(ST) void k_split(SplitArgs a) {
     const uint64_t first = sg.start + t * TKL;
     const uint64_t n = std::min<uint64_t>(TKL, sg.size - t * TKL);
     const uint32_t nd = 1u << a.dbits;
-    uint32_t *row = a.wg_hist + b * nd;
+    uint32_t *row = a.chunk > 1 ? a.wg_hist + b * nd : nullptr;  // chunked: per-workgroup counts
     if constexpr (!SCATTER) {
         for (uint32_t g = tid; g < nd; g += ST) hist[g] = 0;
         __syncthreads();
@@ -372,8 +372,43 @@ __global__ __launch_bounds__(ST) void k_split(SplitArgs a) {
         __syncthreads();
         for (uint32_t g = tid; g < nd; g += ST) {
             const uint32_t c = hist[g];
-            row[g] = c;
+            if (a.chunk > 1) row[g] = c;  // one-tile chunks recount in the scatter pass
             if (c) atomicAdd(&a.child_hist[sg.child_base + g], c);
+        }
+    } else if (a.chunk == 1) {
+        // one tile: its keys stay in registers between the count and the scatter
+        Key<W> key[IPT];
+        uint32_t dig[IPT];
+        for (uint32_t g = tid; g < nd; g += ST) hist[g] = 0;
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < IPT; it++) {
+            uint64_t i = (uint64_t)it * ST + tid;
+            if (i < n) {
+                const uint64_t *p = a.src + (first + i) * W;
+#pragma unroll
+                for (int r = 0; r < W; r++) key[it].w[r] = p[r];
+                dig[it] = key_bits<W>(key[it], a.shift, a.dbits);
+                atomicAdd(&hist[dig[it]], 1u);
+            }
+        }
+        __syncthreads();
+        for (uint32_t g = tid; g < nd; g += ST) {
+            const uint32_t c = hist[g];
+            if (c) cbase[g] = atomicAdd((unsigned long long *)&a.child_cur[sg.child_base + g],
+                                        (unsigned long long)c);
+            hist[g] = 0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < IPT; it++) {
+            uint64_t i = (uint64_t)it * ST + tid;
+            if (i < n) {
+                uint64_t slot = cbase[dig[it]] + atomicAdd(&hist[dig[it]], 1u);
+                uint64_t *d = a.dst + slot * W;
+#pragma unroll
+                for (int r = 0; r < W; r++) d[r] = key[it].w[r];
+            }
         }
     } else {
         for (uint32_t g = tid; g < nd; g += ST) {
@@ -1350,7 +1385,7 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
         sa.dbits = d;
         sa.chunk = spl;
         sa.child_hist = ch;
-        sa.wg_hist = static_cast<uint32_t *>(wg_hist_.ensure(tiles_total * nd * 4));
+        sa.wg_hist = spl > 1 ? static_cast<uint32_t *>(wg_hist_.ensure(tiles_total * nd * 4)) : nullptr;
         launch("k_split_hist", split_keys * 8 * W,
                [&] { k_split<W, false><<<dim3((unsigned)tiles_total), dim3(ST), 0, s>>>(sa); });
         uint64_t *ccur = static_cast<uint64_t *>(child_cur_.ensure(nchild * 8));

@@ -69,6 +69,7 @@ class CountEngine {
     uint64_t n_unique() const { return n_unique_; }
     int words() const { return words_; }
     const std::vector<uint64_t> &part_off() const { return part_off_h_; }
+    const uint64_t *d_part_off() const { return part_u_.as<uint64_t>(); }  // device copy of part_off()
     const uint64_t *keys() const { return out_keys_[slot_].as<uint64_t>(); }
     const uint32_t *counts() const { return out_counts_[slot_].as<uint32_t>(); }
     // Two output slots let a copy stream drain one result while the next

@@ -59,6 +59,7 @@ def test_large_counts_use_wide_codes(gpu_ctx):
     rng = np.random.default_rng(9)
     M = synth_counts(rng, 20000, 6)
     M[::97, 2] = 1_000_000
+    M[:, 4] = np.arange(20000, dtype=np.uint32) % 700  # 700 distinct counts
     out = gpu_ctx.corr_counts(M, 3, 1)
     assert out["stats"]["code_bytes"] == 4
     V = co.transform(M)
@@ -95,3 +96,21 @@ def test_analyze_tables_union_and_correlation(gpu_ctx, tmp_path):
     V = co.transform(matrix)
     assert np.max(np.abs(out["pearson"] - co.pearson(V))) <= TOL_BF16
     assert np.max(np.abs(out["spearman"] - co.spearman(V))) <= TOL_BF16
+
+
+@pytest.mark.parametrize("distinct,expect", [(256, 1), (257, 2)])
+def test_dense_eight_bit_codes(gpu_ctx, distinct, expect):
+    """Columns with at most 256 distinct counts are read as exact 8-bit
+    ranks of their values; one more distinct count falls back to u16."""
+    rng = np.random.default_rng(distinct)
+    M = synth_counts(rng, 50000, 7)
+    M[:, 3] = (np.arange(50000, dtype=np.uint32) * 7919) % distinct * 3  # exactly `distinct` values, up to 768
+    out = gpu_ctx.corr_counts(M, 3, 1)
+    assert out["stats"]["code_bytes"] == expect
+    V = co.transform(M)
+    P, S = co.pearson(V), co.spearman(V)
+    const = np.all(M == M[:1], axis=0)
+    check_corr(out["pearson"], P, const, TOL_BF16)
+    check_corr(out["spearman"], S, const, TOL_BF16)
+    check_corr(out["pearson_fp64"], P, const, TOL_FP64)
+    check_corr(out["spearman_fp64"], S, const, TOL_FP64)
