@@ -133,12 +133,28 @@ def bench_batch24(ctx, scale: float, reps: int) -> tuple[dict, list]:
     def nothing(i, st, keys, counts, off):
         return 0
 
+    # encoded mode: chunk payloads built on the device; the same bytes from both modes
+    def cdigest(i, st, codec, data, boff, rows, mm):
+        n8 = data.size // 8 * 8
+        words = data[:n8].view(np.uint64)
+        return (int(boff[-1]), int(np.bitwise_xor.reduce(words)) if words.size else 0,
+                int(words.sum(dtype=np.uint64)) if words.size else 0, data[n8:].tobytes())
+
+    e_pipe = ctx.count_batch(arrs, k, canon, P, mode=BATCH_PIPELINED, on_result=cdigest, encoded=True)
+    e_plain = ctx.count_batch(arrs, k, canon, P, mode=BATCH_PLAIN, on_result=cdigest, encoded=True)
+    assert e_pipe == e_plain, "plain and pipelined encoded chunks differ"
+    enc_bytes = sum(d[0] for d in e_pipe)
+
+    def nothing_enc(i, st, codec, data, boff, rows, mm):
+        return 0
+
     res = {}
-    for name, mode in (("plain", BATCH_PLAIN), ("pipelined", BATCH_PIPELINED)):
+    for name, mode, enc in (("plain", BATCH_PLAIN, False), ("pipelined", BATCH_PIPELINED, False),
+                            ("plain_encoded", BATCH_PLAIN, True), ("pipelined_encoded", BATCH_PIPELINED, True)):
         best = None
         for _ in range(reps):
             t = time.perf_counter()
-            ctx.count_batch(arrs, k, canon, P, mode=mode, on_result=nothing)
+            ctx.count_batch(arrs, k, canon, P, mode=mode, on_result=nothing_enc if enc else nothing, encoded=enc)
             dt = time.perf_counter() - t
             best = dt if best is None or dt < best else best
         res[name] = {"wall_s": best, "kmers_per_s": kmers / best}
@@ -149,6 +165,10 @@ def bench_batch24(ctx, scale: float, reps: int) -> tuple[dict, list]:
         "plain": res["plain"], "pipelined": res["pipelined"],
         "speedup": res["plain"]["wall_s"] / res["pipelined"]["wall_s"],
         "includes": "H2D from pinned host memory, kernels, D2H of the reduced tables",
+        "encoded": {"plain": res["plain_encoded"], "pipelined": res["pipelined_encoded"],
+                    "speedup": res["plain_encoded"]["wall_s"] / res["pipelined_encoded"]["wall_s"],
+                    "chunk_bytes": int(enc_bytes),
+                    "includes": "H2D of the FASTQ, kernels, DVLE chunk encoding on the device, D2H of the chunks"},
         "bit_exact_plain_vs_pipelined": True, "scale": scale,
     }
     for p in pins:

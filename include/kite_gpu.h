@@ -151,6 +151,21 @@ int kite_gpu_count_batch(kite_gpu_ctx *ctx, uint32_t k, int canonical, uint32_t 
                          const kite_batch_sample *samples, uint32_t n_samples, int mode,
                          kite_batch_result_fn fn, void *user);
 
+/* The same batch, each sample delivered as .kite chunk payloads encoded on
+ * the device (DVLE for k <= 64, RAW above; only these bytes cross PCIe):
+ * partition p's payload is data[byte_off[p], byte_off[p + 1]) holding
+ * rows_off[p + 1] - rows_off[p] rows, with first and last keys
+ * minmax[8p .. 8p+3] and minmax[8p+4 .. 8p+7] (4 words each, unused words 0).
+ * A host writes each sample's .kite from them as the count workers hand
+ * chunks to ConcurrentKiteWriter::append_chunk (src/count/writer.rs:285-348). */
+typedef int (*kite_batch_chunks_fn)(void *user, uint32_t sample, const kite_count_stats *stats, uint32_t codec,
+                                    const uint8_t *data, const uint64_t *byte_off, const uint64_t *rows_off,
+                                    const uint64_t *minmax);
+
+int kite_gpu_count_batch_chunks(kite_gpu_ctx *ctx, uint32_t k, int canonical, uint32_t partitions,
+                                const kite_batch_sample *samples, uint32_t n_samples, int mode,
+                                kite_batch_chunks_fn fn, void *user);
+
 /* Page-locked host memory for FASTQ buffers and results (hipHostMalloc). */
 int kite_gpu_host_alloc(uint64_t bytes, void **out);
 void kite_gpu_host_free(void *p);

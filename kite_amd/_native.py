@@ -64,6 +64,9 @@ class BatchSample(C.Structure):
 
 BATCH_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint32, C.POINTER(CountStats), C.POINTER(C.c_uint64),
                        C.POINTER(C.c_uint32), C.POINTER(C.c_uint64))
+BATCH_CHUNKS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint32, C.POINTER(CountStats), C.c_uint32,
+                              C.POINTER(C.c_uint8), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                              C.POINTER(C.c_uint64))
 BATCH_PLAIN, BATCH_PIPELINED = 0, 1
 
 # name -> (restype, argtypes); every symbol include/kite_gpu.h declares
@@ -101,6 +104,9 @@ GPU_SYMBOLS = {
     "kite_gpu_count_batch": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_uint32,
                                        C.POINTER(BatchSample), C.c_uint32, C.c_int, BATCH_FN,
                                        C.c_void_p]),
+    "kite_gpu_count_batch_chunks": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_uint32,
+                                              C.POINTER(BatchSample), C.c_uint32, C.c_int, BATCH_CHUNKS_FN,
+                                              C.c_void_p]),
     "kite_gpu_host_alloc": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
     "kite_gpu_host_free": (None, [C.c_void_p]),
     "kite_gpu_count_profile": (C.c_int, [C.c_void_p, C.c_int]),
@@ -345,12 +351,19 @@ class GpuContext:
 
     # -- batched count of many samples (kite_gpu_count_batch) ------------
     def count_batch(self, samples, k: int, canonical: bool, partitions: int,
-                    mode: int = BATCH_PIPELINED, on_result=None) -> list:
+                    mode: int = BATCH_PIPELINED, on_result=None, encoded: bool = False) -> list:
         """Count each FASTQ buffer in `samples` (uint8 arrays, ideally from
         PinnedArray) as its own table.  With on_result=None the tables are
         returned as [(CountResult, stats dict)]; otherwise
         on_result(i, stats dict, keys, counts, part_off) receives views that
-        are only valid during the call, and the list holds its return values."""
+        are only valid during the call, and the list holds its return values.
+
+        encoded=True delivers chunk payloads encoded on the device instead:
+        the default result is [(codec, bytes, byte_off, rows_off, minmax,
+        stats dict)] and on_result(i, stats, codec, data, byte_off, rows_off,
+        minmax) receives views valid during the call."""
+        if encoded:
+            return self._count_batch_chunks(samples, k, canonical, partitions, mode, on_result)
         arrs = [np.ascontiguousarray(a, dtype=np.uint8) for a in samples]
         descr = (BatchSample * max(len(arrs), 1))()
         for i, a in enumerate(arrs):
@@ -380,6 +393,41 @@ class GpuContext:
         fn = BATCH_FN(cb)
         rc = self._lib.kite_gpu_count_batch(self._h, k, int(bool(canonical)), P, descr, len(arrs), mode,
                                             fn, None)
+        if errors:
+            raise errors[0]
+        _check(rc)
+        return out
+
+    def _count_batch_chunks(self, samples, k, canonical, partitions, mode, on_result):
+        arrs = [np.ascontiguousarray(a, dtype=np.uint8) for a in samples]
+        descr = (BatchSample * max(len(arrs), 1))()
+        for i, a in enumerate(arrs):
+            descr[i].fastq = a.ctypes.data if a.size else None
+            descr[i].len = a.size
+        P = partitions
+        out: list = [None] * len(arrs)
+        errors: list = []
+
+        def cb(user, i, st, codec, dp, bo, ro, mm):
+            try:
+                d = {f: getattr(st.contents, f) for f, _ in CountStats._fields_}
+                boff = np.ctypeslib.as_array(bo, shape=(P + 1,))
+                total = int(boff[-1])
+                data = np.ctypeslib.as_array(dp, shape=(max(total, 1),))[:total]
+                rows = np.ctypeslib.as_array(ro, shape=(P + 1,))
+                minmax = np.ctypeslib.as_array(mm, shape=(P * 8,)).reshape(P, 8)
+                if on_result is None:
+                    out[i] = (int(codec), data.tobytes(), boff.copy(), rows.copy(), minmax.copy(), d)
+                else:
+                    out[i] = on_result(i, d, int(codec), data, boff, rows, minmax)
+                return 0
+            except Exception as e:  # pragma: no cover - surfaced after the call
+                errors.append(e)
+                return 1
+
+        fn = BATCH_CHUNKS_FN(cb)
+        rc = self._lib.kite_gpu_count_batch_chunks(self._h, k, int(bool(canonical)), P, descr, len(arrs), mode,
+                                                   fn, None)
         if errors:
             raise errors[0]
         _check(rc)

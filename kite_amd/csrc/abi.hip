@@ -75,6 +75,8 @@ struct PinnedBuf {
 // counts and partition offsets, and the event that ends its D2H copy.
 struct BatchSlot {
     PinnedBuf keys, counts;
+    kite::HostPinned enc_bytes, enc_minmax, enc_boff;  // chunk payloads (encoded mode)
+    uint64_t enc_total = 0;
     std::vector<uint64_t> part_off;
     kite_count_stats st{};
     uint32_t sample = 0;
@@ -111,6 +113,7 @@ struct kite_gpu_ctx {
     hipStream_t h2d = nullptr, d2h = nullptr;
     kite::DevBuf batch_in[2];
     BatchSlot slots[2];
+    kite::EncodeBufs enc_slot[2];  // device encode buffers of the two result slots
 };
 
 namespace {
@@ -233,6 +236,9 @@ void kite_gpu_close(kite_gpu_ctx *ctx) {
             ctx->batch_in[b].release();
             ctx->slots[b].keys.release();
             ctx->slots[b].counts.release();
+            ctx->slots[b].enc_bytes.release();
+            ctx->slots[b].enc_minmax.release();
+            ctx->slots[b].enc_boff.release();
             for (hipEvent_t e : {ctx->slots[b].d2h_done, ctx->slots[b].run_done, ctx->slots[b].in_ready})
                 if (e) (void)hipEventDestroy(e);
         }
@@ -615,8 +621,36 @@ void batch_stats(kite_gpu_ctx *c, const kite_batch_sample &smp, kite_count_stats
     st.compact_ms = t.compact_ms;
 }
 
-void batch_plain(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, kite_batch_result_fn fn,
-                 void *user) {
+// Where a finished sample goes: raw rows (kite_batch_result_fn) or chunk
+// payloads encoded on the device (kite_batch_chunks_fn).
+struct BatchSink {
+    kite_batch_result_fn raw = nullptr;
+    kite_batch_chunks_fn chunks = nullptr;
+    void *user = nullptr;
+    bool encoded() const { return chunks != nullptr; }
+    int call(kite_gpu_ctx *c, BatchSlot &sl) const {
+        if (chunks)
+            return chunks(user, sl.sample, &sl.st, c->words == 4 ? KITE_CODEC_RAW : KITE_CODEC_DVLE,
+                          sl.enc_bytes.as<uint8_t>(), sl.enc_boff.as<uint64_t>(), sl.part_off.data(),
+                          sl.enc_minmax.as<uint64_t>());
+        return raw(user, sl.sample, &sl.st, sl.keys.as<uint64_t>(), sl.counts.as<uint32_t>(), sl.part_off.data());
+    }
+};
+
+// Encodes the engine's result slot into `eb` and sizes the slot's host
+// buffers; the payload stays on the device until the caller copies it.
+kite::EncodeDevice encode_slot(kite_gpu_ctx *c, BatchSlot &sl, kite::EncodeBufs &eb) {
+    const uint32_t P = c->partitions;
+    uint64_t *hb = static_cast<uint64_t *>(sl.enc_boff.ensure(((size_t)P + 1) * 8));
+    kite::EncodeDevice ed = kite::encode_on_device(c->count->keys(), c->count->counts(), c->count->d_part_off(),
+                                                   c->count->n_unique(), P, c->words, c->stream, eb, hb);
+    sl.enc_total = ed.total;
+    sl.enc_bytes.ensure(std::max<uint64_t>(ed.total, 1));
+    sl.enc_minmax.ensure((size_t)P * 64);
+    return ed;
+}
+
+void batch_plain(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, const BatchSink &sink) {
     BatchSlot &sl = c->slots[0];
     const int W = c->words;
     for (uint32_t i = 0; i < n; i++) {
@@ -635,17 +669,28 @@ void batch_plain(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, kite
             throw;
         }
         const uint64_t u = c->count->n_unique();
-        uint64_t *keys = static_cast<uint64_t *>(sl.keys.ensure(u * 8 * W));
-        uint32_t *counts = static_cast<uint32_t *>(sl.counts.ensure(u * 4));
-        if (u) {
-            KITE_HIP(hipMemcpy(keys, c->count->keys(), u * 8 * W, hipMemcpyDeviceToHost));
-            KITE_HIP(hipMemcpy(counts, c->count->counts(), u * 4, hipMemcpyDeviceToHost));
+        if (sink.encoded()) {
+            // the engine stream does not synchronize with the null stream:
+            // the copies are queued behind the encode kernels on that stream
+            const kite::EncodeDevice ed = encode_slot(c, sl, c->enc_slot[0]);
+            if (ed.total)
+                KITE_HIP(hipMemcpyAsync(sl.enc_bytes.ptr, ed.bytes, ed.total, hipMemcpyDeviceToHost, c->stream));
+            KITE_HIP(hipMemcpyAsync(sl.enc_minmax.ptr, ed.minmax, (size_t)c->partitions * 64, hipMemcpyDeviceToHost,
+                                    c->stream));
+            KITE_HIP(hipStreamSynchronize(c->stream));
+        } else {
+            uint64_t *keys = static_cast<uint64_t *>(sl.keys.ensure(u * 8 * W));
+            uint32_t *counts = static_cast<uint32_t *>(sl.counts.ensure(u * 4));
+            if (u) {
+                KITE_HIP(hipMemcpy(keys, c->count->keys(), u * 8 * W, hipMemcpyDeviceToHost));
+                KITE_HIP(hipMemcpy(counts, c->count->counts(), u * 4, hipMemcpyDeviceToHost));
+            }
         }
         if (d) KITE_HIP(hipFree(d));
         batch_stats(c, smp[i], sl.st);
         sl.part_off = c->count->part_off();
-        if (fn(user, i, &sl.st, keys, counts, sl.part_off.data()) != 0)
-            throw kite::Error("callback aborted the batch");
+        sl.sample = i;
+        if (sink.call(c, sl) != 0) throw kite::Error("callback aborted the batch");
     }
 }
 
@@ -653,8 +698,7 @@ void batch_plain(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, kite
 // in sample order, and frees each slot for reuse afterwards.
 struct Consumer {
     kite_gpu_ctx *c;
-    kite_batch_result_fn fn;
-    void *user;
+    BatchSink sink;
     std::mutex mu;
     std::condition_variable cv;
     std::deque<int> queue;
@@ -663,7 +707,7 @@ struct Consumer {
     std::string error;
     std::thread th;
 
-    Consumer(kite_gpu_ctx *cc, kite_batch_result_fn f, void *u) : c(cc), fn(f), user(u) {
+    Consumer(kite_gpu_ctx *cc, const BatchSink &sk) : c(cc), sink(sk) {
         th = std::thread([this] { loop(); });
     }
     ~Consumer() { stop(); }
@@ -689,8 +733,7 @@ struct Consumer {
             std::string err;
             hipError_t e = hipEventSynchronize(sl.d2h_done);
             if (e != hipSuccess) err = std::string("HIP error: ") + hipGetErrorString(e);
-            else if (fn(user, sl.sample, &sl.st, sl.keys.as<uint64_t>(), sl.counts.as<uint32_t>(),
-                        sl.part_off.data()) != 0)
+            else if (sink.call(c, sl) != 0)
                 err = "callback aborted the batch";
             {
                 std::lock_guard<std::mutex> g(mu);
@@ -726,8 +769,7 @@ struct Consumer {
     }
 };
 
-void batch_pipelined(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, kite_batch_result_fn fn,
-                     void *user) {
+void batch_pipelined(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, const BatchSink &sink) {
     if (!c->h2d) KITE_HIP(hipStreamCreateWithFlags(&c->h2d, hipStreamNonBlocking));
     if (!c->d2h) KITE_HIP(hipStreamCreateWithFlags(&c->d2h, hipStreamNonBlocking));
     for (int b = 0; b < 2; b++) {
@@ -749,7 +791,7 @@ void batch_pipelined(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, 
         KITE_HIP(hipEventRecord(c->slots[b].in_ready, c->h2d));
     };
 
-    Consumer cons(c, fn, user);
+    Consumer cons(c, sink);
     std::string failure;
     try {
         if (n) h2d(0);
@@ -763,18 +805,31 @@ void batch_pipelined(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, 
             c->count->set_slot(s);
             c->count->set_timing(true);
             c->count->run(c->batch_in[s].as<uint8_t>(), smp[i].len, (int)c->k, c->canonical, c->partitions);
-            KITE_HIP(hipEventRecord(sl.run_done, c->stream));
+            // slot s's host buffers are free once the callback of sample i-2 ran
             if (!cons.acquire(s)) break;
             const uint64_t u = c->count->n_unique();
-            uint64_t *keys = static_cast<uint64_t *>(sl.keys.ensure(u * 8 * W));
-            uint32_t *counts = static_cast<uint32_t *>(sl.counts.ensure(u * 4));
             batch_stats(c, smp[i], sl.st);
             sl.part_off = c->count->part_off();
             sl.sample = i;
-            KITE_HIP(hipStreamWaitEvent(c->d2h, sl.run_done, 0));
-            if (u) {
-                KITE_HIP(hipMemcpyAsync(keys, c->count->keys(), u * 8 * W, hipMemcpyDeviceToHost, c->d2h));
-                KITE_HIP(hipMemcpyAsync(counts, c->count->counts(), u * 4, hipMemcpyDeviceToHost, c->d2h));
+            if (sink.encoded()) {
+                // encode buffer s was last read by the D2H of sample i-2, which the
+                // engine stream waited for before this run
+                const kite::EncodeDevice ed = encode_slot(c, sl, c->enc_slot[s]);
+                KITE_HIP(hipEventRecord(sl.run_done, c->stream));
+                KITE_HIP(hipStreamWaitEvent(c->d2h, sl.run_done, 0));
+                if (ed.total)
+                    KITE_HIP(hipMemcpyAsync(sl.enc_bytes.ptr, ed.bytes, ed.total, hipMemcpyDeviceToHost, c->d2h));
+                KITE_HIP(hipMemcpyAsync(sl.enc_minmax.ptr, ed.minmax, (size_t)c->partitions * 64,
+                                        hipMemcpyDeviceToHost, c->d2h));
+            } else {
+                KITE_HIP(hipEventRecord(sl.run_done, c->stream));
+                uint64_t *keys = static_cast<uint64_t *>(sl.keys.ensure(u * 8 * W));
+                uint32_t *counts = static_cast<uint32_t *>(sl.counts.ensure(u * 4));
+                KITE_HIP(hipStreamWaitEvent(c->d2h, sl.run_done, 0));
+                if (u) {
+                    KITE_HIP(hipMemcpyAsync(keys, c->count->keys(), u * 8 * W, hipMemcpyDeviceToHost, c->d2h));
+                    KITE_HIP(hipMemcpyAsync(counts, c->count->counts(), u * 4, hipMemcpyDeviceToHost, c->d2h));
+                }
             }
             KITE_HIP(hipEventRecord(sl.d2h_done, c->d2h));
             cons.push(s);
@@ -795,11 +850,11 @@ void batch_pipelined(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, 
 
 extern "C" {
 
-int kite_gpu_count_batch(kite_gpu_ctx *ctx, uint32_t k, int canonical, uint32_t partitions,
-                         const kite_batch_sample *samples, uint32_t n_samples, int mode,
-                         kite_batch_result_fn fn, void *user) {
+namespace {
+int count_batch_impl(kite_gpu_ctx *ctx, uint32_t k, int canonical, uint32_t partitions,
+                     const kite_batch_sample *samples, uint32_t n_samples, int mode, const BatchSink &sink) {
     if (!ctx) return set_error(KITE_E_INVALID, "invalid argument: ctx is NULL");
-    if (!fn) return set_error(KITE_E_INVALID, "invalid argument: callback is NULL");
+    if (!sink.raw && !sink.chunks) return set_error(KITE_E_INVALID, "invalid argument: callback is NULL");
     if (n_samples && !samples) return set_error(KITE_E_INVALID, "invalid argument: samples is NULL");
     if (mode != KITE_BATCH_PLAIN && mode != KITE_BATCH_PIPELINED)
         return set_error(KITE_E_INVALID, "invalid argument: unknown batch mode");
@@ -815,12 +870,31 @@ int kite_gpu_count_batch(kite_gpu_ctx *ctx, uint32_t k, int canonical, uint32_t 
             if (samples[i].len > ctx->batch_cap)
                 return set_error(KITE_E_NOMEM, "sample " + std::to_string(i) +
                                                    " exceeds the device budget of one batch");
-        if (mode == KITE_BATCH_PLAIN) batch_plain(ctx, samples, n_samples, fn, user);
-        else batch_pipelined(ctx, samples, n_samples, fn, user);
+        if (mode == KITE_BATCH_PLAIN) batch_plain(ctx, samples, n_samples, sink);
+        else batch_pipelined(ctx, samples, n_samples, sink);
         return KITE_OK;
     });
     if (rc == KITE_E_DEVICE && g_last_error.find("callback aborted") != std::string::npos) rc = KITE_E_CALLBACK;
     return rc;
+}
+}  // namespace
+
+int kite_gpu_count_batch(kite_gpu_ctx *ctx, uint32_t k, int canonical, uint32_t partitions,
+                         const kite_batch_sample *samples, uint32_t n_samples, int mode,
+                         kite_batch_result_fn fn, void *user) {
+    BatchSink sink;
+    sink.raw = fn;
+    sink.user = user;
+    return count_batch_impl(ctx, k, canonical, partitions, samples, n_samples, mode, sink);
+}
+
+int kite_gpu_count_batch_chunks(kite_gpu_ctx *ctx, uint32_t k, int canonical, uint32_t partitions,
+                                const kite_batch_sample *samples, uint32_t n_samples, int mode,
+                                kite_batch_chunks_fn fn, void *user) {
+    BatchSink sink;
+    sink.chunks = fn;
+    sink.user = user;
+    return count_batch_impl(ctx, k, canonical, partitions, samples, n_samples, mode, sink);
 }
 
 int kite_gpu_host_alloc(uint64_t bytes, void **out) {

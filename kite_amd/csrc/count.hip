@@ -158,6 +158,10 @@ __global__ __launch_bounds__(XT) void k_extract(ExtractArgs a) {
     __shared__ uint32_t snl[NSEG];
     __shared__ uint32_t spre[NSEG];
     __shared__ uint32_t ws[XT / 64 + 1];
+    // k-mer end positions of the sub-tile (histogram pass: every lane builds
+    // k-mers; the scatter pass keeps the per-segment loop, which holds more
+    // waves per CU for its stores)
+    __shared__ uint16_t spos[SCATTER ? 1 : XT * SEGB];
     extern __shared__ uint32_t dyn[];  // P u32 counters, then (scatter) P u64 bases
     const uint32_t P = 1u << a.pbits;
     uint32_t *cnt = dyn;
@@ -241,6 +245,7 @@ __global__ __launch_bounds__(XT) void k_extract(ExtractArgs a) {
         }
         __syncthreads();
 
+        // k-mer end positions of this thread's segment
         const int s = tid + HSEG;
         uint32_t run = 0;
 #pragma unroll
@@ -254,13 +259,18 @@ __global__ __launch_bounds__(XT) void k_extract(ExtractArgs a) {
             break;
         }
         const uint32_t vm = svalid[s];
-        if (vm == 0) continue;
-        uint64_t wv[W + 1];
+        uint32_t emask = 0;
+        if (vm) {
+            for (int j = 0; j < SEGB; j++) {
+                run = ((vm >> j) & 1u) ? run + 1 : 0;
+                emask |= (run >= (uint32_t)a.k ? 1u : 0u) << j;
+            }
+        }
+        // the k-mer ending at position j of segment ss: key, partition, count or place
+        auto kmer_at = [&](int ss, int j) {
+            uint64_t wv[W + 1];
 #pragma unroll
-        for (int i = 0; i <= W; i++) wv[i] = scode[s - W + i];
-        for (int j = 0; j < SEGB; j++) {
-            run = ((vm >> j) & 1u) ? run + 1 : 0;
-            if (run < (uint32_t)a.k) continue;
+            for (int q = 0; q <= W; q++) wv[q] = scode[ss - W + q];
             const int sh = 62 - 2 * j;
             Key<W> f;
 #pragma unroll
@@ -296,6 +306,20 @@ __global__ __launch_bounds__(XT) void k_extract(ExtractArgs a) {
 #pragma unroll
                 for (int r = 0; r < W; r++) dst[r] = key.w[r];
             }
+        };
+        if constexpr (!SCATTER) {
+            // one list for the whole sub-tile, so that every lane builds k-mers
+            // (segments of header and quality lines have none)
+            uint32_t nk;
+            uint32_t slot = block_excl_scan<XT, uint32_t>(__popc(emask), ws, &nk);
+            for (uint32_t e = emask; e; e &= e - 1) spos[slot++] = (uint16_t)(s * SEGB + __ffs(e) - 1);
+            __syncthreads();
+            for (uint32_t i = tid; i < nk; i += XT) {
+                const uint32_t pos = spos[i];
+                kmer_at((int)(pos / SEGB), (int)(pos % SEGB));
+            }
+        } else {
+            for (uint32_t e = emask; e; e &= e - 1) kmer_at(s, __ffs(e) - 1);
         }
     }
     if constexpr (!SCATTER) {
@@ -330,15 +354,19 @@ struct SplitArgs {
 
 // One workgroup handles a chunk of `chunk` tiles (TK keys each) of one segment,
 // so that each child receives a run of about TK*chunk/2^dbits keys from it.
-// The histogram pass keeps every chunk's child counts; the scatter pass
-// reserves one range per (chunk, child) with a single atomic and fills it
-// through an LDS cursor.
-template <int W, bool SCATTER>
+// MODE 0 (histogram) adds every chunk's child counts to the global totals
+// (and keeps them per chunk when chunk > 1).  MODE 1 (scatter, one tile)
+// holds the tile's keys in registers, recounts them in LDS and reserves one
+// range per child with a single atomic; the ranges are kept as 32-bit
+// offsets from the segment start (segments below 2^32 keys).  MODE 2
+// (scatter, chunked) reserves from the stored per-chunk counts.
+template <int W, int MODE>
 __global__ __launch_bounds__(ST) void k_split(SplitArgs a) {
     constexpr int IPT = SplitCfg<W>::IPT, TK = SplitCfg<W>::TK;
     const uint64_t TKL = (uint64_t)TK * a.chunk;
     __shared__ uint32_t hist[1 << MAX_DBITS];
-    __shared__ uint64_t cbase[SCATTER ? (1 << MAX_DBITS) : 1];
+    __shared__ uint32_t cbase32[MODE == 1 ? (1 << MAX_DBITS) : 1];
+    __shared__ uint64_t cbase[MODE == 2 ? (1 << MAX_DBITS) : 1];
     const int tid = threadIdx.x;
     const uint64_t b = xcd_tile(blockIdx.x, gridDim.x);
     uint32_t lo = 0, hi = a.nseg;  // largest s with tile_pre[s] <= b
@@ -353,7 +381,7 @@ __global__ __launch_bounds__(ST) void k_split(SplitArgs a) {
     const uint64_t n = std::min<uint64_t>(TKL, sg.size - t * TKL);
     const uint32_t nd = 1u << a.dbits;
     uint32_t *row = a.chunk > 1 ? a.wg_hist + b * nd : nullptr;  // chunked: per-workgroup counts
-    if constexpr (!SCATTER) {
+    if constexpr (MODE == 0) {
         for (uint32_t g = tid; g < nd; g += ST) hist[g] = 0;
         __syncthreads();
         for (uint64_t c0 = 0; c0 < n; c0 += TK) {
@@ -375,10 +403,10 @@ __global__ __launch_bounds__(ST) void k_split(SplitArgs a) {
             if (a.chunk > 1) row[g] = c;  // one-tile chunks recount in the scatter pass
             if (c) atomicAdd(&a.child_hist[sg.child_base + g], c);
         }
-    } else if (a.chunk == 1) {
-        // one tile: its keys stay in registers between the count and the scatter
+    } else if constexpr (MODE == 1) {
+        // one tile: its keys stay in registers between the count and the
+        // scatter; digits are recomputed rather than held
         Key<W> key[IPT];
-        uint32_t dig[IPT];
         for (uint32_t g = tid; g < nd; g += ST) hist[g] = 0;
         __syncthreads();
 #pragma unroll
@@ -388,15 +416,14 @@ __global__ __launch_bounds__(ST) void k_split(SplitArgs a) {
                 const uint64_t *p = a.src + (first + i) * W;
 #pragma unroll
                 for (int r = 0; r < W; r++) key[it].w[r] = p[r];
-                dig[it] = key_bits<W>(key[it], a.shift, a.dbits);
-                atomicAdd(&hist[dig[it]], 1u);
+                atomicAdd(&hist[key_bits<W>(key[it], a.shift, a.dbits)], 1u);
             }
         }
         __syncthreads();
         for (uint32_t g = tid; g < nd; g += ST) {
             const uint32_t c = hist[g];
-            if (c) cbase[g] = atomicAdd((unsigned long long *)&a.child_cur[sg.child_base + g],
-                                        (unsigned long long)c);
+            if (c) cbase32[g] = (uint32_t)(atomicAdd((unsigned long long *)&a.child_cur[sg.child_base + g],
+                                                     (unsigned long long)c) - sg.start);
             hist[g] = 0;
         }
         __syncthreads();
@@ -404,7 +431,8 @@ __global__ __launch_bounds__(ST) void k_split(SplitArgs a) {
         for (int it = 0; it < IPT; it++) {
             uint64_t i = (uint64_t)it * ST + tid;
             if (i < n) {
-                uint64_t slot = cbase[dig[it]] + atomicAdd(&hist[dig[it]], 1u);
+                const uint32_t dg = key_bits<W>(key[it], a.shift, a.dbits);
+                const uint64_t slot = sg.start + cbase32[dg] + atomicAdd(&hist[dg], 1u);
                 uint64_t *d = a.dst + slot * W;
 #pragma unroll
                 for (int r = 0; r < W; r++) d[r] = key[it].w[r];
@@ -593,6 +621,11 @@ __global__ __launch_bounds__(BT) void k_bucket_sort(uint64_t *__restrict__ A,
 
 constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v / 2); }
 
+// Waves per SIMD that the LDS footprint of a hash-sort workgroup allows
+// (about 20 B per item: keys, counts, two table slots); the register budget
+// is capped to match, so registers do not become the tighter limit.
+constexpr int hash_sort_waves(int nt) { return nt <= 256 ? 7 : (nt <= 512 ? 6 : 4); }
+
 // Hash-then-bin variant: every key is inserted into an LDS open-addressing
 // table (a slot holds the first position of its key, claimed by CAS) and
 // duplicates add to that position's count.  The unique keys, carrying their
@@ -603,7 +636,7 @@ constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v / 2); }
 // whole bucket through the LSD passes instead.  Output equals
 // k_bucket_sort's: ascending unique keys with their run lengths.
 template <int W, int IPT, int NT>
-__global__ __launch_bounds__(NT) void k_bucket_hash_sort(uint64_t *__restrict__ A,
+__global__ __launch_bounds__(NT, hash_sort_waves(NT)) void k_bucket_hash_sort(uint64_t *__restrict__ A,
                                                           uint64_t *__restrict__ B,
                                                           uint32_t *__restrict__ counts_tmp,
                                                           const Bucket *__restrict__ buckets,
@@ -1238,7 +1271,9 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
     hipStream_t s = stream_;
     const uint32_t P = 1u << pbits;
     const int CAP = sort_cap(W);
-    const uint32_t spl = (uint32_t)std::max(1, tun.split_chunk);
+    // the one-tile scatter keeps 32-bit offsets inside a segment, which the
+    // input length bounds (every k-mer ends on its own FASTQ byte)
+    const uint32_t spl = (uint32_t)std::max(len > 0xFFFFFFFFull ? 2 : 1, tun.split_chunk);
     const uint64_t TK = (uint64_t)split_tk(W) * spl;  // keys per split workgroup
     EventTimer tm(timing_, s);
     tm.mark();
@@ -1383,11 +1418,12 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
         sa.tile_pre = tpre;
         sa.shift = kbits - prefix - d;
         sa.dbits = d;
-        sa.chunk = spl;
+        const uint32_t chunk = spl;
+        sa.chunk = chunk;
         sa.child_hist = ch;
-        sa.wg_hist = spl > 1 ? static_cast<uint32_t *>(wg_hist_.ensure(tiles_total * nd * 4)) : nullptr;
+        sa.wg_hist = chunk > 1 ? static_cast<uint32_t *>(wg_hist_.ensure(tiles_total * nd * 4)) : nullptr;
         launch("k_split_hist", split_keys * 8 * W,
-               [&] { k_split<W, false><<<dim3((unsigned)tiles_total), dim3(ST), 0, s>>>(sa); });
+               [&] { k_split<W, 0><<<dim3((unsigned)tiles_total), dim3(ST), 0, s>>>(sa); });
         uint64_t *ccur = static_cast<uint64_t *>(child_cur_.ensure(nchild * 8));
         uint32_t *nzr = static_cast<uint32_t *>(nz_rank_.ensure(nchild * 4));
         uint32_t *nzc = static_cast<uint32_t *>(nz_count_.ensure(nact * 4));
@@ -1412,7 +1448,10 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
         KITE_CHECK_LAUNCH();
         sa.child_cur = ccur;
         launch("k_split_scatter", split_keys * 16 * W,
-               [&] { k_split<W, true><<<dim3((unsigned)tiles_total), dim3(ST), 0, s>>>(sa); });
+               [&] {
+                   if (chunk == 1) k_split<W, 1><<<dim3((unsigned)tiles_total), dim3(ST), 0, s>>>(sa);
+                   else k_split<W, 2><<<dim3((unsigned)tiles_total), dim3(ST), 0, s>>>(sa);
+               });
         levels_++;
         nn = (uint32_t)nn_next;
         cur_list = nxt;

@@ -124,9 +124,14 @@ __global__ void k_chunk_meta(const uint64_t *__restrict__ part_off, uint32_t P, 
     }
 }
 
+__global__ void k_put_host_u64(const uint64_t *__restrict__ src, uint32_t n, uint64_t *__restrict__ dst) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
 template <int W>
-void encode_w(const uint64_t *keys, const uint32_t *counts, const uint64_t *part_off, uint64_t n, uint32_t P,
-              hipStream_t s, EncodeBufs &bufs, EncodedRuns &out) {
+EncodeDevice encode_w(const uint64_t *keys, const uint32_t *counts, const uint64_t *part_off, uint64_t n, uint32_t P,
+                      hipStream_t s, EncodeBufs &bufs, uint64_t *host_byte_off) {
     const unsigned T = 256;
     const dim3 gn((unsigned)std::max<uint64_t>(1, ceil_div(n, T))), gp((unsigned)ceil_div((uint64_t)P + 1, T));
     uint8_t *head = static_cast<uint8_t *>(bufs.head.ensure(std::max<uint64_t>(n, 1)));
@@ -146,23 +151,30 @@ void encode_w(const uint64_t *keys, const uint32_t *counts, const uint64_t *part
     scan_to_u64(len, off, n, tmp, s);
     k_chunk_meta<W><<<gp, T, 0, s>>>(part_off, P, off, keys, boff, mm);
     KITE_CHECK_LAUNCH();
-    out.byte_off.resize((size_t)P + 1);
-    KITE_HIP(hipMemcpyAsync(out.byte_off.data(), boff, ((size_t)P + 1) * 8, hipMemcpyDeviceToHost, s));
+    // the offsets go to the host through a kernel write, not a copy engine
+    k_put_host_u64<<<gp, T, 0, s>>>(boff, P + 1, host_byte_off);
+    KITE_CHECK_LAUNCH();
     KITE_HIP(hipStreamSynchronize(s));
-    const uint64_t total = out.byte_off[P];
-    uint8_t *bytes = static_cast<uint8_t *>(bufs.bytes.ensure(std::max<uint64_t>(total, 1)));
+    EncodeDevice ed;
+    ed.total = host_byte_off[P];
+    uint8_t *bytes = static_cast<uint8_t *>(bufs.bytes.ensure(std::max<uint64_t>(ed.total, 1)));
     if (n) {
         k_entry_write<W><<<gn, T, 0, s>>>(keys, counts, head, off, n, bytes);
         KITE_CHECK_LAUNCH();
     }
-    out.bytes.resize(total);
-    out.minmax.resize((size_t)P * 8);
-    if (total) KITE_HIP(hipMemcpyAsync(out.bytes.data(), bytes, total, hipMemcpyDeviceToHost, s));
-    KITE_HIP(hipMemcpyAsync(out.minmax.data(), mm, (size_t)P * 64, hipMemcpyDeviceToHost, s));
-    KITE_HIP(hipStreamSynchronize(s));
+    ed.bytes = bytes;
+    ed.minmax = mm;
+    return ed;
 }
 
 }  // namespace
+
+EncodeDevice encode_on_device(const uint64_t *d_keys, const uint32_t *d_counts, const uint64_t *d_part_off, uint64_t n,
+                              uint32_t P, int W, hipStream_t s, EncodeBufs &bufs, uint64_t *host_byte_off) {
+    if (W == 1) return encode_w<1>(d_keys, d_counts, d_part_off, n, P, s, bufs, host_byte_off);
+    if (W == 2) return encode_w<2>(d_keys, d_counts, d_part_off, n, P, s, bufs, host_byte_off);
+    return encode_w<4>(d_keys, d_counts, d_part_off, n, P, s, bufs, host_byte_off);
+}
 
 void encode_runs_device(const uint64_t *d_keys, const uint32_t *d_counts, const uint64_t *d_part_off,
                         const std::vector<uint64_t> &part_off, uint32_t P, int W, hipStream_t s, EncodeBufs &bufs,
@@ -171,9 +183,14 @@ void encode_runs_device(const uint64_t *d_keys, const uint32_t *d_counts, const 
     out.rows_off = part_off;
     out.codec = W == 4 ? 0x01u : 0x02u;
     out.words = W;
-    if (W == 1) encode_w<1>(d_keys, d_counts, d_part_off, n, P, s, bufs, out);
-    else if (W == 2) encode_w<2>(d_keys, d_counts, d_part_off, n, P, s, bufs, out);
-    else encode_w<4>(d_keys, d_counts, d_part_off, n, P, s, bufs, out);
+    uint64_t *hb = static_cast<uint64_t *>(bufs.host_byte_off.ensure(((size_t)P + 1) * 8));
+    const EncodeDevice ed = encode_on_device(d_keys, d_counts, d_part_off, n, P, W, s, bufs, hb);
+    out.byte_off.assign(hb, hb + P + 1);
+    out.bytes.resize(ed.total);
+    out.minmax.resize((size_t)P * 8);
+    if (ed.total) KITE_HIP(hipMemcpyAsync(out.bytes.data(), ed.bytes, ed.total, hipMemcpyDeviceToHost, s));
+    KITE_HIP(hipMemcpyAsync(out.minmax.data(), ed.minmax, (size_t)P * 64, hipMemcpyDeviceToHost, s));
+    KITE_HIP(hipStreamSynchronize(s));
 }
 
 }  // namespace kite

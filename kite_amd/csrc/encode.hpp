@@ -9,8 +9,45 @@
 
 namespace kite {
 
+// Page-locked host buffer that grows on demand.
+struct HostPinned {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    HostPinned() = default;
+    HostPinned(const HostPinned &) = delete;
+    HostPinned &operator=(const HostPinned &) = delete;
+    ~HostPinned() { release(); }
+    void release() {
+        if (ptr) (void)hipHostFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    void *ensure(size_t need) {
+        if (need == 0) need = 256;
+        if (need > bytes) {
+            release();
+            const size_t want = need + need / 8;
+            KITE_HIP(hipHostMalloc(&ptr, want, hipHostMallocDefault));
+            bytes = want;
+        }
+        return ptr;
+    }
+    template <class T>
+    T *as() const { return static_cast<T *>(ptr); }
+};
+
 struct EncodeBufs {
     DevBuf head, len, off, tmp, byte_off, minmax, bytes;
+    HostPinned host_byte_off;
+};
+
+// Device result of an encode: partition p's payload is bytes[byte_off[p],
+// byte_off[p + 1]) (byte_off on the host, written by the encoder), the key
+// range minmax[8p .. 8p+7] (device).
+struct EncodeDevice {
+    const uint8_t *bytes = nullptr;
+    const uint64_t *minmax = nullptr;
+    uint64_t total = 0;
 };
 
 // One payload per partition, back to back: partition p is
@@ -23,6 +60,12 @@ struct EncodedRuns {
     uint32_t codec = 0;
     int words = 1;
 };
+
+// Runs the encode kernels on stream s and returns with the payload bytes and
+// key ranges on the device; host_byte_off (partitions + 1 entries, page-locked)
+// receives the byte offsets.  The stream is synchronised once, for the total.
+EncodeDevice encode_on_device(const uint64_t *d_keys, const uint32_t *d_counts, const uint64_t *d_part_off, uint64_t n,
+                              uint32_t P, int W, hipStream_t s, EncodeBufs &bufs, uint64_t *host_byte_off);
 
 // d_keys / d_counts: n reduced rows on the device, partition-ordered;
 // d_part_off: the partitions + 1 row offsets on the device (part_off the same

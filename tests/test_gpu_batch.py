@@ -96,3 +96,32 @@ def test_batch_callback_abort(gpu_ctx):
         gpu_ctx.count_batch(arrs, 31, True, 300, mode=BATCH_PIPELINED)
     for p in pins:
         p.free()
+
+
+@pytest.mark.parametrize("mode", [BATCH_PLAIN, BATCH_PIPELINED])
+def test_batch_encoded_chunks(gpu_ctx, mode):
+    """Chunk payloads of every sample, encoded on the device, equal the
+    oracle encoder's bytes on the single-sample tables (k = 31 and 63)."""
+    ws, pins = _samples(0.003)
+    arrs = [p.array for p in pins[:5]] + [np.zeros(0, dtype=np.uint8)]
+    for k, P in ((31, 256), (63, 16)):
+        single = [gpu_ctx.count(a, k, True, P)[0] for a in arrs]
+        res = gpu_ctx.count_batch(arrs, k, True, P, mode=mode, encoded=True)
+        for i, (codec, data, boff, rows, mm, st) in enumerate(res):
+            r = single[i]
+            assert codec == 2
+            assert np.array_equal(rows, r.part_off)
+            assert st["unique"] == len(r.counts)
+            for p in range(P):
+                a, b = int(r.part_off[p]), int(r.part_off[p + 1])
+                chunk = data[int(boff[p]):int(boff[p + 1])]
+                if a == b:
+                    assert len(chunk) == 0, f"k={k} sample={i} partition={p}"
+                    continue
+                # a boolean keeps pytest from diffing megabytes on failure
+                same = chunk == ol.dvle_encode(r.keys[a:b], r.counts[a:b], r.words)
+                assert same, f"k={k} sample={i} partition={p}"
+                assert [int(x) for x in mm[p, :r.words]] == [int(x) for x in r.keys[a]]
+                assert [int(x) for x in mm[p, 4:4 + r.words]] == [int(x) for x in r.keys[b - 1]]
+    for p in pins:
+        p.free()

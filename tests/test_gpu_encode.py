@@ -34,7 +34,8 @@ def test_emitted_chunks_match_oracle_encoder(gpu_ctx, k, canon, P):
         assert rows == b - a and rows > 0
         assert codec == (1 if W == 4 else 2)
         assert raw == len(data)
-        assert data == ol.dvle_encode(res.keys[a:b], res.counts[a:b], W)
+        same = data == ol.dvle_encode(res.keys[a:b], res.counts[a:b], W)
+        assert same, f"partition {part}"
         assert list(kmin[:W]) == [int(x) for x in res.keys[a]] and all(v == 0 for v in kmin[W:])
         assert list(kmax[:W]) == [int(x) for x in res.keys[b - 1]] and all(v == 0 for v in kmax[W:])
         seen.add(part)
