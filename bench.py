@@ -235,6 +235,84 @@ def bench_analyze(ctx, tables: list, min_count: int, cpu_rows_frac_parts: int) -
     return out
 
 
+def bench_dump(w, fq) -> dict:
+    """`kite dump` of the workload's .kite (written by `kite count` on the
+    GPU): the device engine against the host engine (partition workers on
+    this host's cores), wall time to /dev/null, and a hash of each engine's
+    full text (bit-exact check at full size)."""
+    import re
+    import subprocess
+    import tempfile
+    try:
+        import xxhash
+        mkhash = xxhash.xxh3_128
+    except ImportError:  # pragma: no cover
+        import hashlib
+        mkhash = hashlib.sha256
+    kite_bin = ROOT / "kite_amd" / "lib" / "kite"
+    tmp = Path(tempfile.mkdtemp(prefix="kite_dump_", dir="/tmp"))
+    try:
+        fq_path = tmp / "r.fq"
+        fq.tofile(fq_path)
+        kp = tmp / "c.kite"
+        r = subprocess.run([str(kite_bin), "count", "-i", str(fq_path), "-o", str(kp), "-k", str(w.k),
+                            "--canonical", "true" if w.canonical else "false", "--partitions", str(w.partitions)],
+                           capture_output=True, text=True, timeout=600)
+        if r.returncode:
+            raise RuntimeError(f"kite count failed: {r.stderr[-2000:]}")
+        fq_path.unlink()
+        threads = min(16, os.cpu_count() or 1)
+        out = {"kite_bytes": kp.stat().st_size, "host_threads": threads}
+        for engine in ("gpu", "host"):
+            env = dict(os.environ, KITE_DUMP_ENGINE=engine)
+            cmd = [str(kite_bin), "dump", "-i", str(kp), "-t", str(threads)]
+            best = None
+            for _ in range(2):
+                t0 = time.perf_counter()
+                r = subprocess.run(cmd + ["-o", "/dev/null"], capture_output=True, text=True, env=env, timeout=600)
+                wall = time.perf_counter() - t0
+                if r.returncode:
+                    raise RuntimeError(f"kite dump ({engine}) failed: {r.stderr[-2000:]}")
+                best = wall if best is None else min(best, wall)
+            e = {"wall_s": best}
+            m = re.search(r"gpu: (\d+) runs, (\d+) rows, (\d+) lines, (\d+) bytes in (\d+) group.*decode ([\d.]+) ms, "
+                          r"merge ([\d.]+) ms, format ([\d.]+) ms; text ([\d.]+) ms", r.stderr)
+            if m:
+                e.update({"rows": int(m.group(2)), "lines": int(m.group(3)), "text_bytes": int(m.group(4)),
+                          "groups": int(m.group(5)), "decode_ms": float(m.group(6)), "merge_ms": float(m.group(7)),
+                          "format_ms": float(m.group(8)), "text_ms": float(m.group(9))})
+            # full text hash through a pipe
+            h = mkhash()
+            p = subprocess.Popen(cmd + ["-o", "-"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, env=env)
+            n = 0
+            while True:
+                b = p.stdout.read(16 << 20)
+                if not b:
+                    break
+                n += len(b)
+                h.update(b)
+            if p.wait(timeout=600):
+                raise RuntimeError(f"kite dump ({engine}) to stdout failed")
+            e["hash"] = h.hexdigest()
+            e["stdout_bytes"] = n
+            out[engine] = e
+            log(f"dump {engine}: {best:.2f} s wall" + (f", device {e['decode_ms'] + e['merge_ms'] + e['format_ms']:.1f} ms"
+                                                      if m else ""))
+        g, hst = out["gpu"], out["host"]
+        out["bit_exact_gpu_vs_host"] = g["hash"] == hst["hash"] and g["stdout_bytes"] == hst["stdout_bytes"]
+        rows = g.get("rows", 0)
+        out["rows_per_s"] = {"gpu_wall": rows / g["wall_s"], "host_wall": rows / hst["wall_s"],
+                             "gpu_device": rows / ((g["decode_ms"] + g["merge_ms"] + g["format_ms"]) * 1e-3)
+                             if rows else None}
+        out["speedup_wall"] = hst["wall_s"] / g["wall_s"]
+        out["includes"] = ("wall: process start, .kite read, decode, merge, format, text to /dev/null; "
+                           "device: decode + merge + format kernels")
+        return out
+    finally:
+        import shutil
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -247,6 +325,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 24-sample batch and analyze legs")
     ap.add_argument("--no-analyze", action="store_true")
+    ap.add_argument("--no-dump", action="store_true", help="skip the kite dump leg")
     ap.add_argument("--batch-scale", type=float, default=1.0, help="scale of the 24 read counts")
     ap.add_argument("--batch-reps", type=int, default=2)
     ap.add_argument("--analyze-cpu-parts", type=int, default=8,
@@ -337,6 +416,10 @@ def main() -> int:
 
     traffic, traffic_src = traffic_from_profile(dom_name, w.name)
 
+    dump = None
+    if not args.no_dump:
+        dump = bench_dump(w, fq)
+
     batch = analyze = None
     if not args.no_batch:
         del dev
@@ -378,6 +461,7 @@ def main() -> int:
         "kernels": kernels,
         "batch24": batch,
         "analyze": analyze,
+        "dump": dump,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define KITE_GPU_ABI_VERSION 2
+#define KITE_GPU_ABI_VERSION 3
 
 enum {
     KITE_OK = 0,
@@ -249,6 +249,38 @@ int kite_gpu_count_profile(kite_gpu_ctx *ctx, int enable);
  * {"<kernel>": {"ms": total, "launches": n, "bytes": algorithmic}, ...}.
  * Returns the JSON length, or a negative code. */
 int kite_gpu_count_kernel_stats(kite_gpu_ctx *ctx, char *buf, size_t cap);
+
+/* ---- dump -------------------------------------------------------------------
+ * Replaces the partition workers of `kite dump` (stream_partition_slabs,
+ * src/dump/merge.rs:282-428, driven by src/dump/mod.rs:293-348): the stored
+ * runs of every partition are decoded (DVLE / RAW, src/dump/merge.rs:13-240),
+ * merged in key order with u32 saturating sums (src/dump/loser_tree.rs:
+ * 141-175; 256-bit keys: sort + RLE, src/dump/merge.rs:311-371, which equals
+ * the merge for the sorted unique runs writers produce; other 256-bit runs
+ * are refused), filtered by min <= count <= max (src/kiteops/mod.rs:44-56)
+ * and formatted as "KMER\tCOUNT\n" (src/dump/decoder.rs:27-117), partitions
+ * in order 0..P-1.  A DVLE varint without its end byte within 10 (64-bit) or
+ * 19 (128-bit) bytes, or a payload shorter than its rows, is an error. */
+typedef struct kite_dump_run {
+    uint32_t partition;   /* non-decreasing over the array */
+    uint32_t codec;       /* KITE_CODEC_DVLE or KITE_CODEC_RAW; zstd removed by the caller */
+    uint64_t rows;
+    const uint8_t *data;  /* payload, borrowed for the call */
+    uint64_t bytes;
+} kite_dump_run;
+
+typedef struct kite_dump_stats {
+    uint64_t runs, rows_in, rows, lines, bytes, groups;
+    double decode_ms, merge_ms, format_ms;  /* device time per stage */
+    double copy_ms;  /* text phase: D2H pieces overlapped with the writer callback */
+} kite_dump_stats;
+
+/* Receives the text in order, in pieces; non-zero aborts the dump. */
+typedef int (*kite_dump_write_fn)(void *user, const char *text, uint64_t len);
+
+int kite_gpu_dump(kite_gpu_ctx *ctx, uint32_t k, uint32_t key_width, uint32_t partitions, const kite_dump_run *runs,
+                  uint64_t n_runs, int has_min, uint32_t min_count, int has_max, uint32_t max_count,
+                  kite_dump_write_fn fn, void *user, kite_dump_stats *stats);
 
 #ifdef __cplusplus
 }

@@ -69,6 +69,23 @@ BATCH_CHUNKS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint32, C.POINTER(CountSt
                               C.POINTER(C.c_uint64))
 BATCH_PLAIN, BATCH_PIPELINED = 0, 1
 
+
+class DumpRun(C.Structure):
+    _fields_ = [("partition", C.c_uint32), ("codec", C.c_uint32), ("rows", C.c_uint64),
+                ("data", C.c_void_p), ("bytes", C.c_uint64)]
+
+
+class DumpStats(C.Structure):
+    _fields_ = [("runs", C.c_uint64), ("rows_in", C.c_uint64), ("rows", C.c_uint64), ("lines", C.c_uint64),
+                ("bytes", C.c_uint64), ("groups", C.c_uint64), ("decode_ms", C.c_double),
+                ("merge_ms", C.c_double), ("format_ms", C.c_double), ("copy_ms", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+DUMP_WRITE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_char), C.c_uint64)
+
 # name -> (restype, argtypes); every symbol include/kite_gpu.h declares
 GPU_SYMBOLS = {
     "kite_gpu_abi_version": (C.c_int, []),
@@ -107,6 +124,9 @@ GPU_SYMBOLS = {
     "kite_gpu_count_batch_chunks": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_uint32,
                                               C.POINTER(BatchSample), C.c_uint32, C.c_int, BATCH_CHUNKS_FN,
                                               C.c_void_p]),
+    "kite_gpu_dump": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(DumpRun),
+                                C.c_uint64, C.c_int, C.c_uint32, C.c_int, C.c_uint32, DUMP_WRITE_FN,
+                                C.c_void_p, C.POINTER(DumpStats)]),
     "kite_gpu_host_alloc": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
     "kite_gpu_host_free": (None, [C.c_void_p]),
     "kite_gpu_count_profile": (C.c_int, [C.c_void_p, C.c_int]),
@@ -432,6 +452,44 @@ class GpuContext:
             raise errors[0]
         _check(rc)
         return out
+
+    def dump_runs(self, k: int, key_width: int, partitions: int, runs, min_count=None, max_count=None,
+                  write=None) -> tuple[bytes, dict]:
+        """`kite dump` of stored runs on the device (kite_gpu_dump).  runs:
+        (partition, codec, rows, payload bytes) in partition order.  The
+        text is returned, or passed piece by piece to write(memoryview)."""
+        arr = (DumpRun * max(len(runs), 1))()
+        keep = []
+        for i, (part, codec, rows, data) in enumerate(runs):
+            buf = np.frombuffer(bytes(data), dtype=np.uint8) if len(data) else np.zeros(1, np.uint8)
+            keep.append(buf)
+            arr[i].partition, arr[i].codec, arr[i].rows = part, codec, rows
+            arr[i].data = buf.ctypes.data
+            arr[i].bytes = len(data)
+        pieces: list = []
+        errors: list = []
+
+        def cb(user, text, n):
+            try:
+                chunk = C.string_at(text, n)
+                if write is None:
+                    pieces.append(chunk)
+                else:
+                    write(chunk)
+                return 0
+            except Exception as e:  # pragma: no cover - surfaced after the call
+                errors.append(e)
+                return 1
+
+        fn = DUMP_WRITE_FN(cb)
+        st = DumpStats()
+        rc = self._lib.kite_gpu_dump(self._h, k, key_width, partitions, arr, len(runs),
+                                     int(min_count is not None), int(min_count or 0),
+                                     int(max_count is not None), int(max_count or 0), fn, None, C.byref(st))
+        if errors:
+            raise errors[0]
+        _check(rc)
+        return b"".join(pieces), st.as_dict()
 
     def count_profile(self, enable: bool) -> None:
         _check(self._lib.kite_gpu_count_profile(self._h, int(enable)))

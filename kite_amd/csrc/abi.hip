@@ -15,6 +15,7 @@
 #include "../../include/kite_gpu.h"
 #include "count.hpp"
 #include "corr.hpp"
+#include "dump.hpp"
 #include "encode.hpp"
 #include "merge.hpp"
 
@@ -94,6 +95,7 @@ struct kite_gpu_ctx {
     std::unique_ptr<kite::CountEngine> count;
     std::unique_ptr<kite::MergeEngine> merge;
     std::unique_ptr<kite::CorrEngine> corr;
+    std::unique_ptr<kite::DumpEngine> dump;
 
     // count session
     bool session = false;
@@ -114,6 +116,10 @@ struct kite_gpu_ctx {
     kite::DevBuf batch_in[2];
     BatchSlot slots[2];
     kite::EncodeBufs enc_slot[2];  // device encode buffers of the two result slots
+
+    // dump: staged payloads of a partition group, pinned text pieces
+    kite::HostPinned dump_stage, dump_text;
+    kite::DevBuf dump_pay;
 };
 
 namespace {
@@ -218,6 +224,7 @@ int kite_gpu_open(int device, uint64_t hbm_budget_bytes, kite_gpu_ctx **out) {
         c->count = std::make_unique<kite::CountEngine>(c->stream);
         c->merge = std::make_unique<kite::MergeEngine>(c->stream);
         c->corr = std::make_unique<kite::CorrEngine>(c->stream);
+        c->dump = std::make_unique<kite::DumpEngine>(c->stream, c->merge.get());
         *out = c.release();
         return KITE_OK;
     });
@@ -229,9 +236,13 @@ void kite_gpu_close(kite_gpu_ctx *ctx) {
         bind(ctx);
         (void)hipStreamSynchronize(ctx->stream);
         ctx->count.reset();
+        ctx->dump.reset();
         ctx->merge.reset();
         ctx->corr.reset();
         ctx->staging.release();
+        ctx->dump_stage.release();
+        ctx->dump_text.release();
+        ctx->dump_pay.release();
         for (int b = 0; b < 2; b++) {
             ctx->batch_in[b].release();
             ctx->slots[b].keys.release();
@@ -908,6 +919,142 @@ int kite_gpu_host_alloc(uint64_t bytes, void **out) {
 
 void kite_gpu_host_free(void *p) {
     if (p) (void)hipHostFree(p);
+}
+
+
+int kite_gpu_dump(kite_gpu_ctx *ctx, uint32_t k, uint32_t key_width, uint32_t partitions, const kite_dump_run *runs,
+                  uint64_t n_runs, int has_min, uint32_t min_count, int has_max, uint32_t max_count,
+                  kite_dump_write_fn fn, void *user, kite_dump_stats *stats) {
+    if (!ctx || !fn) return set_error(KITE_E_INVALID, "invalid argument: NULL pointer");
+    if (n_runs && !runs) return set_error(KITE_E_INVALID, "invalid argument: runs is NULL");
+    if (key_width != 64 && key_width != 128 && key_width != 256)
+        return set_error(KITE_E_INVALID, "invalid key width " + std::to_string(key_width));
+    const int W = (int)(key_width / 64);
+    if (k < 1 || k > 32u * (uint32_t)W)
+        return set_error(KITE_E_INVALID, "invalid k " + std::to_string(k) + " for key width " +
+                                             std::to_string(key_width));
+    const uint64_t es = 8 * (uint64_t)W + 4;
+    for (uint64_t i = 0; i < n_runs; i++) {
+        const kite_dump_run &r = runs[i];
+        if (r.partition >= partitions) return set_error(KITE_E_INVALID, "invalid run: partition out of range");
+        if (i && r.partition < runs[i - 1].partition)
+            return set_error(KITE_E_INVALID, "invalid run order: partitions must be non-decreasing");
+        if (r.codec != KITE_CODEC_DVLE && r.codec != KITE_CODEC_RAW)
+            return set_error(KITE_E_INVALID, "invalid run codec " + std::to_string(r.codec) +
+                                                 " (zstd payloads are decoded by the caller)");
+        if (r.bytes && !r.data) return set_error(KITE_E_INVALID, "invalid run: data is NULL");
+        const bool raw = W == 4 || r.codec == KITE_CODEC_RAW;
+        if (r.rows && raw && r.bytes / es < r.rows)
+            return set_error(KITE_E_INVALID, "invalid run: raw row payload truncated");
+        if (r.rows && !raw && r.bytes < es)
+            return set_error(KITE_E_INVALID, W == 2 ? "invalid run: u128 DVLE payload too small"
+                                                    : "invalid run: u64 DVLE payload too small");
+    }
+    return guarded([&]() -> int {
+        bind(ctx);
+        kite_dump_stats st{};
+        kite::DumpFilter f;
+        f.has_min = has_min ? 1u : 0u;
+        f.min = min_count;
+        f.has_max = has_max ? 1u : 0u;
+        f.max = max_count;
+        // partition groups bounded by staged bytes and rows
+        // (KITE_DUMP_GROUP_ROWS lowers the row bound, to exercise many groups)
+        constexpr uint64_t PAY_CAP = 1ull << 30;
+        uint64_t ROW_CAP = 96ull << 20;
+        if (const char *e = getenv("KITE_DUMP_GROUP_ROWS")) ROW_CAP = std::max<uint64_t>(1, strtoull(e, nullptr, 10));
+        constexpr uint64_t PIECE = 64ull << 20;  // text bytes per D2H piece
+        hipStream_t s = ctx->stream;
+        uint64_t i = 0;
+        while (i < n_runs) {
+            uint64_t j = i, pay = 0, rows = 0;
+            while (j < n_runs) {
+                // the whole partition of run j joins, or the group ends here
+                uint64_t e = j, pb = 0, pr = 0;
+                while (e < n_runs && runs[e].partition == runs[j].partition) {
+                    pb += (runs[e].bytes + 15) & ~15ull;
+                    pr += runs[e].rows;
+                    e++;
+                }
+                if (j > i && (pay + pb > PAY_CAP || rows + pr > ROW_CAP)) break;
+                pay += pb;
+                rows += pr;
+                j = e;
+            }
+            const uint32_t p0 = runs[i].partition, np = runs[j - 1].partition - p0 + 1;
+            uint8_t *host = static_cast<uint8_t *>(ctx->dump_stage.ensure(pay + 16));
+            std::vector<kite::DumpRun> dr;
+            dr.reserve(j - i);
+            uint64_t off = 0;
+            uint32_t rank = 0;
+            for (uint64_t q = i; q < j; q++) {
+                const kite_dump_run &r = runs[q];
+                rank = (q > i && r.partition == runs[q - 1].partition) ? rank + 1 : 0;
+                if (r.bytes) memcpy(host + off, r.data, r.bytes);
+                kite::DumpRun d{};
+                d.src = off;
+                d.bytes = r.bytes;
+                d.rows = r.rows;
+                d.codec = (W == 4 || r.codec == KITE_CODEC_RAW) ? kite::DUMP_CODEC_RAW : kite::DUMP_CODEC_DVLE;
+                d.part = r.partition - p0;
+                d.table = rank;
+                dr.push_back(d);
+                off += (r.bytes + 15) & ~15ull;
+            }
+            memset(host + off, 0, 16);
+            uint8_t *dev = static_cast<uint8_t *>(ctx->dump_pay.ensure(pay + 16));
+            KITE_HIP(hipMemcpyAsync(dev, host, pay + 16, hipMemcpyHostToDevice, s));
+            uint64_t out_rows = 0, lines = 0;
+            kite::DumpTimes t;
+            const uint64_t bytes = ctx->dump->run(dev, dr, np, (int)k, W, f, &out_rows, &lines, &t);
+            st.decode_ms += t.decode_ms;
+            st.merge_ms += t.merge_ms;
+            st.format_ms += t.format_ms;
+            // text to the caller in pieces, the next piece copied while the
+            // caller takes the current one
+            char *pin = static_cast<char *>(ctx->dump_text.ensure(2 * PIECE));
+            hipEvent_t done[2], t0, t1;
+            for (auto *e : {&done[0], &done[1], &t0, &t1}) KITE_HIP(hipEventCreate(e));
+            struct Ev {
+                hipEvent_t *e[4];
+                ~Ev() {
+                    for (auto *x : e) (void)hipEventDestroy(*x);
+                }
+            } evs{{&done[0], &done[1], &t0, &t1}};
+            const char *text = ctx->dump->text();
+            const uint64_t pieces = kite::ceil_div(bytes, PIECE);
+            KITE_HIP(hipEventRecord(t0, s));
+            auto issue = [&](uint64_t q) {
+                const uint64_t a = q * PIECE, n = std::min<uint64_t>(PIECE, bytes - a);
+                KITE_HIP(hipMemcpyAsync(pin + (q & 1) * PIECE, text + a, n, hipMemcpyDeviceToHost, s));
+                KITE_HIP(hipEventRecord(done[q & 1], s));
+            };
+            if (pieces) issue(0);
+            for (uint64_t q = 0; q < pieces; q++) {
+                KITE_HIP(hipEventSynchronize(done[q & 1]));
+                if (q + 1 < pieces) issue(q + 1);
+                const uint64_t a = q * PIECE, n = std::min<uint64_t>(PIECE, bytes - a);
+                if (fn(user, pin + (q & 1) * PIECE, n) != 0) {
+                    (void)hipStreamSynchronize(s);
+                    return set_error(KITE_E_CALLBACK, "dump writer callback aborted");
+                }
+            }
+            KITE_HIP(hipEventRecord(t1, s));
+            KITE_HIP(hipStreamSynchronize(s));
+            float ms = 0;
+            KITE_HIP(hipEventElapsedTime(&ms, t0, t1));
+            st.copy_ms += ms;
+            st.runs += j - i;
+            st.rows_in += rows;
+            st.rows += out_rows;
+            st.lines += lines;
+            st.bytes += bytes;
+            st.groups++;
+            i = j;
+        }
+        if (stats) *stats = st;
+        return KITE_OK;
+    });
 }
 
 }  // extern "C"

@@ -1,0 +1,69 @@
+// dump.hpp -- `kite dump` on the device: stored runs of a group of
+// partitions are decoded (DVLE or RAW), the runs of each partition are
+// merged with u32-saturating sums, the count filter is applied and the
+// "KMER\tCOUNT\n" lines are formatted, in partition and key order.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "common.hpp"
+#include "merge.hpp"
+
+namespace kite {
+
+constexpr uint32_t DUMP_CODEC_RAW = 0x01u;
+constexpr uint32_t DUMP_CODEC_DVLE = 0x02u;
+
+// One stored run of a partition, staged in device memory.
+struct DumpRun {
+    uint64_t src;    // byte offset of the payload in the staged buffer (16-byte aligned)
+    uint64_t bytes;  // payload bytes (DVLE or RAW; zstd already removed)
+    uint64_t rows;
+    uint64_t row0;   // first row of this run in the decoded row space
+    uint64_t dst;    // first row of this run in its merge table
+    uint32_t codec;  // DUMP_CODEC_DVLE or DUMP_CODEC_RAW
+    uint32_t part;   // partition index within the group
+    uint32_t tile0;  // first byte tile of the run (DVLE)
+    uint32_t table;  // rank of the run among its partition's runs
+};
+
+struct DumpFilter {
+    uint32_t has_min = 0, min = 0, has_max = 0, max = 0;
+};
+
+struct DumpTimes {
+    float decode_ms = 0, merge_ms = 0, format_ms = 0;
+};
+
+class DumpEngine {
+   public:
+    DumpEngine(hipStream_t s, MergeEngine *merge) : stream_(s), merge_(merge) {}
+    DumpEngine(const DumpEngine &) = delete;
+    DumpEngine &operator=(const DumpEngine &) = delete;
+
+    // Decodes `runs` (payloads at d_payload, which has 16 readable bytes past
+    // the last payload), merges the runs of each of the `np` partitions,
+    // filters and formats the lines.  The text stays on the device (text(),
+    // returned byte count).  rows_out receives the rows after the merge,
+    // lines_out the rows that passed the filter.
+    uint64_t run(const uint8_t *d_payload, const std::vector<DumpRun> &runs, uint32_t np, int k, int W,
+                 const DumpFilter &f, uint64_t *rows_out, uint64_t *lines_out, DumpTimes *times);
+    const char *text() const { return text_.as<char>(); }
+    void release();
+
+   private:
+    template <int W>
+    uint64_t run_w(const uint8_t *d_payload, const std::vector<DumpRun> &runs, uint32_t np, int k,
+                   const DumpFilter &f, uint64_t *rows_out, uint64_t *lines_out, DumpTimes *times);
+
+    hipStream_t stream_;
+    MergeEngine *merge_;
+    DevBuf runs_d_, tile_run_, tcnt_, tbase_, ends_, vals_, counts_, scan_tmp_, tlen_, toff_, text_, err_,
+        tabptr_, lines_, loff_;
+    std::vector<DevTable> tabs_;
+    DevTable merged_;
+    uint32_t *host_err_ = nullptr;
+};
+
+}  // namespace kite

@@ -105,6 +105,49 @@ FILE *open_out(const std::string &path) {
     return f;
 }
 
+// The whole dump on the device (kite_gpu_dump): runs of every partition in
+// order, zstd payloads decoded here first; the text is written as it arrives.
+void dump_on_gpu(kite_gpu_ctx *g, const Reader &r, FILE *out) {
+    const Header &h = r.header();
+    const int W = r.words();
+    std::vector<kite_dump_run> runs;
+    std::vector<std::vector<uint8_t>> decoded;
+    runs.reserve(r.chunk_count());
+    for (uint32_t p = 0; p < h.partitions; p++) {
+        auto [first, cnt] = r.partition_chunks(p);
+        for (uint64_t c = first; c < first + cnt; c++) {
+            const ChunkMeta &m = r.chunks()[c];
+            kite_dump_run x{};
+            x.partition = p;
+            x.rows = m.rows;
+            x.codec = (W == 4 || !(m.codec & CODEC_DVLE)) ? KITE_CODEC_RAW : KITE_CODEC_DVLE;
+            if (m.codec & CODEC_ZSTD) {
+                decoded.push_back(r.payload(m));
+                x.data = decoded.back().data();
+                x.bytes = decoded.back().size();
+            } else {
+                x.data = r.raw_payload(m);
+                x.bytes = m.encoded;
+            }
+            runs.push_back(x);
+        }
+    }
+    auto write = [](void *user, const char *text, uint64_t len) -> int {
+        return fwrite(text, 1, len, static_cast<FILE *>(user)) == len ? 0 : 1;
+    };
+    kite_dump_stats st{};
+    if (kite_gpu_dump(g, h.k, h.key_width, h.partitions, runs.data(), runs.size(), h.min_count.has_value(),
+                      h.min_count.value_or(0), h.max_count.has_value(), h.max_count.value_or(0), write, out,
+                      &st) != KITE_OK)
+        throw Error(std::string("dump failed: ") + kite_gpu_last_error());
+    fprintf(stderr,
+            "[kite dump] gpu: %llu runs, %llu rows, %llu lines, %llu bytes in %llu group(s); device "
+            "decode %.1f ms, merge %.1f ms, format %.1f ms; text %.1f ms\n",
+            (unsigned long long)st.runs, (unsigned long long)st.rows, (unsigned long long)st.lines,
+            (unsigned long long)st.bytes, (unsigned long long)st.groups, st.decode_ms, st.merge_ms, st.format_ms,
+            st.copy_ms);
+}
+
 }  // namespace
 
 int run_dump(int argc, char **argv) {
@@ -122,6 +165,33 @@ int run_dump(int argc, char **argv) {
     fprintf(stderr, "[kite dump] k=%u, partitions=%u, total_chunks=%llu\n", h.k, h.partitions,
             (unsigned long long)r.chunk_count());
     FILE *out = open_out(a.get("output"));
+    // engine: the GPU when a device opens (KITE_DUMP_ENGINE=gpu requires
+    // it, KITE_DUMP_ENGINE=host keeps the partition workers on the CPU)
+    const char *ev = getenv("KITE_DUMP_ENGINE");
+    const std::string engine = ev ? ev : "auto";
+    if (engine != "auto" && engine != "gpu" && engine != "host")
+        throw UsageError("KITE_DUMP_ENGINE must be auto, gpu or host");
+    kite_gpu_ctx *gctx = nullptr;
+    if (engine != "host" && kite_gpu_open(0, 0, &gctx) != KITE_OK) {
+        gctx = nullptr;
+        if (engine == "gpu") throw Error(std::string("GPU dump engine unavailable: ") + kite_gpu_last_error());
+    }
+    if (gctx) {
+        fprintf(stderr, "[kite dump] engine: gpu\n");
+        std::string err;
+        try {
+            dump_on_gpu(gctx, r, out);
+        } catch (const std::exception &e) {
+            err = e.what();
+        }
+        kite_gpu_close(gctx);
+        if (out == stdout) fflush(out);
+        else fclose(out);
+        if (!err.empty()) throw Error(err);
+        fprintf(stderr, "[kite dump] done in %.3fs\n",
+                std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+        return 0;
+    }
     const unsigned nt = std::min<unsigned>(worker_threads((unsigned)a.get_u64("threads", 0)),
                                            std::max<uint32_t>(1, h.partitions));
     const int W = r.words(), k = h.k;
