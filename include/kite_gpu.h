@@ -282,6 +282,22 @@ int kite_gpu_dump(kite_gpu_ctx *ctx, uint32_t k, uint32_t key_width, uint32_t pa
                   uint64_t n_runs, int has_min, uint32_t min_count, int has_max, uint32_t max_count,
                   kite_dump_write_fn fn, void *user, kite_dump_stats *stats);
 
+/* ---- histo ------------------------------------------------------------------
+ * Replaces the partition workers of `kite histo` for one input
+ * (src/histo/mod.rs:76-111 over load_partition_counts, src/kiteops/mod.rs:
+ * 433-473): the stored runs (as for kite_gpu_dump) are decoded with u64
+ * counts, the runs of each partition summed per key (u64 saturating), the
+ * header filter applied to each total, and the frequency of every total
+ * counted.  `fn` receives the bins once, values ascending: n_bins values and
+ * their frequencies.  The caller forms the union of bins over its inputs
+ * (merge_histograms, src/histo/mod.rs:171-176).  stats->lines is the number
+ * of keys that passed the filter. */
+typedef int (*kite_histo_fn)(void *user, const uint64_t *values, const uint64_t *freqs, uint64_t n_bins);
+
+int kite_gpu_histo(kite_gpu_ctx *ctx, uint32_t key_width, uint32_t partitions, const kite_dump_run *runs,
+                   uint64_t n_runs, int has_min, uint32_t min_count, int has_max, uint32_t max_count,
+                   kite_histo_fn fn, void *user, kite_dump_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

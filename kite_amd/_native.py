@@ -85,6 +85,7 @@ class DumpStats(C.Structure):
 
 
 DUMP_WRITE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_char), C.c_uint64)
+HISTO_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_uint64)
 
 # name -> (restype, argtypes); every symbol include/kite_gpu.h declares
 GPU_SYMBOLS = {
@@ -127,6 +128,8 @@ GPU_SYMBOLS = {
     "kite_gpu_dump": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(DumpRun),
                                 C.c_uint64, C.c_int, C.c_uint32, C.c_int, C.c_uint32, DUMP_WRITE_FN,
                                 C.c_void_p, C.POINTER(DumpStats)]),
+    "kite_gpu_histo": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(DumpRun), C.c_uint64, C.c_int,
+                                 C.c_uint32, C.c_int, C.c_uint32, HISTO_FN, C.c_void_p, C.POINTER(DumpStats)]),
     "kite_gpu_host_alloc": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
     "kite_gpu_host_free": (None, [C.c_void_p]),
     "kite_gpu_count_profile": (C.c_int, [C.c_void_p, C.c_int]),
@@ -458,14 +461,7 @@ class GpuContext:
         """`kite dump` of stored runs on the device (kite_gpu_dump).  runs:
         (partition, codec, rows, payload bytes) in partition order.  The
         text is returned, or passed piece by piece to write(memoryview)."""
-        arr = (DumpRun * max(len(runs), 1))()
-        keep = []
-        for i, (part, codec, rows, data) in enumerate(runs):
-            buf = np.frombuffer(bytes(data), dtype=np.uint8) if len(data) else np.zeros(1, np.uint8)
-            keep.append(buf)
-            arr[i].partition, arr[i].codec, arr[i].rows = part, codec, rows
-            arr[i].data = buf.ctypes.data
-            arr[i].bytes = len(data)
+        arr, keep = self._runs_array(runs)
         pieces: list = []
         errors: list = []
 
@@ -490,6 +486,39 @@ class GpuContext:
             raise errors[0]
         _check(rc)
         return b"".join(pieces), st.as_dict()
+
+    @staticmethod
+    def _runs_array(runs):
+        arr = (DumpRun * max(len(runs), 1))()
+        keep = []
+        for i, (part, codec, rows, data) in enumerate(runs):
+            buf = np.frombuffer(bytes(data), dtype=np.uint8) if len(data) else np.zeros(1, np.uint8)
+            keep.append(buf)
+            arr[i].partition, arr[i].codec, arr[i].rows = part, codec, rows
+            arr[i].data = buf.ctypes.data
+            arr[i].bytes = len(data)
+        return arr, keep
+
+    def histo_runs(self, key_width: int, partitions: int, runs, min_count=None,
+                   max_count=None) -> tuple[dict, dict]:
+        """`kite histo` of one input's stored runs on the device
+        (kite_gpu_histo): {total: frequency}, and the stats."""
+        arr, keep = self._runs_array(runs)
+        out: dict = {}
+
+        def cb(user, vp, fp, n):
+            for i in range(n):
+                out[int(vp[i])] = int(fp[i])
+            return 0
+
+        fn = HISTO_FN(cb)
+        st = DumpStats()
+        rc = self._lib.kite_gpu_histo(self._h, key_width, partitions, arr, len(runs),
+                                      int(min_count is not None), int(min_count or 0),
+                                      int(max_count is not None), int(max_count or 0), fn, None, C.byref(st))
+        del keep
+        _check(rc)
+        return out, st.as_dict()
 
     def count_profile(self, enable: bool) -> None:
         _check(self._lib.kite_gpu_count_profile(self._h, int(enable)))

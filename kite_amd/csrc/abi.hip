@@ -922,17 +922,17 @@ void kite_gpu_host_free(void *p) {
 }
 
 
-int kite_gpu_dump(kite_gpu_ctx *ctx, uint32_t k, uint32_t key_width, uint32_t partitions, const kite_dump_run *runs,
-                  uint64_t n_runs, int has_min, uint32_t min_count, int has_max, uint32_t max_count,
-                  kite_dump_write_fn fn, void *user, kite_dump_stats *stats) {
-    if (!ctx || !fn) return set_error(KITE_E_INVALID, "invalid argument: NULL pointer");
+}  // extern "C"
+
+namespace {
+
+// Validates the stored runs of a dump or histo call (the reader's checks
+// on each run, src/kiteops/mod.rs:475-585).
+int check_runs(uint32_t key_width, uint32_t partitions, const kite_dump_run *runs, uint64_t n_runs) {
     if (n_runs && !runs) return set_error(KITE_E_INVALID, "invalid argument: runs is NULL");
     if (key_width != 64 && key_width != 128 && key_width != 256)
         return set_error(KITE_E_INVALID, "invalid key width " + std::to_string(key_width));
     const int W = (int)(key_width / 64);
-    if (k < 1 || k > 32u * (uint32_t)W)
-        return set_error(KITE_E_INVALID, "invalid k " + std::to_string(k) + " for key width " +
-                                             std::to_string(key_width));
     const uint64_t es = 8 * (uint64_t)W + 4;
     for (uint64_t i = 0; i < n_runs; i++) {
         const kite_dump_run &r = runs[i];
@@ -950,60 +950,95 @@ int kite_gpu_dump(kite_gpu_ctx *ctx, uint32_t k, uint32_t key_width, uint32_t pa
             return set_error(KITE_E_INVALID, W == 2 ? "invalid run: u128 DVLE payload too small"
                                                     : "invalid run: u64 DVLE payload too small");
     }
+    return KITE_OK;
+}
+
+// Stages the runs group by group (whole partitions, bounded by payload
+// bytes and rows) in device memory and calls fn(d_payload, runs, np,
+// rows_in, n_runs_in_group) for each group.
+template <class F>
+void for_each_run_group(kite_gpu_ctx *ctx, int W, const kite_dump_run *runs, uint64_t n_runs, F &&fn) {
+    // (KITE_DUMP_GROUP_ROWS lowers the row bound, to exercise many groups)
+    constexpr uint64_t PAY_CAP = 1ull << 30;
+    uint64_t ROW_CAP = 96ull << 20;
+    if (const char *e = getenv("KITE_DUMP_GROUP_ROWS")) ROW_CAP = std::max<uint64_t>(1, strtoull(e, nullptr, 10));
+    hipStream_t s = ctx->stream;
+    uint64_t i = 0;
+    while (i < n_runs) {
+        uint64_t j = i, pay = 0, rows = 0;
+        while (j < n_runs) {
+            // the whole partition of run j joins, or the group ends here
+            uint64_t e = j, pb = 0, pr = 0;
+            while (e < n_runs && runs[e].partition == runs[j].partition) {
+                pb += (runs[e].bytes + 15) & ~15ull;
+                pr += runs[e].rows;
+                e++;
+            }
+            if (j > i && (pay + pb > PAY_CAP || rows + pr > ROW_CAP)) break;
+            pay += pb;
+            rows += pr;
+            j = e;
+        }
+        const uint32_t p0 = runs[i].partition, np = runs[j - 1].partition - p0 + 1;
+        uint8_t *host = static_cast<uint8_t *>(ctx->dump_stage.ensure(pay + 16));
+        std::vector<kite::DumpRun> dr;
+        dr.reserve(j - i);
+        uint64_t off = 0;
+        uint32_t rank = 0;
+        for (uint64_t q = i; q < j; q++) {
+            const kite_dump_run &r = runs[q];
+            rank = (q > i && r.partition == runs[q - 1].partition) ? rank + 1 : 0;
+            if (r.bytes) memcpy(host + off, r.data, r.bytes);
+            kite::DumpRun d{};
+            d.src = off;
+            d.bytes = r.bytes;
+            d.rows = r.rows;
+            d.codec = (W == 4 || r.codec == KITE_CODEC_RAW) ? kite::DUMP_CODEC_RAW : kite::DUMP_CODEC_DVLE;
+            d.part = r.partition - p0;
+            d.table = rank;
+            dr.push_back(d);
+            off += (r.bytes + 15) & ~15ull;
+        }
+        memset(host + off, 0, 16);
+        uint8_t *dev = static_cast<uint8_t *>(ctx->dump_pay.ensure(pay + 16));
+        KITE_HIP(hipMemcpyAsync(dev, host, pay + 16, hipMemcpyHostToDevice, s));
+        fn(dev, dr, np, rows, j - i);
+        i = j;
+    }
+}
+
+kite::DumpFilter make_filter(int has_min, uint32_t min_count, int has_max, uint32_t max_count) {
+    kite::DumpFilter f;
+    f.has_min = has_min ? 1u : 0u;
+    f.min = min_count;
+    f.has_max = has_max ? 1u : 0u;
+    f.max = max_count;
+    return f;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kite_gpu_dump(kite_gpu_ctx *ctx, uint32_t k, uint32_t key_width, uint32_t partitions, const kite_dump_run *runs,
+                  uint64_t n_runs, int has_min, uint32_t min_count, int has_max, uint32_t max_count,
+                  kite_dump_write_fn fn, void *user, kite_dump_stats *stats) {
+    if (!ctx || !fn) return set_error(KITE_E_INVALID, "invalid argument: NULL pointer");
+    if (int rc = check_runs(key_width, partitions, runs, n_runs)) return rc;
+    const int W = (int)(key_width / 64);
+    if (k < 1 || k > 32u * (uint32_t)W)
+        return set_error(KITE_E_INVALID, "invalid k " + std::to_string(k) + " for key width " +
+                                             std::to_string(key_width));
     return guarded([&]() -> int {
         bind(ctx);
         kite_dump_stats st{};
-        kite::DumpFilter f;
-        f.has_min = has_min ? 1u : 0u;
-        f.min = min_count;
-        f.has_max = has_max ? 1u : 0u;
-        f.max = max_count;
-        // partition groups bounded by staged bytes and rows
-        // (KITE_DUMP_GROUP_ROWS lowers the row bound, to exercise many groups)
-        constexpr uint64_t PAY_CAP = 1ull << 30;
-        uint64_t ROW_CAP = 96ull << 20;
-        if (const char *e = getenv("KITE_DUMP_GROUP_ROWS")) ROW_CAP = std::max<uint64_t>(1, strtoull(e, nullptr, 10));
+        const kite::DumpFilter f = make_filter(has_min, min_count, has_max, max_count);
         constexpr uint64_t PIECE = 64ull << 20;  // text bytes per D2H piece
         hipStream_t s = ctx->stream;
-        uint64_t i = 0;
-        while (i < n_runs) {
-            uint64_t j = i, pay = 0, rows = 0;
-            while (j < n_runs) {
-                // the whole partition of run j joins, or the group ends here
-                uint64_t e = j, pb = 0, pr = 0;
-                while (e < n_runs && runs[e].partition == runs[j].partition) {
-                    pb += (runs[e].bytes + 15) & ~15ull;
-                    pr += runs[e].rows;
-                    e++;
-                }
-                if (j > i && (pay + pb > PAY_CAP || rows + pr > ROW_CAP)) break;
-                pay += pb;
-                rows += pr;
-                j = e;
-            }
-            const uint32_t p0 = runs[i].partition, np = runs[j - 1].partition - p0 + 1;
-            uint8_t *host = static_cast<uint8_t *>(ctx->dump_stage.ensure(pay + 16));
-            std::vector<kite::DumpRun> dr;
-            dr.reserve(j - i);
-            uint64_t off = 0;
-            uint32_t rank = 0;
-            for (uint64_t q = i; q < j; q++) {
-                const kite_dump_run &r = runs[q];
-                rank = (q > i && r.partition == runs[q - 1].partition) ? rank + 1 : 0;
-                if (r.bytes) memcpy(host + off, r.data, r.bytes);
-                kite::DumpRun d{};
-                d.src = off;
-                d.bytes = r.bytes;
-                d.rows = r.rows;
-                d.codec = (W == 4 || r.codec == KITE_CODEC_RAW) ? kite::DUMP_CODEC_RAW : kite::DUMP_CODEC_DVLE;
-                d.part = r.partition - p0;
-                d.table = rank;
-                dr.push_back(d);
-                off += (r.bytes + 15) & ~15ull;
-            }
-            memset(host + off, 0, 16);
-            uint8_t *dev = static_cast<uint8_t *>(ctx->dump_pay.ensure(pay + 16));
-            KITE_HIP(hipMemcpyAsync(dev, host, pay + 16, hipMemcpyHostToDevice, s));
+        bool aborted = false;
+        for_each_run_group(ctx, W, runs, n_runs, [&](const uint8_t *dev, const std::vector<kite::DumpRun> &dr,
+                                                     uint32_t np, uint64_t rows_in, uint64_t nruns) {
+            if (aborted) return;
             uint64_t out_rows = 0, lines = 0;
             kite::DumpTimes t;
             const uint64_t bytes = ctx->dump->run(dev, dr, np, (int)k, W, f, &out_rows, &lines, &t);
@@ -1036,7 +1071,8 @@ int kite_gpu_dump(kite_gpu_ctx *ctx, uint32_t k, uint32_t key_width, uint32_t pa
                 const uint64_t a = q * PIECE, n = std::min<uint64_t>(PIECE, bytes - a);
                 if (fn(user, pin + (q & 1) * PIECE, n) != 0) {
                     (void)hipStreamSynchronize(s);
-                    return set_error(KITE_E_CALLBACK, "dump writer callback aborted");
+                    aborted = true;
+                    return;
                 }
             }
             KITE_HIP(hipEventRecord(t1, s));
@@ -1044,15 +1080,49 @@ int kite_gpu_dump(kite_gpu_ctx *ctx, uint32_t k, uint32_t key_width, uint32_t pa
             float ms = 0;
             KITE_HIP(hipEventElapsedTime(&ms, t0, t1));
             st.copy_ms += ms;
-            st.runs += j - i;
-            st.rows_in += rows;
+            st.runs += nruns;
+            st.rows_in += rows_in;
             st.rows += out_rows;
             st.lines += lines;
             st.bytes += bytes;
             st.groups++;
-            i = j;
-        }
+        });
+        if (aborted) return set_error(KITE_E_CALLBACK, "dump writer callback aborted");
         if (stats) *stats = st;
+        return KITE_OK;
+    });
+}
+
+int kite_gpu_histo(kite_gpu_ctx *ctx, uint32_t key_width, uint32_t partitions, const kite_dump_run *runs,
+                   uint64_t n_runs, int has_min, uint32_t min_count, int has_max, uint32_t max_count,
+                   kite_histo_fn fn, void *user, kite_dump_stats *stats) {
+    if (!ctx || !fn) return set_error(KITE_E_INVALID, "invalid argument: NULL pointer");
+    if (int rc = check_runs(key_width, partitions, runs, n_runs)) return rc;
+    const int W = (int)(key_width / 64);
+    return guarded([&]() -> int {
+        bind(ctx);
+        kite_dump_stats st{};
+        const kite::DumpFilter f = make_filter(has_min, min_count, has_max, max_count);
+        ctx->dump->histo_begin();
+        for_each_run_group(ctx, W, runs, n_runs, [&](const uint8_t *dev, const std::vector<kite::DumpRun> &dr,
+                                                     uint32_t np, uint64_t rows_in, uint64_t nruns) {
+            uint64_t out_rows = 0;
+            kite::DumpTimes t;
+            ctx->dump->histo(dev, dr, np, W, f, &out_rows, &t);
+            st.decode_ms += t.decode_ms;
+            st.merge_ms += t.merge_ms;
+            st.format_ms += t.format_ms;
+            st.runs += nruns;
+            st.rows_in += rows_in;
+            st.rows += out_rows;
+            st.groups++;
+        });
+        std::vector<uint64_t> values, freqs;
+        ctx->dump->histo_end(values, freqs);
+        for (uint64_t q : freqs) st.lines += q;
+        if (stats) *stats = st;
+        if (fn(user, values.data(), freqs.data(), values.size()) != 0)
+            return set_error(KITE_E_CALLBACK, "histo callback aborted");
         return KITE_OK;
     });
 }

@@ -54,9 +54,14 @@ __device__ __forceinline__ uint64_t rev2(uint64_t x) {
     return ((x >> 1) & 0x5555555555555555ull) | ((x & 0x5555555555555555ull) << 1);
 }
 
-__device__ __forceinline__ uint32_t is_acgt(uint32_t b) {
-    uint32_t l = b | 0x20u;
-    return (l == 'a') | (l == 'c') | (l == 'g') | (l == 't');
+// 0x80 in every zero byte of v, 0 elsewhere (exact, no borrow across bytes)
+__device__ __forceinline__ uint32_t zero_bytes(uint32_t v) {
+    return ~(((v & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v | 0x7F7F7F7Fu);
+}
+
+// The 0x80 flags of zero_bytes as four bits, byte 0 in bit 0.
+__device__ __forceinline__ uint32_t byte_mask4(uint32_t z) {
+    return (((z >> 7) * 0x00204081u) >> 21) & 0xFu;
 }
 
 __device__ __forceinline__ uint32_t newlines_in(uint32_t v) {
@@ -168,7 +173,6 @@ __global__ __launch_bounds__(XT) void k_extract(ExtractArgs a) {
     uint64_t *pbase = reinterpret_cast<uint64_t *>(dyn + ((P + 1) & ~1u));
     const int tid = threadIdx.x;
     const uint32_t wg = (uint32_t)xcd_tile(blockIdx.x, gridDim.x);
-    const uint8_t *sbytes = reinterpret_cast<const uint8_t *>(sb4);
     const uint32_t *sb = reinterpret_cast<const uint32_t *>(sb4);
     const bool live = a.len > 0 && a.fq[0] == '@';
 
@@ -210,19 +214,19 @@ __global__ __launch_bounds__(XT) void k_extract(ExtractArgs a) {
         }
         __syncthreads();
         for (int s = tid; s < NSEG; s += XT) {
-            uint32_t c = 0;
+            uint32_t m = 0;
 #pragma unroll
-            for (int q = 0; q < SEGB / 4; q++) c += newlines_in(sb[s * (SEGB / 4) + q]);
-            snl[s] = c;
+            for (int q = 0; q < SEGB / 4; q++) m |= byte_mask4(zero_bytes(sb[s * (SEGB / 4) + q] ^ 0x0A0A0A0Au)) << (4 * q);
+            snl[s] = m;  // bit j: byte j of the segment is '\n'
         }
         __syncthreads();
         {
             uint32_t total;
-            uint32_t ex = block_excl_scan<XT, uint32_t>(snl[tid], ws, &total);
+            uint32_t ex = block_excl_scan<XT, uint32_t>(__popc(snl[tid]), ws, &total);
             spre[tid] = ex;
             if (tid < NSEG - XT) {
                 uint32_t t = total;
-                for (int q = 0; q < tid; q++) t += snl[XT + q];
+                for (int q = 0; q < tid; q++) t += __popc(snl[XT + q]);
                 spre[XT + tid] = t;
             }
         }
@@ -230,18 +234,32 @@ __global__ __launch_bounds__(XT) void k_extract(ExtractArgs a) {
         // line index (mod 4) at the first halo byte
         const uint32_t l0 = (uint32_t)(a.sub_line0[st] & 3u) - spre[HSEG];
         for (int s = tid; s < NSEG; s += XT) {
-            uint32_t line = (l0 + spre[s]) & 3u;
+            // bytes on sequence lines (line index 1 of the 4-line record)
+            uint32_t line = (l0 + spre[s]) & 3u, l1 = 0, start = 0;
+            for (uint32_t e = snl[s]; e; e &= e - 1) {
+                const uint32_t p = (uint32_t)__ffs(e) - 1;
+                if (line == 1u) l1 |= ((1u << p) - 1u) & ~((1u << start) - 1u);
+                line = (line + 1u) & 3u;
+                start = p + 1;
+            }
+            if (line == 1u && start < 32u) l1 |= ~0u << start;
+            // four bytes at a time: 2-bit codes (A0 C1 G2 T3, case-blind),
+            // first byte most significant, and the ACGT/acgt test
             uint64_t code = 0;
-            uint32_t valid = 0;
+            uint32_t acgt = 0;
 #pragma unroll
-            for (int j = 0; j < SEGB; j++) {
-                uint32_t b = sbytes[s * SEGB + j];
-                code |= (uint64_t)(((b >> 1) ^ (b >> 2)) & 3u) << (62 - 2 * j);
-                valid |= ((line == 1u) & is_acgt(b)) << j;
-                line = (line + (b == '\n')) & 3u;
+            for (int q = 0; q < SEGB / 4; q++) {
+                const uint32_t w = sb[s * (SEGB / 4) + q];
+                const uint32_t c = ((w >> 1) ^ (w >> 2)) & 0x03030303u;
+                const uint32_t p8 = ((c << 6) & 0xC0u) | ((c >> 4) & 0x30u) | ((c >> 14) & 0x0Cu) | (c >> 24);
+                code |= (uint64_t)p8 << (56 - 8 * q);
+                // the lowercase letter each code stands for; a byte is a base
+                // exactly when it equals that letter after case folding
+                const uint32_t expect = __builtin_amdgcn_perm(0u, 0x74676361u, c);
+                acgt |= byte_mask4(zero_bytes((w | 0x20202020u) ^ expect)) << (4 * q);
             }
             scode[s] = code;
-            svalid[s] = live ? valid : 0u;
+            svalid[s] = live ? (acgt & l1) : 0u;
         }
         __syncthreads();
 
@@ -266,8 +284,8 @@ __global__ __launch_bounds__(XT) void k_extract(ExtractArgs a) {
                 emask |= (run >= (uint32_t)a.k ? 1u : 0u) << j;
             }
         }
-        // the k-mer ending at position j of segment ss: key, partition, count or place
-        auto kmer_at = [&](int ss, int j) {
+        // the canonical key of the k-mer ending at position j of segment ss
+        auto kmer_at = [&](int ss, int j) -> Key<W> {
             uint64_t wv[W + 1];
 #pragma unroll
             for (int q = 0; q <= W; q++) wv[q] = scode[ss - W + q];
@@ -297,15 +315,7 @@ __global__ __launch_bounds__(XT) void k_extract(ExtractArgs a) {
                 }
                 if (key_less<W>(rc, f)) key = rc;
             }
-            const uint32_t p = key_partition<W>(key, a.pbits);
-            if constexpr (!SCATTER) {
-                atomicAdd(&cnt[p], 1u);
-            } else {
-                uint64_t slot = pbase[p] + atomicAdd(&cnt[p], 1u);
-                uint64_t *dst = a.out + slot * W;
-#pragma unroll
-                for (int r = 0; r < W; r++) dst[r] = key.w[r];
-            }
+            return key;
         };
         if constexpr (!SCATTER) {
             // one list for the whole sub-tile, so that every lane builds k-mers
@@ -316,10 +326,17 @@ __global__ __launch_bounds__(XT) void k_extract(ExtractArgs a) {
             __syncthreads();
             for (uint32_t i = tid; i < nk; i += XT) {
                 const uint32_t pos = spos[i];
-                kmer_at((int)(pos / SEGB), (int)(pos % SEGB));
+                const Key<W> key = kmer_at((int)(pos / SEGB), (int)(pos % SEGB));
+                atomicAdd(&cnt[key_partition<W>(key, a.pbits)], 1u);
             }
         } else {
-            for (uint32_t e = emask; e; e &= e - 1) kmer_at(s, __ffs(e) - 1);
+            for (uint32_t e = emask; e; e &= e - 1) {
+                const Key<W> key = kmer_at(s, __ffs(e) - 1);
+                const uint32_t p = key_partition<W>(key, a.pbits);
+                uint64_t *dst = a.out + (pbase[p] + atomicAdd(&cnt[p], 1u)) * W;
+#pragma unroll
+                for (int r = 0; r < W; r++) dst[r] = key.w[r];
+            }
         }
     }
     if constexpr (!SCATTER) {

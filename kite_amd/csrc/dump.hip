@@ -127,14 +127,14 @@ __device__ __forceinline__ uint64_t rd_le(const uint8_t *p, int n) {
 template <int W>
 __global__ __launch_bounds__(DT) void k_dump_rows(const uint8_t *__restrict__ pay, const DumpRun *__restrict__ runs,
                                                   uint32_t nr, uint64_t n, const uint64_t *__restrict__ ends,
-                                                  uint64_t *__restrict__ vals, uint64_t *__restrict__ counts,
-                                                  uint32_t *__restrict__ err) {
+                                                  uint32_t trunc32, uint64_t *__restrict__ vals,
+                                                  uint64_t *__restrict__ counts, uint32_t *__restrict__ err) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const DumpRun r = runs[find_run(runs, nr, i)];
     const uint64_t j = i - r.row0;
     uint64_t v[W];
-    uint32_t c;
+    uint64_t c;
     if (r.codec == DUMP_CODEC_RAW || W == 4) {
         const uint8_t *p = pay + r.src + j * (8 * W + 4);
 #pragma unroll
@@ -199,7 +199,7 @@ __global__ __launch_bounds__(DT) void k_dump_rows(const uint8_t *__restrict__ pa
         }
         uint64_t cv = 0;
         for (uint64_t t = 0; t < len; t++) cv |= (uint64_t)(pay[pos + t] & 0x7fu) << (7 * t);
-        c = (uint32_t)cv;
+        c = trunc32 ? (uint64_t)(uint32_t)cv : cv;
         v[0] = lo;
         if constexpr (W >= 2) v[1] = hi;
     }
@@ -409,13 +409,36 @@ __global__ __launch_bounds__(DT) void k_tsv_write(const uint64_t *__restrict__ k
     }
 }
 
+// Frequencies of the totals that pass the filter: values below HISTO_SMALL
+// in per-workgroup LDS bins then one u64 add per used bin, larger values
+// appended to a list (big[0] is its length).
+__global__ __launch_bounds__(DT) void k_histo(const uint64_t *__restrict__ counts, uint64_t n, DumpFilter f,
+                                              uint64_t *__restrict__ small, uint64_t *__restrict__ big) {
+    __shared__ uint32_t bins[HISTO_SMALL];
+    for (uint32_t b = threadIdx.x; b < HISTO_SMALL; b += DT) bins[b] = 0;
+    __syncthreads();
+    for (uint64_t i = (uint64_t)blockIdx.x * DT + threadIdx.x; i < n; i += (uint64_t)gridDim.x * DT) {
+        const uint64_t c = counts[i];
+        if ((f.has_min && c < f.min) || (f.has_max && c > f.max)) continue;
+        if (c < HISTO_SMALL) {
+            atomicAdd(&bins[c], 1u);
+        } else {
+            const uint64_t at = atomicAdd((unsigned long long *)big, 1ull);
+            big[1 + at] = c;
+        }
+    }
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < HISTO_SMALL; b += DT)
+        if (bins[b]) atomicAdd((unsigned long long *)&small[b], (unsigned long long)bins[b]);
+}
+
 inline dim3 grid_for(uint64_t n) { return dim3((unsigned)std::max<uint64_t>(1, ceil_div(n, DT))); }
 
 }  // namespace
 
 void DumpEngine::release() {
     for (DevBuf *b : {&runs_d_, &tile_run_, &tcnt_, &tbase_, &ends_, &vals_, &counts_, &scan_tmp_, &tlen_, &toff_,
-                      &text_, &err_, &tabptr_, &lines_, &loff_})
+                      &text_, &err_, &tabptr_, &lines_, &loff_, &hsmall_, &hbig_d_})
         b->release();
     tabs_.clear();
     merged_.keys.release();
@@ -432,19 +455,24 @@ uint64_t DumpEngine::run(const uint8_t *d_payload, const std::vector<DumpRun> &r
     return run_w<4>(d_payload, runs, np, k, f, rows_out, lines_out, times);
 }
 
+namespace {
+struct Events {
+    hipEvent_t e[4];
+    Events() {
+        for (auto &x : e) KITE_HIP(hipEventCreate(&x));
+    }
+    ~Events() {
+        for (auto &x : e) (void)hipEventDestroy(x);
+    }
+};
+}  // namespace
+
 template <int W>
-uint64_t DumpEngine::run_w(const uint8_t *d_payload, const std::vector<DumpRun> &runs_in, uint32_t np, int k,
-                           const DumpFilter &f, uint64_t *rows_out, uint64_t *lines_out, DumpTimes *times) {
+uint64_t DumpEngine::decode_merge_w(const uint8_t *d_payload, const std::vector<DumpRun> &runs_in, uint32_t np,
+                                    bool trunc32, const uint64_t **keys_out, const uint64_t **counts_out,
+                                    hipEvent_t ev[3]) {
     hipStream_t s = stream_;
     if (!host_err_) KITE_HIP(hipHostMalloc(reinterpret_cast<void **>(&host_err_), 64, hipHostMallocDefault));
-    hipEvent_t ev[4];
-    for (auto &e : ev) KITE_HIP(hipEventCreate(&e));
-    struct EvGuard {
-        hipEvent_t *e;
-        ~EvGuard() {
-            for (int q = 0; q < 4; q++) (void)hipEventDestroy(e[q]);
-        }
-    } evg{ev};
     KITE_HIP(hipEventRecord(ev[0], s));
 
     // runs with rows, tiles of the DVLE ones
@@ -495,7 +523,8 @@ uint64_t DumpEngine::run_w(const uint8_t *d_payload, const std::vector<DumpRun> 
         KITE_CHECK_LAUNCH();
     }
     if (n) {
-        k_dump_rows<W><<<grid_for(n), DT, 0, s>>>(d_payload, druns, nr, n, ends, vals, cnts, derr);
+        k_dump_rows<W><<<grid_for(n), DT, 0, s>>>(d_payload, druns, nr, n, ends, trunc32 ? 1u : 0u, vals, cnts,
+                                                  derr);
         KITE_CHECK_LAUNCH();
         if (!dvle.empty()) {
             k_dvle_prefix<W><<<dim3((unsigned)dvle.size()), DT, 0, s>>>(druns, ddvle, vals);
@@ -544,6 +573,19 @@ uint64_t DumpEngine::run_w(const uint8_t *d_payload, const std::vector<DumpRun> 
         rows = merged_.rows;
     }
     KITE_HIP(hipEventRecord(ev[2], s));
+    *keys_out = keys;
+    *counts_out = counts;
+    return rows;
+}
+
+template <int W>
+uint64_t DumpEngine::run_w(const uint8_t *d_payload, const std::vector<DumpRun> &runs_in, uint32_t np, int k,
+                           const DumpFilter &f, uint64_t *rows_out, uint64_t *lines_out, DumpTimes *times) {
+    hipStream_t s = stream_;
+    Events ev;
+    const uint64_t *keys, *counts;
+    // dump reads every stored count as u32 (src/dump/merge.rs:183, 193)
+    const uint64_t rows = decode_merge_w<W>(d_payload, runs_in, np, true, &keys, &counts, ev.e);
 
     // lines
     const uint64_t nt = ceil_div(rows, DT);
@@ -568,16 +610,80 @@ uint64_t DumpEngine::run_w(const uint8_t *d_payload, const std::vector<DumpRun> 
         k_tsv_write<W><<<dim3((unsigned)nt), DT, 0, s>>>(keys, counts, rows, k, f, toff, text);
         KITE_CHECK_LAUNCH();
     }
-    KITE_HIP(hipEventRecord(ev[3], s));
+    KITE_HIP(hipEventRecord(ev.e[3], s));
     KITE_HIP(hipStreamSynchronize(s));
     if (times) {
-        KITE_HIP(hipEventElapsedTime(&times->decode_ms, ev[0], ev[1]));
-        KITE_HIP(hipEventElapsedTime(&times->merge_ms, ev[1], ev[2]));
-        KITE_HIP(hipEventElapsedTime(&times->format_ms, ev[2], ev[3]));
+        KITE_HIP(hipEventElapsedTime(&times->decode_ms, ev.e[0], ev.e[1]));
+        KITE_HIP(hipEventElapsedTime(&times->merge_ms, ev.e[1], ev.e[2]));
+        KITE_HIP(hipEventElapsedTime(&times->format_ms, ev.e[2], ev.e[3]));
     }
     if (rows_out) *rows_out = rows;
     if (lines_out) *lines_out = hv[1];
     return bytes;
+}
+
+void DumpEngine::histo_begin() {
+    uint64_t *b = static_cast<uint64_t *>(hsmall_.ensure(HISTO_SMALL * 8));
+    KITE_HIP(hipMemsetAsync(b, 0, HISTO_SMALL * 8, stream_));
+    hbig_.clear();
+}
+
+void DumpEngine::histo(const uint8_t *d_payload, const std::vector<DumpRun> &runs, uint32_t np, int W,
+                       const DumpFilter &f, uint64_t *rows_out, DumpTimes *times) {
+    if (W == 1) histo_w<1>(d_payload, runs, np, f, rows_out, times);
+    else if (W == 2) histo_w<2>(d_payload, runs, np, f, rows_out, times);
+    else histo_w<4>(d_payload, runs, np, f, rows_out, times);
+}
+
+template <int W>
+void DumpEngine::histo_w(const uint8_t *d_payload, const std::vector<DumpRun> &runs_in, uint32_t np,
+                         const DumpFilter &f, uint64_t *rows_out, DumpTimes *times) {
+    hipStream_t s = stream_;
+    Events ev;
+    const uint64_t *keys, *counts;
+    // histo reads stored counts as u64 (src/kiteops/mod.rs:475-585)
+    const uint64_t rows = decode_merge_w<W>(d_payload, runs_in, np, false, &keys, &counts, ev.e);
+    uint64_t *small = hsmall_.as<uint64_t>();
+    uint64_t *big = static_cast<uint64_t *>(hbig_d_.ensure((std::max<uint64_t>(rows, 1) + 1) * 8));
+    KITE_HIP(hipMemsetAsync(big, 0, 8, s));
+    if (rows) {
+        const uint64_t nwg = std::min<uint64_t>(ceil_div(rows, DT * 16), 4096);
+        k_histo<<<dim3((unsigned)nwg), DT, 0, s>>>(counts, rows, f, small, big);
+        KITE_CHECK_LAUNCH();
+    }
+    uint64_t nbig = 0;
+    KITE_HIP(hipMemcpyAsync(&nbig, big, 8, hipMemcpyDeviceToHost, s));
+    KITE_HIP(hipEventRecord(ev.e[3], s));
+    KITE_HIP(hipStreamSynchronize(s));
+    if (nbig) {
+        std::vector<uint64_t> v(nbig);
+        KITE_HIP(hipMemcpyAsync(v.data(), big + 1, nbig * 8, hipMemcpyDeviceToHost, s));
+        KITE_HIP(hipStreamSynchronize(s));
+        for (uint64_t x : v) hbig_[x]++;
+    }
+    if (times) {
+        KITE_HIP(hipEventElapsedTime(&times->decode_ms, ev.e[0], ev.e[1]));
+        KITE_HIP(hipEventElapsedTime(&times->merge_ms, ev.e[1], ev.e[2]));
+        KITE_HIP(hipEventElapsedTime(&times->format_ms, ev.e[2], ev.e[3]));
+    }
+    if (rows_out) *rows_out = rows;
+}
+
+void DumpEngine::histo_end(std::vector<uint64_t> &values, std::vector<uint64_t> &freqs) {
+    std::vector<uint64_t> small(HISTO_SMALL);
+    KITE_HIP(hipMemcpyAsync(small.data(), hsmall_.ptr, HISTO_SMALL * 8, hipMemcpyDeviceToHost, stream_));
+    KITE_HIP(hipStreamSynchronize(stream_));
+    values.clear();
+    freqs.clear();
+    for (uint64_t v = 0; v < HISTO_SMALL; v++)
+        if (small[v]) {
+            values.push_back(v);
+            freqs.push_back(small[v]);
+        }
+    for (const auto &kv : hbig_) {  // ascending, all >= HISTO_SMALL
+        values.push_back(kv.first);
+        freqs.push_back(kv.second);
+    }
 }
 
 template uint64_t DumpEngine::run_w<1>(const uint8_t *, const std::vector<DumpRun> &, uint32_t, int,

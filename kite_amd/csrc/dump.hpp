@@ -5,6 +5,7 @@
 #pragma once
 
 #include <cstdint>
+#include <map>
 #include <vector>
 
 #include "common.hpp"
@@ -28,6 +29,8 @@ struct DumpRun {
     uint32_t table;  // rank of the run among its partition's runs
 };
 
+constexpr uint32_t HISTO_SMALL = 8192;  // totals below this are binned in LDS
+
 struct DumpFilter {
     uint32_t has_min = 0, min = 0, has_max = 0, max = 0;
 };
@@ -50,9 +53,24 @@ class DumpEngine {
     uint64_t run(const uint8_t *d_payload, const std::vector<DumpRun> &runs, uint32_t np, int k, int W,
                  const DumpFilter &f, uint64_t *rows_out, uint64_t *lines_out, DumpTimes *times);
     const char *text() const { return text_.as<char>(); }
+
+    // `kite histo` of one input over its partition groups: histo_begin,
+    // then histo per group (decode, merge with u64 sums, filter on the
+    // total, frequency of every total), then histo_end with the bins in
+    // ascending order.
+    void histo_begin();
+    void histo(const uint8_t *d_payload, const std::vector<DumpRun> &runs, uint32_t np, int W, const DumpFilter &f,
+               uint64_t *rows_out, DumpTimes *times);
+    void histo_end(std::vector<uint64_t> &values, std::vector<uint64_t> &freqs);
     void release();
 
    private:
+    template <int W>
+    uint64_t decode_merge_w(const uint8_t *d_payload, const std::vector<DumpRun> &runs, uint32_t np, bool trunc32,
+                            const uint64_t **keys, const uint64_t **counts, hipEvent_t ev[3]);
+    template <int W>
+    void histo_w(const uint8_t *d_payload, const std::vector<DumpRun> &runs, uint32_t np, const DumpFilter &f,
+                 uint64_t *rows_out, DumpTimes *times);
     template <int W>
     uint64_t run_w(const uint8_t *d_payload, const std::vector<DumpRun> &runs, uint32_t np, int k,
                    const DumpFilter &f, uint64_t *rows_out, uint64_t *lines_out, DumpTimes *times);
@@ -60,7 +78,8 @@ class DumpEngine {
     hipStream_t stream_;
     MergeEngine *merge_;
     DevBuf runs_d_, tile_run_, tcnt_, tbase_, ends_, vals_, counts_, scan_tmp_, tlen_, toff_, text_, err_,
-        tabptr_, lines_, loff_;
+        tabptr_, lines_, loff_, hsmall_, hbig_d_;
+    std::map<uint64_t, uint64_t> hbig_;  // frequencies of totals >= HISTO_SMALL
     std::vector<DevTable> tabs_;
     DevTable merged_;
     uint32_t *host_err_ = nullptr;

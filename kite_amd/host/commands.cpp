@@ -105,13 +105,11 @@ FILE *open_out(const std::string &path) {
     return f;
 }
 
-// The whole dump on the device (kite_gpu_dump): runs of every partition in
-// order, zstd payloads decoded here first; the text is written as it arrives.
-void dump_on_gpu(kite_gpu_ctx *g, const Reader &r, FILE *out) {
+// Every stored run of a file in partition order, as the device engine
+// takes them; zstd payloads are decoded into `decoded` first.
+void collect_runs(const Reader &r, std::vector<kite_dump_run> &runs, std::vector<std::vector<uint8_t>> &decoded) {
     const Header &h = r.header();
     const int W = r.words();
-    std::vector<kite_dump_run> runs;
-    std::vector<std::vector<uint8_t>> decoded;
     runs.reserve(r.chunk_count());
     for (uint32_t p = 0; p < h.partitions; p++) {
         auto [first, cnt] = r.partition_chunks(p);
@@ -132,6 +130,30 @@ void dump_on_gpu(kite_gpu_ctx *g, const Reader &r, FILE *out) {
             runs.push_back(x);
         }
     }
+}
+
+// Opens the device engine for a host command unless the environment
+// variable `var` says "host"; "gpu" makes a missing device an error.
+kite_gpu_ctx *open_engine(const char *var) {
+    const char *ev = getenv(var);
+    const std::string engine = ev ? ev : "auto";
+    if (engine != "auto" && engine != "gpu" && engine != "host")
+        throw UsageError(std::string(var) + " must be auto, gpu or host");
+    kite_gpu_ctx *g = nullptr;
+    if (engine != "host" && kite_gpu_open(0, 0, &g) != KITE_OK) {
+        g = nullptr;
+        if (engine == "gpu") throw Error(std::string("GPU engine unavailable: ") + kite_gpu_last_error());
+    }
+    return g;
+}
+
+// The whole dump on the device (kite_gpu_dump): runs of every partition in
+// order; the text is written as it arrives.
+void dump_on_gpu(kite_gpu_ctx *g, const Reader &r, FILE *out) {
+    const Header &h = r.header();
+    std::vector<kite_dump_run> runs;
+    std::vector<std::vector<uint8_t>> decoded;
+    collect_runs(r, runs, decoded);
     auto write = [](void *user, const char *text, uint64_t len) -> int {
         return fwrite(text, 1, len, static_cast<FILE *>(user)) == len ? 0 : 1;
     };
@@ -167,15 +189,7 @@ int run_dump(int argc, char **argv) {
     FILE *out = open_out(a.get("output"));
     // engine: the GPU when a device opens (KITE_DUMP_ENGINE=gpu requires
     // it, KITE_DUMP_ENGINE=host keeps the partition workers on the CPU)
-    const char *ev = getenv("KITE_DUMP_ENGINE");
-    const std::string engine = ev ? ev : "auto";
-    if (engine != "auto" && engine != "gpu" && engine != "host")
-        throw UsageError("KITE_DUMP_ENGINE must be auto, gpu or host");
-    kite_gpu_ctx *gctx = nullptr;
-    if (engine != "host" && kite_gpu_open(0, 0, &gctx) != KITE_OK) {
-        gctx = nullptr;
-        if (engine == "gpu") throw Error(std::string("GPU dump engine unavailable: ") + kite_gpu_last_error());
-    }
+    kite_gpu_ctx *gctx = open_engine("KITE_DUMP_ENGINE");
     if (gctx) {
         fprintf(stderr, "[kite dump] engine: gpu\n");
         std::string err;
@@ -267,9 +281,33 @@ int run_histo(int argc, char **argv) {
         for (size_t i = 0; i < inputs.size(); i++) labels.push_back(infer_label(inputs[i], i));
     const unsigned nt = worker_threads((unsigned)a.get_u64("threads", 0));
     std::vector<std::map<uint64_t, uint64_t>> hs(inputs.size());
+    // engine: the GPU when a device opens (KITE_HISTO_ENGINE as for dump)
+    struct Ctx {
+        kite_gpu_ctx *g;
+        ~Ctx() {
+            if (g) kite_gpu_close(g);
+        }
+    } gctx{open_engine("KITE_HISTO_ENGINE")};
+    if (gctx.g) fprintf(stderr, "[kite histo] engine: gpu\n");
     for (size_t fi = 0; fi < inputs.size(); fi++) {
         Reader r(inputs[fi]);
         const Header &h = r.header();
+        if (gctx.g) {
+            std::vector<kite_dump_run> runs;
+            std::vector<std::vector<uint8_t>> decoded;
+            collect_runs(r, runs, decoded);
+            auto take = [](void *user, const uint64_t *v, const uint64_t *f, uint64_t n) -> int {
+                auto *m = static_cast<std::map<uint64_t, uint64_t> *>(user);
+                for (uint64_t q = 0; q < n; q++) (*m)[v[q]] += f[q];
+                return 0;
+            };
+            kite_dump_stats st{};
+            if (kite_gpu_histo(gctx.g, h.key_width, h.partitions, runs.data(), runs.size(), h.min_count.has_value(),
+                               h.min_count.value_or(0), h.max_count.has_value(), h.max_count.value_or(0), take,
+                               &hs[fi], &st) != KITE_OK)
+                throw Error(std::string("histo failed: ") + kite_gpu_last_error());
+            continue;
+        }
         std::vector<std::map<uint64_t, uint64_t>> local(nt);
         std::atomic<uint32_t> next{0};
         std::string err;

@@ -65,6 +65,23 @@ def test_random_reads_all_key_widths(gpu_ctx, k, canon, P):
     check_same(res, tab, f"k={k}")
 
 
+def test_every_byte_value_in_sequences(gpu_ctx):
+    """Sequence lines drawn from all byte values but the newline: only
+    A, C, G, T in either case are bases, every other byte breaks the k-mer
+    (BASE_ENCODE, src/count/kmer.rs:2-13)."""
+    rng = random.Random(99)
+    pool = bytes(b for b in range(256) if b != 0x0A) + b"ACGTacgt" * 40
+    recs = []
+    for i in range(4000):
+        L = rng.randint(0, 200)
+        s = bytes(rng.choice(pool) for _ in range(L))
+        recs.append(b"@e%d\n" % i + s + b"\n+\n" + b"I" * L + b"\n")
+    fq = b"".join(recs)
+    for k, canon, P in [(3, True, 4), (7, False, 16), (21, True, 64), (40, True, 8)]:
+        res, tab, _ = run_both(gpu_ctx, fq, k, canon, P)
+        check_same(res, tab, f"bytes k={k}")
+
+
 def test_long_reads_cross_subtiles(gpu_ctx):
     rng = random.Random(7)
     recs = []

@@ -183,3 +183,49 @@ def test_dump_abi_runs_and_errors(gpu_ctx):
         gpu_ctx.dump_runs(21, 64, 4, runs[::-1])
     with pytest.raises(KiteError, match="zstd"):
         gpu_ctx.dump_runs(21, 64, 4, [(0, 6, 1, _dvle([(1, 1)], 1))])
+
+
+def test_histo_matches_oracle(tmp_path):
+    """`kite histo` on the device over several inputs (multi-run files,
+    totals above the LDS bin range, header filters) equals the oracle's."""
+    rng = random.Random(21)
+    paths = []
+    for fi, (k, kw, P, mn, mx) in enumerate([(21, 64, 8, None, None), (33, 128, 4, 2, None),
+                                             (70, 256, 2, None, 500000), (15, 64, 16, 3, 9000)]):
+        runs = []
+        for _ in range(30):
+            keys = sorted({rng.randrange(1 << min(2 * k, 40)) for _ in range(120)})
+            runs.append((rng.randrange(P), [(v, rng.choice([1, 1, 2, 3, 7, 8191, 8192, 70000, 400000]))
+                                            for v in keys]))
+        f = tmp_path / f"in{fi}.kite"
+        kf.write(f, k, True, P, kw, runs, min_count=mn, max_count=mx)
+        paths.append(f)
+    args = []
+    for p in paths:
+        args += ["-i", p]
+    ol.kite_histo(paths, tmp_path / "o.tsv")
+    env = dict(os.environ, KITE_HISTO_ENGINE="gpu")
+    r = subprocess.run([str(KITE), "histo", *map(str, args), "-o", str(tmp_path / "g.tsv")], capture_output=True,
+                       text=True, timeout=300, env=env)
+    assert r.returncode == 0, r.stderr
+    assert "engine: gpu" in r.stderr
+    _same_file(tmp_path / "g.tsv", tmp_path / "o.tsv", "histo")
+
+
+def test_histo_abi(gpu_ctx):
+    rows_a = [(3, 5), (10, 1), (1000, 2), (1 << 40, 9000)]
+    rows_b = [(10, 4), (11, 1), (1 << 40, 0xFFFFFFFF)]
+    runs = [(0, 2, len(rows_a), _dvle(rows_a, 1)), (0, 2, len(rows_b), _dvle(rows_b, 1)),
+            (3, 2, 1, _dvle([(77, 5)], 1))]
+    tot = {}
+    for v, c in rows_a + rows_b + [(77, 5)]:
+        tot[(v, v == 77)] = tot.get((v, v == 77), 0) + c
+    exp = {}
+    for c in tot.values():
+        exp[c] = exp.get(c, 0) + 1
+    bins, st = gpu_ctx.histo_runs(64, 4, runs)
+    assert bins == exp  # includes 9000 + 0xFFFFFFFF, a u64 total
+    assert st["lines"] == len(tot)
+    bins, _ = gpu_ctx.histo_runs(64, 4, runs, min_count=5, max_count=5)
+    assert bins == {5: 3}
+    assert gpu_ctx.histo_runs(64, 4, [])[0] == {}
