@@ -155,22 +155,38 @@ struct ExtractArgs {
 
 // Both passes walk identical k-mers; the scatter pass places each key at
 // off[p][wg] + (slot within this workgroup's share of partition p).
-template <int W, bool SCATTER>
+// MODE 0: histogram.  MODE 1: scatter, each lane storing its own keys (any
+// partition count).  MODE 2: scatter in chunks of SC keys grouped by
+// partition in LDS first, so that lanes store runs of neighbouring keys:
+// one 8-byte store per lane costs the cache one request per key, a run of
+// r keys about r/8 (used when a chunk gives runs of two or more keys).
+template <int W>
+struct StageCfg {
+    static constexpr int SC = 1024 / W;    // keys per chunk: SC * 8W bytes alias the sub-tile buffer
+    static constexpr int EPT = SC / XT;    // keys per thread per chunk
+};
+
+template <int W, int MODE>
 __global__ __launch_bounds__(XT) void k_extract(ExtractArgs a) {
+    constexpr bool SCATTER = MODE != 0;
     __shared__ uint4 sb4[EXT_U4];
     __shared__ uint64_t scode[NSEG];
     __shared__ uint32_t svalid[NSEG];
     __shared__ uint32_t snl[NSEG];
     __shared__ uint32_t spre[NSEG];
     __shared__ uint32_t ws[XT / 64 + 1];
-    // k-mer end positions of the sub-tile (histogram pass: every lane builds
-    // k-mers; the scatter pass keeps the per-segment loop, which holds more
-    // waves per CU for its stores)
-    __shared__ uint16_t spos[SCATTER ? 1 : XT * SEGB];
-    extern __shared__ uint32_t dyn[];  // P u32 counters, then (scatter) P u64 bases
+    // k-mer end positions of the sub-tile (histogram and staged passes:
+    // every lane builds k-mers; the direct scatter keeps the per-segment loop)
+    __shared__ uint16_t spos[MODE != 1 ? XT * SEGB : 1];
+    __shared__ uint16_t spart[MODE == 2 ? StageCfg<W>::SC : 1];  // partition of each staged key
+    // P u32 counters, then (scatter) P u64 bases, then (staged) per-chunk
+    // counts and offsets, P u32 each
+    extern __shared__ uint32_t dyn[];
     const uint32_t P = 1u << a.pbits;
     uint32_t *cnt = dyn;
     uint64_t *pbase = reinterpret_cast<uint64_t *>(dyn + ((P + 1) & ~1u));
+    uint32_t *lcnt = dyn + ((P + 1) & ~1u) + 2 * P;
+    uint32_t *loff = lcnt + P;
     const int tid = threadIdx.x;
     const uint32_t wg = (uint32_t)xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t *sb = reinterpret_cast<const uint32_t *>(sb4);
@@ -179,6 +195,7 @@ __global__ __launch_bounds__(XT) void k_extract(ExtractArgs a) {
     for (uint32_t p = tid; p < P; p += XT) {
         cnt[p] = 0;
         if constexpr (SCATTER) pbase[p] = a.off[(uint64_t)p * a.nwg + wg];
+        if constexpr (MODE == 2) lcnt[p] = 0;
     }
 
     // key masks and reverse-complement shift (constant per launch)
@@ -329,13 +346,78 @@ __global__ __launch_bounds__(XT) void k_extract(ExtractArgs a) {
                 const Key<W> key = kmer_at((int)(pos / SEGB), (int)(pos % SEGB));
                 atomicAdd(&cnt[key_partition<W>(key, a.pbits)], 1u);
             }
-        } else {
+        } else if constexpr (MODE == 1) {
             for (uint32_t e = emask; e; e &= e - 1) {
                 const Key<W> key = kmer_at(s, __ffs(e) - 1);
                 const uint32_t p = key_partition<W>(key, a.pbits);
                 uint64_t *dst = a.out + (pbase[p] + atomicAdd(&cnt[p], 1u)) * W;
 #pragma unroll
                 for (int r = 0; r < W; r++) dst[r] = key.w[r];
+            }
+        } else {
+            constexpr int SC = StageCfg<W>::SC, EPT = StageCfg<W>::EPT;
+            const uint32_t PPT = (P + XT - 1) / XT;  // partitions per thread in the offset scan
+            uint32_t nk;
+            uint32_t slot = block_excl_scan<XT, uint32_t>(__popc(emask), ws, &nk);
+            for (uint32_t e = emask; e; e &= e - 1) spos[slot++] = (uint16_t)(s * SEGB + __ffs(e) - 1);
+            __syncthreads();
+            // the sub-tile bytes are no longer needed: they hold the chunk
+            uint64_t *stage = reinterpret_cast<uint64_t *>(sb4);
+            for (uint32_t c0 = 0; c0 < nk; c0 += SC) {
+                const uint32_t m = min((uint32_t)SC, nk - c0);
+                Key<W> key[EPT];
+                uint32_t part[EPT], rank[EPT];
+#pragma unroll
+                for (int u = 0; u < EPT; u++) {
+                    const uint32_t i = (uint32_t)u * XT + tid;
+                    if (i < m) {
+                        const uint32_t pos = spos[c0 + i];
+                        key[u] = kmer_at((int)(pos / SEGB), (int)(pos % SEGB));
+                        part[u] = key_partition<W>(key[u], a.pbits);
+                        rank[u] = atomicAdd(&lcnt[part[u]], 1u);
+                    }
+                }
+                __syncthreads();
+                {
+                    const uint32_t p0 = tid * PPT;
+                    uint32_t sum = 0;
+                    for (uint32_t q = 0; q < PPT; q++)
+                        if (p0 + q < P) sum += lcnt[p0 + q];
+                    uint32_t tot;
+                    uint32_t run = block_excl_scan<XT, uint32_t>(sum, ws, &tot);
+                    for (uint32_t q = 0; q < PPT; q++)
+                        if (p0 + q < P) {
+                            loff[p0 + q] = run;
+                            run += lcnt[p0 + q];
+                        }
+                }
+                __syncthreads();
+#pragma unroll
+                for (int u = 0; u < EPT; u++) {
+                    const uint32_t i = (uint32_t)u * XT + tid;
+                    if (i < m) {
+                        const uint32_t d = loff[part[u]] + rank[u];
+#pragma unroll
+                        for (int r = 0; r < W; r++) stage[d * W + r] = key[u].w[r];
+                        spart[d] = (uint16_t)part[u];
+                    }
+                }
+                __syncthreads();
+                for (uint32_t i = tid; i < m; i += XT) {
+                    const uint32_t p = spart[i];
+                    uint64_t *dst = a.out + (pbase[p] + cnt[p] + (i - loff[p])) * W;
+#pragma unroll
+                    for (int r = 0; r < W; r++) dst[r] = stage[i * W + r];
+                }
+                __syncthreads();
+                for (uint32_t q = 0; q < PPT; q++) {
+                    const uint32_t p = tid * PPT + q;
+                    if (p < P) {
+                        cnt[p] += lcnt[p];
+                        lcnt[p] = 0;
+                    }
+                }
+                __syncthreads();
             }
         }
     }
@@ -1278,6 +1360,7 @@ void CountEngine::run(const uint8_t *d_fq, uint64_t len, int k, int canonical, u
     if (const char *e = getenv("KITE_HASH_CLASSES")) tun.hash_classes = std::min(3, std::max(0, atoi(e)));
     if (const char *e = getenv("KITE_SPLIT_CHUNK")) tun.split_chunk = std::min(64, std::max(1, atoi(e)));
     if (const char *e = getenv("KITE_EXTRACT_SUBS")) tun.extract_subs = std::min(64, std::max(1, atoi(e)));
+    if (const char *e = getenv("KITE_EXTRACT_STAGED")) tun.extract_staged = atoi(e) != 0;
     if (words_ == 1) run_w<1>(d_fq, len, k, canonical, pbits);
     else if (words_ == 2) run_w<2>(d_fq, len, k, canonical, pbits);
     else run_w<4>(d_fq, len, k, canonical, pbits);
@@ -1340,20 +1423,23 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
     ea.hist = static_cast<uint32_t *>(hist_.ensure(mat * 4));
     size_t dyn_hist = (size_t)P * 4;
     size_t dyn_scat = (size_t)((P + 1) & ~1u) * 4 + (size_t)P * 8;
+    // staged scatter when a chunk gives runs of two or more keys per partition
+    const bool staged = tun.extract_staged && 2 * P <= (uint32_t)StageCfg<W>::SC;
+    if (staged) dyn_scat += (size_t)P * 8;
     {
         static bool attr_set[3] = {false, false, false};
         const int wi = W == 1 ? 0 : (W == 2 ? 1 : 2);
         if (!attr_set[wi]) {
             const int max_dyn = (int)(((MAX_PARTITIONS + 1) & ~1u) * 4 + MAX_PARTITIONS * 8);
-            KITE_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_extract<W, false>),
+            KITE_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_extract<W, 0>),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, max_dyn));
-            KITE_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_extract<W, true>),
+            KITE_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_extract<W, 1>),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, max_dyn));
             attr_set[wi] = true;
         }
     }
     launch("k_extract_hist", len + mat * 4,
-           [&] { k_extract<W, false><<<dim3(nwg), dim3(XT), dyn_hist, s>>>(ea); });
+           [&] { k_extract<W, 0><<<dim3(nwg), dim3(XT), dyn_hist, s>>>(ea); });
     uint64_t *off = static_cast<uint64_t *>(off_.ensure((mat + 1) * 8));
     scan_to_u64(ea.hist, off, mat, stmp, s);
     uint64_t *part_off_d = static_cast<uint64_t *>(part_off_d_.ensure((P + 1) * 8));
@@ -1372,8 +1458,10 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
     if (n > 0) {
         ea.off = off;
         ea.out = A;
-        launch("k_extract_scatter", len + mat * 8 + n * 8 * W,
-               [&] { k_extract<W, true><<<dim3(nwg), dim3(XT), dyn_scat, s>>>(ea); });
+        launch("k_extract_scatter", len + mat * 8 + n * 8 * W, [&] {
+            if (staged) k_extract<W, 2><<<dim3(nwg), dim3(XT), dyn_scat, s>>>(ea);
+            else k_extract<W, 1><<<dim3(nwg), dim3(XT), dyn_scat, s>>>(ea);
+        });
     }
     tm.mark();
 

@@ -51,6 +51,7 @@ struct CountTunables {
     int hash_classes = 3;  // size classes [0, hash_classes) use the hash kernel
     int split_chunk = 1;   // split tiles per workgroup (longer runs per child, fewer atomics)
     int extract_subs = 4;  // 8 KiB sub-tiles per extract workgroup
+    bool extract_staged = true;  // group each chunk's keys by partition in LDS before writing
 };
 
 class CountEngine {

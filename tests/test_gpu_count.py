@@ -229,6 +229,26 @@ def test_workgroup_chunking_tunables(gpu_ctx, monkeypatch, chunk, subs):
         check_same(res, tab, f"{w.name} chunk={chunk} subs={subs}")
 
 
+@pytest.mark.parametrize("staged", ["0", "1"])
+def test_extract_scatter_modes(gpu_ctx, monkeypatch, staged):
+    """The extract scatter with keys grouped by partition in LDS (chunks of
+    1024 / W keys) and with direct per-lane stores give the oracle's table,
+    across key widths, partition counts at the staging limit, and long reads
+    whose sub-tiles hold more k-mers than one chunk."""
+    monkeypatch.setenv("KITE_EXTRACT_STAGED", staged)
+    rng = random.Random(5)
+    recs = []
+    for i in range(1500):
+        L = rng.choice([0, 40, 150, 151, 2500])
+        s = "".join(rng.choice("ACGTACGTACGTN") for _ in range(L))
+        recs.append(f"@s{i}\n{s}\n+\n{'I' * L}\n")
+    fq = "".join(recs).encode()
+    for k, canon, P in [(31, True, 512), (31, True, 256), (25, False, 1), (63, True, 256), (45, False, 64),
+                        (101, True, 128), (128, False, 4)]:
+        res, tab, _ = run_both(gpu_ctx, fq, k, canon, P)
+        check_same(res, tab, f"staged={staged} k={k} P={P}")
+
+
 def test_hash_kernel_on_class_two(gpu_ctx, monkeypatch):
     """The hash kernel with 1024 threads on the 4096-key class."""
     monkeypatch.setenv("KITE_HASH_CLASSES", "3")
