@@ -720,10 +720,10 @@ __global__ __launch_bounds__(BT) void k_bucket_sort(uint64_t *__restrict__ A,
 
 constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v / 2); }
 
-// Waves per SIMD that the LDS footprint of a hash-sort workgroup allows
-// (about 20 B per item: keys, counts, two table slots); the register budget
-// is capped to match, so registers do not become the tighter limit.
-constexpr int hash_sort_waves(int nt) { return nt <= 256 ? 7 : (nt <= 512 ? 6 : 4); }
+// Waves per SIMD of every hash-sort class: about 18 B of LDS per item (keys,
+// 16-bit counts, two table slots) fits 8 waves, and the registers are
+// capped to match.
+constexpr int hash_sort_waves(int) { return 8; }
 
 // Hash-then-bin variant: every key is inserted into an LDS open-addressing
 // table (a slot holds the first position of its key, claimed by CAS) and
@@ -731,30 +731,32 @@ constexpr int hash_sort_waves(int nt) { return nt <= 256 ? 7 : (nt <= 512 ? 6 : 
 // counts, are then distributed by one counting pass over 2*CAP bins on
 // their top remaining key bits (order inside a bin does not matter: keys
 // are distinct), and each thread insertion-sorts the few keys of its own
-// contiguous bins.  A bin above LOCAL_MAX keys (skewed key bits) sends the
-// whole bucket through the LSD passes instead.  Output equals
-// k_bucket_sort's: ascending unique keys with their run lengths.
+// contiguous bins.  A bin above LOCAL_MAX keys (skewed key bits) hands the
+// bucket, untouched, to k_bucket_sort through fb_list (the host launches it
+// over the handed-back buckets), which keeps the LSD code and its registers
+// out of this kernel.  Output equals k_bucket_sort's: ascending unique keys
+// with their run lengths.
 template <int W, int IPT, int NT>
 __global__ __launch_bounds__(NT, hash_sort_waves(NT)) void k_bucket_hash_sort(uint64_t *__restrict__ A,
                                                           uint64_t *__restrict__ B,
                                                           uint32_t *__restrict__ counts_tmp,
                                                           const Bucket *__restrict__ buckets,
                                                           const uint32_t *__restrict__ order,
-                                                          uint32_t *__restrict__ ucount, int k) {
+                                                          uint32_t *__restrict__ ucount, int k,
+                                                          uint32_t *__restrict__ fb_list,
+                                                          uint32_t *__restrict__ fb_count) {
     constexpr int NW = NT / 64;
     constexpr int CAP = NT * IPT;
     constexpr int HT = 2 * CAP;  // hash slots (load factor <= 1/2), then bins
     constexpr int BPT = HT / NT; // bins per thread
     constexpr int DB = ilog2c(HT);
     constexpr int LOCAL_MAX = 16;
-    constexpr bool WH_IN_HT = HT >= 256 * NW;  // the LSD histogram reuses the table when it fits
     __shared__ uint64_t skeys[CAP * W];
-    __shared__ uint32_t scnt[CAP];
+    __shared__ uint16_t scnt[CAP];  // counts <= CAP < 2^16; pairs updated as one u32
     __shared__ uint32_t ht[HT];
-    __shared__ uint32_t whist_own[WH_IN_HT ? 1 : 256 * NW];
     __shared__ uint32_t ws[NW + 1];
     __shared__ uint32_t s_fallback;
-    uint32_t *whist = WH_IN_HT ? ht : whist_own;
+    uint32_t *scnt32 = reinterpret_cast<uint32_t *>(scnt);
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const uint32_t bid = order[blockIdx.x];
@@ -777,7 +779,7 @@ __global__ __launch_bounds__(NT, hash_sort_waves(NT)) void k_bucket_hash_sort(ui
     int ipt = (int)((m + NT - 1) / NT);
     uint32_t wbase = (uint32_t)wave * 64u * (uint32_t)ipt;
     for (int j = tid; j < HT; j += NT) ht[j] = 0;
-    for (int j = tid; j < CAP; j += NT) scnt[j] = 0;
+    for (int j = tid; j < CAP / 2; j += NT) scnt32[j] = 0;
     Key<W> key[IPT];
 #pragma unroll
     for (int it = 0; it < IPT; it++) {
@@ -841,7 +843,7 @@ __global__ __launch_bounds__(NT, hash_sort_waves(NT)) void k_bucket_hash_sort(ui
     for (int it = 0; it < IPT; it++) {
         uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
         if (it < ipt && pos < m) {
-            atomicAdd(&scnt[rep[it]], 1u);
+            atomicAdd(&scnt32[rep[it] >> 1], 1u << (16 * (rep[it] & 1u)));
             if (rep[it] == pos) is_rep |= 1u << it;
         }
     }
@@ -890,11 +892,17 @@ __global__ __launch_bounds__(NT, hash_sort_waves(NT)) void k_bucket_hash_sort(ui
             const uint32_t dst = ht[dig[it]] + slot[it];
 #pragma unroll
             for (int r = 0; r < W; r++) skeys[dst * W + r] = key[it].w[r];
-            scnt[dst] = cnt[it];
+            scnt[dst] = (uint16_t)cnt[it];
         }
     }
     __syncthreads();
-    if (!s_fallback) {
+    if (s_fallback) {
+        // a bin above LOCAL_MAX keys (skewed key bits): the bucket is listed
+        // for the LSD kernel, which sorts it from its stored keys
+        if (tid == 0) fb_list[atomicAdd(fb_count, 1u)] = bid;
+        return;
+    }
+    {
         // 3. every thread sorts the keys of its own bins (at most LOCAL_MAX each)
 #pragma unroll
         for (int b = 0; b < BPT; b++) {
@@ -917,82 +925,10 @@ __global__ __launch_bounds__(NT, hash_sort_waves(NT)) void k_bucket_hash_sort(ui
                 }
 #pragma unroll
                 for (int r = 0; r < W; r++) skeys[j * W + r] = kk.w[r];
-                scnt[j] = c;
+                scnt[j] = (uint16_t)c;
             }
         }
         __syncthreads();
-    } else {
-        // skewed key bits: stable LSD passes over the unique keys, reloaded
-        // from skeys[0, U) in the wave-striped layout
-        ipt = (int)((U + NT - 1) / NT);
-        wbase = (uint32_t)wave * 64u * (uint32_t)ipt;
-#pragma unroll
-        for (int it = 0; it < IPT; it++) {
-            uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
-            if (it < ipt && pos < U) {
-#pragma unroll
-                for (int r = 0; r < W; r++) key[it].w[r] = skeys[pos * W + r];
-                cnt[it] = scnt[pos];
-            }
-        }
-        __syncthreads();
-        for (int shift = 0; shift < bits && U > 1; shift += 8) {
-            const int nb = std::min(8, bits - shift);
-            for (int j = tid; j < 256 * NW; j += NT) whist[j] = 0;
-            __syncthreads();
-            uint32_t rank[IPT];
-#pragma unroll
-            for (int it = 0; it < IPT; it++) {
-                if (it < ipt) {
-                    uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
-                    bool valid = pos < U;
-                    uint32_t d = valid ? key_bits<W>(key[it], shift, nb) : 0u;
-                    uint64_t peers = __ballot(valid);
-                    for (int bb = 0; bb < nb; bb++) {
-                        uint64_t mb = __ballot(valid && ((d >> bb) & 1u));
-                        peers &= ((d >> bb) & 1u) ? mb : ~mb;
-                    }
-                    uint32_t below = __popcll(peers & lanemask_lt());
-                    uint32_t basev = whist[d * NW + wave];
-                    rank[it] = basev + below;
-                    dig[it] = d;
-                    if (valid && below == 0) whist[d * NW + wave] = basev + (uint32_t)__popcll(peers);
-                }
-            }
-            __syncthreads();
-            {
-                uint32_t v0 = whist[tid * 4 + 0], v1 = whist[tid * 4 + 1];
-                uint32_t v2 = whist[tid * 4 + 2], v3 = whist[tid * 4 + 3];
-                uint32_t total;
-                uint32_t ex = block_excl_scan<NT, uint32_t>(v0 + v1 + v2 + v3, ws, &total);
-                whist[tid * 4 + 0] = ex;
-                whist[tid * 4 + 1] = ex + v0;
-                whist[tid * 4 + 2] = ex + v0 + v1;
-                whist[tid * 4 + 3] = ex + v0 + v1 + v2;
-            }
-            __syncthreads();
-#pragma unroll
-            for (int it = 0; it < IPT; it++) {
-                uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
-                if (it < ipt && pos < U) {
-                    uint32_t dst = whist[dig[it] * NW + wave] + rank[it];
-#pragma unroll
-                    for (int r = 0; r < W; r++) skeys[dst * W + r] = key[it].w[r];
-                    scnt[dst] = cnt[it];
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int it = 0; it < IPT; it++) {
-                uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
-                if (it < ipt && pos < U) {
-#pragma unroll
-                    for (int r = 0; r < W; r++) key[it].w[r] = skeys[pos * W + r];
-                    cnt[it] = scnt[pos];
-                }
-            }
-            __syncthreads();
-        }
     }
     for (uint32_t q2 = tid; q2 < U; q2 += NT) {
         uint64_t *d = X + (start + q2) * W;
@@ -1314,7 +1250,7 @@ void CountEngine::release() {
                       &out_keys_[1], &out_counts_[0], &out_counts_[1], &segs_, &tile_pre_, &child_hist_, &child_cur_,
                       &ucount_, &uoff_, &part_first_, &part_u_, &order_, &wg_hist_, &nodes_[0], &nodes_[1],
                       &npart_[0], &npart_[1], &act_, &tiles_, &act_rank_, &tile_off_, &lstats_, &seg_node_,
-                      &nz_rank_, &nz_count_, &new_cnt_, &new_base_, &cls_, &cls_count_})
+                      &nz_rank_, &nz_count_, &new_cnt_, &new_base_, &cls_, &cls_count_, &fallback_})
         b->release();
 }
 
@@ -1378,7 +1314,7 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
     EventTimer tm(timing_, s);
     tm.mark();
     part_off_h_.assign(P + 1, 0);
-    n_kmers_ = n_unique_ = n_buckets_ = 0;
+    n_kmers_ = n_unique_ = n_buckets_ = n_fallback_ = 0;
     kstats_.clear();
     pending_.clear();
     levels_ = 0;
@@ -1597,6 +1533,13 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
         k_order_fill<<<grid1(nb), 256, 0, s>>>(cls, (uint32_t)nb, dcls_off, cls_count, ordp);
         KITE_CHECK_LAUNCH();
         const uint32_t *ord = order_.as<uint32_t>();
+        // buckets the hash kernel hands back (a bin above its local limit),
+        // per class in [cls_off[c], ...), counted in fbc[c]
+        uint32_t *fb = static_cast<uint32_t *>(fallback_.ensure(nb * 4 + 16));
+        uint32_t *fbc = fb + nb;
+        k_fill_u32<<<1, 64, 0, s>>>(fbc, 4, 0);
+        KITE_CHECK_LAUNCH();
+        bool hashed = false;
         auto go = [&](auto cls, uint32_t first, uint32_t cnt) {
             constexpr int C = decltype(cls)::value;
             if (!cnt) return;
@@ -1607,9 +1550,10 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
                     constexpr int IPTH = W == 1 ? 4 : (W == 2 ? 2 : 1);
                     constexpr int NTH = SortClass<W, C>::CAP / IPTH;
                     launch("k_bucket_sort", 0, [&] {
-                        k_bucket_hash_sort<W, IPTH, NTH><<<dim3(cnt), dim3(NTH), 0, s>>>(A, B, ctmp, bkts, ord + first,
-                                                                                        ucount, k);
+                        k_bucket_hash_sort<W, IPTH, NTH><<<dim3(cnt), dim3(NTH), 0, s>>>(
+                            A, B, ctmp, bkts, ord + first, ucount, k, fb + first, fbc + C);
                     });
+                    hashed = true;
                     return;
                 }
             }
@@ -1622,6 +1566,25 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
         go(std::integral_constant<int, 1>{}, cls_off[1], cls_off[2] - cls_off[1]);
         go(std::integral_constant<int, 2>{}, cls_off[2], cls_off[3] - cls_off[2]);
         go(std::integral_constant<int, 3>{}, cls_off[3], cls_off[4] - cls_off[3]);
+        if (hashed) {
+            // the handed-back buckets of each class through the LSD kernel
+            uint32_t *hfb = reinterpret_cast<uint32_t *>(hv + 10);
+            to_host(fbc, 3, hfb);
+            KITE_HIP(hipStreamSynchronize(s));
+            const uint32_t nfb[3] = {hfb[0], hfb[1], hfb[2]};
+            n_fallback_ = nfb[0] + nfb[1] + nfb[2];
+            auto lsd = [&](auto cls, uint32_t first, uint32_t cnt) {
+                constexpr int C = decltype(cls)::value;
+                if (!cnt) return;
+                launch("k_bucket_sort", 0, [&] {
+                    k_bucket_sort<W, SortClass<W, C>::IPT><<<dim3(cnt), dim3(BT), 0, s>>>(A, B, ctmp, bkts, fb + first,
+                                                                                         ucount, k);
+                });
+            };
+            lsd(std::integral_constant<int, 0>{}, cls_off[0], nfb[0]);
+            lsd(std::integral_constant<int, 1>{}, cls_off[1], nfb[1]);
+            lsd(std::integral_constant<int, 2>{}, cls_off[2], nfb[2]);
+        }
     }
     tm.mark();
     uint64_t *stmp2 = static_cast<uint64_t *>(scan_tmp_.ensure(

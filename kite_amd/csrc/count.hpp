@@ -81,6 +81,8 @@ class CountEngine {
     void release();
     const StageTimes &times() const { return times_; }
     uint64_t buckets() const { return n_buckets_; }
+    // buckets the hash sort handed to the LSD sort in the last run
+    uint64_t fallback_buckets() const { return n_fallback_; }
     int levels() const { return levels_; }
     void set_timing(bool on) { timing_ = on; }
     void set_kernel_profile(bool on) { kprof_ = on; }
@@ -113,7 +115,7 @@ class CountEngine {
     DevBuf nl_, line0_, scan_tmp_, hist_, off_, part_off_d_;
     DevBuf a_, b_, counts_tmp_, out_keys_[2], out_counts_[2];
     int slot_ = 0;
-    DevBuf segs_, tile_pre_, child_hist_, child_cur_, ucount_, uoff_, part_first_, part_u_;
+    DevBuf segs_, tile_pre_, child_hist_, child_cur_, ucount_, uoff_, part_first_, part_u_, fallback_;
     DevBuf order_, wg_hist_;
     // device bucket plan: node lists (ping-pong), per-level scratch, classes
     DevBuf nodes_[2], npart_[2], act_, tiles_, act_rank_, tile_off_, lstats_, seg_node_;
@@ -122,7 +124,7 @@ class CountEngine {
     size_t pin_bytes_ = 0;
     uint64_t *pinned_host(size_t bytes);
     std::vector<uint64_t> part_off_h_;
-    uint64_t n_kmers_ = 0, n_unique_ = 0, n_buckets_ = 0;
+    uint64_t n_kmers_ = 0, n_unique_ = 0, n_buckets_ = 0, n_fallback_ = 0;
     int words_ = 1, levels_ = 0;
     StageTimes times_;
 };
