@@ -313,6 +313,71 @@ def bench_dump(w, fq) -> dict:
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+def bench_bgzf(ctx, w, fq, threads: int) -> dict:
+    """BGZF input: the workload's FASTQ as BGZF members (64 KiB pieces,
+    zlib level 6 on this host's cores), inflated on the device
+    (kite_gpu_inflate_bgzf: the inflate kernel, all members resident in
+    HBM) against zlib on the host cores, plus a whole count from the BGZF
+    bytes (device inflate + count) with the same result as from plain."""
+    import concurrent.futures as cf
+    import zlib
+    piece = 0xFF00
+    raw = fq.tobytes()
+
+    def member(i):
+        d = raw[i:i + piece]
+        c = zlib.compressobj(6, zlib.DEFLATED, -15, 9)
+        z = c.compress(d) + c.flush()
+        head = (bytes([0x1F, 0x8B, 8, 4, 0, 0, 0, 0, 0, 0xFF, 6, 0, 0x42, 0x43, 2, 0])
+                + (12 + 6 + len(z) + 8 - 1).to_bytes(2, "little"))
+        return head + z + (zlib.crc32(d) & 0xFFFFFFFF).to_bytes(4, "little") + len(d).to_bytes(4, "little")
+
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(threads) as ex:
+        members = list(ex.map(member, range(0, len(raw), piece)))
+    comp = b"".join(members)
+    log(f"bgzf: {len(members)} members, {len(comp) / 1e9:.3f} GB compressed in {time.perf_counter() - t0:.1f}s")
+    # device inflate, twice (the first call sizes the buffers)
+    out = None
+    best = None
+    for _ in range(2):
+        out, ms = ctx.inflate_bgzf(comp)
+        best = ms if best is None else min(best, ms)
+    ok = out == raw
+    del out
+    # zlib on the host cores, member by member (the reference's thread pool)
+    bounds = []
+    pos = 0
+    for m in members:
+        bounds.append((pos, len(m)))
+        pos += len(m)
+
+    def inflate_host(b):
+        o, n = b
+        return len(zlib.decompress(comp[o + 18:o + n - 8], -15))
+
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(threads) as ex:
+        host_bytes = sum(ex.map(inflate_host, bounds))
+    host_s = time.perf_counter() - t0
+    # a whole count from the BGZF bytes
+    t0 = time.perf_counter()
+    ctx.count_begin(w.k, w.canonical, w.partitions)
+    used = ctx.count_feed_bgzf(comp)
+    st = ctx.count_finish()
+    count_s = time.perf_counter() - t0
+    res = {"members": len(members), "compressed_bytes": len(comp), "fastq_bytes": len(raw),
+           "device_inflate_ms": best, "device_inflate_GBps": len(raw) / (best * 1e-3) / 1e9,
+           "inflate_bit_exact": ok,
+           "host_zlib": {"threads": threads, "seconds": host_s, "GBps": host_bytes / host_s / 1e9},
+           "count_from_bgzf": {"wall_s": count_s, "kmers": int(st.kmers), "unique": int(st.unique),
+                               "consumed_all": used == len(comp), "batches": int(st.batches),
+                               "includes": "host->device copy of the compressed bytes, device inflate, count"}}
+    log(f"bgzf: device inflate {best:.1f} ms ({res['device_inflate_GBps']:.1f} GB/s, exact={ok}); "
+        f"host zlib {threads} threads {host_s:.2f} s; count from bgzf {count_s:.3f} s")
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -326,6 +391,7 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the 24-sample batch and analyze legs")
     ap.add_argument("--no-analyze", action="store_true")
     ap.add_argument("--no-dump", action="store_true", help="skip the kite dump leg")
+    ap.add_argument("--no-bgzf", action="store_true", help="skip the BGZF inflate leg")
     ap.add_argument("--batch-scale", type=float, default=1.0, help="scale of the 24 read counts")
     ap.add_argument("--batch-reps", type=int, default=2)
     ap.add_argument("--analyze-cpu-parts", type=int, default=8,
@@ -420,6 +486,12 @@ def main() -> int:
     if not args.no_dump:
         dump = bench_dump(w, fq)
 
+    bgz = None
+    if not args.no_bgzf:
+        bgz = bench_bgzf(ctx, w, fq, args.cpu_threads or min(16, os.cpu_count() or 1))
+        if bgz["count_from_bgzf"]["kmers"] != kmers:
+            raise SystemExit("bgzf count differs from the plain count")
+
     batch = analyze = None
     if not args.no_batch:
         del dev
@@ -462,6 +534,7 @@ def main() -> int:
         "batch24": batch,
         "analyze": analyze,
         "dump": dump,
+        "bgzf": bgz,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

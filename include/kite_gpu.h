@@ -92,6 +92,26 @@ int kite_gpu_count_feed(kite_gpu_ctx *ctx, const uint8_t *fastq, size_t len);
  * buffer must stay valid until kite_gpu_count_finish returns. */
 int kite_gpu_count_feed_device(kite_gpu_ctx *ctx, const void *d_fastq, size_t len);
 
+/* Feeds BGZF-compressed FASTQ: whole BGZF members from data[0, len), the
+ * members scanned as the reference's BGZF reader does (is_bgzf_header /
+ * bgzf_block_total_size, src/count/bgzf.rs:37-101; non-BGZF data is an
+ * error).  A truncated last member is not consumed: *consumed receives the
+ * bytes of the whole members, and the caller passes the rest again with
+ * more data.  Members are inflated on the device, one per thread, straight
+ * into the FASTQ staging buffer (the thread pool of decompress_bgzf_parallel,
+ * src/count/bgzf.rs:152-264).  Batches are cut after the last whole record
+ * (find_fastq_split_4lines, :109-121); a batch that does not start with '@'
+ * yields no k-mers, as a reader chunk.  One session takes either plain
+ * chunks or BGZF input. */
+int kite_gpu_count_feed_bgzf(kite_gpu_ctx *ctx, const uint8_t *data, size_t len, size_t *consumed);
+
+/* Inflates the whole BGZF members of data[0, len) on the device into
+ * out[0, cap) (host memory).  *out_len receives the output size, also when
+ * cap is too small (then KITE_E_INVALID); *device_ms (may be NULL) the
+ * inflate kernel time. */
+int kite_gpu_inflate_bgzf(kite_gpu_ctx *ctx, const uint8_t *data, size_t len, uint8_t *out, size_t cap,
+                          size_t *out_len, double *device_ms);
+
 /* Runs the device pipeline on everything fed; results stay on the device. */
 int kite_gpu_count_finish(kite_gpu_ctx *ctx, kite_count_stats *stats);
 

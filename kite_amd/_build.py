@@ -30,7 +30,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
-GPU_SOURCES = ["scan.hip", "count.hip", "merge.hip", "corr.hip", "encode.hip", "dump.hip", "abi.hip"]
+GPU_SOURCES = ["scan.hip", "count.hip", "merge.hip", "corr.hip", "encode.hip", "dump.hip", "inflate.hip", "abi.hip"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -96,6 +96,9 @@ GPU_SYMBOLS = {
     "kite_gpu_count_begin": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_uint32]),
     "kite_gpu_count_feed": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "kite_gpu_count_feed_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "kite_gpu_count_feed_bgzf": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "kite_gpu_inflate_bgzf": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t,
+                                        C.POINTER(C.c_size_t), C.POINTER(C.c_double)]),
     "kite_gpu_count_finish": (C.c_int, [C.c_void_p, C.POINTER(CountStats)]),
     "kite_gpu_count_emit": (C.c_int, [C.c_void_p, APPEND_FN, C.c_void_p]),
     "kite_gpu_count_result_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64),
@@ -254,6 +257,29 @@ class GpuContext:
 
     def count_feed_device(self, dev_ptr: int, nbytes: int) -> None:
         _check(self._lib.kite_gpu_count_feed_device(self._h, C.c_void_p(dev_ptr), nbytes))
+
+    def count_feed_bgzf(self, data) -> int:
+        """Feed BGZF bytes (whole members are consumed); returns the bytes
+        consumed."""
+        arr = np.frombuffer(bytes(data), dtype=np.uint8) if not isinstance(data, np.ndarray) else data
+        used = C.c_size_t(0)
+        _check(self._lib.kite_gpu_count_feed_bgzf(self._h, arr.ctypes.data if arr.size else None, arr.size,
+                                                  C.byref(used)))
+        return used.value
+
+    def inflate_bgzf(self, data) -> tuple[bytes, float]:
+        """Inflate the whole BGZF members of `data` on the device."""
+        arr = np.frombuffer(bytes(data), dtype=np.uint8) if not isinstance(data, np.ndarray) else data
+        need = C.c_size_t(0)
+        ms = C.c_double(0)
+        rc = self._lib.kite_gpu_inflate_bgzf(self._h, arr.ctypes.data if arr.size else None, arr.size, None, 0,
+                                             C.byref(need), None)
+        if rc != 0 and need.value == 0:
+            _check(rc)
+        out = np.empty(max(need.value, 1), dtype=np.uint8)
+        _check(self._lib.kite_gpu_inflate_bgzf(self._h, arr.ctypes.data if arr.size else None, arr.size,
+                                               out.ctypes.data, out.size, C.byref(need), C.byref(ms)))
+        return out[:need.value].tobytes(), ms.value
 
     def count_finish(self) -> CountStats:
         st = CountStats()

@@ -17,6 +17,7 @@
 #include "corr.hpp"
 #include "dump.hpp"
 #include "encode.hpp"
+#include "inflate.hpp"
 #include "merge.hpp"
 
 namespace {
@@ -120,6 +121,13 @@ struct kite_gpu_ctx {
     // dump: staged payloads of a partition group, pinned text pieces
     kite::HostPinned dump_stage, dump_text;
     kite::DevBuf dump_pay;
+
+    // BGZF input inflated on the device into `staging`: the staged bytes
+    // then hold whole records plus the start of the next one
+    bool bgzf_feed = false;
+    kite::HostPinned bgzf_host;
+    kite::DevBuf bgzf_comp, bgzf_blocks, bgzf_misc, bgzf_tmp;
+    double inflate_ms = 0;
 };
 
 namespace {
@@ -173,6 +181,19 @@ void run_batch(kite_gpu_ctx *c, const uint8_t *ptr, uint64_t len) {
     c->stats.compact_ms = t.compact_ms;
 }
 
+// A batch of inflated BGZF text starts at a record boundary; like a reader
+// chunk it yields no k-mers unless it starts with '@' (src/count/parser.rs:
+// 24-27 with the error ignored at src/count/worker.rs:217).
+void run_bgzf_batch(kite_gpu_ctx *c, uint64_t len) {
+    uint8_t first = 0;
+    KITE_HIP(hipMemcpy(&first, c->staging.ptr, 1, hipMemcpyDeviceToHost));
+    if (first != '@') {
+        c->stats.skipped_chunks++;
+        return;
+    }
+    run_batch(c, c->staging.as<uint8_t>(), len);
+}
+
 void flush_pending(kite_gpu_ctx *c) {
     if (c->direct) {
         const uint8_t *p = c->direct;
@@ -181,9 +202,90 @@ void flush_pending(kite_gpu_ctx *c) {
         c->direct_len = 0;
         run_batch(c, p, n);
     } else if (c->staged) {
-        run_batch(c, c->staging.as<uint8_t>(), c->staged);
+        if (c->bgzf_feed) run_bgzf_batch(c, c->staged);
+        else run_batch(c, c->staging.as<uint8_t>(), c->staged);
         c->staged = 0;
     }
+}
+
+// Inflates the whole BGZF members ms[m0, m1) of `data` into dst (device);
+// returns their output bytes.
+uint64_t inflate_members(kite_gpu_ctx *c, const uint8_t *data, const std::vector<kite::BgzfMember> &ms, size_t m0,
+                         size_t m1, uint8_t *dst) {
+    if (m0 >= m1) return 0;
+    const uint64_t a = ms[m0].offset, b = ms[m1 - 1].offset + ms[m1 - 1].size;
+    uint8_t *h = static_cast<uint8_t *>(c->bgzf_host.ensure(b - a + 16));
+    memcpy(h, data + a, b - a);
+    memset(h + (b - a), 0, 16);
+    uint8_t *dc = static_cast<uint8_t *>(c->bgzf_comp.ensure(b - a + 16));
+    KITE_HIP(hipMemcpyAsync(dc, h, b - a + 16, hipMemcpyHostToDevice, c->stream));
+    std::vector<kite::InflateBlock> bl;
+    bl.reserve(m1 - m0);
+    uint64_t out = 0;
+    for (size_t m = m0; m < m1; m++) {
+        bl.push_back(kite::InflateBlock{ms[m].offset - a + ms[m].data, out, ms[m].clen, ms[m].isize});
+        out += ms[m].isize;
+    }
+    uint32_t *err = static_cast<uint32_t *>(c->bgzf_misc.ensure(64));
+    KITE_HIP(hipMemsetAsync(err, 0, 4, c->stream));
+    hipEvent_t e0, e1;
+    KITE_HIP(hipEventCreate(&e0));
+    KITE_HIP(hipEventCreate(&e1));
+    KITE_HIP(hipEventRecord(e0, c->stream));
+    kite::inflate_blocks(dc, bl, dst, err, c->bgzf_blocks, c->stream);
+    KITE_HIP(hipEventRecord(e1, c->stream));
+    uint32_t he = 0;
+    KITE_HIP(hipMemcpyAsync(&he, err, 4, hipMemcpyDeviceToHost, c->stream));
+    KITE_HIP(hipStreamSynchronize(c->stream));
+    float ms_ = 0;
+    (void)hipEventElapsedTime(&ms_, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->inflate_ms += ms_;
+    if (he & 1u) throw kite::Error("invalid deflate data in a BGZF member");
+    if (he & 2u) throw kite::Error("BGZF member output differs from its ISIZE");
+    return out;
+}
+
+// Runs the staged BGZF text up to its last record boundary (the end of a
+// 4-line group, find_fastq_split_4lines, src/count/bgzf.rs:109-121) and keeps
+// the rest at the front of the staging buffer.  False if the staged text
+// holds no whole record.
+bool cut_and_run_bgzf(kite_gpu_ctx *c) {
+    const uint64_t n = c->staged;
+    unsigned long long *cnt = reinterpret_cast<unsigned long long *>(c->bgzf_misc.as<uint8_t>() + 8);
+    KITE_HIP(hipMemsetAsync(cnt, 0, 8, c->stream));
+    kite::count_newlines_device(c->staging.as<uint8_t>(), n, cnt, c->stream);
+    unsigned long long total = 0;
+    KITE_HIP(hipMemcpyAsync(&total, cnt, 8, hipMemcpyDeviceToHost, c->stream));
+    KITE_HIP(hipStreamSynchronize(c->stream));
+    if (total < 4) return false;
+    // the cut follows the (r+1)-th newline from the end, r = total mod 4
+    const uint64_t need = total % 4 + 1;
+    uint64_t cut = 0;
+    std::vector<uint8_t> win;
+    for (uint64_t w = std::min<uint64_t>(n, 1 << 16);; w = std::min<uint64_t>(n, 4 * w)) {
+        win.resize(w);
+        KITE_HIP(hipMemcpy(win.data(), c->staging.as<uint8_t>() + (n - w), w, hipMemcpyDeviceToHost));
+        uint64_t seen = 0;
+        for (uint64_t q = w; q-- > 0;)
+            if (win[q] == '\n' && ++seen == need) {
+                cut = n - w + q + 1;
+                break;
+            }
+        if (cut || w == n) break;
+    }
+    if (!cut) return false;
+    run_bgzf_batch(c, cut);
+    const uint64_t tail = n - cut;
+    if (tail) {
+        uint8_t *tmp = static_cast<uint8_t *>(c->bgzf_tmp.ensure(tail));
+        KITE_HIP(hipMemcpyAsync(tmp, c->staging.as<uint8_t>() + cut, tail, hipMemcpyDeviceToDevice, c->stream));
+        KITE_HIP(hipMemcpyAsync(c->staging.ptr, tmp, tail, hipMemcpyDeviceToDevice, c->stream));
+        KITE_HIP(hipStreamSynchronize(c->stream));
+    }
+    c->staged = tail;
+    return true;
 }
 
 uint64_t count_newlines(const uint8_t *p, size_t n) {
@@ -240,6 +342,11 @@ void kite_gpu_close(kite_gpu_ctx *ctx) {
         ctx->merge.reset();
         ctx->corr.reset();
         ctx->staging.release();
+        ctx->bgzf_host.release();
+        ctx->bgzf_comp.release();
+        ctx->bgzf_blocks.release();
+        ctx->bgzf_misc.release();
+        ctx->bgzf_tmp.release();
         ctx->dump_stage.release();
         ctx->dump_text.release();
         ctx->dump_pay.release();
@@ -287,12 +394,16 @@ int kite_gpu_count_begin(kite_gpu_ctx *ctx, uint32_t k, int canonical, uint32_t 
         ctx->host_runs.clear();
         ctx->stats = kite_count_stats{};
         ctx->stats.words = (uint32_t)ctx->words;
+        ctx->bgzf_feed = false;
+        ctx->inflate_ms = 0;
         return KITE_OK;
     });
 }
 
 int kite_gpu_count_feed(kite_gpu_ctx *ctx, const uint8_t *fastq, size_t len) {
     if (!ctx || !ctx->session) return set_error(KITE_E_INVALID, "invalid state: no count session");
+    if (ctx->bgzf_feed && ctx->staged)
+        return set_error(KITE_E_INVALID, "invalid state: plain chunks after BGZF input in one count session");
     if (len == 0) return KITE_OK;
     if (!fastq) return set_error(KITE_E_INVALID, "invalid argument: fastq is NULL");
     if (fastq[0] != '@') {
@@ -331,6 +442,62 @@ int kite_gpu_count_feed_device(kite_gpu_ctx *ctx, const void *d_fastq, size_t le
         ctx->direct = static_cast<const uint8_t *>(d_fastq);
         ctx->direct_len = len;
         ctx->stats.input_bytes += len;
+        return KITE_OK;
+    });
+}
+
+int kite_gpu_count_feed_bgzf(kite_gpu_ctx *ctx, const uint8_t *data, size_t len, size_t *consumed) {
+    if (!ctx || !ctx->session) return set_error(KITE_E_INVALID, "invalid state: no count session");
+    if (consumed) *consumed = 0;
+    if (len == 0) return KITE_OK;
+    if (!data) return set_error(KITE_E_INVALID, "invalid argument: data is NULL");
+    if (!ctx->bgzf_feed && (ctx->staged || ctx->direct))
+        return set_error(KITE_E_INVALID, "invalid state: BGZF input after plain chunks in one count session");
+    return guarded([&]() -> int {
+        bind(ctx);
+        size_t used = 0;
+        const std::vector<kite::BgzfMember> ms = kite::scan_bgzf(data, len, &used);
+        ctx->bgzf_feed = true;
+        size_t m = 0;
+        while (m < ms.size()) {
+            size_t j = m;
+            uint64_t add = 0;
+            while (j < ms.size() && ctx->staged + add + ms[j].isize <= ctx->batch_cap) add += ms[j++].isize;
+            if (j == m) {
+                // the batch is full: run it up to its last record, or grow it
+                // when it holds no whole record
+                if (cut_and_run_bgzf(ctx)) continue;
+                add = ms[m].isize;
+                j = m + 1;
+            }
+            staging_reserve(ctx, ctx->staged + add + 16);
+            const uint64_t out = inflate_members(ctx, data, ms, m, j, ctx->staging.as<uint8_t>() + ctx->staged);
+            ctx->staged += out;
+            ctx->stats.input_bytes += out;
+            m = j;
+        }
+        if (consumed) *consumed = used;
+        return KITE_OK;
+    });
+}
+
+int kite_gpu_inflate_bgzf(kite_gpu_ctx *ctx, const uint8_t *data, size_t len, uint8_t *out, size_t cap,
+                          size_t *out_len, double *device_ms) {
+    if (!ctx || !out_len) return set_error(KITE_E_INVALID, "invalid argument: NULL pointer");
+    if (len && !data) return set_error(KITE_E_INVALID, "invalid argument: data is NULL");
+    return guarded([&]() -> int {
+        bind(ctx);
+        const std::vector<kite::BgzfMember> ms = kite::scan_bgzf(data, len, nullptr);
+        uint64_t total = 0;
+        for (const auto &m : ms) total += m.isize;
+        *out_len = total;
+        if (total > cap) return set_error(KITE_E_INVALID, "invalid argument: output buffer too small");
+        if (!out && total) return set_error(KITE_E_INVALID, "invalid argument: out is NULL");
+        const double before = ctx->inflate_ms;
+        uint8_t *dev = static_cast<uint8_t *>(ctx->bgzf_tmp.ensure(total + 16));
+        inflate_members(ctx, data, ms, 0, ms.size(), dev);
+        if (total) KITE_HIP(hipMemcpy(out, dev, total, hipMemcpyDeviceToHost));
+        if (device_ms) *device_ms = ctx->inflate_ms - before;
         return KITE_OK;
     });
 }

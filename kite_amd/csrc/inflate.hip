@@ -1,0 +1,338 @@
+// inflate.hip -- DEFLATE (RFC 1951) decoding of BGZF blocks on the device.
+//
+// BGZF (SAM/BAM specification, section 4.1) is gzip made of independent
+// members of at most 64 KiB of output, each with its compressed size in the
+// BC extra field and its output size (ISIZE) in the trailer.  The reference
+// scans the members and inflates them on a thread pool
+// (src/count/bgzf.rs:37-101, 152-264); here every member is one device
+// thread, writing straight to its offset in the FASTQ staging buffer, so the
+// decompressed text never crosses PCIe.
+//
+// Decoding follows RFC 1951: stored, fixed-Huffman and dynamic-Huffman
+// blocks; canonical Huffman codes decoded from per-length code counts and a
+// symbol list sorted by code (section 3.2.2), kept in LDS per thread; length
+// and distance bases and extra bits from section 3.2.5; back references up
+// to 32 KiB read from the thread's own output.
+#include <algorithm>
+#include <cstring>
+
+#include "inflate.hpp"
+
+namespace kite {
+
+namespace {
+
+constexpr int IT = 64;  // threads (blocks) per workgroup
+
+// RFC 1951 3.2.5: length codes 257..285 and distance codes 0..29
+__constant__ uint16_t c_lbase[29] = {3,  4,  5,  6,  7,  8,  9,  10, 11,  13,  15,  17,  19,  23, 27,
+                                     31, 35, 43, 51, 59, 67, 83, 99, 115, 131, 163, 195, 227, 258};
+__constant__ uint8_t c_lext[29] = {0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 2, 3, 3, 3, 3, 4, 4, 4, 4, 5, 5, 5, 5, 0};
+__constant__ uint16_t c_dbase[30] = {1,   2,   3,   4,   5,   7,    9,    13,   17,   25,   33,   49,   65,    97,    129,
+                                     193, 257, 385, 513, 769, 1025, 1537, 2049, 3073, 4097, 6145, 8193, 12289, 16385, 24577};
+__constant__ uint8_t c_dext[30] = {0, 0, 0, 0, 1, 1, 2, 2, 3, 3, 4, 4, 5, 5, 6, 6, 7, 7, 8, 8, 9, 9, 10, 10, 11, 11, 12, 12, 13, 13};
+// RFC 1951 3.2.7: order of the code length code lengths
+__constant__ uint8_t c_clorder[19] = {16, 17, 18, 0, 8, 7, 9, 6, 10, 5, 11, 4, 12, 3, 13, 2, 14, 1, 15};
+
+enum : uint32_t { E_STREAM = 1, E_SIZE = 2 };
+
+// LSB-first bit reader over [p, end); reads past the end supply zero bytes
+// and are counted, so that consuming them is detected.
+struct Bits {
+    const uint8_t *p, *end;
+    uint64_t buf;
+    int cnt;
+    uint32_t pad;
+    __device__ void fill(int n) {
+        while (cnt < n) {
+            uint64_t v = 0;
+            if (p < end) v = *p++;
+            else pad++;
+            buf |= v << cnt;
+            cnt += 8;
+        }
+    }
+    __device__ uint32_t get(int n) {
+        if (n == 0) return 0;
+        fill(n);
+        const uint32_t r = (uint32_t)(buf & ((1ull << n) - 1));
+        buf >>= n;
+        cnt -= n;
+        return r;
+    }
+    // bits consumed beyond the real input
+    __device__ bool overrun() const { return (int)(8 * pad) > cnt; }
+};
+
+// Canonical code of n symbols from their lengths: count[len] codes per
+// length, symbol[] ordered by code.  Returns < 0 if over-subscribed, > 0 if
+// incomplete, 0 if complete.
+__device__ int construct(uint16_t *count, uint16_t *symbol, uint16_t *offs, const uint8_t *length, int n) {
+    for (int len = 0; len <= 15; len++) count[len] = 0;
+    for (int s = 0; s < n; s++) count[length[s]]++;
+    if (count[0] == n) return 0;
+    int left = 1;
+    for (int len = 1; len <= 15; len++) {
+        left <<= 1;
+        left -= count[len];
+        if (left < 0) return left;
+    }
+    offs[1] = 0;
+    for (int len = 1; len < 15; len++) offs[len + 1] = offs[len] + count[len];
+    for (int s = 0; s < n; s++)
+        if (length[s]) symbol[offs[length[s]]++] = (uint16_t)s;
+    return left;
+}
+
+// One symbol: the code's bits arrive most significant first.
+__device__ int decode(Bits &b, const uint16_t *count, const uint16_t *symbol) {
+    b.fill(15);
+    uint64_t bits = b.buf;
+    int code = 0, first = 0, index = 0;
+    for (int len = 1; len <= 15; len++) {
+        code |= (int)(bits & 1u);
+        bits >>= 1;
+        const int c = count[len];
+        if (code - c < first) {
+            b.buf >>= len;
+            b.cnt -= len;
+            return symbol[index + (code - first)];
+        }
+        index += c;
+        first += c;
+        first <<= 1;
+        code <<= 1;
+    }
+    return -1;
+}
+
+struct Tables {
+    uint16_t lcount[16], lsym[288];
+    uint16_t dcount[16], dsym[32];
+    uint16_t offs[16];
+    uint8_t lengths[320];
+};
+
+__global__ __launch_bounds__(IT) void k_inflate(const uint8_t *__restrict__ comp, const InflateBlock *__restrict__ blocks,
+                                                uint32_t nb, uint8_t *__restrict__ out, uint32_t *__restrict__ err) {
+    extern __shared__ uint16_t dyn_tab[];  // IT Tables (about 66 KiB)
+    const uint32_t i = blockIdx.x * IT + threadIdx.x;
+    if (i >= nb) return;
+    Tables &t = reinterpret_cast<Tables *>(dyn_tab)[threadIdx.x];
+    const InflateBlock bk = blocks[i];
+    uint8_t *o = out + bk.dst;
+    const uint32_t osz = bk.isize;
+    Bits b{comp + bk.src, comp + bk.src + bk.clen, 0, 0, 0};
+    uint32_t pos = 0;
+    uint32_t e = 0;
+    for (;;) {
+        const uint32_t last = b.get(1);
+        const uint32_t type = b.get(2);
+        if (type == 0) {
+            // stored: to the byte boundary, LEN, NLEN, LEN raw bytes
+            b.get(b.cnt & 7);
+            const uint32_t len = b.get(16), nlen = b.get(16);
+            if (len != (~nlen & 0xFFFFu) || pos + len > osz) {
+                e = E_STREAM;
+                break;
+            }
+            for (uint32_t q = 0; q < len; q++) o[pos++] = (uint8_t)b.get(8);
+        } else if (type == 1 || type == 2) {
+            if (type == 1) {
+                // fixed codes (RFC 1951 3.2.6)
+                for (int s = 0; s < 288; s++) t.lengths[s] = s < 144 ? 8 : (s < 256 ? 9 : (s < 280 ? 7 : 8));
+                construct(t.lcount, t.lsym, t.offs, t.lengths, 288);
+                for (int s = 0; s < 30; s++) t.lengths[s] = 5;
+                construct(t.dcount, t.dsym, t.offs, t.lengths, 30);
+            } else {
+                const int nlen = (int)b.get(5) + 257, ndist = (int)b.get(5) + 1, ncode = (int)b.get(4) + 4;
+                if (nlen > 286 || ndist > 30) {
+                    e = E_STREAM;
+                    break;
+                }
+                for (int q = 0; q < 19; q++) t.lengths[c_clorder[q]] = q < ncode ? (uint8_t)b.get(3) : 0;
+                // the code length code must be complete (t.dcount/dsym hold it)
+                if (construct(t.dcount, t.dsym, t.offs, t.lengths, 19) != 0) {
+                    e = E_STREAM;
+                    break;
+                }
+                int idx = 0;
+                while (idx < nlen + ndist) {
+                    int sym = decode(b, t.dcount, t.dsym);
+                    if (sym < 0) {
+                        e = E_STREAM;
+                        break;
+                    }
+                    if (sym < 16) {
+                        t.lengths[idx++] = (uint8_t)sym;
+                        continue;
+                    }
+                    uint8_t len = 0;
+                    int rep;
+                    if (sym == 16) {
+                        if (idx == 0) {
+                            e = E_STREAM;
+                            break;
+                        }
+                        len = t.lengths[idx - 1];
+                        rep = 3 + (int)b.get(2);
+                    } else if (sym == 17) {
+                        rep = 3 + (int)b.get(3);
+                    } else {
+                        rep = 11 + (int)b.get(7);
+                    }
+                    if (idx + rep > nlen + ndist) {
+                        e = E_STREAM;
+                        break;
+                    }
+                    while (rep--) t.lengths[idx++] = len;
+                }
+                if (e) break;
+                if (t.lengths[256] == 0) {  // no end-of-block code
+                    e = E_STREAM;
+                    break;
+                }
+                int r = construct(t.lcount, t.lsym, t.offs, t.lengths, nlen);
+                if (r < 0 || (r > 0 && nlen - t.lcount[0] != 1)) {
+                    e = E_STREAM;
+                    break;
+                }
+                r = construct(t.dcount, t.dsym, t.offs, t.lengths + nlen, ndist);
+                if (r < 0 || (r > 0 && ndist - t.dcount[0] != 1)) {
+                    e = E_STREAM;
+                    break;
+                }
+            }
+            // literals, lengths and distances up to the end-of-block code
+            for (;;) {
+                int sym = decode(b, t.lcount, t.lsym);
+                if (sym < 0) {
+                    e = E_STREAM;
+                    break;
+                }
+                if (sym < 256) {
+                    if (pos >= osz) {
+                        e = E_SIZE;
+                        break;
+                    }
+                    o[pos++] = (uint8_t)sym;
+                    continue;
+                }
+                if (sym == 256) break;
+                sym -= 257;
+                if (sym >= 29) {
+                    e = E_STREAM;
+                    break;
+                }
+                const uint32_t len = c_lbase[sym] + b.get(c_lext[sym]);
+                const int ds = decode(b, t.dcount, t.dsym);
+                if (ds < 0 || ds >= 30) {
+                    e = E_STREAM;
+                    break;
+                }
+                const uint32_t dist = c_dbase[ds] + b.get(c_dext[ds]);
+                if (dist > pos) {
+                    e = E_STREAM;
+                    break;
+                }
+                if (pos + len > osz) {
+                    e = E_SIZE;
+                    break;
+                }
+                for (uint32_t q = 0; q < len; q++, pos++) o[pos] = o[pos - dist];
+            }
+            if (e) break;
+        } else {
+            e = E_STREAM;
+            break;
+        }
+        if (b.overrun()) {
+            e = E_STREAM;
+            break;
+        }
+        if (last) break;
+    }
+    if (!e && pos != osz) e = E_SIZE;
+    if (e) atomicOr(err, e);
+}
+
+// Number of '\n' bytes in d[0, n) added to *total.
+__global__ __launch_bounds__(256) void k_count_newlines(const uint8_t *__restrict__ d, uint64_t n,
+                                                        unsigned long long *__restrict__ total) {
+    __shared__ uint32_t ws[256 / 64 + 1];
+    uint32_t c = 0;
+    const uint64_t nv = n / 16;
+    for (uint64_t v = (uint64_t)blockIdx.x * 256 + threadIdx.x; v < nv; v += (uint64_t)gridDim.x * 256) {
+        const uint4 w = reinterpret_cast<const uint4 *>(d)[v];
+        const uint32_t x[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint32_t y = x[q] ^ 0x0A0A0A0Au;
+            c += __popc(~(((y & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | y | 0x7F7F7F7Fu));
+        }
+    }
+    if (blockIdx.x == 0)
+        for (uint64_t i = nv * 16 + threadIdx.x; i < n; i += 256) c += d[i] == '\n';
+    uint32_t sum;
+    (void)block_excl_scan<256, uint32_t>(c, ws, &sum);
+    if (threadIdx.x == 0 && sum) atomicAdd(total, (unsigned long long)sum);
+}
+
+}  // namespace
+
+void count_newlines_device(const uint8_t *d, uint64_t n, unsigned long long *d_total, hipStream_t s) {
+    if (!n) return;
+    const uint64_t wgs = std::min<uint64_t>(std::max<uint64_t>(1, n / (256 * 16 * 8)), 4096);
+    k_count_newlines<<<dim3((unsigned)wgs), dim3(256), 0, s>>>(d, n, d_total);
+    KITE_CHECK_LAUNCH();
+}
+
+std::vector<BgzfMember> scan_bgzf(const uint8_t *d, size_t len, size_t *consumed) {
+    std::vector<BgzfMember> out;
+    size_t pos = 0;
+    // is_bgzf_header / bgzf_block_total_size (src/count/bgzf.rs:37-101)
+    while (pos + 18 <= len) {
+        const uint8_t *h = d + pos;
+        if (!(h[0] == 0x1f && h[1] == 0x8b && (h[3] & 0x04) && h[12] == 0x42 && h[13] == 0x43 && h[14] == 0x02 &&
+              h[15] == 0x00))
+            throw Error("Non-BGZF data at offset " + std::to_string(pos) + " (expected gzip+BGZF header)");
+        const uint32_t bsize = (uint32_t)(h[16] | (h[17] << 8)) + 1;
+        if (pos + bsize > len) break;  // truncated last member: left for the caller
+        const uint32_t xlen = (uint32_t)(h[10] | (h[11] << 8));
+        if (h[2] != 8 || 12 + xlen + 8 > bsize)
+            throw Error("invalid BGZF member at offset " + std::to_string(pos));
+        // only FEXTRA may be set besides the always-present fields
+        if (h[3] & 0x1a) throw Error("unsupported gzip header flags in BGZF member at offset " + std::to_string(pos));
+        BgzfMember m;
+        m.offset = pos;
+        m.size = bsize;
+        m.data = 12 + xlen;
+        m.clen = bsize - 8 - m.data;
+        const uint8_t *tr = h + bsize - 4;
+        m.isize = (uint32_t)(tr[0] | (tr[1] << 8) | (tr[2] << 16) | ((uint32_t)tr[3] << 24));
+        if (m.isize > 65536) throw Error("BGZF member above 64 KiB of output at offset " + std::to_string(pos));
+        out.push_back(m);
+        pos += bsize;
+    }
+    if (consumed) *consumed = pos;
+    return out;
+}
+
+void inflate_blocks(const uint8_t *d_comp, const std::vector<InflateBlock> &blocks, uint8_t *d_out, uint32_t *d_err,
+                    DevBuf &d_blocks, hipStream_t s) {
+    if (blocks.empty()) return;
+    InflateBlock *db = static_cast<InflateBlock *>(d_blocks.ensure(blocks.size() * sizeof(InflateBlock)));
+    KITE_HIP(hipMemcpyAsync(db, blocks.data(), blocks.size() * sizeof(InflateBlock), hipMemcpyHostToDevice, s));
+    const uint32_t nb = (uint32_t)blocks.size();
+    const size_t lds = sizeof(Tables) * IT;
+    static bool attr = false;
+    if (!attr) {
+        KITE_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_inflate),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        attr = true;
+    }
+    k_inflate<<<dim3((nb + IT - 1) / IT), dim3(IT), lds, s>>>(d_comp, db, nb, d_out, d_err);
+    KITE_CHECK_LAUNCH();
+}
+
+}  // namespace kite

@@ -408,7 +408,31 @@ int run_count(int argc, char **argv) {
     } closer{ctx};
     gpu_check(kite_gpu_count_begin(ctx, (uint32_t)k, canonical, (uint32_t)pl.partitions), "count");
     uint64_t chunks = 0;
-    {
+    // BGZF members are inflated on the device unless KITE_BGZF_ENGINE=host
+    // (then zlib on this host, as for plain gzip)
+    const char *be = getenv("KITE_BGZF_ENGINE");
+    if (comp == Comp::Bgzf && !(be && std::string(be) == "host")) {
+        int fd = open(input.c_str(), O_RDONLY);
+        if (fd < 0) throw Error("failed to open input " + input + ": " + strerror(errno));
+        struct Fd {
+            int f;
+            ~Fd() { close(f); }
+        } fdc{fd};
+        std::vector<uint8_t> buf(64 * MIB);
+        size_t have = 0;
+        for (;;) {
+            ssize_t r = ::read(fd, buf.data() + have, buf.size() - have);
+            if (r < 0) throw Error("read error on BGZF input");
+            have += (size_t)r;
+            size_t used = 0;
+            gpu_check(kite_gpu_count_feed_bgzf(ctx, buf.data(), have, &used), "count");
+            if (used) chunks++;
+            memmove(buf.data(), buf.data() + used, have - used);
+            have -= used;
+            if (r == 0) break;  // a truncated last member is left unread (src/count/bgzf.rs:84-87)
+            if (have == buf.size()) buf.resize(buf.size() * 2);
+        }
+    } else {
         Source src(input, comp);
         Chunker ch(src, pl.chunk);
         std::vector<uint8_t> c;
