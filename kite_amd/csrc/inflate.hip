@@ -5,14 +5,15 @@
 // BC extra field and its output size (ISIZE) in the trailer.  The reference
 // scans the members and inflates them on a thread pool
 // (src/count/bgzf.rs:37-101, 152-264); here every member is one device
-// thread, writing straight to its offset in the FASTQ staging buffer, so the
-// decompressed text never crosses PCIe.
+// workgroup (one wave), writing straight to its offset in the FASTQ staging
+// buffer, so the decompressed text never crosses PCIe.
 //
 // Decoding follows RFC 1951: stored, fixed-Huffman and dynamic-Huffman
-// blocks; canonical Huffman codes decoded from per-length code counts and a
-// symbol list sorted by code (section 3.2.2), kept in LDS per thread; length
-// and distance bases and extra bits from section 3.2.5; back references up
-// to 32 KiB read from the thread's own output.
+// blocks; canonical Huffman codes (section 3.2.2) decoded through a direct
+// table of their first 10 (8) bits, longer codes from per-length code counts
+// and the symbol list sorted by code; length and distance bases and extra
+// bits from section 3.2.5; back references up to 32 KiB read from the
+// member's own output.
 #include <algorithm>
 #include <cstring>
 
@@ -21,8 +22,6 @@
 namespace kite {
 
 namespace {
-
-constexpr int IT = 64;  // threads (blocks) per workgroup
 
 // RFC 1951 3.2.5: length codes 257..285 and distance codes 0..29
 __constant__ uint16_t c_lbase[29] = {3,  4,  5,  6,  7,  8,  9,  10, 11,  13,  15,  17,  19,  23, 27,
@@ -106,154 +105,276 @@ __device__ int decode(Bits &b, const uint16_t *count, const uint16_t *symbol) {
     return -1;
 }
 
-struct Tables {
+// Per-member state in LDS: canonical codes, the direct tables of the
+// first LB / DB code bits, and the token queue.
+constexpr int LB = 10;    // literal/length table bits
+constexpr int DB = 8;     // distance table bits
+constexpr int NTOK = 1024;
+constexpr uint32_t TOK_LIT = 0x80000000u;
+
+struct Member {
+    uint16_t lut_l[1 << LB], lut_d[1 << DB];  // (code length << 9) | symbol, 0 = longer code
     uint16_t lcount[16], lsym[288];
     uint16_t dcount[16], dsym[32];
     uint16_t offs[16];
     uint8_t lengths[320];
+    uint32_t tok[NTOK];  // literal: TOK_LIT | byte; match: (length << 16) | distance - 1
+    uint32_t ntok, done, err;
 };
 
-__global__ __launch_bounds__(IT) void k_inflate(const uint8_t *__restrict__ comp, const InflateBlock *__restrict__ blocks,
+// Direct table of the codes up to `bits` long: entry [reversed code + k <<
+// length] for every k, so that the next `bits` stream bits index it.
+__device__ void fill_lut(uint16_t *lut, int bits, const uint16_t *count, const uint16_t *symbol) {
+    for (int i = 0; i < (1 << bits); i++) lut[i] = 0;
+    int code = 0, idx = 0;
+    for (int len = 1; len <= bits; len++) {
+        for (int q = 0; q < count[len]; q++, idx++, code++) {
+            const uint32_t r = __builtin_bitreverse32((uint32_t)code) >> (32 - len);
+            const uint16_t e = (uint16_t)((len << 9) | symbol[idx]);
+            for (uint32_t k = r; k < (1u << bits); k += 1u << len) lut[k] = e;
+        }
+        code <<= 1;
+    }
+}
+
+__device__ __forceinline__ int decode_fast(Bits &b, const uint16_t *lut, int bits, const uint16_t *count,
+                                           const uint16_t *symbol) {
+    b.fill(15);
+    const uint16_t e = lut[b.buf & ((1u << bits) - 1)];
+    if (e >> 9) {
+        const int len = e >> 9;
+        b.buf >>= len;
+        b.cnt -= len;
+        return e & 511;
+    }
+    return decode(b, count, symbol);
+}
+
+// Block header: the tables of a fixed or dynamic block (lane 0).
+__device__ uint32_t read_tables(Bits &b, Member &t, uint32_t type) {
+    if (type == 1) {
+        // fixed codes (RFC 1951 3.2.6)
+        for (int s = 0; s < 288; s++) t.lengths[s] = s < 144 ? 8 : (s < 256 ? 9 : (s < 280 ? 7 : 8));
+        construct(t.lcount, t.lsym, t.offs, t.lengths, 288);
+        for (int s = 0; s < 30; s++) t.lengths[s] = 5;
+        construct(t.dcount, t.dsym, t.offs, t.lengths, 30);
+    } else {
+        const int nlen = (int)b.get(5) + 257, ndist = (int)b.get(5) + 1, ncode = (int)b.get(4) + 4;
+        if (nlen > 286 || ndist > 30) return E_STREAM;
+        for (int q = 0; q < 19; q++) t.lengths[c_clorder[q]] = q < ncode ? (uint8_t)b.get(3) : 0;
+        // the code length code must be complete (held in the distance arrays)
+        if (construct(t.dcount, t.dsym, t.offs, t.lengths, 19) != 0) return E_STREAM;
+        int idx = 0;
+        while (idx < nlen + ndist) {
+            const int sym = decode(b, t.dcount, t.dsym);
+            if (sym < 0) return E_STREAM;
+            if (sym < 16) {
+                t.lengths[idx++] = (uint8_t)sym;
+                continue;
+            }
+            uint8_t len = 0;
+            int rep;
+            if (sym == 16) {
+                if (idx == 0) return E_STREAM;
+                len = t.lengths[idx - 1];
+                rep = 3 + (int)b.get(2);
+            } else if (sym == 17) {
+                rep = 3 + (int)b.get(3);
+            } else {
+                rep = 11 + (int)b.get(7);
+            }
+            if (idx + rep > nlen + ndist) return E_STREAM;
+            while (rep--) t.lengths[idx++] = len;
+        }
+        if (t.lengths[256] == 0) return E_STREAM;  // no end-of-block code
+        int r = construct(t.lcount, t.lsym, t.offs, t.lengths, nlen);
+        if (r < 0 || (r > 0 && nlen - t.lcount[0] != 1)) return E_STREAM;
+        r = construct(t.dcount, t.dsym, t.offs, t.lengths + nlen, ndist);
+        if (r < 0 || (r > 0 && ndist - t.dcount[0] != 1)) return E_STREAM;
+    }
+    fill_lut(t.lut_l, LB, t.lcount, t.lsym);
+    fill_lut(t.lut_d, DB, t.dcount, t.dsym);
+    return 0;
+}
+
+// One workgroup (one wave) per member.  Lane 0 decodes up to NTOK tokens
+// (literals and length/distance pairs) into LDS; then the wave writes the
+// literals in parallel and copies each match with all lanes, in order: a
+// copy whose source overlaps its destination (distance below the length)
+// goes in rounds that double the copied span, each round reading only
+// bytes written before it.
+__global__ __launch_bounds__(64) void k_inflate(const uint8_t *__restrict__ comp, const InflateBlock *__restrict__ blocks,
                                                 uint32_t nb, uint8_t *__restrict__ out, uint32_t *__restrict__ err) {
-    extern __shared__ uint16_t dyn_tab[];  // IT Tables (about 66 KiB)
-    const uint32_t i = blockIdx.x * IT + threadIdx.x;
-    if (i >= nb) return;
-    Tables &t = reinterpret_cast<Tables *>(dyn_tab)[threadIdx.x];
-    const InflateBlock bk = blocks[i];
+    __shared__ Member t;
+    const uint32_t lane = threadIdx.x;
+    const InflateBlock bk = blocks[blockIdx.x];
     uint8_t *o = out + bk.dst;
     const uint32_t osz = bk.isize;
+    // lane 0's decoder state
     Bits b{comp + bk.src, comp + bk.src + bk.clen, 0, 0, 0};
-    uint32_t pos = 0;
-    uint32_t e = 0;
-    for (;;) {
-        const uint32_t last = b.get(1);
-        const uint32_t type = b.get(2);
-        if (type == 0) {
-            // stored: to the byte boundary, LEN, NLEN, LEN raw bytes
-            b.get(b.cnt & 7);
-            const uint32_t len = b.get(16), nlen = b.get(16);
-            if (len != (~nlen & 0xFFFFu) || pos + len > osz) {
-                e = E_STREAM;
-                break;
-            }
-            for (uint32_t q = 0; q < len; q++) o[pos++] = (uint8_t)b.get(8);
-        } else if (type == 1 || type == 2) {
-            if (type == 1) {
-                // fixed codes (RFC 1951 3.2.6)
-                for (int s = 0; s < 288; s++) t.lengths[s] = s < 144 ? 8 : (s < 256 ? 9 : (s < 280 ? 7 : 8));
-                construct(t.lcount, t.lsym, t.offs, t.lengths, 288);
-                for (int s = 0; s < 30; s++) t.lengths[s] = 5;
-                construct(t.dcount, t.dsym, t.offs, t.lengths, 30);
-            } else {
-                const int nlen = (int)b.get(5) + 257, ndist = (int)b.get(5) + 1, ncode = (int)b.get(4) + 4;
-                if (nlen > 286 || ndist > 30) {
-                    e = E_STREAM;
-                    break;
-                }
-                for (int q = 0; q < 19; q++) t.lengths[c_clorder[q]] = q < ncode ? (uint8_t)b.get(3) : 0;
-                // the code length code must be complete (t.dcount/dsym hold it)
-                if (construct(t.dcount, t.dsym, t.offs, t.lengths, 19) != 0) {
-                    e = E_STREAM;
-                    break;
-                }
-                int idx = 0;
-                while (idx < nlen + ndist) {
-                    int sym = decode(b, t.dcount, t.dsym);
-                    if (sym < 0) {
-                        e = E_STREAM;
+    uint32_t mode = 0;         // 0: between blocks, 1: stored, 2: Huffman codes
+    bool last = false;
+    uint32_t stored_left = 0;  // bytes left in a stored block
+    uint32_t produced = 0;     // output bytes of the tokens decoded so far
+    uint32_t pos = 0;          // output bytes written
+    if (lane == 0) {
+        t.done = 0;
+        t.err = 0;
+    }
+    __syncthreads();
+    while (!t.done) {
+        if (lane == 0) {
+            uint32_t n = 0, e = 0;
+            while (n < NTOK) {
+                if (mode == 0) {
+                    if (last) {
+                        t.done = 1;
                         break;
                     }
-                    if (sym < 16) {
-                        t.lengths[idx++] = (uint8_t)sym;
-                        continue;
-                    }
-                    uint8_t len = 0;
-                    int rep;
-                    if (sym == 16) {
-                        if (idx == 0) {
+                    last = b.get(1) != 0;
+                    const uint32_t type = b.get(2);
+                    if (type == 0) {
+                        b.get(b.cnt & 7);
+                        const uint32_t len = b.get(16), nlen = b.get(16);
+                        if (len != (~nlen & 0xFFFFu) || b.overrun()) {
                             e = E_STREAM;
                             break;
                         }
-                        len = t.lengths[idx - 1];
-                        rep = 3 + (int)b.get(2);
-                    } else if (sym == 17) {
-                        rep = 3 + (int)b.get(3);
+                        stored_left = len;
+                        mode = len ? 1u : 0u;
+                    } else if (type == 1 || type == 2) {
+                        e = read_tables(b, t, type);
+                        if (e) break;
+                        mode = 2;
                     } else {
-                        rep = 11 + (int)b.get(7);
-                    }
-                    if (idx + rep > nlen + ndist) {
                         e = E_STREAM;
                         break;
                     }
-                    while (rep--) t.lengths[idx++] = len;
+                    continue;
                 }
-                if (e) break;
-                if (t.lengths[256] == 0) {  // no end-of-block code
-                    e = E_STREAM;
-                    break;
+                if (mode == 1) {
+                    if (produced >= osz) {
+                        e = E_SIZE;
+                        break;
+                    }
+                    t.tok[n++] = TOK_LIT | b.get(8);
+                    produced++;
+                    if (--stored_left == 0) {
+                        mode = 0;
+                        if (b.overrun()) {
+                            e = E_STREAM;
+                            break;
+                        }
+                    }
+                    continue;
                 }
-                int r = construct(t.lcount, t.lsym, t.offs, t.lengths, nlen);
-                if (r < 0 || (r > 0 && nlen - t.lcount[0] != 1)) {
-                    e = E_STREAM;
-                    break;
-                }
-                r = construct(t.dcount, t.dsym, t.offs, t.lengths + nlen, ndist);
-                if (r < 0 || (r > 0 && ndist - t.dcount[0] != 1)) {
-                    e = E_STREAM;
-                    break;
-                }
-            }
-            // literals, lengths and distances up to the end-of-block code
-            for (;;) {
-                int sym = decode(b, t.lcount, t.lsym);
+                int sym = decode_fast(b, t.lut_l, LB, t.lcount, t.lsym);
                 if (sym < 0) {
                     e = E_STREAM;
                     break;
                 }
                 if (sym < 256) {
-                    if (pos >= osz) {
+                    if (produced >= osz) {
                         e = E_SIZE;
                         break;
                     }
-                    o[pos++] = (uint8_t)sym;
+                    t.tok[n++] = TOK_LIT | (uint32_t)sym;
+                    produced++;
                     continue;
                 }
-                if (sym == 256) break;
+                if (sym == 256) {
+                    mode = 0;
+                    if (b.overrun()) {
+                        e = E_STREAM;
+                        break;
+                    }
+                    continue;
+                }
                 sym -= 257;
                 if (sym >= 29) {
                     e = E_STREAM;
                     break;
                 }
                 const uint32_t len = c_lbase[sym] + b.get(c_lext[sym]);
-                const int ds = decode(b, t.dcount, t.dsym);
+                const int ds = decode_fast(b, t.lut_d, DB, t.dcount, t.dsym);
                 if (ds < 0 || ds >= 30) {
                     e = E_STREAM;
                     break;
                 }
                 const uint32_t dist = c_dbase[ds] + b.get(c_dext[ds]);
-                if (dist > pos) {
+                if (dist > produced) {
                     e = E_STREAM;
                     break;
                 }
-                if (pos + len > osz) {
+                if (produced + len > osz) {
                     e = E_SIZE;
                     break;
                 }
-                for (uint32_t q = 0; q < len; q++, pos++) o[pos] = o[pos - dist];
+                t.tok[n++] = (len << 16) | (dist - 1);
+                produced += len;
             }
-            if (e) break;
-        } else {
-            e = E_STREAM;
-            break;
+            if (e) {
+                t.err = e;
+                t.done = 1;
+            }
+            t.ntok = n;
         }
-        if (b.overrun()) {
-            e = E_STREAM;
-            break;
+        __syncthreads();
+        if (t.err) break;
+        const uint32_t n = t.ntok;
+        // output offsets of the tokens: 16 consecutive tokens per lane
+        constexpr int TPL = NTOK / 64;
+        uint32_t lens[TPL], sum = 0;
+#pragma unroll
+        for (int q = 0; q < TPL; q++) {
+            const uint32_t i = lane * TPL + q;
+            const uint32_t tk = i < n ? t.tok[i] : TOK_LIT;
+            lens[q] = i < n ? ((tk & TOK_LIT) ? 1u : (tk >> 16)) : 0u;
+            sum += lens[q];
         }
-        if (last) break;
+        const uint32_t incl = wave_incl_scan_u32(sum);
+        const uint32_t total = __shfl(incl, 63, 64);
+        uint32_t off = pos + incl - sum;
+        // literals
+#pragma unroll
+        for (int q = 0; q < TPL; q++) {
+            const uint32_t i = lane * TPL + q;
+            if (i < n) {
+                const uint32_t tk = t.tok[i];
+                if (tk & TOK_LIT) o[off] = (uint8_t)tk;
+            }
+            off += lens[q];
+        }
+        __threadfence_block();
+        // matches, in order
+        uint32_t at = pos;
+        for (uint32_t i = 0; i < n; i++) {
+            const uint32_t tk = t.tok[i];
+            if (tk & TOK_LIT) {
+                at++;
+                continue;
+            }
+            const uint32_t len = tk >> 16, dist = (tk & 0xFFFFu) + 1;
+            uint32_t c = dist < len ? dist : len;
+            for (uint32_t j = lane; j < c; j += 64) o[at + j] = o[at - dist + j];
+            __threadfence_block();
+            while (c < len) {
+                // the bytes so far repeat with period c (a multiple of dist)
+                const uint32_t m = (len - c) < c ? (len - c) : c;
+                for (uint32_t j = lane; j < m; j += 64) o[at + c + j] = o[at + j];
+                __threadfence_block();
+                c += m;
+            }
+            at += len;
+        }
+        pos += total;
+        __syncthreads();
     }
-    if (!e && pos != osz) e = E_SIZE;
-    if (e) atomicOr(err, e);
+    if (lane == 0) {
+        uint32_t e = t.err;
+        if (!e && pos != osz) e = E_SIZE;
+        if (e) atomicOr(err, e);
+    }
 }
 
 // Number of '\n' bytes in d[0, n) added to *total.
@@ -324,14 +445,7 @@ void inflate_blocks(const uint8_t *d_comp, const std::vector<InflateBlock> &bloc
     InflateBlock *db = static_cast<InflateBlock *>(d_blocks.ensure(blocks.size() * sizeof(InflateBlock)));
     KITE_HIP(hipMemcpyAsync(db, blocks.data(), blocks.size() * sizeof(InflateBlock), hipMemcpyHostToDevice, s));
     const uint32_t nb = (uint32_t)blocks.size();
-    const size_t lds = sizeof(Tables) * IT;
-    static bool attr = false;
-    if (!attr) {
-        KITE_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_inflate),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        attr = true;
-    }
-    k_inflate<<<dim3((nb + IT - 1) / IT), dim3(IT), lds, s>>>(d_comp, db, nb, d_out, d_err);
+    k_inflate<<<dim3(nb), dim3(64), 0, s>>>(d_comp, db, nb, d_out, d_err);
     KITE_CHECK_LAUNCH();
 }
 
