@@ -1155,7 +1155,6 @@ void for_each_run_group(kite_gpu_ctx *ctx, int W, const kite_dump_run *runs, uin
         for (uint64_t q = i; q < j; q++) {
             const kite_dump_run &r = runs[q];
             rank = (q > i && r.partition == runs[q - 1].partition) ? rank + 1 : 0;
-            if (r.bytes) memcpy(host + off, r.data, r.bytes);
             kite::DumpRun d{};
             d.src = off;
             d.bytes = r.bytes;
@@ -1167,6 +1166,22 @@ void for_each_run_group(kite_gpu_ctx *ctx, int W, const kite_dump_run *runs, uin
             off += (r.bytes + 15) & ~15ull;
         }
         memset(host + off, 0, 16);
+        // the payloads into page-locked staging on several host threads
+        // (one thread copies a few GB/s out of a memory-mapped file)
+        {
+            const uint64_t nrun = j - i;
+            const unsigned nt = (unsigned)std::min<uint64_t>(
+                std::max<uint64_t>(1, pay >> 26), std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+            std::atomic<uint64_t> next{0};
+            auto copy = [&] {
+                for (uint64_t q; (q = next.fetch_add(1)) < nrun;)
+                    if (runs[i + q].bytes) memcpy(host + dr[q].src, runs[i + q].data, runs[i + q].bytes);
+            };
+            std::vector<std::thread> th;
+            for (unsigned t = 1; t < nt; t++) th.emplace_back(copy);
+            copy();
+            for (auto &x : th) x.join();
+        }
         uint8_t *dev = static_cast<uint8_t *>(ctx->dump_pay.ensure(pay + 16));
         KITE_HIP(hipMemcpyAsync(dev, host, pay + 16, hipMemcpyHostToDevice, s));
         fn(dev, dr, np, rows, j - i);
