@@ -1,5 +1,5 @@
 // inflate.hpp -- BGZF blocks inflated on the device (RFC 1951 DEFLATE, one
-// block per thread), the block-parallel decompression of the reference's
+// block per wave), the block-parallel decompression of the reference's
 // BGZF reader (src/count/bgzf.rs:152-264) without its host threads.
 #pragma once
 
