@@ -27,12 +27,13 @@ namespace kite {
 
 // ---- extract geometry -------------------------------------------------------
 constexpr int XT = 256;                 // extract threads
-constexpr int SUB = 8192;               // FASTQ bytes per sub-tile
+constexpr int SUB = 8064;               // FASTQ bytes per sub-tile (with the halo, 8 KiB)
 constexpr int SEGB = 32;                // bytes per segment (one thread)
 constexpr int HALO = 128;               // >= k_max - 1 bytes before each sub-tile
 constexpr int HSEG = HALO / SEGB;       // 4 halo segments
-constexpr int NSEG = SUB / SEGB + HSEG; // 260 segments per sub-tile
+constexpr int NSEG = SUB / SEGB + HSEG; // 256 segments per sub-tile: one per thread
 constexpr int EXT_U4 = (SUB + HALO) / 16;
+static_assert(NSEG == XT, "one segment per extract thread");
 constexpr uint32_t MAX_PARTITIONS = 8192;
 
 // ---- split / sort geometry --------------------------------------------------
@@ -92,7 +93,8 @@ __device__ __forceinline__ uint32_t key_partition(const Key<W> &a, int pbits) {
     uint64_t f = a.w[0];
 #pragma unroll
     for (int r = 1; r < W; r++) f ^= a.w[r];
-    return pbits ? (uint32_t)((f * 0x9E3779B97F4A7C15ull) >> (64 - pbits)) : 0u;
+    // top pbits bits of the product (0 for pbits = 0), without a branch
+    return (uint32_t)(((f * 0x9E3779B97F4A7C15ull) >> 32) >> (32 - pbits));
 }
 
 // Bits [shift, shift + nb) of the key, nb <= 32.
@@ -127,7 +129,9 @@ __global__ __launch_bounds__(XT) void k_nl_count(const uint8_t *__restrict__ fq,
     __shared__ uint32_t ws[XT / 64 + 1];
     uint64_t base = (uint64_t)blockIdx.x * SUB + (uint64_t)threadIdx.x * SEGB;
     uint32_t c = 0;
-    if (base + SEGB <= len) {
+    if ((int)threadIdx.x * SEGB >= SUB) {
+        // past the sub-tile (its last threads)
+    } else if (base + SEGB <= len) {
         const uint4 *p = reinterpret_cast<const uint4 *>(fq + base);
         uint4 a = p[0], b = p[1];
         c = newlines_in(a.x) + newlines_in(a.y) + newlines_in(a.z) + newlines_in(a.w) +
@@ -171,6 +175,7 @@ __global__ __launch_bounds__(XT) void k_extract(ExtractArgs a) {
     constexpr bool SCATTER = MODE != 0;
     __shared__ uint4 sb4[EXT_U4];
     __shared__ uint64_t scode[NSEG];
+    __shared__ uint64_t srcode[NSEG];  // canonical: each segment reverse-complemented
     __shared__ uint32_t svalid[NSEG];
     __shared__ uint32_t snl[NSEG];
     __shared__ uint32_t spre[NSEG];
@@ -206,7 +211,7 @@ __global__ __launch_bounds__(XT) void k_extract(ExtractArgs a) {
         kmask[r] = bits >= 64 ? ~0ull : (bits <= 0 ? 0ull : ((1ull << bits) - 1));
     }
     const int rs = 64 * W - 2 * a.k;
-    const int rq = rs >> 6, rb = rs & 63;
+    const int rq = W == 1 ? 0 : rs >> 6, rb = rs & 63;  // rs < 64 for one word
 
     for (uint32_t si = 0; si < a.subs; si++) {
         uint64_t st = (uint64_t)wg * a.subs + si;
@@ -268,7 +273,9 @@ __global__ __launch_bounds__(XT) void k_extract(ExtractArgs a) {
             for (int q = 0; q < SEGB / 4; q++) {
                 const uint32_t w = sb[s * (SEGB / 4) + q];
                 const uint32_t c = ((w >> 1) ^ (w >> 2)) & 0x03030303u;
-                const uint32_t p8 = ((c << 6) & 0xC0u) | ((c >> 4) & 0x30u) | ((c >> 14) & 0x0Cu) | (c >> 24);
+                // byte b's code to bits 30 - 10b (no two partial products
+                // overlap), then the four codes as one byte, byte 0 on top
+                const uint32_t p8 = (c * 0x40100401u) >> 24;
                 code |= (uint64_t)p8 << (56 - 8 * q);
                 // the lowercase letter each code stands for; a byte is a base
                 // exactly when it equals that letter after case folding
@@ -276,30 +283,43 @@ __global__ __launch_bounds__(XT) void k_extract(ExtractArgs a) {
                 acgt |= byte_mask4(zero_bytes((w | 0x20202020u) ^ expect)) << (4 * q);
             }
             scode[s] = code;
+            srcode[s] = rev2(~code);
             svalid[s] = live ? (acgt & l1) : 0u;
         }
         __syncthreads();
 
-        // k-mer end positions of this thread's segment
+        // k-mer end positions of this thread's segment (the last HSEG
+        // threads own none: their segments are the next sub-tile's halo)
         const int s = tid + HSEG;
-        uint32_t run = 0;
-#pragma unroll
-        for (int q = 1; q <= HSEG; q++) {
-            uint32_t v = svalid[s - q];
-            if (v == 0xFFFFFFFFu) {
-                run += 32;
-                continue;
-            }
-            run += __clz(~v);
-            break;
-        }
-        const uint32_t vm = svalid[s];
         uint32_t emask = 0;
-        if (vm) {
-            for (int j = 0; j < SEGB; j++) {
-                run = ((vm >> j) & 1u) ? run + 1 : 0;
-                emask |= (run >= (uint32_t)a.k ? 1u : 0u) << j;
+        if (s < NSEG) {
+            uint32_t run = 0;  // valid bases just before the segment
+#pragma unroll
+            for (int q = 1; q <= HSEG; q++) {
+                uint32_t v = svalid[s - q];
+                if (v == 0xFFFFFFFFu) {
+                    run += 32;
+                    continue;
+                }
+                run += __clz(~v);
+                break;
             }
+            // byte j ends a k-mer when bytes j-L..j (L = k-1) are valid: no
+            // invalid byte of the segment in that window (each invalid bit
+            // smeared L places up), and j >= L - run where the window
+            // reaches before the segment
+            const uint32_t L = (uint32_t)a.k - 1u;
+            uint32_t sm = ~svalid[s], len = 1;  // sm: invalid bits smeared by [0, len)
+#pragma unroll
+            for (int t = 0; t < 5; t++)
+                if (2 * len <= L + 1) {
+                    sm |= sm << len;
+                    len *= 2;
+                }
+            if (L + 1 - len < 32u) sm |= sm << (L + 1 - len);
+            const uint32_t lk = run < L ? L - run : 0u;  // low bytes whose window lacks bases
+            const uint32_t low = lk >= 32u ? ~0u : ((1u << lk) - 1u);
+            emask = ~(sm | low);
         }
         // the canonical key of the k-mer ending at position j of segment ss
         auto kmer_at = [&](int ss, int j) -> Key<W> {
@@ -310,29 +330,37 @@ __global__ __launch_bounds__(XT) void k_extract(ExtractArgs a) {
             Key<W> f;
 #pragma unroll
             for (int r = 0; r < W; r++) {
+                // (x << 1) << (63 - sh) is x << (64 - sh), and 0 for sh = 0
                 uint64_t lo = wv[W - r] >> sh;
-                uint64_t hi = sh ? (wv[W - r - 1] << (64 - sh)) : 0ull;
+                uint64_t hi = (wv[W - r - 1] << 1) << (63 - sh);
                 f.w[r] = (lo | hi) & kmask[r];
             }
-            Key<W> key = f;
-            if (a.canonical) {
-                uint64_t R[W];
+            // (computed in both modes and selected, which keeps the code
+            // free of branches so that neighbouring k-mers interleave)
+            {
+                // the complemented bases from the k-mer's last one backwards
+                // are the reverse-complemented segments read downwards from
+                // bit 2 * (31 - j) of segment ss: top-aligned, most
+                // significant word first, then shifted down to 2k bits
+                uint64_t top[W];
 #pragma unroll
-                for (int r = 0; r < W; r++) R[r] = rev2(~f.w[W - 1 - r]);
+                for (int r = 0; r < W; r++) {
+                    const uint64_t x = srcode[ss - r], y = srcode[ss - r - 1];
+                    top[r] = (x << sh) | ((y >> 1) >> (63 - sh));
+                }
                 Key<W> rc;
 #pragma unroll
                 for (int r = 0; r < W; r++) {
                     uint64_t lo = 0, hi = 0;
 #pragma unroll
                     for (int t = 0; t < W; t++) {
-                        if (t == r + rq) lo = R[t];
-                        if (t == r + rq + 1) hi = R[t];
+                        if (t == r + rq) lo = top[W - 1 - t];
+                        if (t == r + rq + 1) hi = top[W - 1 - t];
                     }
-                    rc.w[r] = (lo >> rb) | (rb ? (hi << (64 - rb)) : 0ull);
+                    rc.w[r] = (lo >> rb) | ((hi << 1) << (63 - rb));
                 }
-                if (key_less<W>(rc, f)) key = rc;
+                return (a.canonical && key_less<W>(rc, f)) ? rc : f;
             }
-            return key;
         };
         if constexpr (!SCATTER) {
             // one list for the whole sub-tile, so that every lane builds k-mers
@@ -341,10 +369,19 @@ __global__ __launch_bounds__(XT) void k_extract(ExtractArgs a) {
             uint32_t slot = block_excl_scan<XT, uint32_t>(__popc(emask), ws, &nk);
             for (uint32_t e = emask; e; e &= e - 1) spos[slot++] = (uint16_t)(s * SEGB + __ffs(e) - 1);
             __syncthreads();
-            for (uint32_t i = tid; i < nk; i += XT) {
-                const uint32_t pos = spos[i];
-                const Key<W> key = kmer_at((int)(pos / SEGB), (int)(pos % SEGB));
-                atomicAdd(&cnt[key_partition<W>(key, a.pbits)], 1u);
+            // HU k-mers per thread and step, so that their LDS reads overlap
+            constexpr int HU = 4;
+            for (uint32_t i0 = tid; i0 < nk; i0 += HU * XT) {
+                uint32_t q[HU];
+#pragma unroll
+                for (int u = 0; u < HU; u++) {
+                    const uint32_t i = i0 + (uint32_t)u * XT;
+                    const uint32_t pos = spos[i < nk ? i : i0];
+                    q[u] = key_partition<W>(kmer_at((int)(pos / SEGB), (int)(pos % SEGB)), a.pbits);
+                }
+#pragma unroll
+                for (int u = 0; u < HU; u++)
+                    if (i0 + (uint32_t)u * XT < nk) atomicAdd(&cnt[q[u]], 1u);
             }
         } else if constexpr (MODE == 1) {
             for (uint32_t e = emask; e; e &= e - 1) {
@@ -568,6 +605,102 @@ __global__ __launch_bounds__(ST) void k_split(SplitArgs a) {
                     for (int r = 0; r < W; r++) d[r] = key[it].w[r];
                 }
             }
+        }
+    }
+}
+
+// Scatter pass of one tile through LDS (children up to 2^SPLIT_STAGED_BITS):
+// each key's rank among its child's keys comes from the LDS count itself,
+// the tile is regrouped by child in LDS, and the grouped tile is stored in
+// order, so consecutive lanes write consecutive addresses of one child's
+// run (about TK / 2^dbits keys) rather than one scattered store per key.
+// Same tiles, reservations and 32-bit segment offsets as MODE 1; 80 KiB of
+// LDS, two workgroups per CU.
+constexpr int SST = 512;
+constexpr int SPLIT_STAGED_BITS = 11;
+template <int W>
+__global__ __launch_bounds__(SST) void k_split_staged(SplitArgs a) {
+    constexpr int TK = SplitCfg<W>::TK, IPT = TK / SST;
+    constexpr int ND = 1 << SPLIT_STAGED_BITS, CPT = ND / SST;
+    static_assert(TK % SST == 0 && ND % SST == 0, "tile and children split evenly over the threads");
+    __shared__ uint64_t stage[TK * W];
+    __shared__ uint32_t loff[ND];  // child counts, then each child's first LDS position
+    __shared__ uint32_t gofs[ND];  // child's destination minus its LDS position (mod 2^32)
+    const int tid = threadIdx.x;
+    const uint64_t b = xcd_tile(blockIdx.x, gridDim.x);
+    uint32_t lo = 0, hi = a.nseg;  // largest s with tile_pre[s] <= b
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) / 2;
+        if (a.tile_pre[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    const Seg sg = a.segs[lo];
+    const uint64_t t = b - a.tile_pre[lo];
+    const uint64_t first = sg.start + t * TK;
+    const uint32_t n = (uint32_t)std::min<uint64_t>(TK, sg.size - t * TK);
+    const uint32_t nd = 1u << a.dbits;
+    for (uint32_t g = tid; g < nd; g += SST) loff[g] = 0;
+    __syncthreads();
+    Key<W> key[IPT];
+    uint32_t dig[IPT], rk[IPT];
+#pragma unroll
+    for (int it = 0; it < IPT; it++) {
+        const uint32_t i = (uint32_t)it * SST + tid;
+        if (i < n) {
+            const uint64_t *p = a.src + (first + i) * W;
+#pragma unroll
+            for (int r = 0; r < W; r++) key[it].w[r] = p[r];
+            dig[it] = key_bits<W>(key[it], a.shift, a.dbits);
+            rk[it] = atomicAdd(&loff[dig[it]], 1u);
+        }
+    }
+    __syncthreads();
+    {
+        // thread tid owns children [tid*cpt, tid*cpt + cpt)
+        const uint32_t cpt = (nd + SST - 1) / SST, g0 = (uint32_t)tid * cpt;
+        uint32_t c[CPT], sum = 0;
+#pragma unroll
+        for (int q = 0; q < CPT; q++) {
+            c[q] = ((uint32_t)q < cpt && g0 + q < nd) ? loff[g0 + q] : 0u;
+            sum += c[q];
+        }
+        uint32_t tot;
+        // the stage is free until the regrouping below
+        uint32_t run = block_excl_scan<SST, uint32_t>(sum, reinterpret_cast<uint32_t *>(stage), &tot);
+#pragma unroll
+        for (int q = 0; q < CPT; q++) {
+            if ((uint32_t)q < cpt && g0 + q < nd) {
+                const uint32_t g = g0 + q;
+                if (c[q])
+                    gofs[g] = (uint32_t)(atomicAdd((unsigned long long *)&a.child_cur[sg.child_base + g],
+                                                   (unsigned long long)c[q]) - sg.start) - run;
+                loff[g] = run;
+                run += c[q];
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IPT; it++) {
+        const uint32_t i = (uint32_t)it * SST + tid;
+        if (i < n) {
+            const uint32_t d = loff[dig[it]] + rk[it];
+#pragma unroll
+            for (int r = 0; r < W; r++) stage[d * W + r] = key[it].w[r];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IPT; it++) {
+        const uint32_t i = (uint32_t)it * SST + tid;
+        if (i < n) {
+            Key<W> kk;
+#pragma unroll
+            for (int r = 0; r < W; r++) kk.w[r] = stage[i * W + r];
+            const uint32_t g = key_bits<W>(kk, a.shift, a.dbits);
+            uint64_t *d = a.dst + (sg.start + (uint32_t)(gofs[g] + i)) * W;
+#pragma unroll
+            for (int r = 0; r < W; r++) d[r] = kk.w[r];
         }
     }
 }
@@ -1297,6 +1430,7 @@ void CountEngine::run(const uint8_t *d_fq, uint64_t len, int k, int canonical, u
     if (const char *e = getenv("KITE_SPLIT_CHUNK")) tun.split_chunk = std::min(64, std::max(1, atoi(e)));
     if (const char *e = getenv("KITE_EXTRACT_SUBS")) tun.extract_subs = std::min(64, std::max(1, atoi(e)));
     if (const char *e = getenv("KITE_EXTRACT_STAGED")) tun.extract_staged = atoi(e) != 0;
+    if (const char *e = getenv("KITE_SPLIT_STAGED")) tun.split_staged = atoi(e) != 0;
     if (words_ == 1) run_w<1>(d_fq, len, k, canonical, pbits);
     else if (words_ == 2) run_w<2>(d_fq, len, k, canonical, pbits);
     else run_w<4>(d_fq, len, k, canonical, pbits);
@@ -1490,7 +1624,9 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
         sa.child_cur = ccur;
         launch("k_split_scatter", split_keys * 16 * W,
                [&] {
-                   if (chunk == 1) k_split<W, 1><<<dim3((unsigned)tiles_total), dim3(ST), 0, s>>>(sa);
+                   if (chunk == 1 && tun.split_staged && d <= SPLIT_STAGED_BITS)
+                       k_split_staged<W><<<dim3((unsigned)tiles_total), dim3(SST), 0, s>>>(sa);
+                   else if (chunk == 1) k_split<W, 1><<<dim3((unsigned)tiles_total), dim3(ST), 0, s>>>(sa);
                    else k_split<W, 2><<<dim3((unsigned)tiles_total), dim3(ST), 0, s>>>(sa);
                });
         levels_++;

@@ -3,7 +3,7 @@
 // One batch of FASTQ bytes resident in HBM becomes per-partition sorted,
 // fully reduced (key, count) runs, also resident in HBM:
 //
-//   k_nl_count      newline count per 8 KiB sub-tile
+//   k_nl_count      newline count per sub-tile (8064 bytes + 128 halo)
 //   scan            line index at each sub-tile start
 //   k_extract<HIST> 2-bit pack, rolling fwd/rc k-mers, Fibonacci partition
 //                   histogram per (partition, workgroup)
@@ -50,8 +50,9 @@ struct CountTunables {
     bool dedup = true;  // small buckets: hash dedup + one bin pass instead of LSD passes
     int hash_classes = 3;  // size classes [0, hash_classes) use the hash kernel
     int split_chunk = 1;   // split tiles per workgroup (longer runs per child, fewer atomics)
-    int extract_subs = 4;  // 8 KiB sub-tiles per extract workgroup
+    int extract_subs = 4;  // sub-tiles per extract workgroup
     bool extract_staged = true;  // group each chunk's keys by partition in LDS before writing
+    bool split_staged = true;    // group each split tile's keys by child in LDS before writing
 };
 
 class CountEngine {

@@ -249,6 +249,27 @@ def test_extract_scatter_modes(gpu_ctx, monkeypatch, staged):
         check_same(res, tab, f"staged={staged} k={k} P={P}")
 
 
+@pytest.mark.parametrize("staged", ["1", "0"])
+@pytest.mark.parametrize("max_bits", ["12", "5"])
+def test_split_scatter_modes(gpu_ctx, monkeypatch, staged, max_bits):
+    """The MSD split scatter with each tile regrouped by child in LDS and
+    with direct per-lane stores give the oracle's table, for 1-, 2- and
+    4-word keys, with partitions large enough for splits of 11 and 12 bits
+    and, with the digit capped at 5 bits, several split levels."""
+    monkeypatch.setenv("KITE_SPLIT_STAGED", staged)
+    monkeypatch.setenv("KITE_MAX_DBITS", max_bits)
+    from kite_amd.workloads import K31_CANON_50MBP, K63_CANON_100MBP
+    rng = random.Random(11)
+    long_reads = "".join(f"@l{i}\n{s}\n+\n{'I' * len(s)}\n" for i, s in enumerate(
+        "".join(rng.choice("ACGT") for _ in range(400)) for _ in range(3000))).encode()
+    for fq, k, canon, P in ((K31_CANON_50MBP.fastq(n_reads=150000), 31, True, 4),
+                            (K31_CANON_50MBP.fastq(n_reads=150000), 31, True, 1),
+                            (K63_CANON_100MBP.fastq(n_reads=30000), 63, True, 2),
+                            (long_reads, 101, False, 1)):
+        res, tab, _ = run_both(gpu_ctx, fq, k, canon, P)
+        check_same(res, tab, f"split staged={staged} max_bits={max_bits} k={k} P={P}")
+
+
 def test_hash_kernel_on_class_two(gpu_ctx, monkeypatch):
     """The hash kernel with 1024 threads on the 4096-key class."""
     monkeypatch.setenv("KITE_HASH_CLASSES", "3")
