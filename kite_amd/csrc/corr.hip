@@ -12,8 +12,9 @@
 // come exactly from per-column count histograms: means and variances in
 // FP64, ranks as cum(<c) + (h(c) + 1) / 2.  The only per-row work is the
 // centered Gram G = Z^T Z of standardized values z = (v - mean) / sd, done
-// with bf16 MFMA on a hi/lo split of z (hi*hi + hi*lo + lo*hi, FP32 inside
-// a K-tile, FP64 across tiles), or with FP64 MFMA as the baseline.
+// with bf16 MFMA on a hi/lo split of z (hi*hi + hi*lo + lo*hi, the last two
+// as one product whose transpose is added at the end; FP32 inside a K-tile,
+// FP64 across tiles), or with FP64 MFMA as the baseline.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -116,22 +117,30 @@ __device__ __forceinline__ uint32_t staged_code(const uint32_t *__restrict__ sw,
     else return sw[e];
 }
 
-// lut: NM tables (one per method) of `stride` entries per column, each entry
-// packed (hi | lo << 16) bf16 bits; table m starts at m * n * stride.
-// partial: [wave][NM][16][64] FP64 accumulators in MFMA output layout.
+// lut: `stride` entries per column, each holding the NM methods' values as
+// packed bf16 bits (hi | lo2 << 16), where z = hi + lo2 / 2 + O(2^-16 z):
+// lo2 is twice the low part, which is exact in bf16.  The MFMAs accumulate
+// A = H^T H + H^T Lo2 (+ Lo^T Lo), so that (A + A^T) / 2 = H^T H + H^T L +
+// L^T H (+ L^T L) is the split Gram with two products per step instead of
+// three; the host symmetrizes.  partial: [wave][NM][16][64] FP64
+// accumulators in MFMA output layout.
+template <int NM> struct GramEntry { using type = uint32_t; };
+template <> struct GramEntry<2> { using type = uint2; };
+
 template <class T, int NM>
-__global__ __launch_bounds__(GT) void k_gram_bf16(const T *__restrict__ codes, uint64_t N, int n,
+__global__ __launch_bounds__(GT, 2) void k_gram_bf16(const T *__restrict__ codes, uint64_t N, int n,
                                                    const uint32_t *__restrict__ lut, uint32_t stride,
                                                    uint64_t rows_per_wave, int with_lolo,
                                                    double *__restrict__ partial) {
-    __shared__ uint32_t slut[NM][MAX_COLS * LSP];
+    using Entry = typename GramEntry<NM>::type;
+    constexpr uint32_t NONE = 0xFFFFFFFFu;
+    const Entry *glut = reinterpret_cast<const Entry *>(lut);
+    __shared__ Entry slut[MAX_COLS * LSP];
     __shared__ uint32_t sblk[GT / 64][CodeBlock<T>::WORDS];
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
-    for (int m = 0; m < NM; m++)
-        for (int i = threadIdx.x; i < n * LSM; i += GT)
-            slut[m][(i / LSM) * LSP + (i % LSM)] =
-                lut[(uint64_t)m * n * stride + (uint64_t)(i / LSM) * stride + (i % LSM)];
+    for (int i = threadIdx.x; i < n * LSM; i += GT)
+        slut[(i / LSM) * LSP + (i % LSM)] = glut[(uint64_t)(i / LSM) * stride + (i % LSM)];
     __syncthreads();
     const uint64_t wave = (uint64_t)blockIdx.x * (GT / 64) + wv;
     const uint64_t r0 = std::min<uint64_t>(N, wave * rows_per_wave);
@@ -151,30 +160,43 @@ __global__ __launch_bounds__(GT) void k_gram_bf16(const T *__restrict__ codes, u
             facc[m][i] = 0.0;
         }
     int steps = 0;
-    // one 16-row step: codes of rows s0 + kb + t, their table values, the MFMAs
-    auto step = [&](uint32_t s0, uint32_t rows) {
-        uint32_t c[8];
+    // codes of rows s0 + kb + t (t < 8) of this lane's column in the staged block
+    auto codes_at = [&](uint32_t s0, uint32_t rows, uint32_t (&c)[8]) {
 #pragma unroll
         for (int t = 0; t < 8; t++) {
             const uint32_t rl = s0 + (uint32_t)kb + (uint32_t)t;
-            c[t] = (colok && rl < rows) ? staged_code<T>(sw, rl * (uint32_t)n + (uint32_t)col) : 0xFFFFFFFFu;
+            c[t] = (colok && rl < rows) ? staged_code<T>(sw, rl * (uint32_t)n + (uint32_t)col) : NONE;
         }
+    };
+    auto values_at = [&](const uint32_t (&c)[8], Entry (&v)[8]) {
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            Entry e{};
+            if (c[t] != NONE) e = c[t] < (uint32_t)LSM ? slut[ccol * LSP + c[t]] : glut[(uint64_t)ccol * stride + c[t]];
+            v[t] = e;
+        }
+    };
+    // one 16-row step: the MFMAs of every method, and the FP64 flush
+    auto mfma_step = [&](const Entry (&v)[8]) {
 #pragma unroll
         for (int m = 0; m < NM; m++) {
-            bf16x8 hi, lo;
+            bf16x8 hi, lo2;
 #pragma unroll
             for (int t = 0; t < 8; t++) {
-                uint32_t packed = 0;
-                if (c[t] != 0xFFFFFFFFu)
-                    packed = c[t] < (uint32_t)LSM ? slut[m][ccol * LSP + c[t]]
-                                                  : lut[(uint64_t)m * n * stride + (uint64_t)ccol * stride + c[t]];
+                uint32_t packed;
+                if constexpr (NM == 2) packed = m == 0 ? v[t].x : v[t].y;
+                else packed = v[t];
                 hi[t] = bf16_from_bits((uint16_t)(packed & 0xFFFF));
-                lo[t] = bf16_from_bits((uint16_t)(packed >> 16));
+                lo2[t] = bf16_from_bits((uint16_t)(packed >> 16));
             }
             acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi, hi, acc[m], 0, 0, 0);
-            acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi, lo, acc[m], 0, 0, 0);
-            acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lo, hi, acc[m], 0, 0, 0);
-            if (with_lolo) acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lo, lo, acc[m], 0, 0, 0);
+            acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hi, lo2, acc[m], 0, 0, 0);
+            if (with_lolo) {
+                bf16x8 lq;  // lo2 / 4, exact: lo2 * lq = lo * lo
+#pragma unroll
+                for (int t = 0; t < 8; t++) lq[t] = (__bf16)((float)lo2[t] * 0.25f);
+                acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lo2, lq, acc[m], 0, 0, 0);
+            }
         }
         if (++steps == FLUSH_STEPS) {
             steps = 0;
@@ -197,10 +219,33 @@ __global__ __launch_bounds__(GT) void k_gram_bf16(const T *__restrict__ codes, u
             fetch_codes<T>(codes, (rb + RB) * (uint64_t)n, (uint32_t)(std::min<uint64_t>(RB, r1 - rb - RB) * n), w,
                            lane);
         if (rows == RB) {
+            // software pipeline over the block's steps: the table reads of
+            // step q + 1 and the code reads of step q + 2 are in flight
+            // while the MFMAs of step q run
+            constexpr int NS = RB / 16;
+            uint32_t c[8];
+            Entry v[8];
+            codes_at(0, RB, c);
+            values_at(c, v);
+            codes_at(16, RB, c);
 #pragma unroll
-            for (int q = 0; q < RB / 16; q++) step((uint32_t)q * 16u, RB);
+            for (int q = 0; q < NS; q++) {
+                Entry vn[8];
+                if (q + 1 < NS) values_at(c, vn);
+                if (q + 2 < NS) codes_at((uint32_t)(q + 2) * 16u, RB, c);
+                mfma_step(v);
+                if (q + 1 < NS)
+#pragma unroll
+                    for (int t = 0; t < 8; t++) v[t] = vn[t];
+            }
         } else {
-            for (uint32_t s0 = 0; s0 < rows; s0 += 16) step(s0, rows);
+            for (uint32_t s0 = 0; s0 < rows; s0 += 16) {
+                uint32_t c[8];
+                Entry v[8];
+                codes_at(s0, rows, c);
+                values_at(c, v);
+                mfma_step(v);
+            }
         }
         __builtin_amdgcn_wave_barrier();
     }
@@ -495,13 +540,15 @@ void CorrEngine::gram(const void *d_codes, int code_bytes, uint64_t N, int n,
     std::vector<double> out;
     Gs.assign(NM, std::vector<double>(MAX_COLS * MAX_COLS, 0.0));
     if (!fp64) {
+        // entry (column, code) holds the NM methods' packed values side by side
         std::vector<uint32_t> lut(NM * tsz);
         for (int m = 0; m < NM; m++)
             for (size_t i = 0; i < tsz; i++) {
                 const double zv = (*zs[m])[i];
                 uint16_t hi = bf16_bits_rne((float)zv);
                 uint16_t lo = bf16_bits_rne((float)(zv - (double)bf16_to_float(hi)));
-                lut[m * tsz + i] = (uint32_t)hi | ((uint32_t)lo << 16);
+                uint16_t lo2 = bf16_bits_rne(2.0f * bf16_to_float(lo));  // exact doubling
+                lut[i * NM + m] = (uint32_t)hi | ((uint32_t)lo2 << 16);
             }
         lut_.ensure(lut.size() * 4);
         KITE_HIP(hipMemcpyAsync(lut_.ptr, lut.data(), lut.size() * 4, hipMemcpyHostToDevice, stream_));
@@ -543,12 +590,18 @@ void CorrEngine::gram(const void *d_codes, int code_bytes, uint64_t N, int n,
         KITE_HIP(hipMemcpyAsync(out.data(), sum_.ptr, per_wave * 8, hipMemcpyDeviceToHost, stream_));
         KITE_HIP(hipStreamSynchronize(stream_));
         // lane l, element i -> (m, n) = (8*(i/4) + 4*(l/32) + i%4, l%32)
-        for (int mt = 0; mt < NM; mt++)
+        for (int mt = 0; mt < NM; mt++) {
+            std::vector<double> A(MAX_COLS * MAX_COLS);
             for (int i = 0; i < 16; i++)
                 for (int l = 0; l < 64; l++) {
                     int m = 8 * (i / 4) + 4 * (l / 32) + (i % 4), nn = l % 32;
-                    Gs[mt][m * MAX_COLS + nn] = out[(mt * 16 + i) * 64 + l];
+                    A[m * MAX_COLS + nn] = out[(mt * 16 + i) * 64 + l];
                 }
+            // the split Gram is the symmetric part of the accumulated products
+            for (int i = 0; i < MAX_COLS; i++)
+                for (int j = 0; j < MAX_COLS; j++)
+                    Gs[mt][i * MAX_COLS + j] = 0.5 * (A[i * MAX_COLS + j] + A[j * MAX_COLS + i]);
+        }
     } else {
         std::vector<double> lut(NM * tsz);
         for (int m = 0; m < NM; m++) std::copy(zs[m]->begin(), zs[m]->end(), lut.begin() + m * tsz);
