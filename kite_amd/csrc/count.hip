@@ -856,7 +856,7 @@ constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v / 2); }
 // Waves per SIMD of every hash-sort class: about 18 B of LDS per item (keys,
 // 16-bit counts, two table slots) fits 8 waves, and the registers are
 // capped to match.
-constexpr int hash_sort_waves(int) { return 8; }
+constexpr int hash_sort_waves(int ipt) { return ipt > 4 ? 4 : 8; }
 
 // Hash-then-bin variant: every key is inserted into an LDS open-addressing
 // table (a slot holds the first position of its key, claimed by CAS) and
@@ -870,7 +870,7 @@ constexpr int hash_sort_waves(int) { return 8; }
 // out of this kernel.  Output equals k_bucket_sort's: ascending unique keys
 // with their run lengths.
 template <int W, int IPT, int NT>
-__global__ __launch_bounds__(NT, hash_sort_waves(NT)) void k_bucket_hash_sort(uint64_t *__restrict__ A,
+__global__ __launch_bounds__(NT, hash_sort_waves(IPT)) void k_bucket_hash_sort(uint64_t *__restrict__ A,
                                                           uint64_t *__restrict__ B,
                                                           uint32_t *__restrict__ counts_tmp,
                                                           const Bucket *__restrict__ buckets,
@@ -1226,11 +1226,16 @@ struct ClassCaps {
 };
 
 // Size class of every bucket: the smallest LDS footprint that holds it.
-// Counts are aggregated per wave (one atomic per class per wave): the four
-// counters would otherwise take one contended atomic per bucket.
-__global__ __launch_bounds__(256) void k_classify(const Bucket *__restrict__ nodes, uint32_t nb, int kbits,
-                                                  ClassCaps caps, uint32_t *__restrict__ cls,
-                                                  uint32_t *__restrict__ cls_count) {
+// Counts are aggregated per workgroup of CT threads (wave ballots into LDS,
+// then one global atomic per class per workgroup): the four counters would
+// otherwise take one contended atomic per bucket, or per wave.
+constexpr int CT = 1024;
+__global__ __launch_bounds__(CT) void k_classify(const Bucket *__restrict__ nodes, uint32_t nb, int kbits,
+                                                 ClassCaps caps, uint32_t *__restrict__ cls,
+                                                 uint32_t *__restrict__ cls_count) {
+    __shared__ uint32_t wc[SORT_CLASSES];
+    if (threadIdx.x < SORT_CLASSES) wc[threadIdx.x] = 0;
+    __syncthreads();
     const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     int c = -1;
     if (b < nb) {
@@ -1243,29 +1248,44 @@ __global__ __launch_bounds__(256) void k_classify(const Bucket *__restrict__ nod
 #pragma unroll
     for (int q = 0; q < SORT_CLASSES; q++) {
         const uint64_t m = __ballot(c == q);
-        if (m && __lane_id() == (unsigned)__ffsll((unsigned long long)m) - 1u)
-            atomicAdd(&cls_count[q], (uint32_t)__popcll(m));
+        if (m && __lane_id() == (unsigned)__ffsll((unsigned long long)m) - 1u) atomicAdd(&wc[q], (uint32_t)__popcll(m));
     }
+    __syncthreads();
+    if (threadIdx.x < SORT_CLASSES && wc[threadIdx.x]) atomicAdd(&cls_count[threadIdx.x], wc[threadIdx.x]);
 }
 
-// order[cls_off[c] ...]: the buckets of class c (any order inside a class),
-// one cursor atomic per class per wave.
-__global__ __launch_bounds__(256) void k_order_fill(const uint32_t *__restrict__ cls, uint32_t nb,
-                                                    const uint32_t *__restrict__ cls_off,
-                                                    uint32_t *__restrict__ cursor, uint32_t *__restrict__ order) {
+// order[cls_off[c] ...]: the buckets of class c (any order inside a class);
+// each workgroup reserves its share of every class with one cursor atomic
+// and places its buckets by wave ballots.
+__global__ __launch_bounds__(CT) void k_order_fill(const uint32_t *__restrict__ cls, uint32_t nb,
+                                                   const uint32_t *__restrict__ cls_off,
+                                                   uint32_t *__restrict__ cursor, uint32_t *__restrict__ order) {
+    constexpr int NWV = CT / 64;
+    __shared__ uint32_t wofs[NWV][SORT_CLASSES];  // per wave count, then its offset in the workgroup
+    __shared__ uint32_t base[SORT_CLASSES];
     const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     const int c = b < nb ? (int)cls[b] : -1;
-    const uint32_t lane = __lane_id();
+    const uint32_t lane = __lane_id(), wave = threadIdx.x / 64;
+    uint64_t mine = 0;  // ballot of this thread's class
 #pragma unroll
     for (int q = 0; q < SORT_CLASSES; q++) {
         const uint64_t m = __ballot(c == q);
-        if (!m) continue;
-        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1u;
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&cursor[q], (uint32_t)__popcll(m));
-        base = __shfl(base, (int)leader);
-        if (c == q) order[cls_off[q] + base + (uint32_t)__popcll(m & lanemask_lt())] = b;
+        if (c == q) mine = m;
+        if (lane == 0) wofs[wave][q] = (uint32_t)__popcll(m);
     }
+    __syncthreads();
+    if (threadIdx.x < SORT_CLASSES) {
+        const int q = (int)threadIdx.x;
+        uint32_t run = 0;
+        for (int w = 0; w < NWV; w++) {
+            const uint32_t t = wofs[w][q];
+            wofs[w][q] = run;
+            run += t;
+        }
+        base[q] = run ? atomicAdd(&cursor[q], run) : 0u;
+    }
+    __syncthreads();
+    if (c >= 0) order[cls_off[c] + base[c] + wofs[wave][c] + (uint32_t)__popcll(mine & lanemask_lt())] = b;
 }
 
 // First node of every partition p in [0, P] (node partitions ascend).
@@ -1426,11 +1446,11 @@ void CountEngine::run(const uint8_t *d_fq, uint64_t len, int k, int canonical, u
     if (const char *e = getenv("KITE_SPLIT_DIV")) tun.split_target_div = std::max(1, atoi(e));
     if (const char *e = getenv("KITE_SORT_DEDUP")) tun.dedup = atoi(e) != 0;
     if (const char *e = getenv("KITE_MAX_DBITS")) tun.max_digit_bits = std::min(MAX_DBITS, std::max(1, atoi(e)));
-    if (const char *e = getenv("KITE_HASH_CLASSES")) tun.hash_classes = std::min(3, std::max(0, atoi(e)));
+    if (const char *e = getenv("KITE_HASH_CLASSES")) tun.hash_classes = std::min(SORT_CLASSES, std::max(0, atoi(e)));
     if (const char *e = getenv("KITE_SPLIT_CHUNK")) tun.split_chunk = std::min(64, std::max(1, atoi(e)));
     if (const char *e = getenv("KITE_EXTRACT_SUBS")) tun.extract_subs = std::min(64, std::max(1, atoi(e)));
     if (const char *e = getenv("KITE_EXTRACT_STAGED")) tun.extract_staged = atoi(e) != 0;
-    if (const char *e = getenv("KITE_SPLIT_STAGED")) tun.split_staged = atoi(e) != 0;
+    if (const char *e = getenv("KITE_SPLIT_STAGED")) tun.split_staged = atoi(e) != 0 ? 1 : 0;
     if (words_ == 1) run_w<1>(d_fq, len, k, canonical, pbits);
     else if (words_ == 2) run_w<2>(d_fq, len, k, canonical, pbits);
     else run_w<4>(d_fq, len, k, canonical, pbits);
@@ -1441,6 +1461,14 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
     hipStream_t s = stream_;
     const uint32_t P = 1u << pbits;
     const int CAP = sort_cap(W);
+    // measured: one-word keys do best with children up to half the largest
+    // capacity, all of them hashed (the largest class on 1024 threads); wider
+    // keys with a quarter and the LSD kernel for the largest class
+    const int split_div = tun.split_target_div > 0 ? tun.split_target_div : (W == 1 ? 2 : 4);
+    const int hash_classes = tun.hash_classes >= 0 ? tun.hash_classes : (W == 1 ? SORT_CLASSES : 3);
+    // the regrouped split scatter pays for one-word keys (measured slower
+    // for two-word keys, whose tiles give half as long runs per child)
+    const bool split_staged = tun.split_staged >= 0 ? tun.split_staged != 0 : W == 1;
     // the one-tile scatter keeps 32-bit offsets inside a segment, which the
     // input length bounds (every k-mer ends on its own FASTQ byte)
     const uint32_t spl = (uint32_t)std::max(len > 0xFFFFFFFFull ? 2 : 1, tun.split_chunk);
@@ -1573,7 +1601,7 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
         const uint64_t nact = hv[0], tiles_total = hv[1], maxsz = hv[2], split_keys = hv[3];
         if (nact == 0) break;
         int d = 1;
-        while (d < tun.max_digit_bits && (maxsz >> d) > (uint64_t)(CAP / std::max(1, tun.split_target_div))) d++;
+        while (d < tun.max_digit_bits && (maxsz >> d) > (uint64_t)(CAP / split_div)) d++;
         d = std::min(d, kbits - prefix);
         const uint32_t nd = 1u << d;
         const uint64_t nchild = nact * nd;
@@ -1624,7 +1652,7 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
         sa.child_cur = ccur;
         launch("k_split_scatter", split_keys * 16 * W,
                [&] {
-                   if (chunk == 1 && tun.split_staged && d <= SPLIT_STAGED_BITS)
+                   if (chunk == 1 && split_staged && d <= SPLIT_STAGED_BITS)
                        k_split_staged<W><<<dim3((unsigned)tiles_total), dim3(SST), 0, s>>>(sa);
                    else if (chunk == 1) k_split<W, 1><<<dim3((unsigned)tiles_total), dim3(ST), 0, s>>>(sa);
                    else k_split<W, 2><<<dim3((unsigned)tiles_total), dim3(ST), 0, s>>>(sa);
@@ -1655,7 +1683,7 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
         KITE_CHECK_LAUNCH();
         ClassCaps caps;
         for (int c = 0; c < SORT_CLASSES; c++) caps.cap[c] = (uint64_t)class_cap(W, c);
-        k_classify<<<grid1(nb), 256, 0, s>>>(bkts, (uint32_t)nb, kbits, caps, cls, cls_count);
+        k_classify<<<dim3((unsigned)ceil_div(nb, CT)), CT, 0, s>>>(bkts, (uint32_t)nb, kbits, caps, cls, cls_count);
         KITE_CHECK_LAUNCH();
         uint32_t *hcls = reinterpret_cast<uint32_t *>(hv + 5);
         to_host(cls_count, SORT_CLASSES, hcls);
@@ -1666,7 +1694,7 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
         uint32_t cls_off[SORT_CLASSES + 1] = {0};
         for (int c = 0; c < SORT_CLASSES; c++) cls_off[c + 1] = cls_off[c] + hcls[c];
         uint32_t *ordp = static_cast<uint32_t *>(order_.ensure(nb * 4));
-        k_order_fill<<<grid1(nb), 256, 0, s>>>(cls, (uint32_t)nb, dcls_off, cls_count, ordp);
+        k_order_fill<<<dim3((unsigned)ceil_div(nb, CT)), CT, 0, s>>>(cls, (uint32_t)nb, dcls_off, cls_count, ordp);
         KITE_CHECK_LAUNCH();
         const uint32_t *ord = order_.as<uint32_t>();
         // buckets the hash kernel hands back (a bin above its local limit),
@@ -1680,11 +1708,13 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
             constexpr int C = decltype(cls)::value;
             if (!cnt) return;
             // hash kernel: 4 / W items per thread, so a bigger class gets more
-            // threads (up to 1024) rather than more items per thread
-            if constexpr (C < 3) {
-                if (tun.dedup && C < tun.hash_classes) {
-                    constexpr int IPTH = W == 1 ? 4 : (W == 2 ? 2 : 1);
+            // threads (up to 1024) rather than more items per thread; the
+            // largest class takes 8 / W items on 1024 threads
+            {
+                if (tun.dedup && C < hash_classes) {
+                    constexpr int IPTH = (C < 3 ? 4 : 8) / W > 0 ? (C < 3 ? 4 : 8) / W : 1;
                     constexpr int NTH = SortClass<W, C>::CAP / IPTH;
+                    static_assert(NTH <= 1024, "one workgroup per bucket");
                     launch("k_bucket_sort", 0, [&] {
                         k_bucket_hash_sort<W, IPTH, NTH><<<dim3(cnt), dim3(NTH), 0, s>>>(
                             A, B, ctmp, bkts, ord + first, ucount, k, fb + first, fbc + C);
@@ -1705,10 +1735,10 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
         if (hashed) {
             // the handed-back buckets of each class through the LSD kernel
             uint32_t *hfb = reinterpret_cast<uint32_t *>(hv + 10);
-            to_host(fbc, 3, hfb);
+            to_host(fbc, SORT_CLASSES, hfb);
             KITE_HIP(hipStreamSynchronize(s));
-            const uint32_t nfb[3] = {hfb[0], hfb[1], hfb[2]};
-            n_fallback_ = nfb[0] + nfb[1] + nfb[2];
+            const uint32_t nfb[SORT_CLASSES] = {hfb[0], hfb[1], hfb[2], hfb[3]};
+            n_fallback_ = nfb[0] + nfb[1] + nfb[2] + nfb[3];
             auto lsd = [&](auto cls, uint32_t first, uint32_t cnt) {
                 constexpr int C = decltype(cls)::value;
                 if (!cnt) return;
@@ -1720,6 +1750,7 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
             lsd(std::integral_constant<int, 0>{}, cls_off[0], nfb[0]);
             lsd(std::integral_constant<int, 1>{}, cls_off[1], nfb[1]);
             lsd(std::integral_constant<int, 2>{}, cls_off[2], nfb[2]);
+            lsd(std::integral_constant<int, 3>{}, cls_off[3], nfb[3]);
         }
     }
     tm.mark();

@@ -45,14 +45,14 @@ struct KernelStat {
 };
 
 struct CountTunables {
-    int split_target_div = 4;  // split partitions until max child <= largest cap / div
+    int split_target_div = 0;  // split partitions until max child <= largest cap / div (0: by key width)
     int max_digit_bits = 12;
     bool dedup = true;  // small buckets: hash dedup + one bin pass instead of LSD passes
-    int hash_classes = 3;  // size classes [0, hash_classes) use the hash kernel
+    int hash_classes = -1;  // size classes [0, hash_classes) use the hash kernel (-1: by key width)
     int split_chunk = 1;   // split tiles per workgroup (longer runs per child, fewer atomics)
     int extract_subs = 4;  // sub-tiles per extract workgroup
     bool extract_staged = true;  // group each chunk's keys by partition in LDS before writing
-    bool split_staged = true;    // group each split tile's keys by child in LDS before writing
+    int split_staged = -1;       // group each split tile's keys by child in LDS (-1: one-word keys)
 };
 
 class CountEngine {

@@ -270,11 +270,15 @@ def test_split_scatter_modes(gpu_ctx, monkeypatch, staged, max_bits):
         check_same(res, tab, f"split staged={staged} max_bits={max_bits} k={k} P={P}")
 
 
-def test_hash_kernel_on_class_two(gpu_ctx, monkeypatch):
-    """The hash kernel with 1024 threads on the 4096-key class."""
-    monkeypatch.setenv("KITE_HASH_CLASSES", "3")
+@pytest.mark.parametrize("classes,div", [("3", "4"), ("4", "1"), ("4", "2")])
+def test_hash_kernel_on_large_classes(gpu_ctx, monkeypatch, classes, div):
+    """The hash kernel with 1024 threads on the 4096-key class and, with
+    four hash classes, on the largest class (8 / W keys per thread), where
+    a split target of the whole largest capacity fills that class."""
+    monkeypatch.setenv("KITE_HASH_CLASSES", classes)
+    monkeypatch.setenv("KITE_SPLIT_DIV", div)
     from kite_amd.workloads import K31_CANON_50MBP, K63_CANON_100MBP
     for w, n, P in ((K31_CANON_50MBP, 300000, 8), (K63_CANON_100MBP, 40000, 4)):
         fq = w.fastq(n_reads=n)
         res, tab, _ = run_both(gpu_ctx, fq, w.k, w.canonical, P)
-        check_same(res, tab, f"{w.name} hash classes 3")
+        check_same(res, tab, f"{w.name} hash classes {classes} split div {div}")
