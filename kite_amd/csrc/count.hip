@@ -112,6 +112,22 @@ __device__ __forceinline__ uint32_t key_bits(const Key<W> &a, int shift, int nb)
     return (uint32_t)(v & ((1ull << nb) - 1));
 }
 
+// Loads NI keys of a tile, item `it` from position it * STRIDE + tid of
+// src[0, n) (n >= 1).  Positions past n read the last key, so that every
+// load is unconditional and the NI loads are in flight together; a guarded
+// load whose value is used right away would make each wave wait for one
+// load at a time.
+template <int W, int NI, int STRIDE>
+__device__ __forceinline__ void load_tile(const uint64_t *__restrict__ src, uint64_t n, int tid, Key<W> (&key)[NI]) {
+#pragma unroll
+    for (int it = 0; it < NI; it++) {
+        const uint64_t i = (uint64_t)it * STRIDE + (uint64_t)tid;
+        const uint64_t *p = src + (i < n ? i : n - 1) * W;
+#pragma unroll
+        for (int r = 0; r < W; r++) key[it].w[r] = p[r];
+    }
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
 // L2).  This bijection on [0, n) gives XCD x a contiguous range of logical
 // tiles, so tiles that write neighbouring destination lines share an L2 and
@@ -521,17 +537,11 @@ __global__ __launch_bounds__(ST) void k_split(SplitArgs a) {
         for (uint32_t g = tid; g < nd; g += ST) hist[g] = 0;
         __syncthreads();
         for (uint64_t c0 = 0; c0 < n; c0 += TK) {
+            Key<W> key[IPT];
+            load_tile<W, IPT, ST>(a.src + (first + c0) * W, n - c0, tid, key);
 #pragma unroll
-            for (int it = 0; it < IPT; it++) {
-                uint64_t i = c0 + (uint64_t)it * ST + tid;
-                if (i < n) {
-                    Key<W> key;
-                    const uint64_t *p = a.src + (first + i) * W;
-#pragma unroll
-                    for (int r = 0; r < W; r++) key.w[r] = p[r];
-                    atomicAdd(&hist[key_bits<W>(key, a.shift, a.dbits)], 1u);
-                }
-            }
+            for (int it = 0; it < IPT; it++)
+                if (c0 + (uint64_t)it * ST + tid < n) atomicAdd(&hist[key_bits<W>(key[it], a.shift, a.dbits)], 1u);
         }
         __syncthreads();
         for (uint32_t g = tid; g < nd; g += ST) {
@@ -543,18 +553,12 @@ __global__ __launch_bounds__(ST) void k_split(SplitArgs a) {
         // one tile: its keys stay in registers between the count and the
         // scatter; digits are recomputed rather than held
         Key<W> key[IPT];
+        load_tile<W, IPT, ST>(a.src + first * W, n, tid, key);
         for (uint32_t g = tid; g < nd; g += ST) hist[g] = 0;
         __syncthreads();
 #pragma unroll
-        for (int it = 0; it < IPT; it++) {
-            uint64_t i = (uint64_t)it * ST + tid;
-            if (i < n) {
-                const uint64_t *p = a.src + (first + i) * W;
-#pragma unroll
-                for (int r = 0; r < W; r++) key[it].w[r] = p[r];
-                atomicAdd(&hist[key_bits<W>(key[it], a.shift, a.dbits)], 1u);
-            }
-        }
+        for (int it = 0; it < IPT; it++)
+            if ((uint64_t)it * ST + tid < n) atomicAdd(&hist[key_bits<W>(key[it], a.shift, a.dbits)], 1u);
         __syncthreads();
         for (uint32_t g = tid; g < nd; g += ST) {
             const uint32_t c = hist[g];
@@ -585,16 +589,9 @@ __global__ __launch_bounds__(ST) void k_split(SplitArgs a) {
         for (uint64_t c0 = 0; c0 < n; c0 += TK) {
             Key<W> key[IPT];
             uint32_t dig[IPT];
+            load_tile<W, IPT, ST>(a.src + (first + c0) * W, n - c0, tid, key);
 #pragma unroll
-            for (int it = 0; it < IPT; it++) {
-                uint64_t i = c0 + (uint64_t)it * ST + tid;
-                if (i < n) {
-                    const uint64_t *p = a.src + (first + i) * W;
-#pragma unroll
-                    for (int r = 0; r < W; r++) key[it].w[r] = p[r];
-                    dig[it] = key_bits<W>(key[it], a.shift, a.dbits);
-                }
-            }
+            for (int it = 0; it < IPT; it++) dig[it] = key_bits<W>(key[it], a.shift, a.dbits);
 #pragma unroll
             for (int it = 0; it < IPT; it++) {
                 uint64_t i = c0 + (uint64_t)it * ST + tid;
@@ -639,17 +636,14 @@ __global__ __launch_bounds__(SST) void k_split_staged(SplitArgs a) {
     const uint64_t first = sg.start + t * TK;
     const uint32_t n = (uint32_t)std::min<uint64_t>(TK, sg.size - t * TK);
     const uint32_t nd = 1u << a.dbits;
+    Key<W> key[IPT];
+    load_tile<W, IPT, SST>(a.src + first * W, n, tid, key);
     for (uint32_t g = tid; g < nd; g += SST) loff[g] = 0;
     __syncthreads();
-    Key<W> key[IPT];
     uint32_t dig[IPT], rk[IPT];
 #pragma unroll
     for (int it = 0; it < IPT; it++) {
-        const uint32_t i = (uint32_t)it * SST + tid;
-        if (i < n) {
-            const uint64_t *p = a.src + (first + i) * W;
-#pragma unroll
-            for (int r = 0; r < W; r++) key[it].w[r] = p[r];
+        if ((uint32_t)it * SST + tid < n) {
             dig[it] = key_bits<W>(key[it], a.shift, a.dbits);
             rk[it] = atomicAdd(&loff[dig[it]], 1u);
         }
@@ -914,17 +908,21 @@ __global__ __launch_bounds__(NT, hash_sort_waves(IPT)) void k_bucket_hash_sort(u
     for (int j = tid; j < HT; j += NT) ht[j] = 0;
     for (int j = tid; j < CAP / 2; j += NT) scnt32[j] = 0;
     Key<W> key[IPT];
+    // all loads first (positions past the bucket read its last key), then
+    // the LDS copies: the loads are in flight together
 #pragma unroll
     for (int it = 0; it < IPT; it++) {
-        uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
-        if (it < ipt && pos < m) {
-            const uint64_t *p = X + (start + pos) * W;
+        const uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
+        const uint64_t *p = X + (start + (it < ipt && pos < m ? pos : m - 1)) * W;
 #pragma unroll
-            for (int r = 0; r < W; r++) {
-                key[it].w[r] = p[r];
-                skeys[pos * W + r] = key[it].w[r];
-            }
-        }
+        for (int r = 0; r < W; r++) key[it].w[r] = p[r];
+    }
+#pragma unroll
+    for (int it = 0; it < IPT; it++) {
+        const uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
+        if (it < ipt && pos < m)
+#pragma unroll
+            for (int r = 0; r < W; r++) skeys[pos * W + r] = key[it].w[r];
     }
     __syncthreads();
     // 1. hash dedup: the first position of every key collects its count.
