@@ -234,7 +234,18 @@ __global__ __launch_bounds__(XT) void k_extract(ExtractArgs a) {
         if (st >= a.n_sub) break;
         const int64_t base = (int64_t)(st * SUB);
         __syncthreads();
-        for (int i = tid; i < EXT_U4; i += XT) {
+        if (base >= HALO && base + SUB <= (int64_t)a.len) {
+            // the whole sub-tile and its halo are inside: unconditional
+            // loads, all in flight before the first LDS store
+            constexpr int LPT = EXT_U4 / XT;
+            static_assert(EXT_U4 % XT == 0, "whole 16-byte words per thread");
+            uint4 v[LPT];
+            const uint4 *src = reinterpret_cast<const uint4 *>(a.fq + base - HALO);
+#pragma unroll
+            for (int u = 0; u < LPT; u++) v[u] = src[u * XT + tid];
+#pragma unroll
+            for (int u = 0; u < LPT; u++) sb4[u * XT + tid] = v[u];
+        } else for (int i = tid; i < EXT_U4; i += XT) {
             int64_t gb = base - HALO + 16 * (int64_t)i;
             uint4 v = make_uint4(0, 0, 0, 0);
             if (gb >= 0 && gb + 16 <= (int64_t)a.len) {

@@ -15,12 +15,18 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_reduce(const T *__restrict__ in
                                                         uint64_t *__restrict__ sums) {
     __shared__ uint64_t ws[SCAN_T / 64 + 1];
     uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
-    uint64_t acc = 0;
+    // every load unconditional (past n: the last element, not added), so
+    // that they are in flight together
+    T v[SCAN_IPT];
 #pragma unroll
     for (int i = 0; i < SCAN_IPT; i++) {
-        uint64_t idx = base + (uint64_t)i * SCAN_T + threadIdx.x;
-        if (idx < n) acc += (uint64_t)in[idx];
+        const uint64_t idx = base + (uint64_t)i * SCAN_T + threadIdx.x;
+        v[i] = in[idx < n ? idx : n - 1];
     }
+    uint64_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_IPT; i++)
+        if (base + (uint64_t)i * SCAN_T + threadIdx.x < n) acc += (uint64_t)v[i];
     uint64_t total;
     (void)block_excl_scan<SCAN_T, uint64_t>(acc, ws, &total);
     if (threadIdx.x == 0) sums[blockIdx.x] = total;
@@ -55,8 +61,12 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_apply(const T *__restrict__ in,
     uint64_t acc = 0;
 #pragma unroll
     for (int i = 0; i < SCAN_IPT; i++) {
-        uint64_t idx = base + i;
-        v[i] = idx < n ? (uint64_t)in[idx] : 0;
+        const uint64_t idx = base + i;
+        v[i] = (uint64_t)in[idx < n ? idx : n - 1];
+    }
+#pragma unroll
+    for (int i = 0; i < SCAN_IPT; i++) {
+        if (base + i >= n) v[i] = 0;
         acc += v[i];
     }
     uint64_t total;
