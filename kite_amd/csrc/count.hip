@@ -189,7 +189,12 @@ struct StageCfg {
 template <int W, int MODE>
 __global__ __launch_bounds__(XT) void k_extract(ExtractArgs a) {
     constexpr bool SCATTER = MODE != 0;
-    __shared__ uint4 sb4[EXT_U4];
+    // sub-tile bytes, then (histogram and staged passes) the k-mer end
+    // positions; the staged pass reuses both as one chunk's staging once
+    // the positions are read
+    constexpr int SPOS_U4 = MODE != 1 ? XT * SEGB * 2 / 16 : 0;
+    __shared__ uint4 sraw[EXT_U4 + SPOS_U4];
+    uint4 *sb4 = sraw;
     __shared__ uint64_t scode[NSEG];
     __shared__ uint64_t srcode[NSEG];  // canonical: each segment reverse-complemented
     __shared__ uint32_t svalid[NSEG];
@@ -198,7 +203,7 @@ __global__ __launch_bounds__(XT) void k_extract(ExtractArgs a) {
     __shared__ uint32_t ws[XT / 64 + 1];
     // k-mer end positions of the sub-tile (histogram and staged passes:
     // every lane builds k-mers; the direct scatter keeps the per-segment loop)
-    __shared__ uint16_t spos[MODE != 1 ? XT * SEGB : 1];
+    uint16_t *spos = reinterpret_cast<uint16_t *>(sraw + EXT_U4);
     __shared__ uint16_t spart[MODE == 2 ? StageCfg<W>::SC : 1];  // partition of each staged key
     // P u32 counters, then (scatter) P u64 bases, then (staged) per-chunk
     // counts and offsets, P u32 each
@@ -425,6 +430,71 @@ __global__ __launch_bounds__(XT) void k_extract(ExtractArgs a) {
             uint32_t slot = block_excl_scan<XT, uint32_t>(__popc(emask), ws, &nk);
             for (uint32_t e = emask; e; e &= e - 1) spos[slot++] = (uint16_t)(s * SEGB + __ffs(e) - 1);
             __syncthreads();
+            // one chunk for the whole sub-tile when its keys fit the bytes and
+            // positions together (longer runs per partition), staged in
+            // place once every position is read; else chunks of SC keys
+            // staged in the sub-tile bytes alone
+            constexpr int SCB = (EXT_U4 + SPOS_U4) * 16 / (8 * W), EPTB = SCB / XT;
+            static_assert(SCB % XT == 0, "whole keys per thread");
+            if (nk <= (uint32_t)SCB) {
+                Key<W> key[EPTB];
+                uint32_t part[EPTB], rank[EPTB];
+#pragma unroll
+                for (int u = 0; u < EPTB; u++) {
+                    const uint32_t i = (uint32_t)u * XT + tid;
+                    if (i < nk) {
+                        const uint32_t pos = spos[i];
+                        key[u] = kmer_at((int)(pos / SEGB), (int)(pos % SEGB));
+                        part[u] = key_partition<W>(key[u], a.pbits);
+                        rank[u] = atomicAdd(&lcnt[part[u]], 1u);
+                    }
+                }
+                __syncthreads();
+                {
+                    const uint32_t p0 = tid * PPT;
+                    uint32_t sum = 0;
+                    for (uint32_t q = 0; q < PPT; q++)
+                        if (p0 + q < P) sum += lcnt[p0 + q];
+                    uint32_t tot;
+                    uint32_t run = block_excl_scan<XT, uint32_t>(sum, ws, &tot);
+                    for (uint32_t q = 0; q < PPT; q++)
+                        if (p0 + q < P) {
+                            loff[p0 + q] = run;
+                            run += lcnt[p0 + q];
+                        }
+                }
+                __syncthreads();
+                uint64_t *big = reinterpret_cast<uint64_t *>(sraw);
+#pragma unroll
+                for (int u = 0; u < EPTB; u++) {
+                    const uint32_t i = (uint32_t)u * XT + tid;
+                    if (i < nk) {
+                        const uint32_t d = loff[part[u]] + rank[u];
+#pragma unroll
+                        for (int r = 0; r < W; r++) big[d * W + r] = key[u].w[r];
+                    }
+                }
+                __syncthreads();
+                for (uint32_t i = tid; i < nk; i += XT) {
+                    Key<W> kk;
+#pragma unroll
+                    for (int r = 0; r < W; r++) kk.w[r] = big[i * W + r];
+                    const uint32_t p = key_partition<W>(kk, a.pbits);  // recomputed, not stored
+                    uint64_t *dst = a.out + (pbase[p] + cnt[p] + (i - loff[p])) * W;
+#pragma unroll
+                    for (int r = 0; r < W; r++) dst[r] = kk.w[r];
+                }
+                __syncthreads();
+                for (uint32_t q = 0; q < PPT; q++) {
+                    const uint32_t p = tid * PPT + q;
+                    if (p < P) {
+                        cnt[p] += lcnt[p];
+                        lcnt[p] = 0;
+                    }
+                }
+                // (the next sub-tile's fill starts with a barrier)
+                continue;
+            }
             // the sub-tile bytes are no longer needed: they hold the chunk
             uint64_t *stage = reinterpret_cast<uint64_t *>(sb4);
             for (uint32_t c0 = 0; c0 < nk; c0 += SC) {
