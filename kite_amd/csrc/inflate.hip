@@ -42,7 +42,20 @@ struct Bits {
     uint64_t buf;
     int cnt;
     uint32_t pad;
+    // n <= 32.  Four bytes at once from the two aligned words holding them
+    // while eight or more bytes are left (one dependent load pair per four
+    // bytes instead of per byte), then byte by byte.
     __device__ void fill(int n) {
+        if (cnt >= n) return;
+        if (end - p >= 8) {
+            const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+            const uint32_t *w = reinterpret_cast<const uint32_t *>(a & ~static_cast<uintptr_t>(3));
+            const uint64_t two = (uint64_t)w[0] | ((uint64_t)w[1] << 32);
+            buf |= ((two >> (8 * (a & 3))) & 0xFFFFFFFFull) << cnt;  // cnt < 32: no bit lost
+            cnt += 32;
+            p += 4;
+            return;
+        }
         while (cnt < n) {
             uint64_t v = 0;
             if (p < end) v = *p++;
