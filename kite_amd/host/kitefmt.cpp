@@ -106,15 +106,78 @@ Writer::Writer(const std::string &path, const Header &h, size_t staging_bytes, i
     prov.total_chunks = prov.total_rows = 0;
     prov.write(hdr);
     if (fwrite(hdr, 1, 128, f_) != 128) throw Error("write failed");
+    if (regular_) {
+        if (fflush(f_) != 0) throw Error("write failed");
+        fd_ = fileno(f_);
+        for (int i = 0; i < std::max(1, write_lanes); i++) pool_.emplace_back([this] { worker(); });
+    }
 }
 
 Writer::~Writer() {
+    stop_writers();
     if (f_) fclose(f_);
+}
+
+void Writer::write_at(const uint8_t *data, size_t len, uint64_t off) {
+    while (len) {
+        const ssize_t w = pwrite(fd_, data, len, (off_t)off);
+        if (w <= 0) {
+            if (w < 0 && errno == EINTR) continue;
+            throw Error(std::string("write failed: ") + strerror(errno));
+        }
+        data += w;
+        len -= (size_t)w;
+        off += (uint64_t)w;
+    }
+}
+
+void Writer::worker() {
+    for (;;) {
+        Job j;
+        {
+            std::unique_lock<std::mutex> g(mu_);
+            cv_.wait(g, [&] { return stop_ || !jobs_.empty(); });
+            if (jobs_.empty()) return;
+            j = std::move(jobs_.front());
+            jobs_.pop_front();
+            busy_++;
+        }
+        std::string e;
+        try {
+            write_at(j.buf.data(), j.buf.size(), j.off);
+        } catch (const std::exception &x) {
+            e = x.what();
+        }
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            busy_--;
+            queued_bytes_ -= j.buf.size();
+            if (!e.empty() && werr_.empty()) werr_ = e;
+        }
+        cv_done_.notify_all();
+    }
+}
+
+void Writer::wait_writes() {
+    std::unique_lock<std::mutex> g(mu_);
+    cv_done_.wait(g, [&] { return jobs_.empty() && busy_ == 0; });
+    if (!werr_.empty()) throw Error(werr_);
+}
+
+void Writer::stop_writers() {
+    {
+        std::lock_guard<std::mutex> g(mu_);
+        stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto &t : pool_) t.join();
+    pool_.clear();
 }
 
 uint64_t Writer::claim(const uint8_t *data, size_t len) {
     uint64_t off = next_offset_;
-    if (len && fwrite(data, 1, len, f_) != len) throw Error("write failed");
+    if (fd_ >= 0) write_at(data, len, off);  // the caller's buffer: written now
+    else if (len && fwrite(data, 1, len, f_) != len) throw Error("write failed");
     next_offset_ += len;
     return off;
 }
@@ -122,8 +185,24 @@ uint64_t Writer::claim(const uint8_t *data, size_t len) {
 void Writer::flush_partition(uint32_t p) {
     Stage &s = st_[p];
     if (s.buf.empty()) return;
-    uint64_t off = claim(s.buf.data(), s.buf.size());
-    staged_ -= s.buf.size();
+    const size_t n = s.buf.size();
+    uint64_t off;
+    if (fd_ >= 0) {
+        // the staging buffer goes to a writer thread; at most high_ bytes queued
+        off = next_offset_;
+        next_offset_ += n;
+        std::unique_lock<std::mutex> g(mu_);
+        cv_done_.wait(g, [&] { return queued_bytes_ < high_ || !werr_.empty(); });
+        if (!werr_.empty()) throw Error(werr_);
+        queued_bytes_ += n;
+        jobs_.push_back(Job{off, std::move(s.buf)});
+        g.unlock();
+        cv_.notify_one();
+        s.buf = std::vector<uint8_t>();
+    } else {
+        off = claim(s.buf.data(), s.buf.size());
+    }
+    staged_ -= n;
     s.buf.clear();
     for (ChunkMeta m : s.pending) {
         m.offset += off;
@@ -184,8 +263,13 @@ std::pair<uint64_t, uint64_t> Writer::finalize() {
     if (finalized_) throw Error("writer finalized twice");
     finalized_ = true;
     uint64_t total_chunks = 0, total_rows = 0;
+    for (uint32_t p = 0; p < h_.partitions; p++) flush_partition(p);
+    if (fd_ >= 0) {
+        wait_writes();
+        stop_writers();
+        if (fseeko(f_, (off_t)next_offset_, SEEK_SET) != 0) throw Error("write failed");
+    }
     for (uint32_t p = 0; p < h_.partitions; p++) {
-        flush_partition(p);
         auto &c = st_[p].committed;
         std::sort(c.begin(), c.end(), [](const ChunkMeta &a, const ChunkMeta &b) { return a.offset < b.offset; });
         total_chunks += c.size();

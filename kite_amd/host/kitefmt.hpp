@@ -18,6 +18,10 @@
 #include <optional>
 #include <stdexcept>
 #include <string>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 namespace kh {
@@ -97,14 +101,32 @@ class Writer {
     uint64_t claim(const uint8_t *data, size_t len);
     void flush_partition(uint32_t p);
     bool drain_one();
+    // regular files: flushed staging buffers are written at their claimed
+    // offsets by `write_lanes` threads (the file layout does not change)
+    struct Job {
+        uint64_t off;
+        std::vector<uint8_t> buf;
+    };
+    void write_at(const uint8_t *data, size_t len, uint64_t off);
+    void worker();
+    void wait_writes();
+    void stop_writers();
 
     FILE *f_ = nullptr;
+    int fd_ = -1;
     bool regular_ = true;
     bool finalized_ = false;
     Header h_;
     uint64_t next_offset_ = 128;
     size_t S_, high_, low_, staged_ = 0;
     std::vector<Stage> st_;
+    std::vector<std::thread> pool_;
+    std::mutex mu_;
+    std::condition_variable cv_, cv_done_;
+    std::deque<Job> jobs_;
+    size_t busy_ = 0, queued_bytes_ = 0;
+    bool stop_ = false;
+    std::string werr_;
 };
 
 // Memory-mapped reader with the validation of KiteReader::open
