@@ -3,6 +3,8 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
+#include <utility>
 #include <vector>
 
 #include "common.hpp"
@@ -54,8 +56,29 @@ struct EncodeDevice {
 // bytes[byte_off[p], byte_off[p + 1]) and holds rows_off[p + 1] - rows_off[p]
 // rows; minmax[8p .. 8p+3] is its first key, minmax[8p+4 .. 8p+7] its last
 // (4 words each, unused words 0).
+// Allocator whose resize leaves new elements uninitialised (a payload that
+// a device copy fills at once need not be zeroed first).
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = NoInitAlloc<U>;
+    };
+    NoInitAlloc() = default;
+    template <class U>
+    NoInitAlloc(const NoInitAlloc<U> &) noexcept {}
+    template <class U>
+    void construct(U *p) noexcept {
+        ::new (static_cast<void *>(p)) U;
+    }
+    template <class U, class... A>
+    void construct(U *p, A &&...a) {
+        ::new (static_cast<void *>(p)) U(std::forward<A>(a)...);
+    }
+};
+
 struct EncodedRuns {
-    std::vector<uint8_t> bytes;
+    std::vector<uint8_t, NoInitAlloc<uint8_t>> bytes;
     std::vector<uint64_t> byte_off, rows_off, minmax;
     uint32_t codec = 0;
     int words = 1;

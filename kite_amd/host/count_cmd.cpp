@@ -11,6 +11,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <fstream>
 #include <sstream>
 #include <thread>
@@ -433,12 +434,34 @@ int run_count(int argc, char **argv) {
             if (have == buf.size()) buf.resize(buf.size() * 2);
         }
     } else {
+        // the next chunk is read on a second thread while this one is fed
         Source src(input, comp);
         Chunker ch(src, pl.chunk);
-        std::vector<uint8_t> c;
-        while (ch.next(c)) {
-            gpu_check(kite_gpu_count_feed(ctx, c.data(), c.size()), "count");
+        std::vector<uint8_t> c, nx;
+        bool more = ch.next(c);
+        while (more) {
+            bool more_n = false;
+            std::exception_ptr rerr;
+            std::thread rd([&] {
+                try {
+                    more_n = ch.next(nx);
+                } catch (...) {
+                    rerr = std::current_exception();
+                }
+            });
+            int rc;
+            try {
+                rc = kite_gpu_count_feed(ctx, c.data(), c.size());
+            } catch (...) {
+                rd.join();
+                throw;
+            }
+            rd.join();
+            gpu_check(rc, "count");
+            if (rerr) std::rethrow_exception(rerr);
             chunks++;
+            std::swap(c, nx);
+            more = more_n;
         }
     }
     kite_count_stats st{};
