@@ -216,7 +216,7 @@ __device__ uint32_t read_tables(Bits &b, Member &t, uint32_t type) {
 // copy whose source overlaps its destination (distance below the length)
 // goes in rounds that double the copied span, each round reading only
 // bytes written before it.
-__global__ __launch_bounds__(64) void k_inflate(const uint8_t *__restrict__ comp, const InflateBlock *__restrict__ blocks,
+__global__ __launch_bounds__(64, 4) void k_inflate(const uint8_t *__restrict__ comp, const InflateBlock *__restrict__ blocks,
                                                 uint32_t nb, uint8_t *__restrict__ out, uint32_t *__restrict__ err) {
     __shared__ Member t;
     const uint32_t lane = threadIdx.x;
