@@ -11,6 +11,7 @@ int run_histo(int argc, char **argv);
 int run_merge(int argc, char **argv);
 int run_info(int argc, char **argv);
 int run_analyze(int argc, char **argv);
+int run_profile(int argc, char **argv);
 
 // "[kite <cmd>] Command: ..." echo (src/kiteops/mod.rs:72-97).
 void print_command_line(const char *cmd, int argc, char **argv);

@@ -22,6 +22,7 @@ int usage() {
             "  info     Show metadata and summary information for a .kite file.\n"
             "  histo    Generate a k-mer count histogram table.\n"
             "  merge    Merge multiple .kite files into one output file.\n"
+            "  profile  Profile k-mer content across samples into a table.\n"
             "  analyze  Analyze multiple .kite files and write TSV tables.\n");
     return 2;
 }
@@ -37,6 +38,7 @@ int main(int argc, char **argv) {
         if (cmd == "info") return kh::run_info(argc, argv);
         if (cmd == "histo") return kh::run_histo(argc, argv);
         if (cmd == "merge") return kh::run_merge(argc, argv);
+        if (cmd == "profile") return kh::run_profile(argc, argv);
         if (cmd == "analyze") return kh::run_analyze(argc, argv);
         if (cmd == "-h" || cmd == "--help" || cmd == "help") {
             usage();

@@ -1,0 +1,213 @@
+// profile_cmd.cpp -- `kite profile`: one TSV row per k-mer of the union of
+// the inputs, with each input's (header-filtered, merged) count, 0 where the
+// input lacks the k-mer (run_profile / render_partition_rows,
+// src/profile/mod.rs:14-169).  The union is built on the GPU
+// (kite_gpu_union_tables, the device form of for_each_union_row,
+// src/kiteops/mod.rs:625-802); its matrix saturates at u32::MAX, so the few
+// cells at that value are re-read as exact u64 sums from the input's own rows
+// (load_partition_counts sums with saturating u64 adds, :433-470).
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstring>
+#include <thread>
+
+#include "cli.hpp"
+#include "commands.hpp"
+#include "kite_gpu.h"
+#include "kitefmt.hpp"
+#include "merge_host.hpp"
+
+namespace kh {
+
+namespace {
+// Exact count of `key` in rows [lo, hi) of one input's sorted table.
+uint64_t exact_count(const std::vector<uint64_t> &K, const std::vector<uint64_t> &C, uint64_t lo, uint64_t hi,
+                     const uint64_t *key, int W) {
+    auto less = [&](const uint64_t *a, const uint64_t *b) {
+        for (int q = W - 1; q >= 0; q--)
+            if (a[q] != b[q]) return a[q] < b[q];
+        return false;
+    };
+    while (lo < hi) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        if (less(&K[mid * W], key))
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    if (lo < C.size() && !less(key, &K[lo * W]) && !less(&K[lo * W], key)) return C[lo];
+    throw Error("profile: saturated union cell has no matching input row");
+}
+
+std::string fmt_filter(const Header &h) {
+    auto f = [](const std::optional<uint32_t> &v) { return v ? std::to_string(*v) : std::string("none"); };
+    return "min=" + f(h.min_count) + ", max=" + f(h.max_count);
+}
+}  // namespace
+
+int run_profile(int argc, char **argv) {
+    Args a({{"input", 'i', true, true}, {"output", 'o'}, {"label", 0, true, true}, {"threads", 't'},
+            {"progress", 'p', false}, {"test-mode", 0, false}},
+           argc, argv, 2);
+    print_command_line("profile", argc, argv);
+    const auto t0 = std::chrono::steady_clock::now();
+    auto inputs = a.list("input");
+    if (inputs.empty() || !a.has("output"))
+        throw UsageError("the following required arguments were not provided:\n  --input <FILE>...\n  --output <FILE>");
+    const std::string out_path = a.get("output");
+    // infer_sample_labels (src/kiteops/mod.rs:123-162)
+    auto labels = a.list("label");
+    const bool user_labels = !labels.empty();
+    if (user_labels && labels.size() != inputs.size()) {
+        std::string ls, is;
+        for (size_t i = 0; i < labels.size(); i++) ls += (i ? ", " : "") + labels[i];
+        for (size_t i = 0; i < inputs.size(); i++) is += (i ? ", " : "") + inputs[i];
+        throw Error("--label count (" + std::to_string(labels.size()) + ") must match input count (" +
+                    std::to_string(inputs.size()) + "). labels=[" + ls + "], inputs=[" + is + "]");
+    }
+    if (!user_labels)
+        for (size_t i = 0; i < inputs.size(); i++) labels.push_back(infer_label(inputs[i], i));
+    fprintf(stderr, "[kite profile] inputs=%zu, output=%s\n", inputs.size(), out_path.c_str());
+    std::vector<std::unique_ptr<Reader>> rs;
+    for (auto &p : inputs) rs.emplace_back(new Reader(p));
+    {
+        std::string j;
+        for (size_t i = 0; i < labels.size(); i++) j += (i ? "," : "") + labels[i];
+        fprintf(stderr, "[kite profile] labels=%s (%s)\n", j.c_str(),
+                user_labels ? "user supplied" : "inferred from file names");
+    }
+    for (size_t i = 0; i < rs.size(); i++) {
+        const Header &h = rs[i]->header();
+        fprintf(stderr,
+                "[kite profile] input #%2zu label=%s path=%s k=%u canonical=%s partitions=%u key_width=%u "
+                "rows=%llu chunks=%llu filters: %s\n",
+                i + 1, labels[i].c_str(), inputs[i].c_str(), (unsigned)h.k, h.canonical ? "true" : "false",
+                (unsigned)h.partitions, (unsigned)h.key_width, (unsigned long long)h.total_rows,
+                (unsigned long long)rs[i]->chunk_count(), fmt_filter(h).c_str());
+    }
+    {
+        const Header &f = rs[0]->header();
+        for (size_t i = 1; i < rs.size(); i++) {
+            const Header &h = rs[i]->header();
+            if (h.k != f.k || (h.canonical != 0) != (f.canonical != 0) || h.partitions != f.partitions ||
+                h.key_width != f.key_width)
+                throw Error("incompatible .kite inputs: input #" + std::to_string(i + 1) + " (" + inputs[i] +
+                            ") does not match input #1 (" + inputs[0] + ") in k, canonical, partitions or key width");
+        }
+    }
+    const unsigned nt = worker_threads((unsigned)a.get_u64("threads", 0));
+    fprintf(stderr, "[kite profile] worker_threads=%u (parallel partitions in batches)\n", nt);
+    const uint32_t P = rs[0]->partitions();
+    const int W = rs[0]->words(), k = rs[0]->header().k;
+    const size_t n = rs.size();
+    std::vector<std::vector<uint64_t>> K(n), C(n), O(n);
+    uint64_t sum_rows = 0;
+    for (size_t i = 0; i < n; i++) {
+        load_filtered(*rs[i], 0, P, nt, K[i], C[i], O[i]);
+        sum_rows += O[i][P];
+    }
+    std::vector<const uint64_t *> kp(n), cp(n), op(n);
+    for (size_t i = 0; i < n; i++) {
+        kp[i] = K[i].data();
+        cp[i] = C[i].data();
+        op[i] = O[i].data();
+    }
+    std::vector<uint64_t> uk(std::max<uint64_t>(sum_rows, 1) * W), uo(P + 1);
+    std::vector<uint32_t> um(std::max<uint64_t>(sum_rows, 1) * n);
+    {
+        kite_gpu_ctx *ctx = nullptr;
+        if (kite_gpu_open(0, 0, &ctx))
+            throw Error(std::string("failed to open the GPU engine: ") + kite_gpu_last_error());
+        const int rc = kite_gpu_union_tables(ctx, (uint32_t)W, P, (uint32_t)n, kp.data(), cp.data(), op.data(),
+                                             uk.data(), um.data(), uo.data());
+        std::string err = rc ? kite_gpu_last_error() : "";
+        kite_gpu_close(ctx);
+        if (rc) throw Error("GPU union failed: " + err);
+    }
+    const uint64_t N = uo[P];
+
+    // create_output_writer (src/kiteops/mod.rs:110-121): "-" is stdout.
+    FILE *o = stdout;
+    if (out_path != "-") {
+        make_parent_dirs(out_path);
+        o = fopen(out_path.c_str(), "wb");
+        if (!o) throw Error("failed to create " + out_path);
+    }
+    setvbuf(o, nullptr, _IOFBF, 16 << 20);
+    fprintf(o, "kmer");
+    for (auto &l : labels) fprintf(o, "\t%s", l.c_str());
+    fprintf(o, "\n");
+    // Rows are formatted in parallel slabs and written in union order, which
+    // is partition order then key order, as the reference's batches are.
+    const uint64_t slab = 1 << 18;
+    const uint64_t nslab = (N + slab - 1) / slab;
+    std::vector<std::string> buf(nslab);
+    std::vector<std::atomic<int>> ready(nslab);
+    for (auto &x : ready) x = 0;
+    std::atomic<uint64_t> next{0};
+    std::string werr;
+    std::atomic<bool> failed{false};
+    auto work = [&] {
+        for (;;) {
+            const uint64_t s = next++;
+            if (s >= nslab) return;
+            std::string &b = buf[s];
+            const uint64_t r0 = s * slab, r1 = std::min(N, r0 + slab);
+            try {
+                b.resize((r1 - r0) * (k + 1 + n * 21));
+                char *w = b.data();
+                uint32_t p = (uint32_t)(std::upper_bound(uo.begin(), uo.end(), r0) - uo.begin()) - 1;
+                for (uint64_t r = r0; r < r1; r++) {
+                    while (r >= uo[p + 1]) p++;
+                    Key key;
+                    for (int q = 0; q < W; q++) key.w[q] = uk[r * W + q];
+                    decode_key(key, W, k, w);
+                    w += k;
+                    for (size_t j = 0; j < n; j++) {
+                        const uint32_t c = um[r * n + j];
+                        if (c != 0xFFFFFFFFu) {
+                            w += sprintf(w, "\t%u", c);
+                        } else {
+                            const uint64_t e = exact_count(K[j], C[j], O[j][p], O[j][p + 1], &uk[r * W], W);
+                            w += sprintf(w, "\t%llu", (unsigned long long)e);
+                        }
+                    }
+                    *w++ = '\n';
+                }
+                b.resize((size_t)(w - b.data()));
+            } catch (const std::exception &e) {
+                if (!failed.exchange(true)) werr = e.what();
+                b.clear();
+            }
+            ready[s] = 1;
+        }
+    };
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < std::max(1u, nt); t++) th.emplace_back(work);
+    for (uint64_t s = 0; s < nslab; s++) {
+        while (!ready[s]) std::this_thread::sleep_for(std::chrono::microseconds(50));
+        if (!failed) fwrite(buf[s].data(), 1, buf[s].size(), o);
+        std::string().swap(buf[s]);
+    }
+    for (auto &x : th) x.join();
+    const bool bad = fflush(o) != 0 || ferror(o);
+    if (o != stdout) fclose(o);
+    if (failed) throw Error(werr);
+    if (bad) throw Error("failed to write " + out_path);
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (a.flag("test-mode")) {
+        std::string j;
+        for (size_t i = 0; i < labels.size(); i++) j += (i ? "," : "") + labels[i];
+        fprintf(stderr,
+                "[kite profile] test-mode: inputs=%zu labels=%s threads=%llu worker_threads=%u partitions=%u "
+                "rows_written=%llu output=%s elapsed=%.3fs\n",
+                n, j.c_str(), (unsigned long long)a.get_u64("threads", 0), nt, P, (unsigned long long)N,
+                out_path.c_str(), secs);
+    }
+    fprintf(stderr, "[kite profile] done in %.3fs (rows=%llu, output=%s)\n", secs, (unsigned long long)N,
+            out_path.c_str());
+    return 0;
+}
+
+}  // namespace kh
