@@ -263,15 +263,31 @@ def bench_dump(w, fq) -> dict:
         count_wall = time.perf_counter() - t0
         if r.returncode:
             raise RuntimeError(f"kite count failed: {r.stderr[-2000:]}")
-        fq_path.unlink()
         done = [ln for ln in r.stderr.splitlines() if "done:" in ln]
         log(f"kite count CLI: {count_wall:.2f} s wall for {fq.size / 1e9:.3f} GB FASTQ -> "
             f"{kp.stat().st_size / 1e9:.3f} GB .kite ({done[-1].split('done: ')[-1] if done else ''})")
+        # --mode compact: the same count, chunks zstd-compressed on the host
+        # where the reference's policy keeps them
+        kz = tmp / "z.kite"
+        t0 = time.perf_counter()
+        r = subprocess.run([str(kite_bin), "count", "-i", str(fq_path), "-o", str(kz), "-k", str(w.k),
+                            "--canonical", "true" if w.canonical else "false", "--partitions", str(w.partitions),
+                            "--mode", "compact"], capture_output=True, text=True, timeout=600)
+        compact_wall = time.perf_counter() - t0
+        if r.returncode:
+            raise RuntimeError(f"kite count --mode compact failed: {r.stderr[-2000:]}")
+        compact_bytes = kz.stat().st_size
+        kz.unlink()
+        log(f"kite count CLI --mode compact: {compact_wall:.2f} s wall -> {compact_bytes / 1e9:.3f} GB .kite")
+        fq_path.unlink()
         threads = min(16, os.cpu_count() or 1)
         out = {"kite_bytes": kp.stat().st_size, "host_threads": threads,
                "cli_count": {"wall_s": count_wall,
                              "includes": "process start, device init, FASTQ read from the page cache, count, "
-                                         "DVLE encode, .kite write"}}
+                                         "DVLE encode, .kite write"},
+               "cli_count_compact": {"wall_s": compact_wall, "kite_bytes": compact_bytes,
+                                     "includes": "as cli_count, plus zstd level 5-6 of the chunks on the host "
+                                                 "where the chunk codec policy keeps it"}}
         for engine in ("gpu", "host"):
             env = dict(os.environ, KITE_DUMP_ENGINE=engine)
             cmd = [str(kite_bin), "dump", "-i", str(kp), "-t", str(threads)]

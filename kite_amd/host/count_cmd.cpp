@@ -11,8 +11,12 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <condition_variable>
+#include <cstdint>
+#include <deque>
 #include <exception>
 #include <fstream>
+#include <mutex>
 #include <sstream>
 #include <thread>
 
@@ -321,14 +325,147 @@ Plan derive_plan(const std::string &input, Comp comp, int k, uint16_t kw, uint64
     }
 }
 
+// Chunk codec policy of flush_bucket (src/count/worker.rs:340-372): a DVLE
+// or RAW payload is zstd-compressed at the mode's level (fast 0, balanced 2,
+// compact 5; src/cli.rs:13-19) when it is large enough, and the compressed
+// form is kept only when it saves enough.  The thresholds restate
+// min_compression_input_bytes, min_compression_savings_bytes and
+// compression_level_for (src/count/worker.rs:451-501).
+size_t zstd_min_input(int k, bool dvle, int entry_size) {
+    if (entry_size == 12 && dvle && k <= 31) return 16 * 1024;
+    if (entry_size == 20 && dvle) return 32 * 1024;
+    if (entry_size == 36) return 64 * 1024;
+    return 32 * 1024;
+}
+
+size_t zstd_min_savings(size_t payload, size_t logical, bool dvle, int entry_size) {
+    const size_t bias = std::min<size_t>((logical > payload ? logical - payload : 0) / 32, 24 * 1024);
+    const size_t floor = entry_size == 12 && dvle ? 8 * 1024 : (entry_size == 20 && dvle ? 16 * 1024 : 32 * 1024);
+    const size_t m = std::max(floor, payload / 100);
+    return m > bias ? m - bias : 0;
+}
+
+int zstd_level_for(int base, int k, size_t payload, size_t logical, bool dvle, int entry_size) {
+    if (base <= 0) return 0;
+    int level = base;
+    if (dvle && entry_size == 12 && k <= 31 && payload >= 1024 * 1024) level++;
+    // logical >= 2 * payload, with the reference's saturating multiply
+    if ((payload > SIZE_MAX / 2 ? logical == SIZE_MAX : logical >= payload * 2) && payload >= 4 * 1024 * 1024)
+        level++;
+    return std::min(level, 6);
+}
+
+// Receives the encoded chunks from the device.  With a zstd level, chunks
+// that may compress go to a pool of host threads (the reference compresses
+// on its count workers); the Writer takes appends from one thread at a time.
 struct EmitCtx {
     Writer *w;
+    int base_level = 0, k = 0, entry_size = 12;
+    struct Job {
+        uint32_t p, codec;
+        uint64_t rows, raw, kmin[4], kmax[4];
+        std::vector<uint8_t> data;
+    };
+    std::mutex wmu, qmu;
+    std::condition_variable cv_job, cv_space;
+    std::deque<Job> q;
+    size_t q_bytes = 0, q_cap = 256u << 20;
+    bool stop = false;
+    std::string err;
+    std::vector<std::thread> pool;
+    uint64_t zstd_chunks = 0;
+
+    void append(uint32_t p, uint32_t codec, uint64_t rows, uint64_t raw, const uint64_t *kmin, const uint64_t *kmax,
+                const uint8_t *data, size_t len) {
+        std::lock_guard<std::mutex> g(wmu);
+        w->append_chunk(p, codec, rows, raw, kmin, kmax, data, len);
+    }
+    void compress_and_append(Job &j) {
+        const bool dvle = (j.codec & CODEC_DVLE) != 0;
+        const size_t payload = j.data.size(), logical = (size_t)j.rows * (size_t)entry_size;
+        const int level = zstd_level_for(base_level, k, payload, logical, dvle, entry_size);
+        if (level > 0 && payload >= zstd_min_input(k, dvle, entry_size)) {
+            std::vector<uint8_t> z = zstd_compress(j.data.data(), payload, level);
+            if (z.size() + zstd_min_savings(payload, logical, dvle, entry_size) < payload) {
+                append(j.p, j.codec | CODEC_ZSTD, j.rows, j.raw, j.kmin, j.kmax, z.data(), z.size());
+                std::lock_guard<std::mutex> g(qmu);
+                zstd_chunks++;
+                return;
+            }
+        }
+        append(j.p, j.codec, j.rows, j.raw, j.kmin, j.kmax, j.data.data(), payload);
+    }
+    void start(unsigned threads) {
+        for (unsigned t = 0; t < threads; t++)
+            pool.emplace_back([this] {
+                for (;;) {
+                    Job j;
+                    {
+                        std::unique_lock<std::mutex> g(qmu);
+                        cv_job.wait(g, [&] { return stop || !q.empty(); });
+                        if (q.empty()) return;
+                        j = std::move(q.front());
+                        q.pop_front();
+                        q_bytes -= j.data.size();
+                    }
+                    cv_space.notify_all();
+                    try {
+                        compress_and_append(j);
+                    } catch (const std::exception &e) {
+                        std::lock_guard<std::mutex> g(qmu);
+                        if (err.empty()) err = e.what();
+                    }
+                }
+            });
+    }
+    void push(Job &&j) {
+        std::unique_lock<std::mutex> g(qmu);
+        cv_space.wait(g, [&] { return q.empty() || q_bytes + j.data.size() <= q_cap; });
+        q_bytes += j.data.size();
+        q.push_back(std::move(j));
+        g.unlock();
+        cv_job.notify_one();
+    }
+    // Waits for every queued chunk; rethrows the first compression error.
+    void drain() {
+        {
+            std::lock_guard<std::mutex> g(qmu);
+            stop = true;
+        }
+        cv_job.notify_all();
+        for (auto &t : pool) t.join();
+        pool.clear();
+        if (!err.empty()) throw Error(err);
+    }
+    ~EmitCtx() {
+        if (!pool.empty()) {
+            {
+                std::lock_guard<std::mutex> g(qmu);
+                stop = true;
+            }
+            cv_job.notify_all();
+            for (auto &t : pool) t.join();
+        }
+    }
 };
 
 int emit_cb(void *user, uint32_t p, uint32_t codec, uint64_t rows, uint64_t raw, const uint64_t kmin[4],
             const uint64_t kmax[4], const uint8_t *data, size_t len) {
+    EmitCtx &ec = *static_cast<EmitCtx *>(user);
     try {
-        static_cast<EmitCtx *>(user)->w->append_chunk(p, codec, rows, raw, kmin, kmax, data, len);
+        if (ec.pool.empty() || len < zstd_min_input(ec.k, (codec & CODEC_DVLE) != 0, ec.entry_size)) {
+            ec.append(p, codec, rows, raw, kmin, kmax, data, len);
+            return 0;
+        }
+        EmitCtx::Job j;
+        j.p = p;
+        j.codec = codec;
+        j.rows = rows;
+        j.raw = raw;
+        std::copy(kmin, kmin + 4, j.kmin);
+        std::copy(kmax, kmax + 4, j.kmax);
+        j.data.assign(data, data + len);
+        ec.push(std::move(j));
         return 0;
     } catch (...) {
         return 1;
@@ -363,6 +500,12 @@ int run_count(int argc, char **argv) {
     std::optional<uint64_t> buffer;
     if (a.has("buffer")) buffer = a.get_u64("buffer", 0);
     const bool test_mode = a.flag("test-mode");
+    int zstd_level = mode == "balanced" ? 2 : (mode == "compact" ? 5 : 0);
+    if (zstd_level > 0 && !zstd_available()) {
+        fprintf(stderr, "[kite count] warning: libzstd.so.1 is not available; --mode %s writes uncompressed chunks\n",
+                mode.c_str());
+        zstd_level = 0;
+    }
 
     if (k < 1 || k > 128)
         throw Error("invalid k-mer length: --k must be between 1 and 128.\nGot: " + std::to_string(k) + "\nExample: --k 21");
@@ -391,8 +534,10 @@ int run_count(int argc, char **argv) {
     fprintf(stderr, "[kite count] k=%d, canonical=%s, partitions=%llu, min=%s, max=%s\n", k,
             canonical ? "true" : "false", (unsigned long long)pl.partitions,
             mn ? std::to_string(*mn).c_str() : "none", mx ? std::to_string(*mx).c_str() : "none");
+    // codec_policy_label (src/count/mod.rs:952-959)
     fprintf(stderr, "[kite count] mode=%s, codec=%s, engine=gpu (gfx950)\n", mode.c_str(),
-            kw == 256 ? "raw-u256" : "dvle-only");
+            kw == 256 ? (zstd_level > 0 ? "raw-u256+zstd-if-smaller" : "raw-u256")
+                      : (zstd_level > 0 ? "dvle+zstd-if-smaller" : "dvle-only"));
     if (test_mode) {
         fprintf(stderr, "[kite count][auto] input=%.2fGB, read_len=%.1fbp from %llu reads\n",
                 pl.input_bytes / (double)GIB, pl.avg_read_len, (unsigned long long)pl.sampled_reads);
@@ -478,9 +623,19 @@ int run_count(int argc, char **argv) {
         h.min_count = mn;
         h.max_count = mx;
         Writer w(output, h, pl.staging, Writer::suggest_write_lanes((int)pl.workers));
-        EmitCtx ec{&w};
-        gpu_check(kite_gpu_count_emit(ctx, emit_cb, &ec), "count");
+        EmitCtx ec;
+        ec.w = &w;
+        ec.base_level = zstd_level;
+        ec.k = k;
+        ec.entry_size = kw == 64 ? 12 : (kw == 128 ? 20 : 36);
+        if (zstd_level > 0) ec.start(std::max(1u, (unsigned)pl.workers));
+        const int rc = kite_gpu_count_emit(ctx, emit_cb, &ec);
+        ec.drain();
+        gpu_check(rc, "count");
         std::tie(total_chunks, total_rows) = w.finalize();
+        if (zstd_level > 0)
+            fprintf(stderr, "[kite count] zstd chunks=%llu (level %d)\n", (unsigned long long)ec.zstd_chunks,
+                    zstd_level);
     }
     const auto t2 = std::chrono::steady_clock::now();
     const double gpu_s = std::chrono::duration<double>(t1 - t0).count();
