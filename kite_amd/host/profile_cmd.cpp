@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <thread>
 
@@ -101,31 +102,6 @@ int run_profile(int argc, char **argv) {
     const uint32_t P = rs[0]->partitions();
     const int W = rs[0]->words(), k = rs[0]->header().k;
     const size_t n = rs.size();
-    std::vector<std::vector<uint64_t>> K(n), C(n), O(n);
-    uint64_t sum_rows = 0;
-    for (size_t i = 0; i < n; i++) {
-        load_filtered(*rs[i], 0, P, nt, K[i], C[i], O[i]);
-        sum_rows += O[i][P];
-    }
-    std::vector<const uint64_t *> kp(n), cp(n), op(n);
-    for (size_t i = 0; i < n; i++) {
-        kp[i] = K[i].data();
-        cp[i] = C[i].data();
-        op[i] = O[i].data();
-    }
-    std::vector<uint64_t> uk(std::max<uint64_t>(sum_rows, 1) * W), uo(P + 1);
-    std::vector<uint32_t> um(std::max<uint64_t>(sum_rows, 1) * n);
-    {
-        kite_gpu_ctx *ctx = nullptr;
-        if (kite_gpu_open(0, 0, &ctx))
-            throw Error(std::string("failed to open the GPU engine: ") + kite_gpu_last_error());
-        const int rc = kite_gpu_union_tables(ctx, (uint32_t)W, P, (uint32_t)n, kp.data(), cp.data(), op.data(),
-                                             uk.data(), um.data(), uo.data());
-        std::string err = rc ? kite_gpu_last_error() : "";
-        kite_gpu_close(ctx);
-        if (rc) throw Error("GPU union failed: " + err);
-    }
-    const uint64_t N = uo[P];
 
     // create_output_writer (src/kiteops/mod.rs:110-121): "-" is stdout.
     FILE *o = stdout;
@@ -134,67 +110,111 @@ int run_profile(int argc, char **argv) {
         o = fopen(out_path.c_str(), "wb");
         if (!o) throw Error("failed to create " + out_path);
     }
+    struct Closer {
+        FILE *f;
+        ~Closer() {
+            if (f && f != stdout) fclose(f);
+        }
+    } closer{o};
     setvbuf(o, nullptr, _IOFBF, 16 << 20);
     fprintf(o, "kmer");
     for (auto &l : labels) fprintf(o, "\t%s", l.c_str());
     fprintf(o, "\n");
-    // Rows are formatted in parallel slabs and written in union order, which
-    // is partition order then key order, as the reference's batches are.
-    const uint64_t slab = 1 << 18;
-    const uint64_t nslab = (N + slab - 1) / slab;
-    std::vector<std::string> buf(nslab);
-    std::vector<std::atomic<int>> ready(nslab);
-    for (auto &x : ready) x = 0;
-    std::atomic<uint64_t> next{0};
-    std::string werr;
-    std::atomic<bool> failed{false};
-    auto work = [&] {
-        for (;;) {
-            const uint64_t s = next++;
-            if (s >= nslab) return;
-            std::string &b = buf[s];
-            const uint64_t r0 = s * slab, r1 = std::min(N, r0 + slab);
-            try {
-                b.resize((r1 - r0) * (k + 1 + n * 21));
-                char *w = b.data();
-                uint32_t p = (uint32_t)(std::upper_bound(uo.begin(), uo.end(), r0) - uo.begin()) - 1;
-                for (uint64_t r = r0; r < r1; r++) {
-                    while (r >= uo[p + 1]) p++;
-                    Key key;
-                    for (int q = 0; q < W; q++) key.w[q] = uk[r * W + q];
-                    decode_key(key, W, k, w);
-                    w += k;
-                    for (size_t j = 0; j < n; j++) {
-                        const uint32_t c = um[r * n + j];
-                        if (c != 0xFFFFFFFFu) {
-                            w += sprintf(w, "\t%u", c);
-                        } else {
-                            const uint64_t e = exact_count(K[j], C[j], O[j][p], O[j][p + 1], &uk[r * W], W);
-                            w += sprintf(w, "\t%llu", (unsigned long long)e);
-                        }
-                    }
-                    *w++ = '\n';
-                }
-                b.resize((size_t)(w - b.data()));
-            } catch (const std::exception &e) {
-                if (!failed.exchange(true)) werr = e.what();
-                b.clear();
-            }
-            ready[s] = 1;
+
+    kite_gpu_ctx *ctx = nullptr;
+    if (kite_gpu_open(0, 0, &ctx)) throw Error(std::string("failed to open the GPU engine: ") + kite_gpu_last_error());
+    struct CtxCloser {
+        kite_gpu_ctx *c;
+        ~CtxCloser() { kite_gpu_close(c); }
+    } ctx_closer{ctx};
+    // Partitions go in groups of about `group_rows` stored rows over all
+    // inputs (the reference renders partitions in batches of its worker
+    // count), so host memory stays bounded for any input size.
+    uint64_t all_rows = 0;
+    for (auto &r : rs) all_rows += r->header().total_rows;
+    uint64_t group_rows = 64ull << 20;
+    if (const char *e = getenv("KITE_PROFILE_GROUP_ROWS")) group_rows = std::max(1ull, strtoull(e, nullptr, 10));
+    const uint32_t per_group = (uint32_t)std::max<uint64_t>(
+        1, std::min<uint64_t>(P, all_rows ? (group_rows * P) / all_rows : P));
+    uint64_t N = 0;
+    for (uint32_t p0 = 0; p0 < P; p0 += per_group) {
+        const uint32_t p1 = std::min(P, p0 + per_group), G = p1 - p0;
+        std::vector<std::vector<uint64_t>> K(n), C(n), O(n);
+        uint64_t sum_rows = 0;
+        for (size_t i = 0; i < n; i++) {
+            load_filtered(*rs[i], p0, p1, nt, K[i], C[i], O[i]);
+            sum_rows += O[i][G];
         }
-    };
-    std::vector<std::thread> th;
-    for (unsigned t = 0; t < std::max(1u, nt); t++) th.emplace_back(work);
-    for (uint64_t s = 0; s < nslab; s++) {
-        while (!ready[s]) std::this_thread::sleep_for(std::chrono::microseconds(50));
-        if (!failed) fwrite(buf[s].data(), 1, buf[s].size(), o);
-        std::string().swap(buf[s]);
+        std::vector<const uint64_t *> kp(n), cp(n), op(n);
+        for (size_t i = 0; i < n; i++) {
+            kp[i] = K[i].data();
+            cp[i] = C[i].data();
+            op[i] = O[i].data();
+        }
+        std::vector<uint64_t> uk(std::max<uint64_t>(sum_rows, 1) * W), uo(G + 1);
+        std::vector<uint32_t> um(std::max<uint64_t>(sum_rows, 1) * n);
+        if (kite_gpu_union_tables(ctx, (uint32_t)W, G, (uint32_t)n, kp.data(), cp.data(), op.data(), uk.data(),
+                                  um.data(), uo.data()))
+            throw Error(std::string("GPU union failed: ") + kite_gpu_last_error());
+        const uint64_t NG = uo[G];
+        N += NG;
+        // rows formatted in parallel slabs, written in union order (partition
+        // then key order, as the reference's batches are)
+        const uint64_t slab = 1 << 18;
+        const uint64_t nslab = (NG + slab - 1) / slab;
+        std::vector<std::string> buf(nslab);
+        std::vector<std::atomic<int>> ready(nslab);
+        for (auto &x : ready) x = 0;
+        std::atomic<uint64_t> next{0};
+        std::string werr;
+        std::atomic<bool> failed{false};
+        auto work = [&] {
+            for (;;) {
+                const uint64_t sl = next++;
+                if (sl >= nslab) return;
+                std::string &b = buf[sl];
+                const uint64_t r0 = sl * slab, r1 = std::min(NG, r0 + slab);
+                try {
+                    b.resize((r1 - r0) * (k + 1 + n * 21));
+                    char *w = b.data();
+                    uint32_t p = (uint32_t)(std::upper_bound(uo.begin(), uo.end(), r0) - uo.begin()) - 1;
+                    for (uint64_t r = r0; r < r1; r++) {
+                        while (r >= uo[p + 1]) p++;
+                        Key key;
+                        for (int q = 0; q < W; q++) key.w[q] = uk[r * W + q];
+                        decode_key(key, W, k, w);
+                        w += k;
+                        for (size_t j = 0; j < n; j++) {
+                            const uint32_t c = um[r * n + j];
+                            if (c != 0xFFFFFFFFu) {
+                                w += sprintf(w, "\t%u", c);
+                            } else {
+                                const uint64_t e =
+                                    exact_count(K[j], C[j], O[j][p], O[j][p + 1], &uk[r * W], W);
+                                w += sprintf(w, "\t%llu", (unsigned long long)e);
+                            }
+                        }
+                        *w++ = '\n';
+                    }
+                    b.resize((size_t)(w - b.data()));
+                } catch (const std::exception &e) {
+                    if (!failed.exchange(true)) werr = e.what();
+                    b.clear();
+                }
+                ready[sl] = 1;
+            }
+        };
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < std::max(1u, nt); t++) th.emplace_back(work);
+        for (uint64_t sl = 0; sl < nslab; sl++) {
+            while (!ready[sl]) std::this_thread::sleep_for(std::chrono::microseconds(50));
+            if (!failed) fwrite(buf[sl].data(), 1, buf[sl].size(), o);
+            std::string().swap(buf[sl]);
+        }
+        for (auto &x : th) x.join();
+        if (failed) throw Error(werr);
     }
-    for (auto &x : th) x.join();
-    const bool bad = fflush(o) != 0 || ferror(o);
-    if (o != stdout) fclose(o);
-    if (failed) throw Error(werr);
-    if (bad) throw Error("failed to write " + out_path);
+    if (fflush(o) != 0 || ferror(o)) throw Error("failed to write " + out_path);
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (a.flag("test-mode")) {
         std::string j;

@@ -5,6 +5,7 @@ header-filtered u64 counts, zero where absent), formatted the way
 render_partition_rows does (src/profile/mod.rs:140-169): "kmer" then one
 column per label, a row per union k-mer in partition then key order.
 """
+import os
 import random
 import subprocess
 from pathlib import Path
@@ -21,8 +22,8 @@ KITE = ROOT / "kite_amd" / "lib" / "kite"
 GOLD = ROOT / "tests" / "golden"
 
 
-def kite(*args, check=True):
-    r = subprocess.run([str(KITE), *map(str, args)], capture_output=True, text=True, timeout=600)
+def kite(*args, check=True, env=None):
+    r = subprocess.run([str(KITE), *map(str, args)], capture_output=True, text=True, timeout=600, env=env)
     if check and r.returncode != 0:
         raise AssertionError(f"kite {args} failed ({r.returncode}):\n{r.stderr}")
     return r
@@ -62,6 +63,12 @@ def test_profile_saturated_sums_and_filters(tmp_path):
     r = kite("profile", "-i", *paths, "-o", tmp_path / "p.tsv", "--test-mode")
     assert (tmp_path / "p.tsv").read_text() == want
     assert f"rows={len(matrix)}" in r.stderr and "labels=alpha,beta,gamma (inferred from file names)" in r.stderr
+    # the same table when the partitions go through the GPU in groups of
+    # one and of three
+    for rows in ("1", "2000"):
+        env = dict(os.environ, KITE_PROFILE_GROUP_ROWS=rows)
+        kite("profile", "-i", *paths, "-o", tmp_path / f"g{rows}.tsv", env=env)
+        assert (tmp_path / f"g{rows}.tsv").read_text() == want, f"group rows {rows}"
 
 
 @pytest.mark.parametrize("kw,k,canon", [(128, 33, True), (256, 65, False)])
