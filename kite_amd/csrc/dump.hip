@@ -208,7 +208,8 @@ __global__ __launch_bounds__(DT) void k_dump_rows(const uint8_t *__restrict__ pa
     counts[i] = c;
 }
 
-// 3. running sums of the deltas of each DVLE run (one workgroup per run)
+// 3. running sums of the deltas of each DVLE run: a sum per tile of rows,
+// a scan of those sums per run, then each tile on top of its run prefix
 template <int W>
 struct Acc {
     uint64_t w[W];
@@ -235,60 +236,134 @@ __device__ __forceinline__ Acc<W> wave_incl_scan_acc(Acc<W> v) {
     return v;
 }
 
+// One step of a run's running sum: rows [b0, b0 + DT * RPT) of the run
+// starting at row0, each thread RPT consecutive rows, on top of `carry`.
+// Returns the carry for the next step.
+constexpr int PREFIX_RPT = 8;
+constexpr uint64_t PREFIX_TILE = (uint64_t)DT * PREFIX_RPT;
+
+template <int W>
+__device__ Acc<W> prefix_step(uint64_t *__restrict__ vals, uint64_t row0, uint64_t rows, uint64_t b0, Acc<W> carry,
+                              uint64_t (*wsum)[W]) {
+    constexpr int RPT = PREFIX_RPT;
+    constexpr int NW = DT / 64;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    Acc<W> x[RPT], s;
+#pragma unroll
+    for (int w = 0; w < W; w++) s.w[w] = 0;
+#pragma unroll
+    for (int u = 0; u < RPT; u++) {
+        const uint64_t j = b0 + (uint64_t)tid * RPT + u;
+#pragma unroll
+        for (int w = 0; w < W; w++) x[u].w[w] = j < rows ? vals[(row0 + j) * W + w] : 0ull;
+        s = acc_add<W>(s, x[u]);
+    }
+    const Acc<W> inc = wave_incl_scan_acc<W>(s);
+    if (lane == 63)
+#pragma unroll
+        for (int w = 0; w < W; w++) wsum[wave][w] = inc.w[w];
+    __syncthreads();
+    Acc<W> pre = carry, tot = carry;
+    for (int q = 0; q < NW; q++) {
+        Acc<W> ws;
+#pragma unroll
+        for (int w = 0; w < W; w++) ws.w[w] = wsum[q][w];
+        if (q < wave) pre = acc_add<W>(pre, ws);
+        tot = acc_add<W>(tot, ws);
+    }
+    __syncthreads();
+    // exclusive prefix of this thread = block prefix + wave-inclusive - own
+    Acc<W> neg;
+    neg.w[0] = inc.w[0] - s.w[0];
+    if constexpr (W == 2) neg.w[1] = inc.w[1] - s.w[1] - (inc.w[0] < s.w[0] ? 1ull : 0ull);
+    Acc<W> run = acc_add<W>(pre, neg);
+#pragma unroll
+    for (int u = 0; u < RPT; u++) {
+        const uint64_t j = b0 + (uint64_t)tid * RPT + u;
+        run = acc_add<W>(run, x[u]);
+        if (j < rows)
+#pragma unroll
+            for (int w = 0; w < W; w++) vals[(row0 + j) * W + w] = run.w[w];
+    }
+    return tot;
+}
+
+// Running sums of whole runs, one workgroup per run (used on the per-tile
+// sums below, whose runs are short)
 template <int W>
 __global__ __launch_bounds__(DT) void k_dvle_prefix(const DumpRun *__restrict__ runs, const uint32_t *__restrict__ dvle,
                                                     uint64_t *__restrict__ vals) {
-    constexpr int RPT = 8;  // rows per thread per step
-    constexpr int NW = DT / 64;
-    __shared__ uint64_t wsum[NW][W];
+    __shared__ uint64_t wsum[DT / 64][W];
     const DumpRun r = runs[dvle[blockIdx.x]];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     Acc<W> carry;
 #pragma unroll
     for (int w = 0; w < W; w++) carry.w[w] = 0;
-    for (uint64_t b0 = 0; b0 < r.rows; b0 += (uint64_t)DT * RPT) {
-        Acc<W> x[RPT], s;
+    for (uint64_t b0 = 0; b0 < r.rows; b0 += PREFIX_TILE) carry = prefix_step<W>(vals, r.row0, r.rows, b0, carry, wsum);
+}
+
+// A tile of PREFIX_TILE rows (fewer at a run's end) of one DVLE run
+struct PrefixTile {
+    uint64_t row0;  // first row, in the decoded row space
+    uint64_t rows;
+};
+
+// The delta sum of every tile, twice: `incl` is then scanned per run in
+// place, and `own` keeps the tile's own sum
+template <int W>
+__global__ __launch_bounds__(DT) void k_dvle_tile_sum(const PrefixTile *__restrict__ tiles,
+                                                      const uint64_t *__restrict__ vals, uint64_t *__restrict__ incl,
+                                                      uint64_t *__restrict__ own) {
+    __shared__ uint64_t wsum[DT / 64][W];
+    const PrefixTile t = tiles[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    Acc<W> s;
 #pragma unroll
-        for (int w = 0; w < W; w++) s.w[w] = 0;
+    for (int w = 0; w < W; w++) s.w[w] = 0;
 #pragma unroll
-        for (int u = 0; u < RPT; u++) {
-            const uint64_t j = b0 + (uint64_t)tid * RPT + u;
+    for (int u = 0; u < PREFIX_RPT; u++) {
+        const uint64_t j = (uint64_t)tid * PREFIX_RPT + u;
+        Acc<W> x;
 #pragma unroll
-            for (int w = 0; w < W; w++) x[u].w[w] = j < r.rows ? vals[(r.row0 + j) * W + w] : 0ull;
-            s = acc_add<W>(s, x[u]);
-        }
-        const Acc<W> inc = wave_incl_scan_acc<W>(s);
-        if (lane == 63)
+        for (int w = 0; w < W; w++) x.w[w] = j < t.rows ? vals[(t.row0 + j) * W + w] : 0ull;
+        s = acc_add<W>(s, x);
+    }
+    const Acc<W> inc = wave_incl_scan_acc<W>(s);
+    if (lane == 63)
 #pragma unroll
-            for (int w = 0; w < W; w++) wsum[wave][w] = inc.w[w];
-        __syncthreads();
-        Acc<W> pre = carry, tot = carry;
-        for (int q = 0; q < NW; q++) {
+        for (int w = 0; w < W; w++) wsum[wave][w] = inc.w[w];
+    __syncthreads();
+    if (tid == 0) {
+        Acc<W> tot;
+#pragma unroll
+        for (int w = 0; w < W; w++) tot.w[w] = 0;
+        for (int q = 0; q < DT / 64; q++) {
             Acc<W> ws;
 #pragma unroll
             for (int w = 0; w < W; w++) ws.w[w] = wsum[q][w];
-            if (q < wave) pre = acc_add<W>(pre, ws);
             tot = acc_add<W>(tot, ws);
         }
-        __syncthreads();
-        // exclusive prefix of this thread = block prefix + wave-inclusive - own
-        Acc<W> neg;
-        {
-            // inc - s, as a wrapping subtraction
-            neg.w[0] = inc.w[0] - s.w[0];
-            if constexpr (W == 2) neg.w[1] = inc.w[1] - s.w[1] - (inc.w[0] < s.w[0] ? 1ull : 0ull);
-        }
-        Acc<W> run = acc_add<W>(pre, neg);
 #pragma unroll
-        for (int u = 0; u < RPT; u++) {
-            const uint64_t j = b0 + (uint64_t)tid * RPT + u;
-            run = acc_add<W>(run, x[u]);
-            if (j < r.rows)
-#pragma unroll
-                for (int w = 0; w < W; w++) vals[(r.row0 + j) * W + w] = run.w[w];
-        }
-        carry = tot;
+        for (int w = 0; w < W; w++) incl[(uint64_t)blockIdx.x * W + w] = own[(uint64_t)blockIdx.x * W + w] = tot.w[w];
     }
+}
+
+// Each tile's running sums on top of the sum of the run's earlier tiles
+// (incl - own: the inclusive tile prefix less the tile itself)
+template <int W>
+__global__ __launch_bounds__(DT) void k_dvle_tile_apply(const PrefixTile *__restrict__ tiles,
+                                                        const uint64_t *__restrict__ incl,
+                                                        const uint64_t *__restrict__ own, uint64_t *__restrict__ vals) {
+    __shared__ uint64_t wsum[DT / 64][W];
+    const PrefixTile t = tiles[blockIdx.x];
+    Acc<W> a, b, carry;
+#pragma unroll
+    for (int w = 0; w < W; w++) {
+        a.w[w] = incl[(uint64_t)blockIdx.x * W + w];
+        b.w[w] = own[(uint64_t)blockIdx.x * W + w];
+    }
+    carry.w[0] = a.w[0] - b.w[0];
+    if constexpr (W == 2) carry.w[1] = a.w[1] - b.w[1] - (a.w[0] < b.w[0] ? 1ull : 0ull);
+    (void)prefix_step<W>(vals, t.row0, t.rows, 0, carry, wsum);
 }
 
 // 4. rows of each run into its merge table
@@ -498,6 +573,22 @@ uint64_t DumpEngine::decode_merge_w(const uint8_t *d_payload, const std::vector<
     }
     const uint32_t nr = (uint32_t)runs.size();
     const uint64_t T = tile_run.size();
+    // prefix tiles of the DVLE runs; truns[d] spans run d's tiles in the
+    // tile list, so a run-wise scan over the tile sums reuses k_dvle_prefix
+    std::vector<PrefixTile> ptiles;
+    std::vector<DumpRun> truns;
+    std::vector<uint32_t> tids;
+    for (uint32_t d : dvle) {
+        const DumpRun &r = runs[d];
+        DumpRun tr{};
+        tr.row0 = ptiles.size();
+        for (uint64_t b0 = 0; b0 < r.rows; b0 += PREFIX_TILE)
+            ptiles.push_back(PrefixTile{r.row0 + b0, std::min<uint64_t>(PREFIX_TILE, r.rows - b0)});
+        tr.rows = ptiles.size() - tr.row0;
+        tids.push_back((uint32_t)truns.size());
+        truns.push_back(tr);
+    }
+    if (ptiles.size() > 0x7FFFFFFFull) throw Error("dump group above 2^31 prefix tiles");
     DumpRun *druns = static_cast<DumpRun *>(runs_d_.ensure(std::max<size_t>(nr, 1) * sizeof(DumpRun)));
     uint32_t *dtile = static_cast<uint32_t *>(tile_run_.ensure(std::max<uint64_t>(T, 1) * 4 + dvle.size() * 4 + 4));
     uint32_t *ddvle = dtile + std::max<uint64_t>(T, 1);
@@ -526,8 +617,21 @@ uint64_t DumpEngine::decode_merge_w(const uint8_t *d_payload, const std::vector<
         k_dump_rows<W><<<grid_for(n), DT, 0, s>>>(d_payload, druns, nr, n, ends, trunc32 ? 1u : 0u, vals, cnts,
                                                   derr);
         KITE_CHECK_LAUNCH();
-        if (!dvle.empty()) {
-            k_dvle_prefix<W><<<dim3((unsigned)dvle.size()), DT, 0, s>>>(druns, ddvle, vals);
+        if (!ptiles.empty()) {
+            const uint64_t NPT = ptiles.size();
+            PrefixTile *dpt = static_cast<PrefixTile *>(ptile_.ensure(NPT * sizeof(PrefixTile)));
+            DumpRun *dtr = static_cast<DumpRun *>(ptrun_.ensure(truns.size() * sizeof(DumpRun) + truns.size() * 4));
+            uint32_t *dtid = reinterpret_cast<uint32_t *>(dtr + truns.size());
+            uint64_t *incl = static_cast<uint64_t *>(psum_.ensure(NPT * 2 * 8 * W));
+            uint64_t *own = incl + NPT * W;
+            KITE_HIP(hipMemcpyAsync(dpt, ptiles.data(), NPT * sizeof(PrefixTile), hipMemcpyHostToDevice, s));
+            KITE_HIP(hipMemcpyAsync(dtr, truns.data(), truns.size() * sizeof(DumpRun), hipMemcpyHostToDevice, s));
+            KITE_HIP(hipMemcpyAsync(dtid, tids.data(), tids.size() * 4, hipMemcpyHostToDevice, s));
+            k_dvle_tile_sum<W><<<dim3((unsigned)NPT), DT, 0, s>>>(dpt, vals, incl, own);
+            KITE_CHECK_LAUNCH();
+            k_dvle_prefix<W><<<dim3((unsigned)truns.size()), DT, 0, s>>>(dtr, dtid, incl);
+            KITE_CHECK_LAUNCH();
+            k_dvle_tile_apply<W><<<dim3((unsigned)NPT), DT, 0, s>>>(dpt, incl, own, vals);
             KITE_CHECK_LAUNCH();
         }
     }

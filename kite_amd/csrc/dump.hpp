@@ -78,7 +78,7 @@ class DumpEngine {
     hipStream_t stream_;
     MergeEngine *merge_;
     DevBuf runs_d_, tile_run_, tcnt_, tbase_, ends_, vals_, counts_, scan_tmp_, tlen_, toff_, text_, err_,
-        tabptr_, lines_, loff_, hsmall_, hbig_d_;
+        tabptr_, lines_, loff_, hsmall_, hbig_d_, ptile_, ptrun_, psum_;
     std::map<uint64_t, uint64_t> hbig_;  // frequencies of totals >= HISTO_SMALL
     std::vector<DevTable> tabs_;
     DevTable merged_;
