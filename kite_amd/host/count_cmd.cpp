@@ -545,6 +545,9 @@ int run_count(int argc, char **argv) {
                 (unsigned long long)(pl.chunk / MIB), (unsigned long long)(pl.staging / MIB));
         for (auto &r : pl.reasons) fprintf(stderr, "[kite count][auto] %s\n", r.c_str());
     }
+    // KITE_COUNT_PLAN_ONLY=1 stops after the plan (no device is opened, no
+    // output written): the planner can be checked on a host without a GPU
+    if (const char *po = getenv("KITE_COUNT_PLAN_ONLY"); po && std::string(po) == "1") return 0;
 
     kite_gpu_ctx *ctx = nullptr;
     gpu_check(kite_gpu_open(0, 0, &ctx), "failed to open the GPU engine");
