@@ -103,3 +103,32 @@ def dump(text: bytes, k: int, canonical: bool, P: int, min_count=None, max_count
                 continue
             out.append(f"{decode(v, k)}\t{n}\n")
     return "".join(out).encode()
+
+
+def exact_windows(fq, k: int, reads_per_block: int = 100_000) -> int:
+    """Valid k-mer windows of a FASTQ image (numpy uint8 array of whole
+    newline-terminated 4-line records): sequence lines (line 2 of every
+    record) split at every byte that is not A/C/G/T in either case; a run
+    of L valid bases holds max(0, L - k + 1) windows.  Vectorised over
+    reads of equal length, for workload-sized inputs."""
+    import numpy as np
+    nl = np.flatnonzero(fq == 0x0A)
+    assert nl.size % 4 == 0
+    starts = nl[0::4] + 1
+    lens = nl[1::4] - starts
+    valid = np.zeros(256, dtype=np.int32)
+    for b in b"ACGTacgt":
+        valid[b] = 1
+    total = 0
+    for L in np.unique(lens):
+        L = int(L)
+        if L < k:
+            continue
+        sel = np.flatnonzero(lens == L)
+        for a in range(0, sel.size, reads_per_block):
+            s = starts[sel[a:a + reads_per_block]]
+            bad = 1 - valid[fq[s[:, None] + np.arange(L)[None, :]]]
+            cs = np.zeros((s.size, L + 1), dtype=np.int32)
+            np.cumsum(bad, axis=1, out=cs[:, 1:])
+            total += int(np.count_nonzero(cs[:, k:] - cs[:, :L - k + 1] == 0))
+    return total

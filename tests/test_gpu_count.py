@@ -170,11 +170,8 @@ def test_full_size_properties(gpu_ctx):
         a, b = int(res.part_off[p]), int(res.part_off[p + 1])
         assert np.all(part[a:b] == p)
         assert np.all(np.diff(k[a:b].astype(np.int64)) > 0) if b - a > 1 else True
-    # expected windows: every read is 150 bp; windows with an N are dropped
-    exp = 0
-    for s in ol.parse_sequences(fq[:2_000_000].tobytes())[:1000]:
-        exp += len(brute.kmers(s.decode(), 31, True))
-    assert exp > 0
+    # every valid window of the FASTQ bytes is one input k-mer
+    assert st.kmers == brute.exact_windows(fq, w.k)
 
 
 @pytest.mark.parametrize("dedup", ["0", "1"])

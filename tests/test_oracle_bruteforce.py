@@ -118,3 +118,26 @@ def test_writer_reader_roundtrip_multirun(tmp_path):
     assert t.part_off.tolist() == [0, 1, 3, 3, 5]
     assert t.keys[:, 0].tolist() == [4, 3, 9, 10, 20]
     assert t.counts.tolist() == [5, 5, 2, 1, 1]
+
+
+def test_exact_windows_counts():
+    """brute.exact_windows (the vectorised window count used at workload
+    size) on hand-checked records and against brute.kmers on random
+    records of several lengths with N, lower case and other bytes."""
+    fq = np.frombuffer(b"@a\nACGTA\n+\nIIIII\n@b\nACNGTACG\n+\nIIIIIIII\n@c\nac\n+\nII\n", dtype=np.uint8)
+    # k=2: a has 4; b has AC (1) + GTACG (4); c has 1 -> 10
+    assert brute.exact_windows(fq, 2) == 10
+    # k=5: a has 1; b: GTACG has 1; c none -> 2
+    assert brute.exact_windows(fq, 5) == 2
+    assert brute.exact_windows(fq, 6) == 0
+    rng = random.Random(17)
+    recs = []
+    for i in range(600):
+        L = rng.choice([0, 3, 31, 150, 151])
+        s = "".join(rng.choice("ACGTACGTACGTacgtNRx") for _ in range(L))
+        recs.append(f"@w{i}\n{s}\n+\n{'I' * L}\n")
+    text = "".join(recs)
+    arr = np.frombuffer(text.encode(), dtype=np.uint8)
+    for k in (1, 5, 21, 31):
+        exp = sum(len(brute.kmers(s, k, False)) for s in brute.fastq_sequences(text.encode()))
+        assert brute.exact_windows(arr, k, reads_per_block=37) == exp, k
