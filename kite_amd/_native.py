@@ -5,8 +5,11 @@ when no HIP device is present.  There is no CPU fallback.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import sys
+import weakref
 from pathlib import Path
 
 import numpy as np
@@ -15,6 +18,17 @@ LIB_DIR = Path(__file__).resolve().parent / "lib"
 
 _gpu = None
 _synth = None
+# live contexts and pinned arrays: released by an exit hook before the
+# interpreter (and, after it, the HIP runtime) shuts down
+_live: "weakref.WeakSet" = weakref.WeakSet()
+
+
+def _release_all() -> None:
+    for obj in list(_live):
+        try:
+            obj._release()
+        except Exception:  # pragma: no cover - best effort at exit
+            pass
 
 
 class KiteError(RuntimeError):
@@ -157,6 +171,9 @@ def load_gpu() -> C.CDLL:
             fn.restype = res
             fn.argtypes = args
         _gpu = lib
+        # registered after torch's own hooks when torch is already imported,
+        # so it runs before them (atexit runs hooks last-in first-out)
+        atexit.register(_release_all)
     return _gpu
 
 
@@ -184,6 +201,7 @@ class PinnedArray:
         _check(lib.kite_gpu_host_alloc(nbytes, C.byref(p)))
         self._lib, self.ptr, self.nbytes = lib, p.value, nbytes
         self.array = np.ctypeslib.as_array((C.c_uint8 * max(nbytes, 1)).from_address(p.value))[:nbytes]
+        _live.add(self)
 
     def free(self) -> None:
         if self.ptr:
@@ -191,7 +209,13 @@ class PinnedArray:
             self._lib.kite_gpu_host_free(C.c_void_p(self.ptr))
             self.ptr = None
 
+    _release = free
+
     def __del__(self):  # pragma: no cover
+        # no HIP call while the interpreter finalizes: the exit hook has
+        # already released what was still live
+        if sys.is_finalizing():
+            return
         try:
             self.free()
         except Exception:
@@ -227,11 +251,14 @@ class GpuContext:
         _check(lib.kite_gpu_open(device, hbm_budget_bytes, C.byref(h)))
         self._h = h
         self._lib = lib
+        _live.add(self)
 
     def close(self) -> None:
         if self._h:
             self._lib.kite_gpu_close(self._h)
             self._h = None
+
+    _release = close
 
     def __enter__(self):
         return self
@@ -240,6 +267,8 @@ class GpuContext:
         self.close()
 
     def __del__(self):  # pragma: no cover
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

@@ -1500,6 +1500,12 @@ uint64_t *CountEngine::pinned_host(size_t bytes) {
 }
 
 CountEngine::~CountEngine() {
+    (void)hipStreamSynchronize(stream_);
+    for (auto &p : pending_) {
+        ev_pool_.push_back(p.a);
+        ev_pool_.push_back(p.b);
+    }
+    for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
     if (pin_) (void)hipHostFree(pin_);
 }
 
@@ -1557,6 +1563,10 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
     part_off_h_.assign(P + 1, 0);
     n_kmers_ = n_unique_ = n_buckets_ = n_fallback_ = 0;
     kstats_.clear();
+    for (auto &p : pending_) {  // events of a run whose stats were not collected
+        ev_pool_.push_back(p.a);
+        ev_pool_.push_back(p.b);
+    }
     pending_.clear();
     levels_ = 0;
 

@@ -513,7 +513,7 @@ inline dim3 grid_for(uint64_t n) { return dim3((unsigned)std::max<uint64_t>(1, c
 
 void DumpEngine::release() {
     for (DevBuf *b : {&runs_d_, &tile_run_, &tcnt_, &tbase_, &ends_, &vals_, &counts_, &scan_tmp_, &tlen_, &toff_,
-                      &text_, &err_, &tabptr_, &lines_, &loff_, &hsmall_, &hbig_d_})
+                      &text_, &err_, &tabptr_, &lines_, &loff_, &hsmall_, &hbig_d_, &ptile_, &ptrun_, &psum_})
         b->release();
     tabs_.clear();
     merged_.keys.release();

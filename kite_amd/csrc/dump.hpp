@@ -42,6 +42,7 @@ struct DumpTimes {
 class DumpEngine {
    public:
     DumpEngine(hipStream_t s, MergeEngine *merge) : stream_(s), merge_(merge) {}
+    ~DumpEngine() { release(); }  // frees the page-locked error word too
     DumpEngine(const DumpEngine &) = delete;
     DumpEngine &operator=(const DumpEngine &) = delete;
 
