@@ -259,6 +259,17 @@ int kite_gpu_analyze_tables(kite_gpu_ctx *ctx, uint32_t words, uint32_t partitio
 int kite_gpu_pca(kite_gpu_ctx *ctx, const double *cov, uint32_t n, uint32_t dims, double *scores,
                  double *eigenvalues);
 
+/* kite_gpu_pca without any allocation: no counterpart in the reference.  It
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * Jacobi works in a fixed per-GPU device array of 2 * 64 * 64 + 64 FP64
+ * (66,048 bytes, held until the process ends), so n is limited to 1..64
+ * (larger n: KITE_E_INVALID).  Same arguments and results as kite_gpu_pca.
+ * Calls on one GPU share that array: within a process they take turns (a
+ * per-GPU lock), so overlapping calls wait rather than race. */
+int kite_gpu_pca_noalloc(kite_gpu_ctx *ctx, const double *cov, uint32_t n, uint32_t dims, double *scores,
+                         double *eigenvalues);
+
 /* Turns per-kernel profiling on or off for later batches: HIP events around
  * every launch on the engine's stream, plus each kernel's algorithmic HBM
  * bytes.  Counterpart of the reference's --test-mode stage timers

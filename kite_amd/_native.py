@@ -136,6 +136,8 @@ GPU_SYMBOLS = {
                                           C.c_void_p, C.c_void_p, C.POINTER(CorrStats)]),
     "kite_gpu_pca": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
                                C.c_void_p]),
+    "kite_gpu_pca_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
+                                       C.c_void_p]),
     "kite_gpu_count_batch": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_uint32,
                                        C.POINTER(BatchSample), C.c_uint32, C.c_int, BATCH_FN,
                                        C.c_void_p]),
@@ -425,6 +427,16 @@ class GpuContext:
         ev = np.zeros(n)
         _check(self._lib.kite_gpu_pca(self._h, c.ctypes.data, n, dims, scores.ctypes.data,
                                       ev.ctypes.data))
+        return scores, ev
+
+    def pca_noalloc(self, cov: np.ndarray, dims: int) -> tuple[np.ndarray, np.ndarray]:
+        """kite_gpu_pca_noalloc: the same PCA in fixed device memory (n <= 64)."""
+        c = np.ascontiguousarray(cov, dtype=np.float64)
+        n = c.shape[0]
+        scores = np.zeros((n, dims))
+        ev = np.zeros(n)
+        _check(self._lib.kite_gpu_pca_noalloc(self._h, c.ctypes.data, n, dims, scores.ctypes.data,
+                                              ev.ctypes.data))
         return scores, ev
 
     # -- batched count of many samples (kite_gpu_count_batch) ------------

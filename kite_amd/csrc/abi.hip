@@ -89,6 +89,37 @@ struct BatchSlot {
 
 }  // namespace
 
+// Working memory of kite_gpu_pca_noalloc, the entry point for callers that
+// cannot allocate device memory: the matrix being rotated and the
+// eigenvector accumulator (PCA_SCRATCH_N x PCA_SCRATCH_N FP64 each), then
+// the eigenvalues.  Each GPU holds its own copy for the life of the process.
+constexpr uint32_t PCA_SCRATCH_N = 64;
+constexpr size_t PCA_SCRATCH_DOUBLES = 2ull * PCA_SCRATCH_N * PCA_SCRATCH_N + PCA_SCRATCH_N;
+static __device__ __align__(256) double scratch[PCA_SCRATCH_DOUBLES];
+
+namespace {
+constexpr int SCRATCH_MAX_DEVICES = 256;
+std::mutex g_scratch_init_mu;
+// device address of `scratch` on each GPU (the host-side name is only a
+// stand-in, never a device address)
+double *g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+// calls on one GPU share its array: they take that GPU's lock in turn
+std::mutex g_scratch_use_mu[SCRATCH_MAX_DEVICES];
+
+// Called with `device` current, once per device (later calls return at once).
+void init_scratch(int device) {
+    if (device < 0 || device >= SCRATCH_MAX_DEVICES)
+        throw kite::Error("invalid device ordinal for the PCA working memory");
+    std::lock_guard<std::mutex> g(g_scratch_init_mu);
+    if (g_scratch_ptr[device]) return;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        throw kite::Error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<double *>(p);
+}
+}  // namespace
+
 struct kite_gpu_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -349,6 +380,7 @@ int kite_gpu_open(int device, uint64_t hbm_budget_bytes, kite_gpu_ctx **out) {
         auto c = std::make_unique<kite_gpu_ctx>();
         c->device = device;
         KITE_HIP(hipSetDevice(device));
+        init_scratch(device);
         KITE_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         size_t free_b = 0, total_b = 0;
         KITE_HIP(hipMemGetInfo(&free_b, &total_b));
@@ -786,6 +818,49 @@ int kite_gpu_pca(kite_gpu_ctx *ctx, const double *cov, uint32_t n, uint32_t dims
                 const bool have = d < n;
                 const double sc = have ? std::sqrt(std::max(0.0, ev[order[d]])) : 0.0;
                 for (uint32_t i = 0; i < n; i++) scores[i * dims + d] = have ? V[i * n + order[d]] * sc : 0.0;
+            }
+        return KITE_OK;
+    });
+}
+
+int kite_gpu_pca_noalloc(kite_gpu_ctx *ctx, const double *cov, uint32_t n, uint32_t dims, double *scores,
+                         double *eigenvalues) {
+    if (!ctx || !cov) return set_error(KITE_E_INVALID, "invalid argument: NULL pointer");
+    if (n < 1 || n > PCA_SCRATCH_N)
+        return set_error(KITE_E_INVALID, "invalid argument: kite_gpu_pca_noalloc supports 1.." +
+                                             std::to_string(PCA_SCRATCH_N) + " samples (its fixed working memory)");
+    return guarded([&]() -> int {
+        bind(ctx);
+        double *A = g_scratch_ptr[ctx->device];
+        if (!A) return set_error(KITE_E_DEVICE, "PCA working memory not set up on this device");
+        double *V = A + (size_t)PCA_SCRATCH_N * PCA_SCRATCH_N, *E = V + (size_t)PCA_SCRATCH_N * PCA_SCRATCH_N;
+        // host copies on the stack: this path allocates nothing
+        double ev[PCA_SCRATCH_N], vec[PCA_SCRATCH_N * PCA_SCRATCH_N];
+        {
+            std::lock_guard<std::mutex> g(g_scratch_use_mu[ctx->device]);
+            KITE_HIP(hipMemcpyAsync(A, cov, (size_t)n * n * 8, hipMemcpyHostToDevice, ctx->stream));
+            kite::jacobi_device(A, V, E, (int)n, ctx->stream);
+            KITE_HIP(hipMemcpyAsync(ev, E, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream));
+            KITE_HIP(hipMemcpyAsync(vec, V, (size_t)n * n * 8, hipMemcpyDeviceToHost, ctx->stream));
+            KITE_HIP(hipStreamSynchronize(ctx->stream));
+        }
+        // eigenvalues descending, equal ones in index order (a stable sort)
+        int order[PCA_SCRATCH_N];
+        for (uint32_t i = 0; i < n; i++) {
+            int x = (int)i, j = (int)i;
+            while (j > 0 && ev[order[j - 1]] < ev[x]) {
+                order[j] = order[j - 1];
+                j--;
+            }
+            order[j] = x;
+        }
+        if (eigenvalues)
+            for (uint32_t d = 0; d < n; d++) eigenvalues[d] = ev[order[d]];
+        if (scores)
+            for (uint32_t d = 0; d < dims; d++) {
+                const bool have = d < n;
+                const double sc = have ? std::sqrt(std::max(0.0, ev[order[d]])) : 0.0;
+                for (uint32_t i = 0; i < n; i++) scores[i * dims + d] = have ? vec[i * n + order[d]] * sc : 0.0;
             }
         return KITE_OK;
     });

@@ -731,6 +731,12 @@ void CorrEngine::run(const void *d_codes, int code_bytes, uint64_t N, int n, uin
     }
 }
 
+void jacobi_device(double *A, double *V, double *evals, int n, hipStream_t s) {
+    if (n < 1 || n > 64) throw Error("invalid argument: PCA supports 1..64 samples");
+    k_jacobi<<<1, 64, 0, s>>>(A, V, n, JACOBI_SWEEPS, evals);
+    KITE_CHECK_LAUNCH();
+}
+
 void CorrEngine::pca(const std::vector<double> &cov, int n, std::vector<double> &evals, std::vector<double> &evecs) {
     if (n < 1 || n > 64) throw Error("invalid argument: PCA supports 1..64 samples");
     DevBuf a, v, e;
@@ -738,8 +744,7 @@ void CorrEngine::pca(const std::vector<double> &cov, int n, std::vector<double> 
     v.ensure((size_t)n * n * 8);
     e.ensure((size_t)n * 8);
     KITE_HIP(hipMemcpyAsync(a.ptr, cov.data(), (size_t)n * n * 8, hipMemcpyHostToDevice, stream_));
-    k_jacobi<<<1, 64, 0, stream_>>>(a.as<double>(), v.as<double>(), n, 60, e.as<double>());
-    KITE_CHECK_LAUNCH();
+    jacobi_device(a.as<double>(), v.as<double>(), e.as<double>(), n, stream_);
     evals.resize(n);
     evecs.resize((size_t)n * n);
     KITE_HIP(hipMemcpyAsync(evals.data(), e.ptr, (size_t)n * 8, hipMemcpyDeviceToHost, stream_));

@@ -25,6 +25,14 @@ struct CorrOutput {
     int fp64_passes = 0;                               // passes of the FP64 Gram (one per table)
 };
 
+// Cyclic Jacobi sweeps of the PCA eigendecomposition.
+constexpr int JACOBI_SWEEPS = 60;
+
+// Eigendecomposition of the symmetric n x n FP64 matrix A (device, row
+// major, n <= 64, overwritten) on one wave: eigenvectors into the columns of
+// V (n x n), eigenvalues (unsorted) into evals.  Queued on stream s.
+void jacobi_device(double *A, double *V, double *evals, int n, hipStream_t s);
+
 class CorrEngine {
    public:
     explicit CorrEngine(hipStream_t s) : stream_(s) {}
