@@ -1,23 +1,21 @@
 #!/usr/bin/env python3
 """Throughput of the gfx950 exact k-mer count on one MI355X.
 
-One step = one pass of the count pipeline over the whole workload: FASTQ
-bytes resident in HBM -> per-partition sorted, fully reduced (k-mer, count)
-runs resident in HBM.  The default workload is the single-sample k=31
-canonical count (50 Mbp genome with 10% diverged repeats, 5,000,000 reads
-of 150 bp, 1% substitution errors, 0.01% N, 256 partitions).
+One step = the 24-sample cohort (k=31 canonical, 256 partitions; a 20 Mbp
+base genome plus 2% SNVs per sample, 1% read errors, log-uniform 2e5..8e6
+reads of 150 bp per sample) counted from pinned host memory through the
+pipelined batch: H2D of the FASTQ, count, DVLE chunk encode on the device,
+D2H of the chunk payloads a .kite file holds.  `value` = input k-mers of
+all 24 samples per step / time per step.  Beside it: the plain per-sample
+path on the same cohort ("batched vs plain"), the dominant kernel's
+roofline (HIP events over the timed steps) and the PCIe rate against the
+pinned-copy peak measured in the same run.
 
-Two more measurements ride on the same line:
-
-* ``batch24``: 24 independent samples (20 Mbp base genome + 2% SNVs per
-  sample, log-uniform 2e5..8e6 reads) counted from pinned host memory, once
-  one at a time ("plain": per-sample device allocation, H2D, kernels, D2H,
-  synchronise) and once through the pipelined batch (persistent buffers,
-  copies overlapped with kernels).  Rate = total input k-mers / wall time,
-  both including the H2D of the FASTQ and the D2H of the reduced tables.
-* ``analyze``: Pearson + Spearman of the union count matrix of those 24
-  tables (--min 2), split-bf16 MFMA Gram against the FP64 MFMA Gram, and
-  the FP64 oracle on a row sample.
+More measurements ride on the same line: Pearson + Spearman of the
+cohort's union at --min 2 (split-bf16 MFMA Gram against the FP64 Gram and
+the FP64 oracle), the single-sample k=31 count with its FASTQ resident in
+HBM (the count kernels' own rate), `kite dump`, BGZF inflate, and the CPU
+baseline (the oracle's count on a bounded sample of the cohort).
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -44,35 +42,45 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(w, n_reads: int, threads: int) -> dict:
-    """Time the oracle (the C restatement of the reference `kite count`) on
-    the first n_reads reads of the same workload, on this host's cores."""
+def cpu_baseline(arrs, names, threads: int, reps: int = 3) -> dict:
+    """Time the oracle -- a C restatement of the reference `kite count`
+    (parse, k-mers, partition, sort, run-length counts; output /dev/null) --
+    on a bounded sample of the cohort: whole samples, each its own count run
+    as the reference would do them, summed; median of `reps` passes."""
     exe = ROOT / "oracle" / "build" / "kite_oracle"
     if not exe.exists():
-        return {"value": None, "unit": "k-mers/s", "cores": 0, "kind": "port",
-                "sample": "oracle binary missing"}
-    fq = w.fastq(n_reads=n_reads)
+        return {"value": None, "unit": "k-mers/s", "cores": 0, "kind": "port", "sample": "oracle binary missing"}
     tmpdir = "/dev/shm" if os.path.isdir("/dev/shm") else tempfile.gettempdir()
-    path = Path(tempfile.mkstemp(dir=tmpdir, suffix=".fq")[1])
+    paths = []
     try:
-        fq.tofile(path)
-        del fq
-        # chunk size as the reference planner picks it for a large budget: 64 MiB
-        cmd = [str(exe), "count", str(path), "/dev/null", str(w.k), str(int(w.canonical)),
-               str(w.partitions), str(64 << 20), str(threads)]
-        best = None
-        for _ in range(3):  # median of 3 in the reference plan; keep the bounded sample cheap
-            r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
-            if r.returncode != 0:
-                raise RuntimeError(r.stderr)
-            res = json.loads(r.stdout.strip().splitlines()[-1])
-            best = res if best is None or res["count_s"] < best["count_s"] else best
-        return {"value": best["kmers"] / best["count_s"], "unit": "k-mers/s", "cores": threads,
-                "kind": "port",
-                "sample": f"first {n_reads} reads of the workload ({best['kmers']} k-mers), "
-                          f"best of 3, {threads} threads, output /dev/null"}
+        for a in arrs:
+            p = Path(tempfile.mkstemp(dir=tmpdir, suffix=".fq")[1])
+            a.tofile(p)
+            paths.append(p)
+        rates, kmers = [], 0
+        for _ in range(reps):
+            secs, kmers = 0.0, 0
+            for p in paths:
+                # chunk size as the reference planner picks it for a large budget: 64 MiB
+                cmd = [str(exe), "count", str(p), "/dev/null", "31", "1", "256", str(64 << 20), str(threads)]
+                r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+                if r.returncode != 0:
+                    raise RuntimeError(r.stderr)
+                res = json.loads(r.stdout.strip().splitlines()[-1])
+                secs += res["count_s"]
+                kmers += res["kmers"]
+            rates.append(kmers / secs)
+        return {"value": float(np.median(rates)), "unit": "k-mers/s", "cores": threads, "kind": "port",
+                "host_nproc": os.cpu_count(),
+                "sample": f"cohort samples {', '.join(names)} ({kmers} k-mers), each its own count run, "
+                          f"median of {reps} passes, {threads} threads, output /dev/null",
+                "note": "simplified restatement of the reference count (per-thread partition vectors, then "
+                        "sort + run-length counts per partition); it has neither the reference's "
+                        "write-combined scatter nor its MSD/LSD radix sorter",
+                "rates": rates}
     finally:
-        path.unlink(missing_ok=True)
+        for p in paths:
+            p.unlink(missing_ok=True)
 
 
 def traffic_from_profile(kernel: str, workload: str):
@@ -88,11 +96,12 @@ def traffic_from_profile(kernel: str, workload: str):
     return None, None
 
 
-def bench_batch24(ctx, scale: float, reps: int) -> tuple[dict, list]:
-    """Plain vs pipelined count of the 24-sample cohort from pinned memory."""
+def make_batch24(scale: float):
+    """The 24-sample cohort's FASTQ in pinned host memory (one PinnedArray
+    per sample) and its workloads."""
     import ctypes as C
 
-    from kite_amd import BATCH_PIPELINED, BATCH_PLAIN, PinnedArray, synth_reads
+    from kite_amd import PinnedArray, synth_reads
     from kite_amd._native import load_synth
     from kite_amd.workloads import batch24
 
@@ -107,14 +116,34 @@ def bench_batch24(ctx, scale: float, reps: int) -> tuple[dict, list]:
         pa = PinnedArray(need)
         synth_reads(spec, g, 0, w.n_reads, out=pa.array)
         pins.append(pa)
-    arrs = [p.array for p in pins]
-    total_bytes = sum(a.size for a in arrs)
-    log(f"batch24: {len(arrs)} samples, {sum(w.n_reads for w in ws)} reads, {total_bytes / 1e9:.2f} GB "
+    total = sum(p.nbytes for p in pins)
+    log(f"batch24: {len(pins)} samples, {sum(w.n_reads for w in ws)} reads, {total / 1e9:.2f} GB "
         f"pinned FASTQ (generated in {time.time() - t0:.1f}s)")
-    k, canon, P = ws[0].k, ws[0].canonical, ws[0].partitions
+    return ws, pins
 
-    # untimed: size the persistent buffers, keep the tables, and check that
-    # both modes give identical tables (checksums per sample)
+
+def pcie_peak(ctx) -> dict:
+    """Page-locked host <-> device copy rates on this box through the
+    engine's own runtime (kite_gpu_copy_peak): H2D alone, D2H alone, both
+    at once on two streams; 1 GiB copies, best of 3."""
+    out = ctx.copy_peak(1 << 30)
+    log(f"pcie: H2D {out['h2d_GBps']:.1f} GB/s, D2H {out['d2h_GBps']:.1f} GB/s, "
+        f"both at once {out['both_GBps']:.1f} GB/s (page-locked, 1 GiB copies)")
+    return out
+
+
+def bench_batch24(ctx, pins, steps: int, warmup: int, plain_reps: int) -> tuple[dict, list, dict, int]:
+    """The headline: the 24-sample cohort from pinned host memory through
+    the pipelined batch (H2D of the FASTQ, count, DVLE chunk encode on the
+    device, D2H of the chunks), timed per pass over all 24 samples, against
+    the plain per-sample path (allocation, H2D, count, encode, D2H, sync)."""
+    from kite_amd import BATCH_PIPELINED, BATCH_PLAIN, BATCH_SERIAL
+
+    arrs = [p.array for p in pins]
+    k, canon, P = 31, True, 256
+
+    # untimed: size the ring buffers, keep the raw tables for analyze, and
+    # check that both modes give identical tables and identical chunk bytes
     def digest(i, st, keys, counts, off):
         return (st["kmers"], st["unique"], int(np.bitwise_xor.reduce(keys.ravel())) if keys.size else 0,
                 int(counts.astype(np.uint64).sum()), int(off[-1]))
@@ -129,51 +158,117 @@ def bench_batch24(ctx, scale: float, reps: int) -> tuple[dict, list]:
     d_plain = ctx.count_batch(arrs, k, canon, P, mode=BATCH_PLAIN, on_result=digest)
     assert d_pipe == d_plain, "plain and pipelined batch tables differ"
     kmers = sum(d[0] for d in d_pipe)
+    unique = sum(d[1] for d in d_pipe)
 
-    def nothing(i, st, keys, counts, off):
-        return 0
-
-    # encoded mode: chunk payloads built on the device; the same bytes from both modes
     def cdigest(i, st, codec, data, boff, rows, mm):
         n8 = data.size // 8 * 8
         words = data[:n8].view(np.uint64)
         return (int(boff[-1]), int(np.bitwise_xor.reduce(words)) if words.size else 0,
-                int(words.sum(dtype=np.uint64)) if words.size else 0, data[n8:].tobytes())
+                int(words.sum(dtype=np.uint64)) if words.size else 0, data[n8:].tobytes(),
+                int(np.bitwise_xor.reduce(mm.ravel())))
 
     e_pipe = ctx.count_batch(arrs, k, canon, P, mode=BATCH_PIPELINED, on_result=cdigest, encoded=True)
     e_plain = ctx.count_batch(arrs, k, canon, P, mode=BATCH_PLAIN, on_result=cdigest, encoded=True)
     assert e_pipe == e_plain, "plain and pipelined encoded chunks differ"
     enc_bytes = sum(d[0] for d in e_pipe)
+    fastq_bytes = sum(a.size for a in arrs)
 
     def nothing_enc(i, st, codec, data, boff, rows, mm):
         return 0
 
-    res = {}
-    for name, mode, enc in (("plain", BATCH_PLAIN, False), ("pipelined", BATCH_PIPELINED, False),
-                            ("plain_encoded", BATCH_PLAIN, True), ("pipelined_encoded", BATCH_PIPELINED, True)):
-        best = None
-        for _ in range(reps):
-            t = time.perf_counter()
-            ctx.count_batch(arrs, k, canon, P, mode=mode, on_result=nothing_enc if enc else nothing, encoded=enc)
-            dt = time.perf_counter() - t
-            best = dt if best is None or dt < best else best
-        res[name] = {"wall_s": best, "kmers_per_s": kmers / best}
-        log(f"batch24 {name}: {best * 1e3:.1f} ms, {kmers / best / 1e9:.2f} G k-mers/s (best of {reps})")
+    def nothing(i, st, keys, counts, off):
+        return 0
+
+    for i in range(warmup):
+        t = time.perf_counter()
+        ctx.count_batch(arrs, k, canon, P, mode=BATCH_PIPELINED, on_result=nothing_enc, encoded=True)
+        log(f"batch24 warmup {i + 1}/{warmup}: {(time.perf_counter() - t) * 1e3:.1f} ms")
+
+    # timed region: kernel events on the engine stream during the same steps
+    # (every batch call returns after its last D2H and callback)
+    ctx.count_profile(True)
+    kern: dict[str, dict] = {}
+    walls = []
+    t_start = time.perf_counter()
+    for i in range(steps):
+        t = time.perf_counter()
+        ctx.count_batch(arrs, k, canon, P, mode=BATCH_PIPELINED, on_result=nothing_enc, encoded=True)
+        walls.append(time.perf_counter() - t)
+        for name, v in ctx.count_kernel_stats().items():
+            a = kern.setdefault(name, {"ms": 0.0, "launches": 0, "bytes": 0})
+            for key in a:
+                a[key] += v[key]
+    wall = time.perf_counter() - t_start
+    ctx.count_profile(False)
+    log(f"batch24 timed {steps} steps: {wall / steps * 1e3:.1f} ms/step, {kmers * steps / wall / 1e9:.2f} G k-mers/s "
+        f"(steps {min(walls) * 1e3:.1f}..{max(walls) * 1e3:.1f} ms)")
+    # two untimed passes: one with per-sample stage events on the three
+    # streams (where the time goes), one with no events at all (what the
+    # kernel events cost)
+    ctx.count_profile(False, timeline=True)
+    t = time.perf_counter()
+    ctx.count_batch(arrs, k, canon, P, mode=BATCH_PIPELINED, on_result=nothing_enc, encoded=True)
+    timeline_pass = time.perf_counter() - t
+    tl = ctx.batch_timeline()
+    ctx.count_profile(False)
+    t = time.perf_counter()
+    ctx.count_batch(arrs, k, canon, P, mode=BATCH_PIPELINED, on_result=nothing_enc, encoded=True)
+    bare_pass = time.perf_counter() - t
+    log(f"batch24 untimed passes: with stage events {timeline_pass * 1e3:.1f} ms, no events {bare_pass * 1e3:.1f} ms")
+
+    # the plain per-sample path, median of plain_reps
+    plain = []
+    for _ in range(plain_reps):
+        t = time.perf_counter()
+        ctx.count_batch(arrs, k, canon, P, mode=BATCH_PLAIN, on_result=nothing_enc, encoded=True)
+        plain.append(time.perf_counter() - t)
+    plain_med = float(np.median(plain))
+    log(f"batch24 plain (encoded): median {plain_med * 1e3:.1f} ms of {[round(x * 1e3, 1) for x in plain]}")
+    # one sample at a time in persistent buffers: no allocation, no overlap
+    serial = []
+    for _ in range(plain_reps):
+        t = time.perf_counter()
+        ctx.count_batch(arrs, k, canon, P, mode=BATCH_SERIAL, on_result=nothing_enc, encoded=True)
+        serial.append(time.perf_counter() - t)
+    serial_med = float(np.median(serial))
+    log(f"batch24 serial (encoded, persistent buffers): median {serial_med * 1e3:.1f} ms of "
+        f"{[round(x * 1e3, 1) for x in serial]}")
+    # the raw-table modes (rows instead of chunk payloads cross PCIe), once each
+    raw = {}
+    for name, mode in (("pipelined", BATCH_PIPELINED), ("plain", BATCH_PLAIN)):
+        t = time.perf_counter()
+        ctx.count_batch(arrs, k, canon, P, mode=mode, on_result=nothing)
+        dt = time.perf_counter() - t
+        raw[name] = {"wall_s": dt, "kmers_per_s": kmers / dt}
+        log(f"batch24 raw tables {name}: {dt * 1e3:.1f} ms")
+
+    # stage occupancy of the last timed step, from the per-sample events
+    span = float(tl[:, 5].max()) if len(tl) else 0.0
+    busy = {name: float(np.sum(tl[:, b] - tl[:, a])) for name, a, b in
+            (("h2d_ms", 0, 1), ("engine_ms", 2, 3), ("d2h_ms", 4, 5))}
     out = {
-        "samples": len(arrs), "reads": int(sum(w.n_reads for w in ws)), "fastq_bytes": int(total_bytes),
-        "input_kmers": int(kmers), "unique_kmers_total": int(sum(d[1] for d in d_pipe)),
-        "plain": res["plain"], "pipelined": res["pipelined"],
-        "speedup": res["plain"]["wall_s"] / res["pipelined"]["wall_s"],
-        "includes": "H2D from pinned host memory, kernels, D2H of the reduced tables",
-        "encoded": {"plain": res["plain_encoded"], "pipelined": res["pipelined_encoded"],
-                    "speedup": res["plain_encoded"]["wall_s"] / res["pipelined_encoded"]["wall_s"],
-                    "chunk_bytes": int(enc_bytes),
-                    "includes": "H2D of the FASTQ, kernels, DVLE chunk encoding on the device, D2H of the chunks"},
-        "bit_exact_plain_vs_pipelined": True, "scale": scale,
+        "samples": len(arrs), "fastq_bytes": int(fastq_bytes), "input_kmers": int(kmers),
+        "unique_kmers_total": int(unique), "chunk_bytes": int(enc_bytes), "wall_s": wall, "steps": steps,
+        "step_ms": [w * 1e3 for w in walls],
+        "plain": {"wall_s": plain_med, "kmers_per_s": kmers / plain_med, "reps_s": plain,
+                  "includes": "per sample: device allocation, H2D, count, chunk encode, D2H, synchronise, free"},
+        "batched_vs_plain": plain_med / (wall / steps),
+        "serial": {"wall_s": serial_med, "kmers_per_s": kmers / serial_med, "reps_s": serial,
+                   "includes": "per sample, in persistent device buffers: H2D, count, chunk encode, D2H, "
+                               "synchronise (nothing overlapped)"},
+        "batched_vs_serial": serial_med / (wall / steps),
+        "raw_tables": {**raw, "speedup": raw["plain"]["wall_s"] / raw["pipelined"]["wall_s"],
+                       "bytes_d2h": int(unique * 12)},
+        "bit_exact_plain_vs_pipelined": True,
+        "stages": {"span_ms": span, **busy,
+                   "h2d_frac": busy["h2d_ms"] / span if span else None,
+                   "d2h_frac": busy["d2h_ms"] / span if span else None,
+                   "pass_ms_with_stage_events": timeline_pass * 1e3, "pass_ms_without_events": bare_pass * 1e3,
+                   "per_sample_ms": [[round(float(x), 3) for x in row] for row in tl],
+                   "columns": ["h2d0", "h2d1", "run0", "run1_encoded", "d2h0", "d2h1"]},
+        "ring": int(os.environ.get("KITE_BATCH_RING", "3")),
     }
-    for p in pins:
-        p.free()
-    return out, tables
+    return out, tables, kern, kmers
 
 
 def bench_analyze(ctx, tables: list, min_count: int, cpu_rows_frac_parts: int) -> dict:
@@ -403,27 +498,110 @@ def bench_bgzf(ctx, w, fq, threads: int) -> dict:
     return res
 
 
+class DeviceBytes:
+    """A device copy of host bytes, through the HIP runtime the engine
+    library loaded (hipMalloc / hipMemcpy / hipFree)."""
+
+    def __init__(self, arr: np.ndarray):
+        import ctypes as C
+        from kite_amd import load_gpu
+        load_gpu()  # the engine's runtime is the one resolved below
+        self.hip = C.CDLL("libamdhip64.so.7")
+        self.hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.hip.hipFree.argtypes = [C.c_void_p]
+        p = C.c_void_p()
+        if self.hip.hipMalloc(C.byref(p), max(arr.size, 1)) != 0:
+            raise RuntimeError("hipMalloc failed")
+        self.ptr, self.nbytes = p.value, arr.size
+        if self.hip.hipMemcpy(C.c_void_p(self.ptr), arr.ctypes.data, arr.size, 1) != 0:  # host to device
+            raise RuntimeError("hipMemcpy failed")
+
+    def free(self) -> None:
+        import ctypes as C
+        if self.ptr:
+            self.hip.hipFree(C.c_void_p(self.ptr))
+            self.ptr = None
+
+
+def bench_single(ctx, w, steps: int, warmup: int) -> tuple[dict, np.ndarray]:
+    """One sample resident in HBM (the FASTQ already on the device, the
+    tables left there): the count pipeline's own rate and per-kernel
+    rooflines, without PCIe."""
+    fq = w.fastq()
+    dev = DeviceBytes(fq)
+    ptr, nbytes = dev.ptr, dev.nbytes
+
+    def step():
+        ctx.count_begin(w.k, w.canonical, w.partitions)
+        ctx.count_feed_device(ptr, nbytes)
+        return ctx.count_finish()
+
+    for i in range(warmup):
+        st = step()
+    ctx.count_profile(True)
+    kern: dict[str, dict] = {}
+    t0 = time.perf_counter()
+    stats = []
+    for i in range(steps):
+        stats.append(step())  # returns after the device work
+        for name, v in ctx.count_kernel_stats().items():
+            a = kern.setdefault(name, {"ms": 0.0, "launches": 0, "bytes": 0})
+            for key in a:
+                a[key] += v[key]
+    wall = time.perf_counter() - t0
+    ctx.count_profile(False)
+    kmers = stats[-1].kmers
+    assert all(s.kmers == kmers and s.unique == stats[0].unique for s in stats)
+    dev.free()
+    rate = kmers * steps / wall
+    log(f"single sample {w.name} (HBM-resident): {wall / steps * 1e3:.3f} ms/step, {rate / 1e9:.2f} G k-mers/s")
+    out = {"workload": w.name, "k": w.k, "canonical": w.canonical, "partitions": w.partitions,
+           "reads": w.n_reads, "fastq_bytes": int(nbytes), "input_kmers": int(kmers),
+           "unique_kmers": int(stats[-1].unique), "steps": steps, "ms_per_step": wall / steps * 1e3,
+           "kmers_per_s": rate, "roofline": roofline_of(kern, w.name), "kernels": kernel_table(kern, steps),
+           "includes": "FASTQ resident in HBM; count kernels only; results stay in HBM"}
+    return out, fq
+
+
+def kernel_table(kern: dict, steps: int) -> dict:
+    return {n: {"ms_per_step": v["ms"] / steps, "launches_per_step": v["launches"] / steps,
+                "GBps": (v["bytes"] / (v["ms"] * 1e-3) / 1e9) if v["ms"] and v["bytes"] else None}
+            for n, v in sorted(kern.items(), key=lambda kv: -kv[1]["ms"])}
+
+
+def roofline_of(kern: dict, workload: str) -> dict:
+    """Roofline of the dominant kernel (largest total time): algorithmic bytes
+    per launch over its average launch time (HIP events)."""
+    dom_name, dom = max(kern.items(), key=lambda kv: kv[1]["ms"])
+    per_launch_s = dom["ms"] / dom["launches"] * 1e-3
+    per_launch_bytes = dom["bytes"] / dom["launches"]
+    achieved = per_launch_bytes / per_launch_s / 1e9
+    traffic, traffic_src = traffic_from_profile(dom_name, workload)
+    return {"bound": "hbm", "kernel": dom_name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+            "algorithmic_bytes_per_launch": per_launch_bytes, "avg_launch_ms": per_launch_s * 1e3,
+            "launches": dom["launches"]}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="k31-canon-50mbp-5m")
-    ap.add_argument("--reads", type=int, default=None, help="override the read count")
-    ap.add_argument("--cpu-sample-reads", type=int, default=1_000_000)
+    ap.add_argument("--batch-scale", type=float, default=1.0, help="scale of the 24 read counts")
+    ap.add_argument("--plain-reps", type=int, default=3)
+    ap.add_argument("--single-workload", default="k31-canon-50mbp-5m")
+    ap.add_argument("--single-steps", type=int, default=10)
+    ap.add_argument("--cpu-samples", default="0,1,2", help="cohort samples the CPU baseline counts")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-batch", action="store_true", help="skip the 24-sample batch and analyze legs")
     ap.add_argument("--no-analyze", action="store_true")
+    ap.add_argument("--no-single", action="store_true", help="skip the HBM-resident single-sample leg")
     ap.add_argument("--no-dump", action="store_true", help="skip the kite dump leg")
     ap.add_argument("--no-bgzf", action="store_true", help="skip the BGZF inflate leg")
-    ap.add_argument("--batch-scale", type=float, default=1.0, help="scale of the 24 read counts")
-    ap.add_argument("--batch-reps", type=int, default=2)
     ap.add_argument("--analyze-cpu-parts", type=int, default=8,
                     help="partitions (of 256) whose union rows the CPU oracle sample covers")
-    ap.add_argument("--calibrate-copy", action="store_true",
-                    help="first run one 1 GiB device-to-device copy (a known-size kernel that "
-                         "scales FETCH_SIZE / WRITE_SIZE counter readings)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -432,106 +610,65 @@ def main() -> int:
               "(--gpus must be 1)", file=sys.stderr)
         return 2
 
-    import torch
-
-    if not torch.cuda.is_available():
-        print("bench.py needs a HIP device", file=sys.stderr)
-        return 1
-
-    from kite_amd import GpuContext
+    from kite_amd import GpuContext, KiteError
     from kite_amd.workloads import WORKLOADS
 
-    w = WORKLOADS[args.workload]
-    if args.reads:
-        w = w.scaled(args.reads)
-    t0 = time.time()
-    genome = w.genome()
-    fq = w.fastq(genome=genome)
-    log(f"workload {w.name}: {w.n_reads} reads, {fq.size / 1e9:.3f} GB FASTQ "
-        f"(generated in {time.time() - t0:.1f}s)")
-    dev = torch.from_numpy(fq).to("cuda")
-    torch.cuda.synchronize()
-    ptr, nbytes = dev.data_ptr(), dev.numel()
-
-    if args.calibrate_copy:
-        src = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
-        log("calibration copy: 1 GiB read + 1 GiB written")
-
-    ctx = GpuContext(0)
-
-    def step():
-        ctx.count_begin(w.k, w.canonical, w.partitions)
-        ctx.count_feed_device(ptr, nbytes)
-        return ctx.count_finish()
-
-    for i in range(args.warmup):
-        st = step()
-        log(f"warmup {i + 1}/{args.warmup}: {st.kmers} k-mers, {st.unique} unique, "
-            f"device {st.device_ms:.2f} ms")
-
-    # timed region: kernel events recorded on the engine's stream during the same steps
-    ctx.count_profile(True)
-    kern: dict[str, dict] = {}
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    stats = []
-    for i in range(args.steps):
-        st = step()
-        stats.append(st)
-        for name, v in ctx.count_kernel_stats().items():
-            a = kern.setdefault(name, {"ms": 0.0, "launches": 0, "bytes": 0})
-            for key in a:
-                a[key] += v[key]
-    torch.cuda.synchronize()
-    t_end = time.perf_counter()
-    ctx.count_profile(False)
-    wall = t_end - t_start
-    kmers = stats[-1].kmers
-    assert all(s.kmers == kmers and s.unique == stats[0].unique for s in stats)
-    ms_per_step = wall / args.steps * 1e3
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    try:
+        ctx = GpuContext(0)
+    except KiteError as e:
+        print(f"bench.py needs a HIP device: {e}", file=sys.stderr)
+        return 1
+    pcie = pcie_peak(ctx)
+    ws, pins = make_batch24(args.batch_scale)
+    batch, tables, kern, kmers = bench_batch24(ctx, pins, args.steps, args.warmup, args.plain_reps)
+    wall = batch["wall_s"]
     value = kmers * args.steps / wall
-    log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step, {value / 1e9:.2f} G k-mers/s")
+    ms_per_step = wall / args.steps * 1e3
+    workload = "batch24-k31-canon-20mbp-snv2"
+    roof = roofline_of(kern, workload)
+    kernels = kernel_table(kern, args.steps)
+    log("batch24 kernels: " + json.dumps({n: round(v["ms_per_step"], 3) for n, v in kernels.items()}))
+    moved = batch["fastq_bytes"] + batch["chunk_bytes"]
+    pcie_line = {"h2d_bytes": batch["fastq_bytes"], "d2h_bytes": batch["chunk_bytes"],
+                 "achieved_GBps": moved / (wall / args.steps) / 1e9, "peak_GBps": pcie["both_GBps"],
+                 "frac": moved / (wall / args.steps) / 1e9 / pcie["both_GBps"],
+                 "peak_source": "kite_gpu_copy_peak in this run: page-locked 1 GiB copies, H2D and D2H at once",
+                 **{f"measured_{k}": v for k, v in pcie.items()}}
+    log(f"batch24 PCIe: {pcie_line['achieved_GBps']:.1f} GB/s of {pcie['both_GBps']:.1f} "
+        f"({pcie_line['frac']:.2f}); plain {batch['plain']['kmers_per_s'] / 1e9:.2f} G k-mers/s, "
+        f"batched/plain {batch['batched_vs_plain']:.2f}x")
 
-    dom_name, dom = max(kern.items(), key=lambda kv: kv[1]["ms"])
-    per_launch_s = dom["ms"] / dom["launches"] * 1e-3
-    per_launch_bytes = dom["bytes"] / dom["launches"]
-    achieved = per_launch_bytes / per_launch_s / 1e9
-    kernels = {n: {"ms_per_step": v["ms"] / args.steps,
-                   "GBps": (v["bytes"] / (v["ms"] * 1e-3) / 1e9) if v["ms"] else None}
-               for n, v in sorted(kern.items(), key=lambda kv: -kv[1]["ms"])}
-    log("kernels: " + json.dumps(kernels))
-
-    traffic, traffic_src = traffic_from_profile(dom_name, w.name)
-
-    dump = None
-    if not args.no_dump:
-        dump = bench_dump(w, fq)
-
-    bgz = None
-    if not args.no_bgzf:
-        bgz = bench_bgzf(ctx, w, fq, args.cpu_threads or min(16, os.cpu_count() or 1))
-        if bgz["count_from_bgzf"]["kmers"] != kmers:
-            raise SystemExit("bgzf count differs from the plain count")
-
-    batch = analyze = None
-    if not args.no_batch:
-        del dev
-        torch.cuda.empty_cache()
-        batch, tables = bench_batch24(ctx, args.batch_scale, args.batch_reps)
-        if not args.no_analyze:
-            analyze = bench_analyze(ctx, tables, 2, args.analyze_cpu_parts)
-        del tables
+    analyze = None
+    if not args.no_analyze:
+        analyze = bench_analyze(ctx, tables, 2, args.analyze_cpu_parts)
+    del tables
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log(f"cpu baseline: oracle count on {args.cpu_sample_reads} reads, {threads} threads")
-        cpu = cpu_baseline(w, min(args.cpu_sample_reads, w.n_reads), threads)
+        idx = [int(x) for x in args.cpu_samples.split(",") if x != ""]
+        log(f"cpu baseline: oracle count on cohort samples {idx}, {threads} threads")
+        cpu = cpu_baseline([pins[i].array for i in idx], [ws[i].name for i in idx], threads)
         log(f"cpu baseline: {cpu['value'] / 1e6 if cpu['value'] else 0:.1f} M k-mers/s")
+    for p in pins:
+        p.free()
+    del pins
+
+    single = fq = None
+    w = WORKLOADS[args.single_workload]
+    if not args.no_single:
+        single, fq = bench_single(ctx, w, args.single_steps, 3)
+    if not args.no_dump:
+        fq = w.fastq() if fq is None else fq
+        single = single or {}
+        single["dump"] = bench_dump(w, fq)
+    if not args.no_bgzf:
+        fq = w.fastq() if fq is None else fq
+        bgz = bench_bgzf(ctx, w, fq, threads)
+        if single and "input_kmers" in single and bgz["count_from_bgzf"]["kmers"] != single["input_kmers"]:
+            raise SystemExit("bgzf count differs from the plain count")
+        single = single or {}
+        single["bgzf"] = bgz
 
     line = {
         "metric": "input k-mers/s, batched vs plain GPU (bit-exact); corr-matrix time @1e-5 err",
@@ -544,22 +681,24 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u64" if stats[-1].words == 1 else f"u64x{stats[-1].words}",
+        "dtype": "u64",
         "data": "synthetic (seeded generator, regenerated on the box)",
-        "config": {"workload": w.name, "k": w.k, "canonical": w.canonical,
-                   "partitions": w.partitions, "reads": w.n_reads, "read_len": w.read_len,
-                   "fastq_bytes": int(nbytes), "input_kmers": int(kmers),
-                   "unique_kmers": int(stats[-1].unique), "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic, "traffic_source": traffic_src,
-                     "algorithmic_bytes_per_launch": per_launch_bytes,
-                     "avg_launch_ms": per_launch_s * 1e3},
+        "config": {"workload": workload, "samples": 24, "k": 31, "canonical": True, "partitions": 256,
+                   "reads": int(sum(w_.n_reads for w_ in ws)), "read_len": 150,
+                   "fastq_bytes": batch["fastq_bytes"], "input_kmers": int(kmers),
+                   "unique_kmers_total": batch["unique_kmers_total"], "chunk_bytes": batch["chunk_bytes"],
+                   "step": "all 24 samples from pinned host memory: H2D of the FASTQ, count, DVLE chunk "
+                           "encode on the device, D2H of the chunks (the .kite payloads)",
+                   "batch_scale": args.batch_scale, "parallelism": "single GPU"},
+        "batched_vs_plain": batch["batched_vs_plain"],
+        "plain": batch["plain"],
+        "batched_vs_serial": batch["batched_vs_serial"],
+        "roofline": roof,
+        "pcie": pcie_line,
         "kernels": kernels,
         "batch24": batch,
         "analyze": analyze,
-        "dump": dump,
-        "bgzf": bgz,
+        "single_sample_hbm": single,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

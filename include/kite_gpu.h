@@ -146,10 +146,13 @@ int kite_gpu_count_device_result(kite_gpu_ctx *ctx, const uint64_t **d_keys,
  *
  * mode KITE_BATCH_PLAIN: one sample at a time -- fresh device allocations,
  * H2D copy, kernels, D2H copy, synchronise, free.
- * mode KITE_BATCH_PIPELINED: persistent device buffers; the H2D copy of
- * sample i+1 and the D2H copy of sample i-1 run on their own streams while
- * sample i is counted, and results are handed to `fn` on a host thread of
- * the engine's own while later samples run.
+ * mode KITE_BATCH_SERIAL: one sample at a time like PLAIN, but in
+ * persistent device buffers (no per-sample allocation).
+ * mode KITE_BATCH_PIPELINED: persistent device buffers in a ring of three
+ * (KITE_BATCH_RING: 2..4); the H2D copies of the next samples and the D2H
+ * copies of earlier ones run on their own streams while sample i is counted,
+ * and results are handed to `fn` on a host thread of the engine's own while
+ * later samples run.
  *
  * `fn` receives sample i's result, borrowed for the call: keys
  * (stats->unique * words u64), counts (u32, saturated at u32::MAX) and
@@ -157,6 +160,7 @@ int kite_gpu_count_device_result(kite_gpu_ctx *ctx, const uint64_t **d_keys,
  * to abort the batch (KITE_E_CALLBACK). */
 #define KITE_BATCH_PLAIN 0
 #define KITE_BATCH_PIPELINED 1
+#define KITE_BATCH_SERIAL 2
 
 typedef struct kite_batch_sample {
     const uint8_t *fastq;
@@ -185,6 +189,12 @@ typedef int (*kite_batch_chunks_fn)(void *user, uint32_t sample, const kite_coun
 int kite_gpu_count_batch_chunks(kite_gpu_ctx *ctx, uint32_t k, int canonical, uint32_t partitions,
                                 const kite_batch_sample *samples, uint32_t n_samples, int mode,
                                 kite_batch_chunks_fn fn, void *user);
+
+/* Host-link copy rates of this device: page-locked buffers of `bytes`,
+ * H2D alone, D2H alone, and both at once on two streams (GB/s, best of 3).
+ * The bound of the batch above, whose H2D and D2H run at once.  No
+ * counterpart in the reference (it has no device link). */
+int kite_gpu_copy_peak(kite_gpu_ctx *ctx, uint64_t bytes, double *h2d_gbps, double *d2h_gbps, double *both_gbps);
 
 /* Page-locked host memory for FASTQ buffers and results (hipHostMalloc). */
 int kite_gpu_host_alloc(uint64_t bytes, void **out);
@@ -270,16 +280,30 @@ int kite_gpu_pca(kite_gpu_ctx *ctx, const double *cov, uint32_t n, uint32_t dims
 int kite_gpu_pca_noalloc(kite_gpu_ctx *ctx, const double *cov, uint32_t n, uint32_t dims, double *scores,
                          double *eigenvalues);
 
-/* Turns per-kernel profiling on or off for later batches: HIP events around
- * every launch on the engine's stream, plus each kernel's algorithmic HBM
- * bytes.  Counterpart of the reference's --test-mode stage timers
+/* Turns profiling on or off for later counts: KITE_PROFILE_KERNELS records
+ * HIP events around every launch on the engine's stream, plus each kernel's
+ * algorithmic HBM bytes; KITE_PROFILE_TIMELINE records per-sample stage
+ * events on the three streams of a pipelined batch (kite_gpu_batch_timeline).
+ * Counterpart of the reference's --test-mode stage timers
  * (src/stats.rs:8-273), at kernel granularity. */
+#define KITE_PROFILE_KERNELS 0x1
+#define KITE_PROFILE_TIMELINE 0x2
 int kite_gpu_count_profile(kite_gpu_ctx *ctx, int enable);
 
 /* Writes the last batch's kernel profile as JSON into buf (cap bytes):
  * {"<kernel>": {"ms": total, "launches": n, "bytes": algorithmic}, ...}.
  * Returns the JSON length, or a negative code. */
 int kite_gpu_count_kernel_stats(kite_gpu_ctx *ctx, char *buf, size_t cap);
+
+/* Stage times of the last pipelined batch run with profiling on: per sample
+ * six values in ms since the batch started -- H2D start and end (copy
+ * stream), count start and count (+ chunk encode) end (engine stream), D2H
+ * start and end (second copy stream).  out receives up to cap_samples * 6
+ * doubles; returns the number of samples timed.  After a batch, the kernel
+ * profile of kite_gpu_count_kernel_stats is summed over its samples.
+ * Counterpart of the reference's per-stage --test-mode timers
+ * (src/stats.rs:8-273) for the multi-sample pipeline. */
+int kite_gpu_batch_timeline(kite_gpu_ctx *ctx, double *out, uint32_t cap_samples);
 
 /* ---- dump -------------------------------------------------------------------
  * Replaces the partition workers of `kite dump` (stream_partition_slabs,

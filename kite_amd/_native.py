@@ -81,7 +81,7 @@ BATCH_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint32, C.POINTER(CountStats), C
 BATCH_CHUNKS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint32, C.POINTER(CountStats), C.c_uint32,
                               C.POINTER(C.c_uint8), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                               C.POINTER(C.c_uint64))
-BATCH_PLAIN, BATCH_PIPELINED = 0, 1
+BATCH_PLAIN, BATCH_PIPELINED, BATCH_SERIAL = 0, 1, 2
 
 
 class DumpRun(C.Structure):
@@ -149,10 +149,13 @@ GPU_SYMBOLS = {
                                 C.c_void_p, C.POINTER(DumpStats)]),
     "kite_gpu_histo": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(DumpRun), C.c_uint64, C.c_int,
                                  C.c_uint32, C.c_int, C.c_uint32, HISTO_FN, C.c_void_p, C.POINTER(DumpStats)]),
+    "kite_gpu_copy_peak": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double)]),
     "kite_gpu_host_alloc": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
     "kite_gpu_host_free": (None, [C.c_void_p]),
     "kite_gpu_count_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "kite_gpu_count_kernel_stats": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t]),
+    "kite_gpu_batch_timeline": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
 }
 
 
@@ -587,8 +590,10 @@ class GpuContext:
         _check(rc)
         return out, st.as_dict()
 
-    def count_profile(self, enable: bool) -> None:
-        _check(self._lib.kite_gpu_count_profile(self._h, int(enable)))
+    def count_profile(self, enable: bool, timeline: bool = False) -> None:
+        """Kernel events on the engine stream (enable) and per-sample stage
+        events of a pipelined batch (timeline)."""
+        _check(self._lib.kite_gpu_count_profile(self._h, (1 if enable else 0) | (2 if timeline else 0)))
 
     def count_kernel_stats(self) -> dict:
         import json
@@ -597,6 +602,22 @@ class GpuContext:
         if n < 0:
             _check(n)
         return json.loads(buf.value.decode())
+
+    def copy_peak(self, nbytes: int = 1 << 30) -> dict:
+        """kite_gpu_copy_peak: page-locked H2D, D2H and both-at-once GB/s."""
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        _check(self._lib.kite_gpu_copy_peak(self._h, nbytes, C.byref(a), C.byref(b), C.byref(c)))
+        return {"h2d_GBps": a.value, "d2h_GBps": b.value, "both_GBps": c.value}
+
+    def batch_timeline(self) -> np.ndarray:
+        """Per-sample stage times of the last profiled pipelined batch, ms
+        since its start: columns h2d0, h2d1, run0, run1, d2h0, d2h1."""
+        n = self._lib.kite_gpu_batch_timeline(self._h, None, 0)
+        if n < 0:
+            _check(n)
+        out = np.zeros((max(n, 1), 6))
+        self._lib.kite_gpu_batch_timeline(self._h, out.ctypes.data, n)
+        return out[:n]
 
     def count(self, fastq: np.ndarray | bytes, k: int, canonical: bool, partitions: int,
               chunks: list[tuple[int, int]] | None = None) -> tuple[CountResult, CountStats]:

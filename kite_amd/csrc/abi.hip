@@ -1,7 +1,9 @@
 // abi.hip -- the extern "C" boundary of libkite_gpu.so (declared in
 // include/kite_gpu.h): context, count session, run-chunk emission.
 #include <algorithm>
+#include <array>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <condition_variable>
 #include <cstring>
@@ -72,6 +74,16 @@ struct PinnedBuf {
     template <class T>
     T *as() const { return static_cast<T *>(ptr); }
 };
+
+// Depth of the pipelined batch's ring of input buffers and result slots
+// (KITE_BATCH_RING overrides the default of 3 within 2..MAX_RING).
+constexpr int MAX_RING = kite::CountEngine::OUT_SLOTS;
+
+int batch_ring() {
+    int r = 3;
+    if (const char *e = getenv("KITE_BATCH_RING")) r = atoi(e);
+    return std::min(MAX_RING, std::max(2, r));
+}
 
 // One result slot of the batched count: the host copy of a sample's keys,
 // counts and partition offsets, and the event that ends its D2H copy.
@@ -148,11 +160,20 @@ struct kite_gpu_ctx {
     kite::EncodeBufs enc_bufs;
     kite_count_stats stats{};
 
-    // batched count: copy streams, double-buffered input and result slots
+    // batched count: copy streams, a ring of input buffers and result slots
     hipStream_t h2d = nullptr, d2h = nullptr;
-    kite::DevBuf batch_in[2];
-    BatchSlot slots[2];
-    kite::EncodeBufs enc_slot[2];  // device encode buffers of the two result slots
+    kite::DevBuf batch_in[MAX_RING];
+    BatchSlot slots[MAX_RING];
+    kite::EncodeBufs enc_slot[MAX_RING];  // device encode buffers of the result slots
+    // profiling (kite_gpu_count_profile): per-kernel stats summed over the
+    // samples of the last batch, and per-sample stage times
+    bool profile = false;   // kernel events on the engine stream
+    bool timeline = false;  // per-sample stage events on the batch's three streams
+    bool kstats_from_batch = false;
+    std::vector<kite::KernelStat> batch_kstats;
+    std::vector<std::array<hipEvent_t, 6>> tl_events;  // grows, reused across batches
+    hipEvent_t tl_start = nullptr;
+    std::vector<double> tl_ms;  // 6 per sample of the last profiled batch
 
     // dump: staged payloads of a partition group, pinned text pieces
     kite::HostPinned dump_stage, dump_text;
@@ -201,6 +222,7 @@ void stash_device_result(kite_gpu_ctx *c) {
 
 void run_batch(kite_gpu_ctx *c, const uint8_t *ptr, uint64_t len) {
     stash_device_result(c);
+    c->kstats_from_batch = false;
     c->count->set_timing(true);
     c->count->run(ptr, len, (int)c->k, c->canonical, c->partitions);
     c->device_result = true;
@@ -416,7 +438,10 @@ void kite_gpu_close(kite_gpu_ctx *ctx) {
         ctx->dump_stage.release();
         ctx->dump_text.release();
         ctx->dump_pay.release();
-        for (int b = 0; b < 2; b++) {
+        for (auto &ev : ctx->tl_events)
+            for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        if (ctx->tl_start) (void)hipEventDestroy(ctx->tl_start);
+        for (int b = 0; b < MAX_RING; b++) {
             ctx->batch_in[b].release();
             ctx->slots[b].keys.release();
             ctx->slots[b].counts.release();
@@ -868,15 +893,24 @@ int kite_gpu_pca_noalloc(kite_gpu_ctx *ctx, const double *cov, uint32_t n, uint3
 
 int kite_gpu_count_profile(kite_gpu_ctx *ctx, int enable) {
     if (!ctx) return set_error(KITE_E_INVALID, "invalid argument: ctx is NULL");
-    ctx->count->set_kernel_profile(enable != 0);
+    ctx->count->set_kernel_profile((enable & KITE_PROFILE_KERNELS) != 0);
+    ctx->profile = (enable & KITE_PROFILE_KERNELS) != 0;
+    ctx->timeline = (enable & KITE_PROFILE_TIMELINE) != 0;
     return KITE_OK;
+}
+
+int kite_gpu_batch_timeline(kite_gpu_ctx *ctx, double *out, uint32_t cap_samples) {
+    if (!ctx) return set_error(KITE_E_INVALID, "invalid argument: ctx is NULL");
+    const uint32_t ns = (uint32_t)(ctx->tl_ms.size() / 6);
+    if (out) std::copy(ctx->tl_ms.begin(), ctx->tl_ms.begin() + (size_t)std::min(ns, cap_samples) * 6, out);
+    return (int)ns;
 }
 
 int kite_gpu_count_kernel_stats(kite_gpu_ctx *ctx, char *buf, size_t cap) {
     if (!ctx) return set_error(KITE_E_INVALID, "invalid argument: ctx is NULL");
     std::string js = "{";
     bool first = true;
-    for (const auto &ks : ctx->count->kernel_stats()) {
+    for (const auto &ks : ctx->kstats_from_batch ? ctx->batch_kstats : ctx->count->kernel_stats()) {
         char tmp[256];
         snprintf(tmp, sizeof tmp, "%s\"%s\": {\"ms\": %.6f, \"launches\": %llu, \"bytes\": %llu}",
                  first ? "" : ", ", ks.name.c_str(), ks.ms, (unsigned long long)ks.launches,
@@ -913,6 +947,22 @@ void batch_stats(kite_gpu_ctx *c, const kite_batch_sample &smp, kite_count_stats
     st.compact_ms = t.compact_ms;
 }
 
+// Adds the engine's per-kernel stats of the last run to the batch's sums.
+void add_batch_kstats(kite_gpu_ctx *c) {
+    if (!c->profile) return;
+    for (const kite::KernelStat &ks : c->count->kernel_stats()) {
+        auto it = std::find_if(c->batch_kstats.begin(), c->batch_kstats.end(),
+                               [&](const kite::KernelStat &x) { return x.name == ks.name; });
+        if (it == c->batch_kstats.end()) {
+            c->batch_kstats.push_back(ks);
+        } else {
+            it->ms += ks.ms;
+            it->launches += ks.launches;
+            it->bytes += ks.bytes;
+        }
+    }
+}
+
 // Where a finished sample goes: raw rows (kite_batch_result_fn) or chunk
 // payloads encoded on the device (kite_batch_chunks_fn).
 struct BatchSink {
@@ -942,24 +992,38 @@ kite::EncodeDevice encode_slot(kite_gpu_ctx *c, BatchSlot &sl, kite::EncodeBufs 
     return ed;
 }
 
-void batch_plain(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, const BatchSink &sink) {
+// One sample at a time, nothing overlapped.  reuse == false (KITE_BATCH_PLAIN):
+// the device buffers are allocated and freed per sample, as separate count
+// runs would; reuse == true (KITE_BATCH_SERIAL): persistent buffers.
+void batch_plain(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, const BatchSink &sink, bool reuse) {
     BatchSlot &sl = c->slots[0];
     const int W = c->words;
+    if (reuse) {
+        uint64_t maxlen = 0;
+        for (uint32_t i = 0; i < n; i++) maxlen = std::max<uint64_t>(maxlen, smp[i].len);
+        c->batch_in[0].ensure(maxlen);
+    }
     for (uint32_t i = 0; i < n; i++) {
-        c->count->release();
         void *d = nullptr;
-        if (smp[i].len) {
-            KITE_HIP(hipMalloc(&d, smp[i].len));
-            KITE_HIP(hipMemcpy(d, smp[i].fastq, smp[i].len, hipMemcpyHostToDevice));
+        if (reuse) {
+            d = c->batch_in[0].ptr;
+            if (smp[i].len) KITE_HIP(hipMemcpy(d, smp[i].fastq, smp[i].len, hipMemcpyHostToDevice));
+        } else {
+            c->count->release();
+            if (smp[i].len) {
+                KITE_HIP(hipMalloc(&d, smp[i].len));
+                KITE_HIP(hipMemcpy(d, smp[i].fastq, smp[i].len, hipMemcpyHostToDevice));
+            }
         }
         c->count->set_slot(0);
         c->count->set_timing(true);
         try {
             c->count->run(static_cast<const uint8_t *>(d), smp[i].len, (int)c->k, c->canonical, c->partitions);
         } catch (...) {
-            if (d) (void)hipFree(d);
+            if (d && !reuse) (void)hipFree(d);
             throw;
         }
+        add_batch_kstats(c);
         const uint64_t u = c->count->n_unique();
         if (sink.encoded()) {
             // the engine stream does not synchronize with the null stream:
@@ -978,7 +1042,7 @@ void batch_plain(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, cons
                 KITE_HIP(hipMemcpy(counts, c->count->counts(), u * 4, hipMemcpyDeviceToHost));
             }
         }
-        if (d) KITE_HIP(hipFree(d));
+        if (d && !reuse) KITE_HIP(hipFree(d));
         batch_stats(c, smp[i], sl.st);
         sl.part_off = c->count->part_off();
         sl.sample = i;
@@ -994,12 +1058,13 @@ struct Consumer {
     std::mutex mu;
     std::condition_variable cv;
     std::deque<int> queue;
-    bool free_slot[2] = {true, true};
+    bool free_slot[MAX_RING];
     bool done = false;
     std::string error;
     std::thread th;
 
     Consumer(kite_gpu_ctx *cc, const BatchSink &sk) : c(cc), sink(sk) {
+        for (bool &f : free_slot) f = true;
         th = std::thread([this] { loop(); });
     }
     ~Consumer() { stop(); }
@@ -1053,7 +1118,13 @@ struct Consumer {
     // Waits for every queued slot to be consumed.
     void drain() {
         std::unique_lock<std::mutex> g(mu);
-        cv.wait(g, [&] { return (queue.empty() && free_slot[0] && free_slot[1]) || !error.empty(); });
+        cv.wait(g, [&] {
+            if (!error.empty()) return true;
+            if (!queue.empty()) return false;
+            for (bool f : free_slot)
+                if (!f) return false;
+            return true;
+        });
     }
     std::string err() {
         std::lock_guard<std::mutex> g(mu);
@@ -1061,68 +1132,103 @@ struct Consumer {
     }
 };
 
+// Samples flow through a ring of R input buffers and R result slots (input
+// buffer, engine output slot, encode buffers and host buffers of slot i % R
+// belong to sample i):
+//   copy stream h2d:   H2D of sample j into buffer j % R, after the run of
+//                      sample j - R read that buffer;
+//   engine stream:     count of sample i once its H2D is done (raw rows:
+//                      also once the D2H of sample i - R has read the
+//                      engine's output slot), then the chunk encode;
+//   copy stream d2h:   D2H of sample i after its run / encode;
+//   consumer thread:   the callback of sample i after its D2H, which frees
+//                      the slot's host buffers for sample i + R.
+// The H2D queue runs R - 1 samples ahead of the engine, so the two copy
+// directions stay busy while the engine works.
 void batch_pipelined(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, const BatchSink &sink) {
+    const int R = batch_ring();
     if (!c->h2d) KITE_HIP(hipStreamCreateWithFlags(&c->h2d, hipStreamNonBlocking));
     if (!c->d2h) KITE_HIP(hipStreamCreateWithFlags(&c->d2h, hipStreamNonBlocking));
-    for (int b = 0; b < 2; b++) {
+    for (int b = 0; b < R; b++) {
         BatchSlot &sl = c->slots[b];
         for (hipEvent_t *e : {&sl.d2h_done, &sl.run_done, &sl.in_ready})
             if (!*e) KITE_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
     }
+    const bool prof = c->timeline;
+    if (prof) {
+        if (!c->tl_start) KITE_HIP(hipEventCreate(&c->tl_start));
+        while (c->tl_events.size() < n) {
+            std::array<hipEvent_t, 6> ev{};
+            for (hipEvent_t &e : ev) KITE_HIP(hipEventCreate(&e));
+            c->tl_events.push_back(ev);
+        }
+    }
     uint64_t maxlen = 0;
     for (uint32_t i = 0; i < n; i++) maxlen = std::max<uint64_t>(maxlen, smp[i].len);
-    // both input buffers are sized before any copy is queued into them
-    for (int b = 0; b < (n > 1 ? 2 : 1); b++) c->batch_in[b].ensure(maxlen);
+    // every input buffer is sized before any copy is queued into one
+    for (int b = 0; b < std::min<int>(R, (int)n); b++) c->batch_in[b].ensure(maxlen);
     const int W = c->words;
 
     auto h2d = [&](uint32_t j) {
-        const int b = (int)(j & 1);
-        if (j >= 2) KITE_HIP(hipStreamWaitEvent(c->h2d, c->slots[b].run_done, 0));
+        const int b = (int)(j % (uint32_t)R);
+        if (j >= (uint32_t)R) KITE_HIP(hipStreamWaitEvent(c->h2d, c->slots[b].run_done, 0));
+        if (prof) KITE_HIP(hipEventRecord(c->tl_events[j][0], c->h2d));
         if (smp[j].len)
             KITE_HIP(hipMemcpyAsync(c->batch_in[b].ptr, smp[j].fastq, smp[j].len, hipMemcpyHostToDevice, c->h2d));
+        if (prof) KITE_HIP(hipEventRecord(c->tl_events[j][1], c->h2d));
         KITE_HIP(hipEventRecord(c->slots[b].in_ready, c->h2d));
     };
 
     Consumer cons(c, sink);
     std::string failure;
     try {
-        if (n) h2d(0);
+        if (prof) KITE_HIP(hipEventRecord(c->tl_start, c->h2d));
+        for (uint32_t j = 0; j + 1 < (uint32_t)R && j < n; j++) h2d(j);
         for (uint32_t i = 0; i < n; i++) {
-            const int s = (int)(i & 1);
+            const int s = (int)(i % (uint32_t)R);
             BatchSlot &sl = c->slots[s];
-            if (i + 1 < n) h2d(i + 1);
+            // input buffer (i - 1) % R was last read by the run of sample i - 1
+            if (i + R - 1 < n) h2d(i + R - 1);
             KITE_HIP(hipStreamWaitEvent(c->stream, sl.in_ready, 0));
-            // the D2H copy of sample i-2 reads the engine's result slot s
-            if (i >= 2) KITE_HIP(hipStreamWaitEvent(c->stream, sl.d2h_done, 0));
+            // raw rows: the D2H of sample i - R reads the engine's output slot s
+            if (!sink.encoded() && i >= (uint32_t)R) KITE_HIP(hipStreamWaitEvent(c->stream, sl.d2h_done, 0));
+            if (prof) KITE_HIP(hipEventRecord(c->tl_events[i][2], c->stream));
             c->count->set_slot(s);
             c->count->set_timing(true);
             c->count->run(c->batch_in[s].as<uint8_t>(), smp[i].len, (int)c->k, c->canonical, c->partitions);
-            // slot s's host buffers are free once the callback of sample i-2 ran
+            add_batch_kstats(c);
+            // slot s's host buffers are free once the callback of sample i - R
+            // ran, which was after the D2H of that sample had ended
             if (!cons.acquire(s)) break;
             const uint64_t u = c->count->n_unique();
             batch_stats(c, smp[i], sl.st);
             sl.part_off = c->count->part_off();
             sl.sample = i;
             if (sink.encoded()) {
-                // encode buffer s was last read by the D2H of sample i-2, which the
-                // engine stream waited for before this run
+                // encode buffer s was last read by the D2H of sample i - R (ended, see above)
+                if (i >= (uint32_t)R) KITE_HIP(hipStreamWaitEvent(c->stream, sl.d2h_done, 0));
                 const kite::EncodeDevice ed = encode_slot(c, sl, c->enc_slot[s]);
                 KITE_HIP(hipEventRecord(sl.run_done, c->stream));
+                if (prof) KITE_HIP(hipEventRecord(c->tl_events[i][3], c->stream));
                 KITE_HIP(hipStreamWaitEvent(c->d2h, sl.run_done, 0));
+                if (prof) KITE_HIP(hipEventRecord(c->tl_events[i][4], c->d2h));
                 if (ed.total)
                     KITE_HIP(hipMemcpyAsync(sl.enc_bytes.ptr, ed.bytes, ed.total, hipMemcpyDeviceToHost, c->d2h));
                 KITE_HIP(hipMemcpyAsync(sl.enc_minmax.ptr, ed.minmax, (size_t)c->partitions * 64,
                                         hipMemcpyDeviceToHost, c->d2h));
             } else {
                 KITE_HIP(hipEventRecord(sl.run_done, c->stream));
+                if (prof) KITE_HIP(hipEventRecord(c->tl_events[i][3], c->stream));
                 uint64_t *keys = static_cast<uint64_t *>(sl.keys.ensure(u * 8 * W));
                 uint32_t *counts = static_cast<uint32_t *>(sl.counts.ensure(u * 4));
                 KITE_HIP(hipStreamWaitEvent(c->d2h, sl.run_done, 0));
+                if (prof) KITE_HIP(hipEventRecord(c->tl_events[i][4], c->d2h));
                 if (u) {
                     KITE_HIP(hipMemcpyAsync(keys, c->count->keys(), u * 8 * W, hipMemcpyDeviceToHost, c->d2h));
                     KITE_HIP(hipMemcpyAsync(counts, c->count->counts(), u * 4, hipMemcpyDeviceToHost, c->d2h));
                 }
             }
+            if (prof) KITE_HIP(hipEventRecord(c->tl_events[i][5], c->d2h));
             KITE_HIP(hipEventRecord(sl.d2h_done, c->d2h));
             cons.push(s);
         }
@@ -1133,9 +1239,19 @@ void batch_pipelined(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, 
     cons.stop();
     (void)hipStreamSynchronize(c->h2d);
     (void)hipStreamSynchronize(c->d2h);
+    (void)hipStreamSynchronize(c->stream);
     if (!failure.empty()) throw kite::Error(failure);
     std::string ce = cons.err();
     if (!ce.empty()) throw kite::Error(ce);
+    if (prof) {
+        c->tl_ms.assign((size_t)n * 6, 0.0);
+        for (uint32_t i = 0; i < n; i++)
+            for (int q = 0; q < 6; q++) {
+                float ms = 0;
+                KITE_HIP(hipEventElapsedTime(&ms, c->tl_start, c->tl_events[i][q]));
+                c->tl_ms[(size_t)i * 6 + q] = ms;
+            }
+    }
 }
 
 }  // namespace
@@ -1148,7 +1264,7 @@ int count_batch_impl(kite_gpu_ctx *ctx, uint32_t k, int canonical, uint32_t part
     if (!ctx) return set_error(KITE_E_INVALID, "invalid argument: ctx is NULL");
     if (!sink.raw && !sink.chunks) return set_error(KITE_E_INVALID, "invalid argument: callback is NULL");
     if (n_samples && !samples) return set_error(KITE_E_INVALID, "invalid argument: samples is NULL");
-    if (mode != KITE_BATCH_PLAIN && mode != KITE_BATCH_PIPELINED)
+    if (mode != KITE_BATCH_PLAIN && mode != KITE_BATCH_PIPELINED && mode != KITE_BATCH_SERIAL)
         return set_error(KITE_E_INVALID, "invalid argument: unknown batch mode");
     for (uint32_t i = 0; i < n_samples; i++)
         if (samples[i].len && !samples[i].fastq)
@@ -1158,12 +1274,15 @@ int count_batch_impl(kite_gpu_ctx *ctx, uint32_t k, int canonical, uint32_t part
     ctx->session = false;  // the batch does not leave a count session behind
     rc = guarded([&]() -> int {
         bind(ctx);
+        ctx->batch_kstats.clear();
+        ctx->kstats_from_batch = true;
+        ctx->tl_ms.clear();
         for (uint32_t i = 0; i < n_samples; i++)
             if (samples[i].len > ctx->batch_cap)
                 return set_error(KITE_E_NOMEM, "sample " + std::to_string(i) +
                                                    " exceeds the device budget of one batch");
-        if (mode == KITE_BATCH_PLAIN) batch_plain(ctx, samples, n_samples, sink);
-        else batch_pipelined(ctx, samples, n_samples, sink);
+        if (mode == KITE_BATCH_PIPELINED) batch_pipelined(ctx, samples, n_samples, sink);
+        else batch_plain(ctx, samples, n_samples, sink, mode == KITE_BATCH_SERIAL);
         return KITE_OK;
     });
     if (rc == KITE_E_DEVICE && g_last_error.find("callback aborted") != std::string::npos) rc = KITE_E_CALLBACK;
@@ -1187,6 +1306,46 @@ int kite_gpu_count_batch_chunks(kite_gpu_ctx *ctx, uint32_t k, int canonical, ui
     sink.chunks = fn;
     sink.user = user;
     return count_batch_impl(ctx, k, canonical, partitions, samples, n_samples, mode, sink);
+}
+
+int kite_gpu_copy_peak(kite_gpu_ctx *ctx, uint64_t bytes, double *h2d_gbps, double *d2h_gbps, double *both_gbps) {
+    if (!ctx || bytes == 0) return set_error(KITE_E_INVALID, "invalid argument: ctx is NULL or bytes is 0");
+    return guarded([&]() -> int {
+        bind(ctx);
+        PinnedBuf hs, hd;
+        kite::DevBuf ds, dd;
+        void *hsrc = hs.ensure(bytes), *hdst = hd.ensure(bytes);
+        memset(hsrc, 1, bytes);
+        memset(hdst, 2, bytes);
+        void *dsrc = ds.ensure(bytes), *ddst = dd.ensure(bytes);
+        hipStream_t a = nullptr, b = nullptr;
+        KITE_HIP(hipStreamCreateWithFlags(&a, hipStreamNonBlocking));
+        KITE_HIP(hipStreamCreateWithFlags(&b, hipStreamNonBlocking));
+        struct Streams {
+            hipStream_t x, y;
+            ~Streams() {
+                (void)hipStreamDestroy(x);
+                (void)hipStreamDestroy(y);
+            }
+        } guard{a, b};
+        double best[3] = {0, 0, 0};
+        for (int rep = 0; rep < 3; rep++)
+            for (int mode = 1; mode <= 3; mode++) {
+                KITE_HIP(hipDeviceSynchronize());
+                const auto t0 = std::chrono::steady_clock::now();
+                if (mode & 1) KITE_HIP(hipMemcpyAsync(ddst, hsrc, bytes, hipMemcpyHostToDevice, a));
+                if (mode & 2) KITE_HIP(hipMemcpyAsync(hdst, dsrc, bytes, hipMemcpyDeviceToHost, b));
+                KITE_HIP(hipStreamSynchronize(a));
+                KITE_HIP(hipStreamSynchronize(b));
+                const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                const double gbps = (double)bytes * (mode == 3 ? 2 : 1) / sec / 1e9;
+                best[mode - 1] = std::max(best[mode - 1], gbps);
+            }
+        if (h2d_gbps) *h2d_gbps = best[0];
+        if (d2h_gbps) *d2h_gbps = best[1];
+        if (both_gbps) *both_gbps = best[2];
+        return KITE_OK;
+    });
 }
 
 int kite_gpu_host_alloc(uint64_t bytes, void **out) {

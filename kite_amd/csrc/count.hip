@@ -1478,8 +1478,12 @@ void CountEngine::collect_kernel_stats() {
 
 void CountEngine::release() {
     KITE_HIP(hipStreamSynchronize(stream_));
-    for (DevBuf *b : {&nl_, &line0_, &scan_tmp_, &hist_, &off_, &part_off_d_, &a_, &b_, &counts_tmp_, &out_keys_[0],
-                      &out_keys_[1], &out_counts_[0], &out_counts_[1], &segs_, &tile_pre_, &child_hist_, &child_cur_,
+    for (int q = 0; q < OUT_SLOTS; q++) {
+        out_keys_[q].release();
+        out_counts_[q].release();
+    }
+    for (DevBuf *b : {&nl_, &line0_, &scan_tmp_, &hist_, &off_, &part_off_d_, &a_, &b_, &counts_tmp_, &segs_,
+                      &tile_pre_, &child_hist_, &child_cur_,
                       &ucount_, &uoff_, &part_first_, &part_u_, &order_, &wg_hist_, &nodes_[0], &nodes_[1],
                       &npart_[0], &npart_[1], &act_, &tiles_, &act_rank_, &tile_off_, &lstats_, &seg_node_,
                       &nz_rank_, &nz_count_, &new_cnt_, &new_base_, &cls_, &cls_count_, &fallback_})

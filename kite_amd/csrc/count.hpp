@@ -74,9 +74,10 @@ class CountEngine {
     const uint64_t *d_part_off() const { return part_u_.as<uint64_t>(); }  // device copy of part_off()
     const uint64_t *keys() const { return out_keys_[slot_].as<uint64_t>(); }
     const uint32_t *counts() const { return out_counts_[slot_].as<uint32_t>(); }
-    // Two output slots let a copy stream drain one result while the next
-    // run writes the other.
-    void set_slot(int s) { slot_ = s & 1; }
+    // Several output slots let a copy stream drain earlier results while the
+    // next run writes another slot.
+    static constexpr int OUT_SLOTS = 4;
+    void set_slot(int s) { slot_ = s % OUT_SLOTS; }
     int slot() const { return slot_; }
     // Frees every device buffer (the next run allocates afresh).
     void release();
@@ -114,7 +115,7 @@ class CountEngine {
     std::vector<PendingEv> pending_;
     std::vector<hipEvent_t> ev_pool_;
     DevBuf nl_, line0_, scan_tmp_, hist_, off_, part_off_d_;
-    DevBuf a_, b_, counts_tmp_, out_keys_[2], out_counts_[2];
+    DevBuf a_, b_, counts_tmp_, out_keys_[OUT_SLOTS], out_counts_[OUT_SLOTS];
     int slot_ = 0;
     DevBuf segs_, tile_pre_, child_hist_, child_cur_, ucount_, uoff_, part_first_, part_u_, fallback_;
     DevBuf order_, wg_hist_;

@@ -125,3 +125,51 @@ def test_batch_encoded_chunks(gpu_ctx, mode):
                 assert [int(x) for x in mm[p, 4:4 + r.words]] == [int(x) for x in r.keys[b - 1]]
     for p in pins:
         p.free()
+
+
+@pytest.mark.parametrize("ring", ["2", "3", "4"])
+def test_batch_ring_depths_and_serial_mode(gpu_ctx, monkeypatch, ring):
+    """Every ring depth of the pipelined batch, with more samples than slots
+    (each slot reused several times, sizes up and down), and the serial mode
+    (persistent buffers, no overlap): raw tables and chunk payloads equal the
+    single-sample counts."""
+    from kite_amd import BATCH_SERIAL
+    monkeypatch.setenv("KITE_BATCH_RING", ring)
+    ws, pins = _samples(0.002)
+    arrs = [p.array for p in pins] + [pins[0].array[:0], pins[5].array]
+    single = [gpu_ctx.count(a, 31, True, 256)[0] for a in arrs]
+    for mode in (BATCH_PIPELINED, BATCH_SERIAL):
+        res = gpu_ctx.count_batch(arrs, 31, True, 256, mode=mode)
+        for i, (r, st) in enumerate(res):
+            _same(r, single[i], f"ring={ring} mode={mode} sample={i}")
+        enc = gpu_ctx.count_batch(arrs, 31, True, 256, mode=mode, encoded=True)
+        for i, (codec, data, boff, rows, mm, st) in enumerate(enc):
+            r = single[i]
+            assert np.array_equal(rows, r.part_off)
+            for p in (0, 17, 255):
+                a, b = int(r.part_off[p]), int(r.part_off[p + 1])
+                same = data[int(boff[p]):int(boff[p + 1])] == ol.dvle_encode(r.keys[a:b], r.counts[a:b], 1)
+                assert same, f"ring={ring} mode={mode} sample={i} partition={p}"
+    for p in pins:
+        p.free()
+
+
+def test_batch_stage_timeline_order(gpu_ctx):
+    """The per-sample stage events of a profiled pipelined batch follow the
+    pipeline's dependencies: each count starts after its own H2D ended, each
+    D2H after its count, and the D2H copies run in sample order."""
+    ws, pins = _samples(0.002)
+    arrs = [p.array for p in pins]
+    gpu_ctx.count_profile(False, timeline=True)
+    try:
+        gpu_ctx.count_batch(arrs, 31, True, 256, mode=BATCH_PIPELINED, encoded=True)
+        tl = gpu_ctx.batch_timeline()
+    finally:
+        gpu_ctx.count_profile(False)
+    assert tl.shape == (len(arrs), 6)
+    eps = 0.05  # ms: events of different streams around one dependency
+    for i, (h0, h1, r0, r1, d0, d1) in enumerate(tl):
+        assert 0 <= h0 <= h1 <= r0 + eps and r0 <= r1 and d0 <= d1, i
+        assert d0 >= r1 - eps, i
+        if i:
+            assert d0 >= tl[i - 1][5] - eps and h0 >= tl[i - 1][1] - eps, i
