@@ -229,7 +229,11 @@ int kite_gpu_union_tables(kite_gpu_ctx *ctx, uint32_t words, uint32_t partitions
  * 1-based average-tie ranks over all rows (average_ranks + pearson_slice,
  * :406-438, :559-578).  The Gram runs on bf16 MFMA with a hi/lo split (FP32
  * within a K-tile, FP64 across tiles); column means and variances, the
- * reduction and the normalisation are FP64.  Any output pointer may be NULL. */
+ * reduction and the normalisation are FP64.  Any output pointer may be NULL.
+ * Every count is read through its rank among its column's distinct counts
+ * (1-, 2- or 4-byte codes, by the most distinct counts in a column), so any
+ * count value works; 1..4096 samples (more than 32 as pairs of 16-column
+ * blocks through the same 32 x 32 Gram). */
 #define KITE_CORR_PEARSON 0x1u
 #define KITE_CORR_SPEARMAN 0x2u
 #define KITE_CORR_FP64_BASELINE 0x1u /* flags: also run the FP64 MFMA Gram */
@@ -255,7 +259,9 @@ int kite_gpu_corr_counts(kite_gpu_ctx *ctx, const uint32_t *counts, uint64_t N, 
 
 /* The same on the union of n_inputs tables (see kite_gpu_union_tables),
  * built and consumed on the device; out_keys / out_matrix / out_part_off
- * optionally return the union (for exact_count_matrix.tsv). */
+ * optionally return the union (for exact_count_matrix.tsv; out_matrix cells
+ * saturate at u32::MAX).  When an input count exceeds u32::MAX the union
+ * the correlation reads holds exact u64 counts. */
 int kite_gpu_analyze_tables(kite_gpu_ctx *ctx, uint32_t words, uint32_t partitions, uint32_t n_inputs,
                             const uint64_t *const *keys, const uint64_t *const *counts,
                             const uint64_t *const *part_off, uint32_t methods, uint32_t flags,
@@ -264,8 +270,9 @@ int kite_gpu_analyze_tables(kite_gpu_ctx *ctx, uint32_t words, uint32_t partitio
                             kite_corr_stats *stats);
 
 /* PCA of an n x n covariance (pca_scores, src/analyze/mod.rs:1605-1625):
- * FP64 cyclic Jacobi on the device, eigenvalues descending; scores
- * (n x dims) = eigenvector * sqrt(max(eigenvalue, 0)).  n <= 64. */
+ * FP64 cyclic Jacobi on the device (one wave for n <= 64, a round-robin
+ * parallel Jacobi on one workgroup above), eigenvalues descending; scores
+ * (n x dims) = eigenvector * sqrt(max(eigenvalue, 0)).  n <= 1024. */
 int kite_gpu_pca(kite_gpu_ctx *ctx, const double *cov, uint32_t n, uint32_t dims, double *scores,
                  double *eigenvalues);
 

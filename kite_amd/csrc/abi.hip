@@ -740,30 +740,22 @@ void copy_out(const std::vector<double> &v, double *dst) {
     if (dst && !v.empty()) std::copy(v.begin(), v.end(), dst);
 }
 
-int corr_device(kite_gpu_ctx *ctx, const uint32_t *d_counts, uint64_t N, uint32_t n, uint32_t methods,
+// d_cells: N x n count matrix on the device, u32 (cell_bytes 4) or u64 (8).
+int corr_device(kite_gpu_ctx *ctx, const void *d_cells, int cell_bytes, uint64_t N, uint32_t n, uint32_t methods,
                 uint32_t flags, double *pearson, double *spearman, double *cov, double *pearson_fp64,
                 double *spearman_fp64, kite_corr_stats *stats) {
-    if (n < 1 || n > 32) return set_error(KITE_E_INVALID, "invalid argument: analyze supports 1..32 samples on the GPU");
-    // maximum count decides the code width the Gram reads
-    const uint32_t maxc = ctx->corr->max_count(d_counts, N * n);
-    if (maxc > (64u << 20))
-        return set_error(KITE_E_INVALID, "counts above 2^26 are not supported by the correlation engine");
-    int cb = maxc < 65535 ? 2 : 4;
+    if (n < 1 || n > (uint32_t)kite::MAX_SAMPLES)
+        return set_error(KITE_E_INVALID, "invalid argument: analyze supports 1.." +
+                                             std::to_string(kite::MAX_SAMPLES) + " samples");
+    // rank codes: the Gram reads 1, 2 or 4 bytes a cell
     kite::DevBuf codes;
-    const void *dc = d_counts;
-    if (ctx->corr->dense_codes(d_counts, N, (int)n, maxc, codes)) {
-        cb = 1;  // every column has at most 256 distinct counts
-        dc = codes.ptr;
-    } else if (cb == 2) {
-        ctx->corr->to_codes(d_counts, N * n, codes, 2);
-        dc = codes.ptr;
-    }
+    const int cb = ctx->corr->prepare(d_cells, cell_bytes, N, (int)n, codes);
     kite::CorrRequest req;
     req.methods = methods;
     req.fp64_baseline = (flags & KITE_CORR_FP64_BASELINE) != 0;
     req.with_lolo = (flags & KITE_CORR_LOLO) != 0;
     kite::CorrOutput out;
-    ctx->corr->run(dc, cb, N, (int)n, maxc, req, out);
+    ctx->corr->run(codes.ptr, N, (int)n, req, out);
     copy_out(out.pearson, pearson);
     copy_out(out.spearman, spearman);
     copy_out(out.cov, cov);
@@ -796,7 +788,7 @@ int kite_gpu_corr_counts(kite_gpu_ctx *ctx, const uint32_t *counts, uint64_t N, 
         kite::DevBuf d;
         d.ensure(std::max<uint64_t>(N * n, 1) * 4);
         if (N) KITE_HIP(hipMemcpy(d.ptr, counts, N * n * 4, hipMemcpyHostToDevice));
-        return corr_device(ctx, d.as<uint32_t>(), N, n, methods, flags, pearson, spearman, cov, pearson_fp64,
+        return corr_device(ctx, d.ptr, 4, N, n, methods, flags, pearson, spearman, cov, pearson_fp64,
                            spearman_fp64, stats);
     });
 }
@@ -813,14 +805,26 @@ int kite_gpu_analyze_tables(kite_gpu_ctx *ctx, uint32_t words, uint32_t partitio
         std::vector<kite::DevTable> tabs;
         int rc = load_tables(ctx, words, partitions, n_inputs, keys, counts, part_off, tabs);
         if (rc) return rc;
+        // counts above u32::MAX keep their exact u64 value in the matrix the
+        // correlation reads (the reference sums in u64, kiteops/mod.rs:433-470)
+        const bool wide = ctx->merge->max_count(tabs) > 0xFFFFFFFFull;
         kite::DevTable u;
         ctx->merge->merge_all(tabs, partitions, (int)words, u);
         kite::DevBuf m;
-        ctx->merge->union_matrix(tabs, u, partitions, (int)words, m);
+        ctx->merge->union_matrix(tabs, u, partitions, (int)words, m, wide ? 8 : 4);
         tabs.clear();
         ctx->merge->download(u, (int)words, partitions, out_keys, nullptr, out_part_off);
-        if (out_matrix && u.rows) KITE_HIP(hipMemcpy(out_matrix, m.ptr, u.rows * n_inputs * 4, hipMemcpyDeviceToHost));
-        return corr_device(ctx, m.as<uint32_t>(), u.rows, n_inputs, methods, flags, pearson, spearman, cov,
+        if (out_matrix && u.rows) {
+            if (!wide) {
+                KITE_HIP(hipMemcpy(out_matrix, m.ptr, u.rows * n_inputs * 4, hipMemcpyDeviceToHost));
+            } else {
+                std::vector<uint64_t> h(u.rows * n_inputs);
+                KITE_HIP(hipMemcpy(h.data(), m.ptr, h.size() * 8, hipMemcpyDeviceToHost));
+                for (size_t i = 0; i < h.size(); i++)
+                    out_matrix[i] = h[i] > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)h[i];
+            }
+        }
+        return corr_device(ctx, m.ptr, wide ? 8 : 4, u.rows, n_inputs, methods, flags, pearson, spearman, cov,
                            pearson_fp64, spearman_fp64, stats);
     });
 }
@@ -828,7 +832,7 @@ int kite_gpu_analyze_tables(kite_gpu_ctx *ctx, uint32_t words, uint32_t partitio
 int kite_gpu_pca(kite_gpu_ctx *ctx, const double *cov, uint32_t n, uint32_t dims, double *scores,
                  double *eigenvalues) {
     if (!ctx || !cov) return set_error(KITE_E_INVALID, "invalid argument: NULL pointer");
-    if (n < 1 || n > 64) return set_error(KITE_E_INVALID, "invalid argument: PCA supports 1..64 samples");
+    if (n < 1 || n > 1024) return set_error(KITE_E_INVALID, "invalid argument: PCA supports 1..1024 samples");
     return guarded([&]() -> int {
         bind(ctx);
         std::vector<double> c(cov, cov + (size_t)n * n), ev, V;

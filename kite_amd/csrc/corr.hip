@@ -33,24 +33,111 @@ constexpr int MAX_COLS = 32;      // one 32x32 MFMA tile of columns
 constexpr int GT = 256;           // Gram workgroup threads (4 waves)
 constexpr int FLUSH_STEPS = 32;   // bf16 steps (16 rows each) per FP32 flush: 512 rows
 
+// ---- rank codes ------------------------------------------------------------
+// A cell's code is the rank of its count among the distinct counts of its
+// column.  Counts below `small` are ranked from a per-column histogram;
+// the (rare) larger ones from a sorted per-column list built on the host.
+
+// Largest cell of a u32 or u64 count matrix.
 template <class T>
-__global__ __launch_bounds__(256) void k_col_hist(const T *__restrict__ codes, uint64_t N, int n,
-                                                  uint32_t stride, uint32_t *__restrict__ hist) {
+__global__ void k_max_cells(const T *__restrict__ p, uint64_t n, unsigned long long *__restrict__ out) {
+    uint64_t m = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        m = m > (uint64_t)p[i] ? m : (uint64_t)p[i];
+    for (int d = 32; d > 0; d >>= 1) {
+        const uint64_t o = __shfl_xor(m, d, 64);
+        m = m > o ? m : o;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)m);
+}
+
+// Per-column histogram of the counts below `small` (blockIdx.y: a group of
+// MAX_COLS columns; counts below HIST_SMALL aggregated in LDS first), and
+// the number of cells at or above `small`.
+template <class T>
+__global__ __launch_bounds__(256) void k_rank_hist(const T *__restrict__ cells, uint64_t N, uint32_t n, uint32_t small,
+                                                   uint32_t *__restrict__ hist, unsigned long long *__restrict__ n_big) {
     __shared__ uint32_t sm[MAX_COLS * HIST_SMALL];
-    for (int i = threadIdx.x; i < n * HIST_SMALL; i += 256) sm[i] = 0;
+    __shared__ uint32_t sbig;
+    const uint32_t c0 = blockIdx.y * MAX_COLS;
+    const uint32_t nc = min((uint32_t)MAX_COLS, n - c0);
+    for (uint32_t i = threadIdx.x; i < (uint32_t)(MAX_COLS * HIST_SMALL); i += 256) sm[i] = 0;
+    if (threadIdx.x == 0) sbig = 0;
     __syncthreads();
-    const uint64_t total = N * (uint64_t)n;
+    const uint64_t total = N * (uint64_t)nc;
+    uint32_t big = 0;
     for (uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (uint64_t)gridDim.x * 256) {
-        uint32_t c = (uint32_t)codes[e];
-        uint32_t j = (uint32_t)(e % (uint64_t)n);
-        if (c < HIST_SMALL) atomicAdd(&sm[j * HIST_SMALL + c], 1u);
-        else atomicAdd(&hist[(uint64_t)j * stride + c], 1u);
+        const uint64_t r = e / nc;
+        const uint32_t j = (uint32_t)(e - r * nc);
+        const uint64_t v = (uint64_t)cells[r * n + c0 + j];
+        if (v < (uint64_t)HIST_SMALL && v < (uint64_t)small) atomicAdd(&sm[j * HIST_SMALL + (uint32_t)v], 1u);
+        else if (v < (uint64_t)small) atomicAdd(&hist[(uint64_t)(c0 + j) * small + v], 1u);
+        else big++;
     }
+    if (big) atomicAdd(&sbig, big);
     __syncthreads();
-    for (int i = threadIdx.x; i < n * HIST_SMALL; i += 256) {
-        uint32_t v = sm[i];
-        if (v) atomicAdd(&hist[(uint64_t)(i / HIST_SMALL) * stride + (i % HIST_SMALL)], v);
+    const uint32_t lim = min((uint32_t)HIST_SMALL, small);
+    for (uint32_t i = threadIdx.x; i < nc * HIST_SMALL; i += 256) {
+        const uint32_t j = i / HIST_SMALL, v = i % HIST_SMALL;
+        if (v < lim && sm[i]) atomicAdd(&hist[(uint64_t)(c0 + j) * small + v], sm[i]);
     }
+    if (threadIdx.x == 0 && sbig) atomicAdd(n_big, (unsigned long long)sbig);
+}
+
+// The cells at or above `small` as (column, count) pairs, in any order.
+struct BigCell {
+    uint64_t col, value;
+};
+
+template <class T>
+__global__ void k_rank_big(const T *__restrict__ cells, uint64_t cells_n, uint32_t n, uint32_t small,
+                           BigCell *__restrict__ out, uint64_t cap, unsigned long long *__restrict__ cursor) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cells_n) return;
+    const uint64_t v = (uint64_t)cells[i];
+    if (v < (uint64_t)small) return;
+    const uint64_t at = atomicAdd(cursor, 1ull);
+    if (at < cap) out[at] = BigCell{i % n, v};
+}
+
+// Rank codes: map[column][count] for counts below `small`; above, the
+// column's count of small distinct values plus the position in its sorted
+// list of large distinct values.
+template <class T, class C>
+__global__ void k_rank_codes(const T *__restrict__ cells, uint64_t cells_n, uint32_t n, uint32_t small,
+                             const uint32_t *__restrict__ map, const uint64_t *__restrict__ big_off,
+                             const uint64_t *__restrict__ big_vals, const uint32_t *__restrict__ big_base,
+                             C *__restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cells_n) return;
+    const uint32_t j = (uint32_t)(i % n);
+    const uint64_t v = (uint64_t)cells[i];
+    uint32_t code;
+    if (v < (uint64_t)small) {
+        code = map[(uint64_t)j * small + v];
+    } else {
+        uint64_t lo = big_off[j], hi = big_off[j + 1];
+        while (lo < hi) {
+            const uint64_t mid = lo + (hi - lo) / 2;
+            if (big_vals[mid] < v) lo = mid + 1;
+            else hi = mid;
+        }
+        code = big_base[j] + (uint32_t)(lo - big_off[j]);
+    }
+    out[i] = (C)code;
+}
+
+// A 32-column code matrix of up to two 16-column blocks of the n-column
+// one: out[r][c] = codes[r][cols[c]], or code 0 where cols[c] < 0 (a
+// padding column, whose value table is all zero).
+template <class C>
+__global__ void k_gather_cols(const C *__restrict__ codes, uint64_t N, uint32_t n, const int *__restrict__ cols,
+                              C *__restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N * MAX_COLS) return;
+    const uint64_t r = i / MAX_COLS;
+    const int c = cols[i % MAX_COLS];
+    out[i] = c >= 0 ? codes[r * n + (uint32_t)c] : (C)0;
 }
 
 __device__ __forceinline__ __bf16 bf16_from_bits(uint16_t b) {
@@ -415,31 +502,124 @@ __global__ __launch_bounds__(64) void k_jacobi(double *__restrict__ A, double *_
     }
 }
 
-__global__ void k_max_u32(const uint32_t *__restrict__ p, uint64_t n, uint32_t *__restrict__ out) {
-    uint32_t m = 0;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        m = std::max(m, p[i]);
-    for (int d = 32; d > 0; d >>= 1) m = std::max(m, (uint32_t)__shfl_xor((int)m, d, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(out, m);
-}
-
 __global__ void k_zero_cells(uint32_t *p, uint64_t n) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
 }
 
-// 8-bit codes: each count becomes the rank of its value among the distinct
-// values of its column (map: [column][stride] u8).
-__global__ void k_dense_codes(const uint32_t *__restrict__ in, uint64_t cells, uint32_t n,
-                              const uint8_t *__restrict__ map, uint32_t stride, uint8_t *__restrict__ out) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < cells) out[i] = map[(uint64_t)(i % n) * stride + in[i]];
-}
+// Round-robin (parallel) cyclic Jacobi for 64 < n <= JACOBI_RR_MAX on one
+// workgroup, A and V in device memory.  Each round rotates m / 2 disjoint
+// pairs (m = n rounded up to even; index n is a dummy when n is odd): the
+// rotation of pair (p, q) zeroes A[p][q] as the one-wave kernel's does, all
+// pairs' column rotations are applied (A J, V J), then all row rotations
+// (J^T A).  The pairing follows the circle method: index 0 stays, the
+// others shift by one place a round, so a sweep of m - 1 rounds meets
+// every pair once.
+constexpr int JACOBI_RR_MAX = 1024;
 
-template <class T>
-__global__ void k_u32_to_codes(const uint32_t *__restrict__ in, uint64_t n, T *__restrict__ out) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = (T)in[i];
+__global__ __launch_bounds__(1024) void k_jacobi_rr(double *__restrict__ A, double *__restrict__ V, int n,
+                                                    int max_sweeps, double *__restrict__ evals) {
+    __shared__ int ring[JACOBI_RR_MAX];
+    __shared__ int pp[JACOBI_RR_MAX / 2], qq[JACOBI_RR_MAX / 2];
+    __shared__ double cc[JACOBI_RR_MAX / 2], ss[JACOBI_RR_MAX / 2];
+    __shared__ double red[2][16];
+    __shared__ int stop;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int m = (n + 1) & ~1, half = m / 2;
+    for (int i = tid; i < n * n; i += nt) V[i] = (i / n == i % n) ? 1.0 : 0.0;
+    for (int i = tid; i < m; i += nt) ring[i] = i;
+    __syncthreads();
+    for (int sweep = 0; sweep < max_sweeps; sweep++) {
+        // convergence: off-diagonal against total squared mass
+        double off = 0, tot = 0;
+        for (int i = tid; i < n * n; i += nt) {
+            const double x = A[i] * A[i];
+            tot += x;
+            if (i / n != i % n) off += x;
+        }
+        for (int d = 32; d > 0; d >>= 1) {
+            off += __shfl_xor(off, d, 64);
+            tot += __shfl_xor(tot, d, 64);
+        }
+        if ((tid & 63) == 0) {
+            red[0][tid >> 6] = off;
+            red[1][tid >> 6] = tot;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double o = 0, t = 0;
+            for (int w = 0; w < (nt + 63) / 64; w++) {
+                o += red[0][w];
+                t += red[1][w];
+            }
+            stop = (o <= 1e-30 * t || o == 0.0) ? 1 : 0;
+        }
+        __syncthreads();
+        if (stop) break;
+        for (int round = 0; round < m - 1; round++) {
+            // pairs of this round and their rotations (from the current A)
+            for (int i = tid; i < half; i += nt) {
+                int p = ring[i], q = ring[m - 1 - i];
+                if (p > q) {
+                    const int t = p;
+                    p = q;
+                    q = t;
+                }
+                double c = 1.0, sn = 0.0;
+                if (q < n) {
+                    const double apq = A[p * n + q];
+                    if (apq != 0.0) {
+                        const double app = A[p * n + p], aqq = A[q * n + q];
+                        const double theta = (aqq - app) / (2.0 * apq);
+                        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                        c = 1.0 / sqrt(t * t + 1.0);
+                        sn = t * c;
+                    }
+                }
+                pp[i] = p;
+                qq[i] = q;
+                cc[i] = c;
+                ss[i] = sn;
+            }
+            __syncthreads();
+            // columns: A <- A J, V <- V J
+            for (int e = tid; e < half * n; e += nt) {
+                const int i = e / n, k = e - i * n, p = pp[i], q = qq[i];
+                if (q >= n || ss[i] == 0.0) continue;
+                const double c = cc[i], sn = ss[i];
+                const double akp = A[k * n + p], akq = A[k * n + q];
+                A[k * n + p] = c * akp - sn * akq;
+                A[k * n + q] = sn * akp + c * akq;
+                const double vkp = V[k * n + p], vkq = V[k * n + q];
+                V[k * n + p] = c * vkp - sn * vkq;
+                V[k * n + q] = sn * vkp + c * vkq;
+            }
+            __syncthreads();
+            // rows: A <- J^T A
+            for (int e = tid; e < half * n; e += nt) {
+                const int i = e / n, k = e - i * n, p = pp[i], q = qq[i];
+                if (q >= n || ss[i] == 0.0) continue;
+                const double c = cc[i], sn = ss[i];
+                const double apk = A[p * n + k], aqk = A[q * n + k];
+                A[p * n + k] = c * apk - sn * aqk;
+                A[q * n + k] = sn * apk + c * aqk;
+            }
+            __syncthreads();
+            // the pair's off-diagonal entries are zero by construction
+            for (int i = tid; i < half; i += nt)
+                if (qq[i] < n && ss[i] != 0.0) {
+                    A[pp[i] * n + qq[i]] = 0.0;
+                    A[qq[i] * n + pp[i]] = 0.0;
+                }
+            // circle method: slots 1..m-1 shift by one
+            int nv = 0;
+            if (tid < m && tid > 0) nv = ring[tid == 1 ? m - 1 : tid - 1];
+            __syncthreads();
+            if (tid < m && tid > 0) ring[tid] = nv;
+            __syncthreads();
+        }
+    }
+    for (int k = tid; k < n; k += nt) evals[k] = A[k * n + k];
 }
 
 // ---- host -----------------------------------------------------------------
@@ -458,42 +638,167 @@ float bf16_to_float(uint16_t b) {
 }
 }  // namespace
 
-void CorrEngine::column_stats(const void *d_codes, int code_bytes, uint64_t N, int n, uint32_t maxc) {
-    const uint32_t stride = std::max<uint32_t>(maxc + 1, HIST_SMALL);
-    stride_ = stride;
-    hist_.ensure((uint64_t)n * stride * 4);
-    {
-        const uint64_t cells = (uint64_t)n * stride;
-        k_zero_cells<<<(uint32_t)((cells + 255) / 256), 256, 0, stream_>>>(hist_.as<uint32_t>(), cells);
-        KITE_CHECK_LAUNCH();
-    }
-    const uint64_t total = N * (uint64_t)n;
-    const uint32_t grid = (uint32_t)std::min<uint64_t>(4096, std::max<uint64_t>(1, (total + 255) / 256));
-    if (total) {
-        if (code_bytes == 2)
-            k_col_hist<uint16_t><<<grid, 256, 0, stream_>>>((const uint16_t *)d_codes, N, n, stride, hist_.as<uint32_t>());
+int CorrEngine::prepare(const void *d_cells, int cell_bytes, uint64_t N, int n, DevBuf &codes) {
+    if (n < 1 || n > MAX_SAMPLES)
+        throw Error("invalid argument: sample count must be 1.." + std::to_string(MAX_SAMPLES));
+    if (cell_bytes != 4 && cell_bytes != 8) throw Error("internal: count cells must be u32 or u64");
+    const uint64_t cells = N * (uint64_t)n;
+    auto grid_of = [](uint64_t work) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(4096, (work + 255) / 256)); };
+    // largest count: sizes the histogram of the small counts
+    DevBuf mx;
+    mx.ensure(8);
+    k_zero_cells<<<1, 64, 0, stream_>>>(mx.as<uint32_t>(), 2);
+    KITE_CHECK_LAUNCH();
+    if (cells) {
+        if (cell_bytes == 4)
+            k_max_cells<uint32_t><<<grid_of(cells), 256, 0, stream_>>>((const uint32_t *)d_cells, cells,
+                                                                       mx.as<unsigned long long>());
         else
-            k_col_hist<uint32_t><<<grid, 256, 0, stream_>>>((const uint32_t *)d_codes, N, n, stride, hist_.as<uint32_t>());
+            k_max_cells<uint64_t><<<grid_of(cells), 256, 0, stream_>>>((const uint64_t *)d_cells, cells,
+                                                                       mx.as<unsigned long long>());
         KITE_CHECK_LAUNCH();
     }
-    hist_h_.resize((uint64_t)n * stride);
-    KITE_HIP(hipMemcpyAsync(hist_h_.data(), hist_.ptr, hist_h_.size() * 4, hipMemcpyDeviceToHost, stream_));
+    uint64_t maxv = 0;
+    KITE_HIP(hipMemcpyAsync(&maxv, mx.ptr, 8, hipMemcpyDeviceToHost, stream_));
     KITE_HIP(hipStreamSynchronize(stream_));
+    // counts below `small` have a histogram slot: at most 2^16 per column and
+    // 2^26 slots in all; larger counts go through a list
+    const uint64_t slot_cap = std::max<uint64_t>(HIST_SMALL, (1ull << 26) / (uint64_t)n);
+    const uint32_t small = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({maxv + 1, 65536ull, slot_cap}));
+    const uint64_t hslots = (uint64_t)n * small;
+    hist_.ensure(hslots * 4 + 16);
+    unsigned long long *d_nbig = reinterpret_cast<unsigned long long *>(hist_.as<uint32_t>() + hslots + (hslots & 1));
+    k_zero_cells<<<(uint32_t)((hslots + 2 + 255) / 256), 256, 0, stream_>>>(hist_.as<uint32_t>(), hslots + 2 + (hslots & 1));
+    KITE_CHECK_LAUNCH();
+    if (cells) {
+        const dim3 g(grid_of(N * (uint64_t)std::min(n, MAX_COLS)), (unsigned)((n + MAX_COLS - 1) / MAX_COLS));
+        if (cell_bytes == 4)
+            k_rank_hist<uint32_t><<<g, 256, 0, stream_>>>((const uint32_t *)d_cells, N, (uint32_t)n, small,
+                                                         hist_.as<uint32_t>(), d_nbig);
+        else
+            k_rank_hist<uint64_t><<<g, 256, 0, stream_>>>((const uint64_t *)d_cells, N, (uint32_t)n, small,
+                                                         hist_.as<uint32_t>(), d_nbig);
+        KITE_CHECK_LAUNCH();
+    }
+    std::vector<uint32_t> hsmall(hslots);
+    uint64_t nbig = 0;
+    KITE_HIP(hipMemcpyAsync(hsmall.data(), hist_.ptr, hslots * 4, hipMemcpyDeviceToHost, stream_));
+    KITE_HIP(hipMemcpyAsync(&nbig, d_nbig, 8, hipMemcpyDeviceToHost, stream_));
+    KITE_HIP(hipStreamSynchronize(stream_));
+    // the large counts, sorted by (column, count)
+    std::vector<BigCell> big(nbig);
+    if (nbig) {
+        DevBuf lst;
+        lst.ensure(nbig * sizeof(BigCell) + 8);
+        unsigned long long *cur = reinterpret_cast<unsigned long long *>(lst.as<uint8_t>() + nbig * sizeof(BigCell));
+        k_zero_cells<<<1, 64, 0, stream_>>>(reinterpret_cast<uint32_t *>(cur), 2);
+        KITE_CHECK_LAUNCH();
+        const uint32_t g = (uint32_t)((cells + 255) / 256);
+        if (cell_bytes == 4)
+            k_rank_big<uint32_t><<<g, 256, 0, stream_>>>((const uint32_t *)d_cells, cells, (uint32_t)n, small,
+                                                        lst.as<BigCell>(), nbig, cur);
+        else
+            k_rank_big<uint64_t><<<g, 256, 0, stream_>>>((const uint64_t *)d_cells, cells, (uint32_t)n, small,
+                                                        lst.as<BigCell>(), nbig, cur);
+        KITE_CHECK_LAUNCH();
+        KITE_HIP(hipMemcpyAsync(big.data(), lst.ptr, nbig * sizeof(BigCell), hipMemcpyDeviceToHost, stream_));
+        KITE_HIP(hipStreamSynchronize(stream_));
+        std::sort(big.begin(), big.end(), [](const BigCell &x, const BigCell &y) {
+            return x.col != y.col ? x.col < y.col : x.value < y.value;
+        });
+    }
+    // distinct counts of every column, ascending, with multiplicities
+    std::vector<std::vector<uint64_t>> vals(n), mult(n);
+    std::vector<uint32_t> map(hslots, 0), big_base(n, 0);
+    std::vector<uint64_t> big_off(n + 1, 0), big_vals;
+    size_t bi = 0;
+    for (int j = 0; j < n; j++) {
+        const uint32_t *h = hsmall.data() + (uint64_t)j * small;
+        for (uint32_t v = 0; v < small; v++)
+            if (h[v]) {
+                map[(uint64_t)j * small + v] = (uint32_t)vals[j].size();
+                vals[j].push_back(v);
+                mult[j].push_back(h[v]);
+            }
+        big_base[j] = (uint32_t)vals[j].size();
+        big_off[j] = big_vals.size();
+        for (; bi < big.size() && big[bi].col == (uint64_t)j; bi++) {
+            if (vals[j].size() > big_base[j] && vals[j].back() == big[bi].value) {
+                mult[j].back()++;
+            } else {
+                big_vals.push_back(big[bi].value);
+                vals[j].push_back(big[bi].value);
+                mult[j].push_back(1);
+            }
+        }
+    }
+    big_off[n] = big_vals.size();
+    uint64_t D = 1;
+    for (int j = 0; j < n; j++) D = std::max<uint64_t>(D, vals[j].size());
+    if (D > 0xFFFFFFFFull) throw Error("internal: more than 2^32 distinct counts in one column");
+    const int cb = D <= 256 ? 1 : (D <= 65536 ? 2 : 4);
+    const uint32_t stride = (uint32_t)std::max<uint64_t>(D, HIST_SMALL);
+    stride_ = stride;
+    code_bytes_ = cb;
+    hist_h_.assign((uint64_t)n * stride, 0);
+    dense_vals_.assign((uint64_t)n * stride, 0);
+    for (int j = 0; j < n; j++)
+        for (size_t c = 0; c < vals[j].size(); c++) {
+            hist_h_[(uint64_t)j * stride + c] = mult[j][c];
+            dense_vals_[(uint64_t)j * stride + c] = vals[j][c];
+        }
+    // codes on the device
+    DevBuf dmap, dboff, dbvals, dbbase;
+    dmap.ensure(hslots * 4);
+    dboff.ensure((n + 1) * 8);
+    dbvals.ensure(std::max<size_t>(big_vals.size(), 1) * 8);
+    dbbase.ensure(n * 4);
+    KITE_HIP(hipMemcpyAsync(dmap.ptr, map.data(), hslots * 4, hipMemcpyHostToDevice, stream_));
+    KITE_HIP(hipMemcpyAsync(dboff.ptr, big_off.data(), (n + 1) * 8, hipMemcpyHostToDevice, stream_));
+    if (!big_vals.empty())
+        KITE_HIP(hipMemcpyAsync(dbvals.ptr, big_vals.data(), big_vals.size() * 8, hipMemcpyHostToDevice, stream_));
+    KITE_HIP(hipMemcpyAsync(dbbase.ptr, big_base.data(), n * 4, hipMemcpyHostToDevice, stream_));
+    codes.ensure(std::max<uint64_t>(cells, 1) * cb);
+    if (cells) {
+        const uint32_t g = (uint32_t)((cells + 255) / 256);
+        auto launch = [&](auto cell_tag, auto code_tag) {
+            using T = decltype(cell_tag);
+            using Cd = decltype(code_tag);
+            k_rank_codes<T, Cd><<<g, 256, 0, stream_>>>((const T *)d_cells, cells, (uint32_t)n, small,
+                                                        dmap.as<uint32_t>(), dboff.as<uint64_t>(),
+                                                        dbvals.as<uint64_t>(), dbbase.as<uint32_t>(), codes.as<Cd>());
+        };
+        if (cell_bytes == 4) {
+            if (cb == 1) launch(uint32_t{}, uint8_t{});
+            else if (cb == 2) launch(uint32_t{}, uint16_t{});
+            else launch(uint32_t{}, uint32_t{});
+        } else {
+            if (cb == 1) launch(uint64_t{}, uint8_t{});
+            else if (cb == 2) launch(uint64_t{}, uint16_t{});
+            else launch(uint64_t{}, uint32_t{});
+        }
+        KITE_CHECK_LAUNCH();
+    }
+    KITE_HIP(hipStreamSynchronize(stream_));  // host tables above are locals
+    return cb;
 }
 
-// Standardized value per (column, count): z = (v - mean) / sd, sd over N.
+// Standardized value per (column, code): z = (v - mean) / sd, sd over N.
+// Codes are the ranks of the column's distinct counts, so the value of a
+// code is ln(1 + count) (Pearson) or the 1-based average-tie rank of the
+// count (Spearman): the codes before it cover the smaller counts.
 void CorrEngine::make_lut(int method, uint64_t N, int n, std::vector<double> &z, std::vector<double> &sd) {
     const uint32_t stride = stride_;
     z.assign((uint64_t)n * stride, 0.0);
     sd.assign(n, 0.0);
     for (int j = 0; j < n; j++) {
-        const uint32_t *h = hist_h_.data() + (uint64_t)j * stride;
+        const uint64_t *h = hist_h_.data() + (uint64_t)j * stride;
         std::vector<double> val(stride, 0.0);
         double cum = 0;
         for (uint32_t c = 0; c < stride; c++) {
-            if (method == 0) val[c] = std::log1p((double)(dense_ ? dense_vals_[(uint64_t)j * stride + c] : c));
+            if (method == 0) val[c] = std::log1p((double)dense_vals_[(uint64_t)j * stride + c]);
             else val[c] = cum + ((double)h[c] + 1.0) / 2.0;  // average tie rank, 1-based
-            cum += h[c];
+            cum += (double)h[c];
         }
         double mean = 0;
         for (uint32_t c = 0; c < stride; c++) mean += (double)h[c] * val[c];
@@ -665,80 +970,171 @@ void CorrEngine::gram(const void *d_codes, int code_bytes, uint64_t N, int n,
     KITE_HIP(hipEventDestroy(e1));
 }
 
-void CorrEngine::run(const void *d_codes, int code_bytes, uint64_t N, int n, uint32_t maxc, const CorrRequest &req,
-                     CorrOutput &out) {
-    if (n < 1 || n > MAX_COLS) throw Error("invalid argument: sample count must be 1.." + std::to_string(MAX_COLS));
-    if (code_bytes == 1) {
-        if (!dense_) throw Error("internal: 8-bit codes without their value table");
-    } else {
-        dense_ = false;
-        column_stats(d_codes, code_bytes, N, n, maxc);
-    }
+void CorrEngine::run(const void *d_codes, uint64_t N, int n, const CorrRequest &req, CorrOutput &out) {
+    if (n < 1 || n > MAX_SAMPLES)
+        throw Error("invalid argument: sample count must be 1.." + std::to_string(MAX_SAMPLES));
+    const int code_bytes = code_bytes_;
     // the requested methods share one pass over the codes
     std::vector<int> methods;
     for (int method = 0; method < 2; method++)
         if (req.methods & (1u << method)) methods.push_back(method);
     if (methods.empty()) return;
     std::vector<std::vector<double>> z(methods.size()), sd(methods.size());
-    std::vector<const std::vector<double> *> zp;
-    for (size_t q = 0; q < methods.size(); q++) {
-        make_lut(methods[q], N, n, z[q], sd[q]);
-        zp.push_back(&z[q]);
-    }
-    std::vector<std::vector<double>> G, G64;
-    float ms = 0, ms64 = 0;
-    gram(d_codes, code_bytes, N, n, zp, false, req.with_lolo, G, &ms);
-    std::vector<float> ms64_each(methods.size(), 0.f);
-    if (req.fp64_baseline) {
-        // the FP64 Gram is bound by FP64 MFMA throughput, not by reading the
-        // codes: one pass per table is its faster form (measured), so the
-        // baseline takes that form
-        G64.resize(methods.size());
-        for (size_t q = 0; q < methods.size(); q++) {
-            std::vector<std::vector<double>> g1;
-            gram(d_codes, code_bytes, N, n, {zp[q]}, true, false, g1, &ms64_each[q]);
-            G64[q] = std::move(g1[0]);
-            ms64 += ms64_each[q];
+    for (size_t q = 0; q < methods.size(); q++) make_lut(methods[q], N, n, z[q], sd[q]);
+    // Gram of every requested table, n x n.  Up to MAX_COLS columns in one
+    // 32-column tile; more as pairs of 16-column blocks, each pair's codes
+    // gathered into a 32-column matrix for the same kernels.
+    const size_t NQ = methods.size();
+    std::vector<std::vector<double>> G(NQ, std::vector<double>((size_t)n * n, 0.0)), G64;
+    std::vector<float> ms64_each(NQ, 0.f);
+    float ms = 0;
+    int passes = 0;
+    auto tile_gram = [&](const void *dc, const std::vector<int> &cols, bool fp64, int q_only, float *tms) {
+        // cols: the tile's columns of the n-column matrix (-1: padding)
+        const int tn = (int)cols.size();
+        std::vector<std::vector<double>> tz;
+        std::vector<size_t> qs;
+        for (size_t q = 0; q < NQ; q++)
+            if (q_only < 0 || (int)q == q_only) qs.push_back(q);
+        for (size_t q : qs) {
+            std::vector<double> t((size_t)tn * stride_, 0.0);
+            for (int c = 0; c < tn; c++)
+                if (cols[c] >= 0)
+                    std::copy(z[q].begin() + (size_t)cols[c] * stride_, z[q].begin() + (size_t)(cols[c] + 1) * stride_,
+                              t.begin() + (size_t)c * stride_);
+            tz.push_back(std::move(t));
         }
+        std::vector<const std::vector<double> *> zp;
+        for (auto &t : tz) zp.push_back(&t);
+        std::vector<std::vector<double>> Gt;
+        gram(dc, code_bytes, N, tn, zp, fp64, req.with_lolo && !fp64, Gt, tms);
+        return std::make_pair(qs, Gt);
+    };
+    auto scatter = [&](std::vector<std::vector<double>> &dst, const std::vector<size_t> &qs,
+                       const std::vector<std::vector<double>> &Gt, const std::vector<int> &cols,
+                       const std::vector<char> &take) {
+        const int tn = (int)cols.size();
+        for (size_t w = 0; w < qs.size(); w++)
+            for (int i = 0; i < tn; i++)
+                for (int j = 0; j < tn; j++)
+                    if (cols[i] >= 0 && cols[j] >= 0 && take[(size_t)i * tn + j])
+                        dst[qs[w]][(size_t)cols[i] * n + cols[j]] = Gt[w][(size_t)i * MAX_COLS + j];
+    };
+    if (req.fp64_baseline) G64.assign(NQ, std::vector<double>((size_t)n * n, 0.0));
+    if (n <= MAX_COLS) {
+        std::vector<int> cols(n);
+        for (int c = 0; c < n; c++) cols[c] = c;
+        std::vector<char> all((size_t)n * n, 1);
+        auto r = tile_gram(d_codes, cols, false, -1, &ms);
+        scatter(G, r.first, r.second, cols, all);
+        passes = 1;
+        if (req.fp64_baseline)
+            // the FP64 Gram is bound by FP64 MFMA throughput, not by reading
+            // the codes: one pass per table is its faster form (measured)
+            for (size_t q = 0; q < NQ; q++) {
+                auto r64 = tile_gram(d_codes, cols, true, (int)q, &ms64_each[q]);
+                scatter(G64, r64.first, r64.second, cols, all);
+            }
+    } else {
+        const int HB = MAX_COLS / 2, nb = (n + HB - 1) / HB;
+        DevBuf tmp, dcols;
+        tmp.ensure(std::max<uint64_t>(N, 1) * MAX_COLS * code_bytes);
+        dcols.ensure(MAX_COLS * sizeof(int));
+        std::vector<char> diag_done(nb, 0);
+        for (int a = 0; a < nb; a++)
+            for (int b = a + 1; b < nb; b++) {
+                std::vector<int> cols(MAX_COLS, -1);
+                for (int c = 0; c < HB; c++) {
+                    if (a * HB + c < n) cols[c] = a * HB + c;
+                    if (b * HB + c < n) cols[HB + c] = b * HB + c;
+                }
+                // this pair gives the (a, b) block; the diagonal blocks come
+                // from the first pair that holds them
+                std::vector<char> take((size_t)MAX_COLS * MAX_COLS, 0);
+                for (int i = 0; i < MAX_COLS; i++)
+                    for (int j = 0; j < MAX_COLS; j++) {
+                        const bool ia = i < HB, ja = j < HB;
+                        if (ia != ja) take[(size_t)i * MAX_COLS + j] = 1;
+                        else if (ia && !diag_done[a]) take[(size_t)i * MAX_COLS + j] = 1;
+                        else if (!ia && !diag_done[b]) take[(size_t)i * MAX_COLS + j] = 1;
+                    }
+                KITE_HIP(hipMemcpyAsync(dcols.ptr, cols.data(), MAX_COLS * sizeof(int), hipMemcpyHostToDevice,
+                                        stream_));
+                const uint64_t cells = N * MAX_COLS;
+                if (cells) {
+                    const uint32_t g = (uint32_t)((cells + 255) / 256);
+                    if (code_bytes == 1)
+                        k_gather_cols<uint8_t><<<g, 256, 0, stream_>>>((const uint8_t *)d_codes, N, (uint32_t)n,
+                                                                       dcols.as<int>(), tmp.as<uint8_t>());
+                    else if (code_bytes == 2)
+                        k_gather_cols<uint16_t><<<g, 256, 0, stream_>>>((const uint16_t *)d_codes, N, (uint32_t)n,
+                                                                        dcols.as<int>(), tmp.as<uint16_t>());
+                    else
+                        k_gather_cols<uint32_t><<<g, 256, 0, stream_>>>((const uint32_t *)d_codes, N, (uint32_t)n,
+                                                                        dcols.as<int>(), tmp.as<uint32_t>());
+                    KITE_CHECK_LAUNCH();
+                }
+                float t = 0;
+                auto r = tile_gram(tmp.ptr, cols, false, -1, &t);
+                ms += t;
+                scatter(G, r.first, r.second, cols, take);
+                passes++;
+                if (req.fp64_baseline)
+                    for (size_t q = 0; q < NQ; q++) {
+                        float t64 = 0;
+                        auto r64 = tile_gram(tmp.ptr, cols, true, (int)q, &t64);
+                        ms64_each[q] += t64;
+                        scatter(G64, r64.first, r64.second, cols, take);
+                    }
+                diag_done[a] = diag_done[b] = 1;
+            }
     }
+    float ms64 = 0;
+    for (float x : ms64_each) ms64 += x;
     out.gram_ms_total = ms;
     out.fp64_ms_total = ms64;
-    out.passes = 1;
-    out.fp64_passes = req.fp64_baseline ? (int)methods.size() : 0;
+    out.passes = passes;
+    out.fp64_passes = req.fp64_baseline ? (int)NQ * passes : 0;
     auto corr_of = [&](const std::vector<double> &g, std::vector<double> &r) {
         r.assign((size_t)n * n, 0.0);
         for (int i = 0; i < n; i++)
             for (int j = 0; j < n; j++) {
-                double den = std::sqrt(std::max(0.0, g[i * MAX_COLS + i]) * std::max(0.0, g[j * MAX_COLS + j]));
-                r[i * n + j] = den > 0 ? g[i * MAX_COLS + j] / den : 0.0;
+                double den = std::sqrt(std::max(0.0, g[(size_t)i * n + i]) * std::max(0.0, g[(size_t)j * n + j]));
+                r[(size_t)i * n + j] = den > 0 ? g[(size_t)i * n + j] / den : 0.0;
             }
     };
-    for (size_t q = 0; q < methods.size(); q++) {
+    for (size_t q = 0; q < NQ; q++) {
         const int method = methods[q];
         corr_of(G[q], method == 0 ? out.pearson : out.spearman);
         if (req.fp64_baseline) corr_of(G64[q], method == 0 ? out.pearson_fp64 : out.spearman_fp64);
         // one pass: its time is shared by the methods it computed
-        (method == 0 ? out.gram_ms_pearson : out.gram_ms_spearman) = ms / (float)methods.size();
+        (method == 0 ? out.gram_ms_pearson : out.gram_ms_spearman) = ms / (float)NQ;
         (method == 0 ? out.fp64_ms_pearson : out.fp64_ms_spearman) = ms64_each[q];
         if (method == 0) {
-            // covariance of log1p values (N - 1 divisor) from the standardized Gram
+            // covariance of log1p values (N - 1 divisor): r_ij sigma_i sigma_j N / (N - 1),
+            // with the FP64 column deviations and r from the Gram (its diagonal
+            // is then the exact FP64 variance)
             out.cov.assign((size_t)n * n, 0.0);
             if (N > 1)
                 for (int i = 0; i < n; i++)
                     for (int j = 0; j < n; j++)
-                        out.cov[i * n + j] = G[q][i * MAX_COLS + j] * sd[q][i] * sd[q][j] / (double)(N - 1);
+                        out.cov[(size_t)i * n + j] =
+                            out.pearson[(size_t)i * n + j] * sd[q][i] * sd[q][j] * (double)N / (double)(N - 1);
         }
     }
 }
 
 void jacobi_device(double *A, double *V, double *evals, int n, hipStream_t s) {
-    if (n < 1 || n > 64) throw Error("invalid argument: PCA supports 1..64 samples");
-    k_jacobi<<<1, 64, 0, s>>>(A, V, n, JACOBI_SWEEPS, evals);
+    if (n < 1 || n > JACOBI_RR_MAX)
+        throw Error("invalid argument: PCA supports 1.." + std::to_string(JACOBI_RR_MAX) + " samples");
+    if (n <= 64) k_jacobi<<<1, 64, 0, s>>>(A, V, n, JACOBI_SWEEPS, evals);
+    else k_jacobi_rr<<<1, 1024, 0, s>>>(A, V, n, JACOBI_SWEEPS, evals);
     KITE_CHECK_LAUNCH();
 }
 
 void CorrEngine::pca(const std::vector<double> &cov, int n, std::vector<double> &evals, std::vector<double> &evecs) {
-    if (n < 1 || n > 64) throw Error("invalid argument: PCA supports 1..64 samples");
+    if (n < 1 || n > JACOBI_RR_MAX)
+        throw Error("invalid argument: PCA supports 1.." + std::to_string(JACOBI_RR_MAX) + " samples");
     DevBuf a, v, e;
     a.ensure((size_t)n * n * 8);
     v.ensure((size_t)n * n * 8);
@@ -750,71 +1146,6 @@ void CorrEngine::pca(const std::vector<double> &cov, int n, std::vector<double> 
     KITE_HIP(hipMemcpyAsync(evals.data(), e.ptr, (size_t)n * 8, hipMemcpyDeviceToHost, stream_));
     KITE_HIP(hipMemcpyAsync(evecs.data(), v.ptr, (size_t)n * n * 8, hipMemcpyDeviceToHost, stream_));
     KITE_HIP(hipStreamSynchronize(stream_));
-}
-
-uint32_t CorrEngine::max_count(const uint32_t *d_counts, uint64_t cells) {
-    DevBuf m;
-    m.ensure(4);
-    k_zero_cells<<<1, 64, 0, stream_>>>(m.as<uint32_t>(), 1);
-    KITE_CHECK_LAUNCH();
-    if (cells) {
-        uint32_t g = (uint32_t)std::min<uint64_t>(2048, (cells + 255) / 256);
-        k_max_u32<<<g, 256, 0, stream_>>>(d_counts, cells, m.as<uint32_t>());
-        KITE_CHECK_LAUNCH();
-    }
-    uint32_t h = 0;
-    KITE_HIP(hipMemcpyAsync(&h, m.ptr, 4, hipMemcpyDeviceToHost, stream_));
-    KITE_HIP(hipStreamSynchronize(stream_));
-    return h;
-}
-
-bool CorrEngine::dense_codes(const uint32_t *d_counts, uint64_t N, int n, uint32_t maxc, DevBuf &codes) {
-    dense_ = false;
-    if (maxc >= (1u << 20) || n < 1 || n > MAX_COLS) return false;
-    const uint64_t cells = N * (uint64_t)n;
-    // histogram of the raw counts (the value table needs it anyway)
-    column_stats(d_counts, 4, N, n, maxc);
-    const uint32_t stride0 = stride_;
-    std::vector<uint8_t> map((uint64_t)n * stride0, 0);
-    std::vector<uint32_t> vals((uint64_t)n * 256, 0), hcode((uint64_t)n * 256, 0);
-    for (int j = 0; j < n; j++) {
-        const uint32_t *h = hist_h_.data() + (uint64_t)j * stride0;
-        uint32_t code = 0;
-        for (uint32_t v = 0; v < stride0; v++) {
-            if (!h[v]) continue;
-            if (code == 256) return false;  // more than 256 distinct counts in this column
-            map[(uint64_t)j * stride0 + v] = (uint8_t)code;
-            vals[(uint64_t)j * 256 + code] = v;
-            hcode[(uint64_t)j * 256 + code] = h[v];
-            code++;
-        }
-    }
-    DevBuf dmap;
-    dmap.ensure(map.size());
-    KITE_HIP(hipMemcpyAsync(dmap.ptr, map.data(), map.size(), hipMemcpyHostToDevice, stream_));
-    codes.ensure(std::max<uint64_t>(cells, 1));
-    if (cells) {
-        k_dense_codes<<<(uint32_t)((cells + 255) / 256), 256, 0, stream_>>>(d_counts, cells, (uint32_t)n,
-                                                                            dmap.as<uint8_t>(), stride0,
-                                                                            codes.as<uint8_t>());
-        KITE_CHECK_LAUNCH();
-    }
-    KITE_HIP(hipStreamSynchronize(stream_));  // `map` is a local vector
-    // from here on the value table is indexed by code
-    hist_h_.swap(hcode);
-    dense_vals_.swap(vals);
-    stride_ = 256;
-    dense_ = true;
-    return true;
-}
-
-void CorrEngine::to_codes(const uint32_t *d_counts, uint64_t cells, DevBuf &codes, int code_bytes) {
-    codes.ensure(std::max<uint64_t>(cells, 1) * code_bytes);
-    if (!cells) return;
-    const uint32_t g = (uint32_t)((cells + 255) / 256);
-    if (code_bytes == 2) k_u32_to_codes<uint16_t><<<g, 256, 0, stream_>>>(d_counts, cells, codes.as<uint16_t>());
-    else k_u32_to_codes<uint32_t><<<g, 256, 0, stream_>>>(d_counts, cells, codes.as<uint32_t>());
-    KITE_CHECK_LAUNCH();
 }
 
 }  // namespace kite

@@ -33,36 +33,36 @@ constexpr int JACOBI_SWEEPS = 60;
 // V (n x n), eigenvalues (unsorted) into evals.  Queued on stream s.
 void jacobi_device(double *A, double *V, double *evals, int n, hipStream_t s);
 
+// Columns (samples) the correlation takes.
+constexpr int MAX_SAMPLES = 4096;
+
 class CorrEngine {
    public:
     explicit CorrEngine(hipStream_t s) : stream_(s) {}
-    // d_codes: N x n row-major count values (u16 or u32), all <= maxc, or
-    // 8-bit codes from dense_codes() (code_bytes 1).
-    void run(const void *d_codes, int code_bytes, uint64_t N, int n, uint32_t maxc, const CorrRequest &req,
-             CorrOutput &out);
+    // Rank codes of a count matrix (N x n row major, u32 or u64 cells,
+    // device): every cell becomes the rank of its count among the distinct
+    // counts of its column, 1, 2 or 4 bytes wide (returned); the value
+    // tables behind the codes are kept for run().
+    int prepare(const void *d_cells, int cell_bytes, uint64_t N, int n, DevBuf &codes);
+    // Correlation of the codes prepare() made.
+    void run(const void *d_codes, uint64_t N, int n, const CorrRequest &req, CorrOutput &out);
     void pca(const std::vector<double> &cov, int n, std::vector<double> &evals, std::vector<double> &evecs);
-    void to_codes(const uint32_t *d_counts, uint64_t cells, DevBuf &codes, int code_bytes);
-    // 8-bit codes when every column has at most 256 distinct counts: code =
-    // rank of the count among its column's values (exact).  Returns false
-    // (and leaves codes alone) otherwise; run() then takes code_bytes 1.
-    bool dense_codes(const uint32_t *d_counts, uint64_t N, int n, uint32_t maxc, DevBuf &codes);
-    uint32_t max_count(const uint32_t *d_counts, uint64_t cells);
 
    private:
-    void column_stats(const void *d_codes, int code_bytes, uint64_t N, int n, uint32_t maxc);
     void make_lut(int method, uint64_t N, int n, std::vector<double> &z, std::vector<double> &sd);
-    // One pass over the codes computes the Gram of every table in zs (1 or 2).
+    // One pass over the codes (n <= 32 columns) computes the Gram of every
+    // table in zs (1 or 2); Gs come back in a 32 x 32 layout.
     void gram(const void *d_codes, int code_bytes, uint64_t N, int n, const std::vector<const std::vector<double> *> &zs,
               bool fp64, bool with_lolo, std::vector<std::vector<double>> &Gs, float *ms);
 
     void reduce_partials(uint64_t waves, uint32_t per_wave);
 
     hipStream_t stream_;
-    uint32_t stride_ = 0;
+    uint32_t stride_ = 0;   // value-table entries per column (>= 256)
+    int code_bytes_ = 1;
     DevBuf hist_, lut_, partial_, sum_, stage_;
-    std::vector<uint32_t> hist_h_;
-    bool dense_ = false;                 // tables are indexed by 8-bit codes
-    std::vector<uint32_t> dense_vals_;   // [column][code] -> count value
+    std::vector<uint64_t> hist_h_;      // [column][code] -> multiplicity
+    std::vector<uint64_t> dense_vals_;  // [column][code] -> count value
 };
 
 }  // namespace kite

@@ -119,7 +119,8 @@ struct PlaceArgs {
     const uint64_t *ik, *ic, *ioff;  // one input table
     uint64_t rows;
     uint32_t P, col, ncols;
-    uint32_t *matrix;            // union rows x ncols, u32 saturated
+    void *matrix;                // union rows x ncols: u32 saturated, or u64 when wide
+    uint32_t wide;
 };
 
 template <int W>
@@ -140,7 +141,19 @@ __global__ void k_union_place(PlaceArgs a) {
         else h = mid;
     }
     uint64_t c = a.ic[r];
-    a.matrix[l * a.ncols + a.col] = c > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)c;
+    if (a.wide) static_cast<uint64_t *>(a.matrix)[l * a.ncols + a.col] = c;
+    else static_cast<uint32_t *>(a.matrix)[l * a.ncols + a.col] = c > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)c;
+}
+
+__global__ void k_max_u64(const uint64_t *__restrict__ p, uint64_t n, unsigned long long *__restrict__ out) {
+    uint64_t m = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        m = m > p[i] ? m : p[i];
+    for (int d = 32; d > 0; d >>= 1) {
+        const uint64_t o = __shfl_xor(m, d, 64);
+        m = m > o ? m : o;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)m);
 }
 
 __global__ void k_zero_u32(uint32_t *p, uint64_t n) {
@@ -255,20 +268,39 @@ void MergeEngine::merge_all(std::vector<DevTable> &tabs, uint32_t P, int W, DevT
     }
 }
 
+uint64_t MergeEngine::max_count(const std::vector<DevTable> &tabs) {
+    DevBuf m;
+    m.ensure(8);
+    k_zero_u32<<<1, 64, 0, stream_>>>(m.as<uint32_t>(), 2);
+    KITE_CHECK_LAUNCH();
+    for (const DevTable &t : tabs)
+        if (t.rows) {
+            k_max_u64<<<(uint32_t)std::min<uint64_t>(2048, blocks(t.rows, 256)), 256, 0, stream_>>>(
+                t.counts.as<uint64_t>(), t.rows, m.as<unsigned long long>());
+            KITE_CHECK_LAUNCH();
+        }
+    uint64_t h = 0;
+    KITE_HIP(hipMemcpyAsync(&h, m.ptr, 8, hipMemcpyDeviceToHost, stream_));
+    KITE_HIP(hipStreamSynchronize(stream_));
+    return h;
+}
+
 void MergeEngine::union_matrix(const std::vector<DevTable> &tabs, const DevTable &u, uint32_t P, int W,
-                               DevBuf &matrix) {
+                               DevBuf &matrix, int cell_bytes) {
     const uint32_t nc = (uint32_t)tabs.size();
     const uint64_t cells = u.rows * nc;
-    matrix.ensure(std::max<uint64_t>(cells, 1) * 4);
+    const bool wide = cell_bytes == 8;
+    matrix.ensure(std::max<uint64_t>(cells, 1) * (wide ? 8 : 4));
     if (cells) {
-        k_zero_u32<<<blocks(cells, 256), 256, 0, stream_>>>(matrix.as<uint32_t>(), cells);
+        const uint64_t words = cells * (wide ? 2 : 1);
+        k_zero_u32<<<blocks(words, 256), 256, 0, stream_>>>(matrix.as<uint32_t>(), words);
         KITE_CHECK_LAUNCH();
     }
     for (uint32_t c = 0; c < nc; c++) {
         const DevTable &t = tabs[c];
         if (!t.rows) continue;
         PlaceArgs pa{u.keys.as<uint64_t>(), u.off.as<uint64_t>(), t.keys.as<uint64_t>(), t.counts.as<uint64_t>(),
-                     t.off.as<uint64_t>(), t.rows, P, c, nc, matrix.as<uint32_t>()};
+                     t.off.as<uint64_t>(), t.rows, P, c, nc, matrix.ptr, wide ? 1u : 0u};
         if (W == 1) k_union_place<1><<<blocks(t.rows, 256), 256, 0, stream_>>>(pa);
         else if (W == 2) k_union_place<2><<<blocks(t.rows, 256), 256, 0, stream_>>>(pa);
         else k_union_place<4><<<blocks(t.rows, 256), 256, 0, stream_>>>(pa);

@@ -28,8 +28,12 @@ class MergeEngine {
     void upload_table(const HostTable &t, DevTable &d, int W);
     // Sorted union of all tables with u64-saturating count sums.
     void merge_all(std::vector<DevTable> &tabs, uint32_t P, int W, DevTable &result);
-    // u32-saturated count matrix (union rows x tables) of the union `u`.
-    void union_matrix(const std::vector<DevTable> &tabs, const DevTable &u, uint32_t P, int W, DevBuf &matrix);
+    // Count matrix (union rows x tables) of the union `u`: u32 cells
+    // saturated at u32::MAX (cell_bytes 4), or exact u64 cells (8).
+    void union_matrix(const std::vector<DevTable> &tabs, const DevTable &u, uint32_t P, int W, DevBuf &matrix,
+                      int cell_bytes = 4);
+    // Largest count over the tables.
+    uint64_t max_count(const std::vector<DevTable> &tabs);
     void download(const DevTable &t, int W, uint32_t P, uint64_t *keys, uint64_t *counts, uint64_t *part_off);
 
     template <int W>

@@ -131,8 +131,21 @@ int run_analyze(int argc, char **argv) {
                                 spear.data(), cov.data(), p64.data(), s64.data(), &st))
         throw Error(std::string("GPU analyze failed: ") + kite_gpu_last_error());
     const uint64_t N = uo[P];
-    for (auto &v : K) std::vector<uint64_t>().swap(v);
-    for (auto &v : C) std::vector<uint64_t>().swap(v);
+    // a cell the u32 matrix saturates prints the input's exact u64 sum
+    // (src/kiteops/mod.rs:433-470 sums in u64): keep the loaded rows only
+    // when some input has such a count
+    bool wide = false;
+    for (auto &c : C)
+        for (uint64_t x : c) wide |= x > 0xFFFFFFFFull;
+    if (!wide) {
+        for (auto &v : K) std::vector<uint64_t>().swap(v);
+        for (auto &v : C) std::vector<uint64_t>().swap(v);
+    }
+    auto cell = [&](uint64_t r, uint32_t p, size_t j) -> uint64_t {
+        const uint32_t c = um[r * n + j];
+        if (c != 0xFFFFFFFFu || !wide) return c;
+        return exact_count(K[j], C[j], O[j][p], O[j][p + 1], &uk[r * W], W);
+    };
     fprintf(stderr, "[kite analyze] union rows=%llu, samples=%zu, gram %.3f ms (all methods, one pass), codes=%u B\n",
             (unsigned long long)N, n, st.gram_ms_total, st.code_bytes);
     if (flags) {
@@ -164,14 +177,16 @@ int run_analyze(int argc, char **argv) {
                 if (s >= nslab) return;
                 std::string &b = buf[s];
                 uint64_t r0 = s * slab, r1 = std::min(N, r0 + slab);
-                b.resize((r1 - r0) * (k + 1 + n * 11));
+                b.resize((r1 - r0) * (k + 1 + n * 21));
                 char *w = b.data();
+                uint32_t p = (uint32_t)(std::upper_bound(uo.begin(), uo.end(), r0) - uo.begin()) - 1;
                 for (uint64_t r = r0; r < r1; r++) {
+                    while (r >= uo[p + 1]) p++;
                     Key key;
                     for (int q = 0; q < W; q++) key.w[q] = uk[r * W + q];
                     decode_key(key, W, k, w);
                     w += k;
-                    for (size_t j = 0; j < n; j++) w += sprintf(w, "\t%u", um[r * n + j]);
+                    for (size_t j = 0; j < n; j++) w += sprintf(w, "\t%llu", (unsigned long long)cell(r, p, j));
                     *w++ = '\n';
                 }
                 b.resize((size_t)(w - b.data()));
@@ -194,8 +209,10 @@ int run_analyze(int argc, char **argv) {
         fprintf(o, "sample\ttotal_raw_counts\tnonzero_kmers\n");
         for (size_t j = 0; j < n; j++) {
             uint64_t tot = 0, nz = 0;
+            uint32_t p = 0;
             for (uint64_t r = 0; r < N; r++) {
-                uint32_t c = um[r * n + j];
+                while (r >= uo[p + 1]) p++;
+                const uint64_t c = cell(r, p, j);
                 uint64_t s = tot + c;
                 tot = s < tot ? ~0ull : s;
                 nz += c > 0;
@@ -210,9 +227,11 @@ int run_analyze(int argc, char **argv) {
         const uint32_t dims = (uint32_t)std::min<size_t>(n, 10);
         std::vector<double> sc(n * std::max<uint32_t>(dims, 2), 0.0), ev(n);
         const uint32_t d2 = std::max<uint32_t>(dims, 2);
-        if (n <= 64) {
+        if (n <= 1024) {
             if (kite_gpu_pca(ctx, cov.data(), (uint32_t)n, d2, sc.data(), ev.data()))
                 throw Error(std::string("GPU PCA failed: ") + kite_gpu_last_error());
+        } else {
+            fprintf(stderr, "[kite analyze] warning: PCA is computed for up to 1024 samples; embedding left at 0\n");
         }
         FILE *o = fopen((out_dir + "/embedding_pca.tsv").c_str(), "wb");
         if (!o) throw Error("failed to create embedding_pca.tsv");

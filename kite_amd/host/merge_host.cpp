@@ -142,4 +142,23 @@ MergeStats merge_files(const std::vector<std::unique_ptr<Reader>> &rs, Writer &w
     return ms;
 }
 
+// Exact count of `key` in rows [lo, hi) of one input's sorted table.
+uint64_t exact_count(const std::vector<uint64_t> &K, const std::vector<uint64_t> &C, uint64_t lo, uint64_t hi,
+                     const uint64_t *key, int W) {
+    auto less = [&](const uint64_t *a, const uint64_t *b) {
+        for (int q = W - 1; q >= 0; q--)
+            if (a[q] != b[q]) return a[q] < b[q];
+        return false;
+    };
+    while (lo < hi) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        if (less(&K[mid * W], key))
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    if (lo < C.size() && !less(key, &K[lo * W]) && !less(&K[lo * W], key)) return C[lo];
+    throw Error("saturated union cell has no matching input row");
+}
+
 }  // namespace kh

@@ -22,4 +22,10 @@ MergeStats merge_files(const std::vector<std::unique_ptr<Reader>> &rs, Writer &w
 void load_filtered(const Reader &r, uint32_t p0, uint32_t p1, unsigned threads, std::vector<uint64_t> &keys,
                    std::vector<uint64_t> &counts, std::vector<uint64_t> &off);
 
+// Exact u64 count of `key` in rows [lo, hi) of one input's loaded table (a
+// union cell saturated at u32::MAX is re-read from here).  Throws when the
+// key is absent.
+uint64_t exact_count(const std::vector<uint64_t> &K, const std::vector<uint64_t> &C, uint64_t lo, uint64_t hi,
+                     const uint64_t *key, int W);
+
 }  // namespace kh

@@ -22,25 +22,6 @@
 namespace kh {
 
 namespace {
-// Exact count of `key` in rows [lo, hi) of one input's sorted table.
-uint64_t exact_count(const std::vector<uint64_t> &K, const std::vector<uint64_t> &C, uint64_t lo, uint64_t hi,
-                     const uint64_t *key, int W) {
-    auto less = [&](const uint64_t *a, const uint64_t *b) {
-        for (int q = W - 1; q >= 0; q--)
-            if (a[q] != b[q]) return a[q] < b[q];
-        return false;
-    };
-    while (lo < hi) {
-        const uint64_t mid = lo + (hi - lo) / 2;
-        if (less(&K[mid * W], key))
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    if (lo < C.size() && !less(key, &K[lo * W]) && !less(&K[lo * W], key)) return C[lo];
-    throw Error("profile: saturated union cell has no matching input row");
-}
-
 std::string fmt_filter(const Header &h) {
     auto f = [](const std::optional<uint32_t> &v) { return v ? std::to_string(*v) : std::string("none"); };
     return "min=" + f(h.min_count) + ", max=" + f(h.max_count);
