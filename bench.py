@@ -132,6 +132,32 @@ def pcie_peak(ctx) -> dict:
     return out
 
 
+def size_class(nbytes: int) -> int:
+    """The engine's input size classes for tuned settings (below 64 MiB,
+    512 MiB, 4 GiB, above)."""
+    return 0 if nbytes < 64 << 20 else (1 if nbytes < 512 << 20 else (2 if nbytes < 4 << 30 else 3))
+
+
+def autotune_cohort(ctx, arrs, names, k: int, canon: bool, P: int) -> dict:
+    """kite_gpu_autotune on the largest sample of every size class in the
+    cohort; the context keeps the choices for the batch that follows."""
+    by_cls: dict[int, list[int]] = {}
+    for i, a in enumerate(arrs):
+        by_cls.setdefault(size_class(a.size), []).append(i)
+    out = {}
+    for cls, idx in sorted(by_cls.items()):
+        i = max(idx, key=lambda j: arrs[j].size)
+        t = time.perf_counter()
+        rep = ctx.autotune(arrs[i], k, canon, P)
+        rep["sample"] = names[i]
+        rep["samples_in_class"] = len(idx)
+        rep["seconds"] = time.perf_counter() - t
+        log(f"autotune class {cls} on {names[i]} ({arrs[i].size / 1e9:.2f} GB): {rep['chosen']} "
+            f"{rep['default_ms']:.2f} -> {rep['tuned_ms']:.2f} ms in {rep['trials']} trials ({rep['seconds']:.1f} s)")
+        out[str(cls)] = rep
+    return out
+
+
 def bench_batch24(ctx, pins, steps: int, warmup: int, plain_reps: int) -> tuple[dict, list, dict, int]:
     """The headline: the 24-sample cohort from pinned host memory through
     the pipelined batch (H2D of the FASTQ, count, DVLE chunk encode on the
@@ -524,11 +550,16 @@ class DeviceBytes:
             self.ptr = None
 
 
-def bench_single(ctx, w, steps: int, warmup: int) -> tuple[dict, np.ndarray]:
+def bench_single(ctx, w, steps: int, warmup: int, autotune: bool) -> tuple[dict, np.ndarray]:
     """One sample resident in HBM (the FASTQ already on the device, the
     tables left there): the count pipeline's own rate and per-kernel
-    rooflines, without PCIe."""
+    rooflines, without PCIe.  Autotuned on its own sample first (its genome
+    differs from the cohort's)."""
     fq = w.fastq()
+    tune = None
+    if autotune:
+        tune = ctx.autotune(fq, w.k, w.canonical, w.partitions)
+        log(f"autotune {w.name}: {tune['chosen']} {tune['default_ms']:.2f} -> {tune['tuned_ms']:.2f} ms")
     dev = DeviceBytes(fq)
     ptr, nbytes = dev.ptr, dev.nbytes
 
@@ -560,6 +591,7 @@ def bench_single(ctx, w, steps: int, warmup: int) -> tuple[dict, np.ndarray]:
            "reads": w.n_reads, "fastq_bytes": int(nbytes), "input_kmers": int(kmers),
            "unique_kmers": int(stats[-1].unique), "steps": steps, "ms_per_step": wall / steps * 1e3,
            "kmers_per_s": rate, "roofline": roofline_of(kern, w.name), "kernels": kernel_table(kern, steps),
+           "autotune": tune,
            "includes": "FASTQ resident in HBM; count kernels only; results stay in HBM"}
     return out, fq
 
@@ -596,6 +628,7 @@ def main() -> int:
     ap.add_argument("--cpu-samples", default="0,1,2", help="cohort samples the CPU baseline counts")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="count with the built-in default settings")
     ap.add_argument("--no-analyze", action="store_true")
     ap.add_argument("--no-single", action="store_true", help="skip the HBM-resident single-sample leg")
     ap.add_argument("--no-dump", action="store_true", help="skip the kite dump leg")
@@ -621,6 +654,9 @@ def main() -> int:
         return 1
     pcie = pcie_peak(ctx)
     ws, pins = make_batch24(args.batch_scale)
+    tune = None
+    if not args.no_autotune:
+        tune = autotune_cohort(ctx, [p.array for p in pins], [w_.name for w_ in ws], 31, True, 256)
     batch, tables, kern, kmers = bench_batch24(ctx, pins, args.steps, args.warmup, args.plain_reps)
     wall = batch["wall_s"]
     value = kmers * args.steps / wall
@@ -657,7 +693,7 @@ def main() -> int:
     single = fq = None
     w = WORKLOADS[args.single_workload]
     if not args.no_single:
-        single, fq = bench_single(ctx, w, args.single_steps, 3)
+        single, fq = bench_single(ctx, w, args.single_steps, 3, not args.no_autotune)
     if not args.no_dump:
         fq = w.fastq() if fq is None else fq
         single = single or {}
@@ -689,7 +725,10 @@ def main() -> int:
                    "unique_kmers_total": batch["unique_kmers_total"], "chunk_bytes": batch["chunk_bytes"],
                    "step": "all 24 samples from pinned host memory: H2D of the FASTQ, count, DVLE chunk "
                            "encode on the device, D2H of the chunks (the .kite payloads)",
-                   "batch_scale": args.batch_scale, "parallelism": "single GPU"},
+                   "batch_scale": args.batch_scale, "parallelism": "single GPU",
+                   "autotune": ({c: {"chosen": r["chosen"], "default_ms": r["default_ms"], "tuned_ms": r["tuned_ms"],
+                                     "sample": r["sample"], "trials": r["trials"]} for c, r in tune.items()}
+                                if tune else "off (built-in defaults)")},
         "batched_vs_plain": batch["batched_vs_plain"],
         "plain": batch["plain"],
         "batched_vs_serial": batch["batched_vs_serial"],
@@ -697,6 +736,7 @@ def main() -> int:
         "pcie": pcie_line,
         "kernels": kernels,
         "batch24": batch,
+        "autotune": tune,
         "analyze": analyze,
         "single_sample_hbm": single,
         "cpu_baseline": cpu,

@@ -190,6 +190,26 @@ int kite_gpu_count_batch_chunks(kite_gpu_ctx *ctx, uint32_t k, int canonical, ui
                                 const kite_batch_sample *samples, uint32_t n_samples, int mode,
                                 kite_batch_chunks_fn fn, void *user);
 
+/* Autotuner of the count's own settings: times the count of the sample
+ * fastq[0, len) (host memory, whole records) under candidate settings --
+ * the MSD split target (1, 2, 4, 8: children split until they hold at most
+ * the largest LDS sort bucket / target), the maximum radix digit width of
+ * a split level (8, 10, 11, 12 bits), how many sort size classes use the
+ * hash-then-bin kernel (2, 3, 4) and the extract tile (1, 2, 4, 8 sub-tiles
+ * per workgroup) -- one setting at a time with the others held, the best of
+ * three device timings each, a change kept when it wins by 1%.  Every
+ * setting must give the same k-mer and unique counts.  The winner is used
+ * by every later count on this context with the same key width and
+ * partition count and an input of the same size class (below 64 MiB,
+ * 512 MiB, 4 GiB, above).  report (may be NULL) receives a JSON summary:
+ * the chosen settings, default and tuned device ms, and every trial.
+ * Counterpart of the reference's run-time planner and k-aware sort choice
+ * (derive_heuristic_plan, src/count/mod.rs:624-864; radix_sort_u64_kmer,
+ * src/count/sorter.rs:67-93).  A count session's device result is first
+ * kept as encoded runs. */
+int kite_gpu_autotune(kite_gpu_ctx *ctx, uint32_t k, int canonical, uint32_t partitions, const uint8_t *fastq,
+                      size_t len, char *report, size_t cap);
+
 /* Host-link copy rates of this device: page-locked buffers of `bytes`,
  * H2D alone, D2H alone, and both at once on two streams (GB/s, best of 3).
  * The bound of the batch above, whose H2D and D2H run at once.  No

@@ -149,6 +149,8 @@ GPU_SYMBOLS = {
                                 C.c_void_p, C.POINTER(DumpStats)]),
     "kite_gpu_histo": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(DumpRun), C.c_uint64, C.c_int,
                                  C.c_uint32, C.c_int, C.c_uint32, HISTO_FN, C.c_void_p, C.POINTER(DumpStats)]),
+    "kite_gpu_autotune": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_uint32, C.c_void_p, C.c_size_t,
+                                    C.c_char_p, C.c_size_t]),
     "kite_gpu_copy_peak": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                      C.POINTER(C.c_double)]),
     "kite_gpu_host_alloc": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
@@ -601,6 +603,16 @@ class GpuContext:
         n = self._lib.kite_gpu_count_kernel_stats(self._h, buf, len(buf))
         if n < 0:
             _check(n)
+        return json.loads(buf.value.decode())
+
+    def autotune(self, fastq: np.ndarray, k: int, canonical: bool, partitions: int) -> dict:
+        """kite_gpu_autotune on one sample: the chosen count settings, kept
+        on this context for inputs of the sample's size class."""
+        import json
+        arr = np.ascontiguousarray(fastq, dtype=np.uint8)
+        buf = C.create_string_buffer(1 << 16)
+        _check(self._lib.kite_gpu_autotune(self._h, k, int(bool(canonical)), partitions, arr.ctypes.data, arr.size,
+                                           buf, len(buf)))
         return json.loads(buf.value.decode())
 
     def copy_peak(self, nbytes: int = 1 << 30) -> dict:

@@ -8,10 +8,12 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 #include "../../include/kite_gpu.h"
@@ -174,6 +176,9 @@ struct kite_gpu_ctx {
     std::vector<std::array<hipEvent_t, 6>> tl_events;  // grows, reused across batches
     hipEvent_t tl_start = nullptr;
     std::vector<double> tl_ms;  // 6 per sample of the last profiled batch
+    // count settings chosen by kite_gpu_autotune, per (key words, partitions,
+    // input size class)
+    std::map<std::tuple<int, uint32_t, int>, kite::CountTunables> tuned;
 
     // dump: staged payloads of a partition group, pinned text pieces
     kite::HostPinned dump_stage, dump_text;
@@ -190,6 +195,18 @@ struct kite_gpu_ctx {
 namespace {
 
 void bind(kite_gpu_ctx *c) { KITE_HIP(hipSetDevice(c->device)); }
+
+// Input size classes of the tuned settings: below 64 MiB, 512 MiB, 4 GiB, above.
+int size_class(uint64_t len) {
+    return len < (64ull << 20) ? 0 : (len < (512ull << 20) ? 1 : (len < (4ull << 30) ? 2 : 3));
+}
+
+// The settings a count of `len` input bytes starts from: the autotuner's
+// choice for this key width, partition count and size class, if any.
+void apply_tuned(kite_gpu_ctx *c, uint64_t len) {
+    auto it = c->tuned.find(std::make_tuple(c->words, c->partitions, size_class(len)));
+    c->count->set_base_tunables(it != c->tuned.end() ? it->second : kite::CountTunables{});
+}
 
 // Keep the staged FASTQ bytes when the staging buffer grows.
 void staging_reserve(kite_gpu_ctx *c, uint64_t need) {
@@ -224,6 +241,7 @@ void run_batch(kite_gpu_ctx *c, const uint8_t *ptr, uint64_t len) {
     stash_device_result(c);
     c->kstats_from_batch = false;
     c->count->set_timing(true);
+    apply_tuned(c, len);
     c->count->run(ptr, len, (int)c->k, c->canonical, c->partitions);
     c->device_result = true;
     const kite::StageTimes &t = c->count->times();
@@ -1022,6 +1040,7 @@ void batch_plain(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, cons
         c->count->set_slot(0);
         c->count->set_timing(true);
         try {
+            apply_tuned(c, smp[i].len);
             c->count->run(static_cast<const uint8_t *>(d), smp[i].len, (int)c->k, c->canonical, c->partitions);
         } catch (...) {
             if (d && !reuse) (void)hipFree(d);
@@ -1199,6 +1218,7 @@ void batch_pipelined(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, 
             if (prof) KITE_HIP(hipEventRecord(c->tl_events[i][2], c->stream));
             c->count->set_slot(s);
             c->count->set_timing(true);
+            apply_tuned(c, smp[i].len);
             c->count->run(c->batch_in[s].as<uint8_t>(), smp[i].len, (int)c->k, c->canonical, c->partitions);
             add_batch_kstats(c);
             // slot s's host buffers are free once the callback of sample i - R
@@ -1310,6 +1330,97 @@ int kite_gpu_count_batch_chunks(kite_gpu_ctx *ctx, uint32_t k, int canonical, ui
     sink.chunks = fn;
     sink.user = user;
     return count_batch_impl(ctx, k, canonical, partitions, samples, n_samples, mode, sink);
+}
+
+int kite_gpu_autotune(kite_gpu_ctx *ctx, uint32_t k, int canonical, uint32_t partitions, const uint8_t *fastq,
+                      size_t len, char *report, size_t cap) {
+    if (!ctx) return set_error(KITE_E_INVALID, "invalid argument: ctx is NULL");
+    if (k < 1 || k > 128) return set_error(KITE_E_INVALID, "invalid k-mer length: k must be between 1 and 128");
+    if (partitions == 0 || (partitions & (partitions - 1)))
+        return set_error(KITE_E_INVALID, "invalid partition count: must be a power of 2");
+    if (!fastq || len == 0) return set_error(KITE_E_INVALID, "invalid argument: empty sample");
+    return guarded([&]() -> int {
+        bind(ctx);
+        // a count session's device result is kept as encoded runs first
+        if (ctx->session) stash_device_result(ctx);
+        const int W = k <= 31 ? 1 : (k <= 64 ? 2 : 4);
+        kite::DevBuf d;
+        d.ensure(len);
+        KITE_HIP(hipMemcpy(d.ptr, fastq, len, hipMemcpyHostToDevice));
+        // the defaults, spelled out (the engine resolves -1 by key width)
+        kite::CountTunables best;
+        best.split_target_div = W == 1 ? 2 : 4;
+        best.hash_classes = W == 1 ? 4 : 3;
+        best.max_digit_bits = 12;
+        best.extract_subs = 4;
+        uint64_t ref_unique = 0, ref_kmers = 0;
+        int trials = 0;
+        std::string log;
+        auto time_of = [&](const kite::CountTunables &t) -> double {
+            ctx->count->set_base_tunables(t);
+            ctx->count->set_timing(true);
+            double m = 1e300;
+            for (int rep = 0; rep < 3; rep++) {
+                ctx->count->run(d.as<uint8_t>(), len, (int)k, canonical ? 1 : 0, partitions);
+                if (!trials && !rep) {
+                    ref_unique = ctx->count->n_unique();
+                    ref_kmers = ctx->count->n_kmers();
+                }
+                if (ctx->count->n_unique() != ref_unique || ctx->count->n_kmers() != ref_kmers)
+                    throw kite::Error("autotune: a setting changed the count result");
+                m = std::min<double>(m, ctx->count->times().total_ms);
+            }
+            trials++;
+            char buf[160];
+            snprintf(buf, sizeof buf, "%s[%d, %d, %d, %d, %.4f]", log.empty() ? "" : ", ", t.split_target_div,
+                     t.max_digit_bits, t.hash_classes, t.extract_subs, m);
+            log += buf;
+            return m;
+        };
+        // coordinate descent: one knob at a time over its candidates, the
+        // others held; a change must win by 1% to count as one
+        const double default_ms = time_of(best);
+        double best_ms = default_ms;
+        for (int pass = 0; pass < 2; pass++) {
+            bool changed = false;
+            auto try_knob = [&](int kite::CountTunables::*field, std::initializer_list<int> cands) {
+                for (int v : cands) {
+                    if (best.*field == v) continue;
+                    kite::CountTunables t = best;
+                    t.*field = v;
+                    const double m = time_of(t);
+                    if (m < best_ms * 0.99) {
+                        best = t;
+                        best_ms = m;
+                        changed = true;
+                    }
+                }
+            };
+            try_knob(&kite::CountTunables::split_target_div, {1, 2, 4, 8});
+            try_knob(&kite::CountTunables::max_digit_bits, {8, 10, 11, 12});
+            try_knob(&kite::CountTunables::hash_classes, {2, 3, 4});
+            try_knob(&kite::CountTunables::extract_subs, {1, 2, 4, 8});
+            if (!changed) break;
+        }
+        const int cls = size_class(len);
+        ctx->tuned[std::make_tuple(W, partitions, cls)] = best;
+        ctx->count->set_base_tunables(kite::CountTunables{});
+        if (report) {
+            char head[512];
+            snprintf(head, sizeof head,
+                     "{\"words\": %d, \"partitions\": %u, \"size_class\": %d, \"input_bytes\": %llu, "
+                     "\"default_ms\": %.4f, \"tuned_ms\": %.4f, \"chosen\": {\"split_target_div\": %d, "
+                     "\"max_digit_bits\": %d, \"hash_classes\": %d, \"extract_subs\": %d}, \"trials\": %d, "
+                     "\"trial_columns\": [\"split_target_div\", \"max_digit_bits\", \"hash_classes\", "
+                     "\"extract_subs\", \"ms\"], \"trial_log\": [",
+                     W, partitions, cls, (unsigned long long)len, default_ms, best_ms, best.split_target_div,
+                     best.max_digit_bits, best.hash_classes, best.extract_subs, trials);
+            const std::string js = std::string(head) + log + "]}";
+            if (cap <= js.size()) return set_error(KITE_E_INVALID, "report buffer too small");
+            memcpy(report, js.c_str(), js.size() + 1);
+        }
+        return KITE_OK;
+    });
 }
 
 int kite_gpu_copy_peak(kite_gpu_ctx *ctx, uint64_t bytes, double *h2d_gbps, double *d2h_gbps, double *both_gbps) {

@@ -1531,7 +1531,7 @@ void CountEngine::run(const uint8_t *d_fq, uint64_t len, int k, int canonical, u
     int pbits = __builtin_ctz(partitions);
     words_ = k <= 31 ? 1 : (k <= 64 ? 2 : 4);
     // tunables: built-in defaults, overridden per run by the environment
-    tun = CountTunables{};
+    tun = base_;
     if (const char *e = getenv("KITE_SPLIT_DIV")) tun.split_target_div = std::max(1, atoi(e));
     if (const char *e = getenv("KITE_SORT_DEDUP")) tun.dedup = atoi(e) != 0;
     if (const char *e = getenv("KITE_MAX_DBITS")) tun.max_digit_bits = std::min(MAX_DBITS, std::max(1, atoi(e)));

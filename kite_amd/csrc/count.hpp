@@ -95,6 +95,10 @@ class CountEngine {
     // and `fastq` input bytes (used by the batch planner).
     static uint64_t bytes_needed(uint64_t n, int words, uint64_t fastq, uint32_t partitions);
 
+    // Settings the next runs start from (an autotuner's choice, or the
+    // built-in defaults); KITE_* environment overrides still apply on top.
+    void set_base_tunables(const CountTunables &t) { base_ = t; }
+
    private:
     template <int W>
     void run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical, int pbits);
@@ -105,6 +109,7 @@ class CountEngine {
     void collect_kernel_stats();
 
     hipStream_t stream_;
+    CountTunables base_;
     bool timing_ = false;
     bool kprof_ = false;
     std::vector<KernelStat> kstats_;

@@ -279,3 +279,20 @@ def test_hash_kernel_on_large_classes(gpu_ctx, monkeypatch, classes, div):
         fq = w.fastq(n_reads=n)
         res, tab, _ = run_both(gpu_ctx, fq, w.k, w.canonical, P)
         check_same(res, tab, f"{w.name} hash classes {classes} split div {div}")
+
+
+def test_autotune_choice_keeps_parity(gpu_ctx):
+    """kite_gpu_autotune times its candidate settings on a sample; later
+    counts of that size class on the context use the choice, and must still
+    give the oracle's table (one-word and two-word keys)."""
+    from kite_amd.workloads import K31_CANON_50MBP, K63_CANON_100MBP
+    from kite_amd import GpuContext
+    with GpuContext(0) as ctx:  # its own context: the tuned settings stay with it
+        for w, n in ((K31_CANON_50MBP, 150000), (K63_CANON_100MBP, 40000)):
+            fq = w.fastq(n_reads=n)
+            rep = ctx.autotune(fq, w.k, w.canonical, w.partitions)
+            assert rep["words"] == (1 if w.k <= 31 else 2) and rep["size_class"] == 0
+            assert rep["trials"] >= 12 and rep["tuned_ms"] <= rep["default_ms"]
+            assert rep["chosen"]["max_digit_bits"] in (8, 10, 11, 12)
+            res, tab, _ = run_both(ctx, fq, w.k, w.canonical, w.partitions)
+            check_same(res, tab, f"{w.name} after autotune {rep['chosen']}")
