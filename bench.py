@@ -550,6 +550,24 @@ class DeviceBytes:
             self.ptr = None
 
 
+def calibrate_copy(nbytes: int = 1 << 30) -> None:
+    """One device-to-device copy of a known size (the runtime's copy kernel):
+    the reference point that scales FETCH_SIZE / WRITE_SIZE readings."""
+    import ctypes as C
+    src = DeviceBytes(np.zeros(nbytes, dtype=np.uint8))
+    dst = DeviceBytes(np.zeros(1, dtype=np.uint8))
+    p = C.c_void_p()
+    if src.hip.hipMalloc(C.byref(p), nbytes) != 0:
+        raise RuntimeError("hipMalloc failed")
+    if src.hip.hipMemcpy(p, C.c_void_p(src.ptr), nbytes, 3) != 0:  # device to device
+        raise RuntimeError("hipMemcpy failed")
+    src.hip.hipDeviceSynchronize()
+    src.hip.hipFree(p)
+    src.free()
+    dst.free()
+    log(f"calibration copy: {nbytes >> 20} MiB read + {nbytes >> 20} MiB written")
+
+
 def bench_single(ctx, w, steps: int, warmup: int, autotune: bool) -> tuple[dict, np.ndarray]:
     """One sample resident in HBM (the FASTQ already on the device, the
     tables left there): the count pipeline's own rate and per-kernel
@@ -616,6 +634,23 @@ def roofline_of(kern: dict, workload: str) -> dict:
             "launches": dom["launches"]}
 
 
+def single_only(ctx, args) -> int:
+    """--no-batch: the single-sample leg alone, printed as the line (for
+    profiling commands; not the headline configuration)."""
+    from kite_amd.workloads import WORKLOADS
+    w = WORKLOADS[args.single_workload]
+    single, fq = bench_single(ctx, w, args.single_steps, 3, not args.no_autotune)
+    line = {"metric": "input k-mers/s (single sample, FASTQ resident in HBM; profiling run)",
+            "value": single["kmers_per_s"], "unit": "k-mers/s", "n_gpus": 1, "steps": single["steps"],
+            "warmup": 3, "ms_per_step": single["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "u64", "data": "synthetic (seeded generator, regenerated on the box)",
+            "config": {"workload": w.name, "k": w.k, "canonical": w.canonical, "partitions": w.partitions},
+            "roofline": single["roofline"], "kernels": single["kernels"], "autotune": single["autotune"]}
+    print(json.dumps(line), flush=True)
+    ctx.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -629,6 +664,11 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="count with the built-in default settings")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="profiling runs: only the HBM-resident single-sample leg, printed as the line")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run one 1 GiB device-to-device copy (a known-size kernel that scales "
+                         "FETCH_SIZE / WRITE_SIZE counter readings)")
     ap.add_argument("--no-analyze", action="store_true")
     ap.add_argument("--no-single", action="store_true", help="skip the HBM-resident single-sample leg")
     ap.add_argument("--no-dump", action="store_true", help="skip the kite dump leg")
@@ -652,6 +692,10 @@ def main() -> int:
     except KiteError as e:
         print(f"bench.py needs a HIP device: {e}", file=sys.stderr)
         return 1
+    if args.calibrate_copy:
+        calibrate_copy()
+    if args.no_batch:
+        return single_only(ctx, args)
     pcie = pcie_peak(ctx)
     ws, pins = make_batch24(args.batch_scale)
     tune = None
