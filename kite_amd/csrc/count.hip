@@ -1558,10 +1558,6 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
     // the regrouped split scatter pays for one-word keys (measured slower
     // for two-word keys, whose tiles give half as long runs per child)
     const bool split_staged = tun.split_staged >= 0 ? tun.split_staged != 0 : W == 1;
-    // the one-tile scatter keeps 32-bit offsets inside a segment, which the
-    // input length bounds (every k-mer ends on its own FASTQ byte)
-    const uint32_t spl = (uint32_t)std::max(len > 0xFFFFFFFFull ? 2 : 1, tun.split_chunk);
-    const uint64_t TK = (uint64_t)split_tk(W) * spl;  // keys per split workgroup
     EventTimer tm(timing_, s);
     tm.mark();
     part_off_h_.assign(P + 1, 0);
@@ -1641,6 +1637,12 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
     std::copy(pin, pin + P + 1, part_off_h_.begin());
     const uint64_t n = part_off_h_[P];
     n_kmers_ = n;
+    // the one-tile split scatter keeps 32-bit offsets inside a segment, and
+    // a segment never outgrows its partition
+    uint64_t max_part = 0;
+    for (uint32_t p = 0; p < P; p++) max_part = std::max(max_part, part_off_h_[p + 1] - part_off_h_[p]);
+    const uint32_t spl = (uint32_t)std::max(max_part > 0xFFFFFFFFull ? 2 : 1, tun.split_chunk);
+    const uint64_t TK = (uint64_t)split_tk(W) * spl;  // keys per split workgroup
 
     // 3. scatter keys into partition order
     uint64_t *A = static_cast<uint64_t *>(a_.ensure(std::max<uint64_t>(n, 1) * 8 * W));

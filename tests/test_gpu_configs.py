@@ -202,3 +202,26 @@ def test_auto_partitions_count_dump(tmp_path):
     ol.write_table(tmp_path / "o.kite", t)
     ol.kite_dump(tmp_path / "o.kite", tmp_path / "o.tsv")
     assert _dump_bytes(tmp_path / "g.kite") == (tmp_path / "o.tsv").read_bytes()
+
+
+def test_config3_full_size_properties(gpu_ctx):
+    """k=63 canonical, all 10,000,000 reads of 250 bp (5.15 GB of FASTQ: an
+    input above 4 GiB), P=1024: the counts add up to the exact window count
+    of the FASTQ bytes, every partition's 128-bit keys are strictly
+    ascending, and every key sits in its Fibonacci partition of lo ^ hi
+    (src/count/kmer.rs:111-116)."""
+    from kite_amd.workloads import K63_CANON_100MBP as w
+    fq = w.fastq()
+    assert fq.size > 1 << 32
+    res, st = gpu_ctx.count(fq, w.k, w.canonical, w.partitions)
+    assert st.kmers == exact_windows(fq, w.k)
+    del fq
+    assert int(res.counts.astype(np.uint64).sum()) == st.kmers
+    lo, hi = res.keys[:, 0], res.keys[:, 1]
+    part = ((lo ^ hi) * np.uint64(0x9E3779B97F4A7C15)) >> np.uint64(64 - 10)
+    off = res.part_off.astype(np.int64)
+    assert np.array_equal(np.searchsorted(part, np.arange(w.partitions + 1), side="left"), off)
+    assert np.all(np.diff(part.astype(np.int64)) >= 0)
+    same_part = part[1:] == part[:-1]
+    up = (hi[1:] > hi[:-1]) | ((hi[1:] == hi[:-1]) & (lo[1:] > lo[:-1]))
+    assert np.all(up[same_part])
