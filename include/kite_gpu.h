@@ -195,8 +195,9 @@ int kite_gpu_count_batch_chunks(kite_gpu_ctx *ctx, uint32_t k, int canonical, ui
  * the MSD split target (1, 2, 4, 8: children split until they hold at most
  * the largest LDS sort bucket / target), the maximum radix digit width of
  * a split level (8, 10, 11, 12 bits), how many sort size classes use the
- * hash-then-bin kernel (2, 3, 4) and the extract tile (1, 2, 4, 8 sub-tiles
- * per workgroup) -- one setting at a time with the others held, the best of
+ * hash-then-bin kernel (2, 3, 4), the extract tile (1, 2, 4, 8 sub-tiles
+ * per workgroup) and the size above which a node is split again (the
+ * largest bucket capacity / 1 or / 2) -- one setting at a time with the others held, the best of
  * three device timings each, a change kept when it wins by 1%.  Every
  * setting must give the same k-mer and unique counts.  The winner is used
  * by every later count on this context with the same key width and

@@ -1372,8 +1372,8 @@ int kite_gpu_autotune(kite_gpu_ctx *ctx, uint32_t k, int canonical, uint32_t par
             }
             trials++;
             char buf[160];
-            snprintf(buf, sizeof buf, "%s[%d, %d, %d, %d, %.4f]", log.empty() ? "" : ", ", t.split_target_div,
-                     t.max_digit_bits, t.hash_classes, t.extract_subs, m);
+            snprintf(buf, sizeof buf, "%s[%d, %d, %d, %d, %d, %.4f]", log.empty() ? "" : ", ", t.split_target_div,
+                     t.max_digit_bits, t.hash_classes, t.extract_subs, t.split_above_div, m);
             log += buf;
             return m;
         };
@@ -1400,6 +1400,7 @@ int kite_gpu_autotune(kite_gpu_ctx *ctx, uint32_t k, int canonical, uint32_t par
             try_knob(&kite::CountTunables::max_digit_bits, {8, 10, 11, 12});
             try_knob(&kite::CountTunables::hash_classes, {2, 3, 4});
             try_knob(&kite::CountTunables::extract_subs, {1, 2, 4, 8});
+            try_knob(&kite::CountTunables::split_above_div, {1, 2});
             if (!changed) break;
         }
         const int cls = size_class(len);
@@ -1410,11 +1411,12 @@ int kite_gpu_autotune(kite_gpu_ctx *ctx, uint32_t k, int canonical, uint32_t par
             snprintf(head, sizeof head,
                      "{\"words\": %d, \"partitions\": %u, \"size_class\": %d, \"input_bytes\": %llu, "
                      "\"default_ms\": %.4f, \"tuned_ms\": %.4f, \"chosen\": {\"split_target_div\": %d, "
-                     "\"max_digit_bits\": %d, \"hash_classes\": %d, \"extract_subs\": %d}, \"trials\": %d, "
-                     "\"trial_columns\": [\"split_target_div\", \"max_digit_bits\", \"hash_classes\", "
-                     "\"extract_subs\", \"ms\"], \"trial_log\": [",
+                     "\"max_digit_bits\": %d, \"hash_classes\": %d, \"extract_subs\": %d, "
+                     "\"split_above_div\": %d}, \"trials\": %d, \"trial_columns\": [\"split_target_div\", "
+                     "\"max_digit_bits\", \"hash_classes\", \"extract_subs\", \"split_above_div\", \"ms\"], "
+                     "\"trial_log\": [",
                      W, partitions, cls, (unsigned long long)len, default_ms, best_ms, best.split_target_div,
-                     best.max_digit_bits, best.hash_classes, best.extract_subs, trials);
+                     best.max_digit_bits, best.hash_classes, best.extract_subs, best.split_above_div, trials);
             const std::string js = std::string(head) + log + "]}";
             if (cap <= js.size()) return set_error(KITE_E_INVALID, "report buffer too small");
             memcpy(report, js.c_str(), js.size() + 1);

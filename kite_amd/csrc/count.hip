@@ -1540,6 +1540,7 @@ void CountEngine::run(const uint8_t *d_fq, uint64_t len, int k, int canonical, u
     if (const char *e = getenv("KITE_EXTRACT_SUBS")) tun.extract_subs = std::min(64, std::max(1, atoi(e)));
     if (const char *e = getenv("KITE_EXTRACT_STAGED")) tun.extract_staged = atoi(e) != 0;
     if (const char *e = getenv("KITE_SPLIT_STAGED")) tun.split_staged = atoi(e) != 0 ? 1 : 0;
+    if (const char *e = getenv("KITE_SPLIT_ABOVE_DIV")) tun.split_above_div = std::min(4, std::max(1, atoi(e)));
     if (words_ == 1) run_w<1>(d_fq, len, k, canonical, pbits);
     else if (words_ == 2) run_w<2>(d_fq, len, k, canonical, pbits);
     else run_w<4>(d_fq, len, k, canonical, pbits);
@@ -1685,7 +1686,8 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
             std::max<uint64_t>(scan_elems, scan_temp_elems((uint64_t)nn + 1)) * 8));
         k_fill_u32<<<1, 64, 0, s>>>(reinterpret_cast<uint32_t *>(lstats), 4, 0);
         KITE_CHECK_LAUNCH();
-        k_level_flags<<<grid1(nn), 256, 0, s>>>(nodes, nn, (uint64_t)CAP, kbits, TK, act, tiles, lstats);
+        k_level_flags<<<grid1(nn), 256, 0, s>>>(nodes, nn, (uint64_t)CAP / (uint64_t)std::max(1, tun.split_above_div),
+                                                kbits, TK, act, tiles, lstats);
         KITE_CHECK_LAUNCH();
         scan_to_u64(act, act_rank, nn, stmp, s);
         scan_to_u64(tiles, tile_off, nn, stmp, s);

@@ -53,6 +53,7 @@ struct CountTunables {
     int extract_subs = 4;  // sub-tiles per extract workgroup
     bool extract_staged = true;  // group each chunk's keys by partition in LDS before writing
     int split_staged = -1;       // group each split tile's keys by child in LDS (-1: one-word keys)
+    int split_above_div = 1;     // split nodes above the largest bucket capacity / this (1 or 2)
 };
 
 class CountEngine {
