@@ -64,52 +64,71 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
+// Wave scans by DPP lane moves (no LDS round trips): row_shr:1/2/4/8 scan
+// each row of 16 lanes, then row_bcast:15 adds the end of row 0 (2) to row
+// 1 (3), and row_bcast:31 adds the end of rows 0-1 to rows 2-3.  Lanes whose
+// source is outside the row keep 0.  Every lane of the wave must be active.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xF, false);
+}
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t v) {
+    const uint32_t lo = dpp_u32<CTRL, ROWS>((uint32_t)v);
+    const uint32_t hi = dpp_u32<CTRL, ROWS>((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
 // Inclusive scan of a 64-bit value across one wave (64 lanes).
 __device__ __forceinline__ uint64_t wave_incl_scan_u64(uint64_t v) {
-    unsigned lane = __lane_id();
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint64_t o = __shfl_up(v, d, 64);
-        if (lane >= (unsigned)d) v += o;
-    }
+    v += dpp_u64<0x111, 0xF>(v);
+    v += dpp_u64<0x112, 0xF>(v);
+    v += dpp_u64<0x114, 0xF>(v);
+    v += dpp_u64<0x118, 0xF>(v);
+    v += dpp_u64<0x142, 0xA>(v);
+    v += dpp_u64<0x143, 0xC>(v);
     return v;
 }
 
 __device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v) {
-    unsigned lane = __lane_id();
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t o = __shfl_up(v, d, 64);
-        if (lane >= (unsigned)d) v += o;
-    }
+    v += dpp_u32<0x111, 0xF>(v);
+    v += dpp_u32<0x112, 0xF>(v);
+    v += dpp_u32<0x114, 0xF>(v);
+    v += dpp_u32<0x118, 0xF>(v);
+    v += dpp_u32<0x142, 0xA>(v);
+    v += dpp_u32<0x143, 0xC>(v);
     return v;
 }
 
-// Exclusive block scan; `ws` is shared scratch of at least THREADS/64 entries.
-// Returns the exclusive prefix; *total receives the block sum.
-template <int THREADS, class T>
+// Exclusive block scan; `ws` is shared scratch of at least THREADS/64 + 1
+// entries.  Returns the exclusive prefix; *total receives the block sum.
+// Wave 0 scans the wave totals.  With `reuse` false the closing barrier is
+// left out: the caller then passes another barrier before `ws` is written
+// again.
+template <int THREADS, class T, bool reuse = true>
 __device__ __forceinline__ T block_excl_scan(T v, T *ws, T *total) {
     constexpr int NW = THREADS / 64;
-    unsigned lane = __lane_id();
-    unsigned wave = threadIdx.x / 64;
+    static_assert(NW <= 64, "one wave scans the wave totals");
+    const unsigned lane = threadIdx.x & 63;
+    const unsigned wave = threadIdx.x / 64;
     T inc;
     if constexpr (sizeof(T) == 8) inc = wave_incl_scan_u64(v);
     else inc = wave_incl_scan_u32(v);
     if (lane == 63) ws[wave] = inc;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        T s = 0;
-        for (int w = 0; w < NW; w++) {
-            T t = ws[w];
-            ws[w] = s;
-            s += t;
-        }
-        ws[NW] = s;
+    if (wave == 0) {
+        const T t = lane < (unsigned)NW ? ws[lane] : T(0);
+        T s;
+        if constexpr (sizeof(T) == 8) s = wave_incl_scan_u64(t);
+        else s = wave_incl_scan_u32(t);
+        if (lane < (unsigned)NW) ws[lane] = s - t;
+        if (lane == (unsigned)NW - 1) ws[NW] = s;
     }
     __syncthreads();
     T r = ws[wave] + inc - v;
     *total = ws[NW];
-    __syncthreads();
+    if constexpr (reuse) __syncthreads();
     return r;
 }
 

@@ -936,14 +936,16 @@ constexpr int hash_sort_waves(int ipt) { return ipt > 4 ? 4 : 8; }
 // Hash-then-bin variant: every key is inserted into an LDS open-addressing
 // table (a slot holds the first position of its key, claimed by CAS) and
 // duplicates add to that position's count.  The unique keys, carrying their
-// counts, are then distributed by one counting pass over 2*CAP bins on
+// counts, are then distributed by one counting pass over CAP / 2 bins on
 // their top remaining key bits (order inside a bin does not matter: keys
-// are distinct), and each thread insertion-sorts the few keys of its own
-// contiguous bins.  A bin above LOCAL_MAX keys (skewed key bits) hands the
-// bucket, untouched, to k_bucket_sort through fb_list (the host launches it
-// over the handed-back buckets), which keeps the LSD code and its registers
-// out of this kernel.  Output equals k_bucket_sort's: ascending unique keys
-// with their run lengths.
+// are distinct), and each key's place inside its bin is its rank among the
+// bin's keys (every key compares itself with the others of its bin at once,
+// instead of a per-thread insertion sort whose steps wait on each other).
+// A bin above LOCAL_MAX keys (skewed key bits) hands the bucket, untouched,
+// to k_bucket_sort through fb_list (the host launches it over the
+// handed-back buckets), which keeps the LSD code and its registers out of
+// this kernel.  Output equals k_bucket_sort's: ascending unique keys with
+// their run lengths.
 template <int W, int IPT, int NT>
 __global__ __launch_bounds__(NT, hash_sort_waves(IPT)) void k_bucket_hash_sort(uint64_t *__restrict__ A,
                                                           uint64_t *__restrict__ B,
@@ -956,12 +958,14 @@ __global__ __launch_bounds__(NT, hash_sort_waves(IPT)) void k_bucket_hash_sort(u
     constexpr int NW = NT / 64;
     constexpr int CAP = NT * IPT;
     constexpr int HT = 2 * CAP;  // hash slots (load factor <= 1/2), then bins
-    constexpr int BPT = HT / NT; // bins per thread
-    constexpr int DB = ilog2c(HT);
+    constexpr int NB = HT / 4 >= NT ? HT / 4 : NT;  // bins of the counting pass
+    constexpr int BPT = NB / NT; // bins per thread
+    constexpr int DB = ilog2c(NB);
     constexpr int LOCAL_MAX = 16;
+    static_assert(NB <= HT && NB % NT == 0, "bins alias the hash table");
     __shared__ uint64_t skeys[CAP * W];
     __shared__ uint16_t scnt[CAP];  // counts <= CAP < 2^16; pairs updated as one u32
-    __shared__ uint32_t ht[HT];
+    __shared__ uint32_t ht[HT + 1];  // [NB]: the number of unique keys once scanned
     __shared__ uint32_t ws[NW + 1];
     __shared__ uint32_t s_fallback;
     uint32_t *scnt32 = reinterpret_cast<uint32_t *>(scnt);
@@ -1060,11 +1064,11 @@ __global__ __launch_bounds__(NT, hash_sort_waves(IPT)) void k_bucket_hash_sort(u
         }
     }
     __syncthreads();
-    // 2. one counting pass of the representatives over HT bins on their top
+    // 2. one counting pass of the representatives over NB bins on their top
     // remaining key bits
     const int db = bits < DB ? bits : DB;
     const int dshift = bits - db;
-    for (int j = tid; j < HT; j += NT) ht[j] = 0;
+    for (int j = tid; j < NB; j += NT) ht[j] = 0;
     if (tid == 0) s_fallback = 0;
     __syncthreads();
     uint32_t cnt[IPT], slot[IPT], dig[IPT];
@@ -1078,7 +1082,7 @@ __global__ __launch_bounds__(NT, hash_sort_waves(IPT)) void k_bucket_hash_sort(u
         }
     }
     __syncthreads();
-    uint32_t boff[BPT + 1], U;
+    uint32_t U;
     {
         uint32_t v[BPT], sum = 0, big = 0;
 #pragma unroll
@@ -1087,14 +1091,14 @@ __global__ __launch_bounds__(NT, hash_sort_waves(IPT)) void k_bucket_hash_sort(u
             sum += v[b];
             big |= v[b] > (uint32_t)LOCAL_MAX;
         }
-        uint32_t run = block_excl_scan<NT, uint32_t>(sum, ws, &U);
+        // (ws is not written again in this kernel)
+        uint32_t run = block_excl_scan<NT, uint32_t, false>(sum, ws, &U);
 #pragma unroll
         for (int b = 0; b < BPT; b++) {
-            boff[b] = run;
             ht[tid * BPT + b] = run;
             run += v[b];
         }
-        boff[BPT] = run;
+        if (tid == NT - 1) ht[NB] = run;
         if (big) s_fallback = 1;
     }
     __syncthreads();
@@ -1115,29 +1119,36 @@ __global__ __launch_bounds__(NT, hash_sort_waves(IPT)) void k_bucket_hash_sort(u
         return;
     }
     {
-        // 3. every thread sorts the keys of its own bins (at most LOCAL_MAX each)
+        // 3. the place of each unique key inside its bin (at most LOCAL_MAX
+        // keys): its rank among the bin's keys, four compares per step
+        uint32_t fin[IPT];
 #pragma unroll
-        for (int b = 0; b < BPT; b++) {
-            const uint32_t lo = boff[b], hi = boff[b + 1];
-            for (uint32_t i = lo + 1; i < hi; i++) {
-                Key<W> kk;
+        for (int it = 0; it < IPT; it++) {
+            fin[it] = 0;
+            if ((is_rep >> it) & 1u) {
+                const uint32_t lo = ht[dig[it]], hi = ht[dig[it] + 1];
+                uint32_t rank = 0;
+                for (uint32_t j = lo; j < hi; j += 4) {
+                    Key<W> o[4];
 #pragma unroll
-                for (int r = 0; r < W; r++) kk.w[r] = skeys[i * W + r];
-                const uint32_t c = scnt[i];
-                uint32_t j = i;
-                while (j > lo) {
-                    Key<W> pk;
+                    for (int q = 0; q < 4; q++) {
+                        const uint32_t jj = j + q < hi ? j + q : hi - 1;
 #pragma unroll
-                    for (int r = 0; r < W; r++) pk.w[r] = skeys[(j - 1) * W + r];
-                    if (!key_less<W>(kk, pk)) break;
+                        for (int r = 0; r < W; r++) o[q].w[r] = skeys[jj * W + r];
+                    }
 #pragma unroll
-                    for (int r = 0; r < W; r++) skeys[j * W + r] = pk.w[r];
-                    scnt[j] = scnt[j - 1];
-                    j--;
+                    for (int q = 0; q < 4; q++) rank += (j + q < hi && key_less<W>(o[q], key[it])) ? 1u : 0u;
                 }
+                fin[it] = lo + rank;
+            }
+        }
+        __syncthreads();
 #pragma unroll
-                for (int r = 0; r < W; r++) skeys[j * W + r] = kk.w[r];
-                scnt[j] = (uint16_t)c;
+        for (int it = 0; it < IPT; it++) {
+            if ((is_rep >> it) & 1u) {
+#pragma unroll
+                for (int r = 0; r < W; r++) skeys[fin[it] * W + r] = key[it].w[r];
+                scnt[fin[it]] = (uint16_t)cnt[it];
             }
         }
         __syncthreads();
