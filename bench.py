@@ -122,10 +122,16 @@ def make_batch24(scale: float):
     return ws, pins
 
 
+# MI355X's host link: PCIe 5.0 x16, 32 GT/s per lane with 128b/130b
+# encoding, 63.0 GB/s each way, 126.0 GB/s with both directions busy.
+PCIE5_X16_BOTH_GBPS = 2 * 32e9 * 16 * 128 / 130 / 8 / 1e9
+
+
 def pcie_peak(ctx) -> dict:
     """Page-locked host <-> device copy rates on this box through the
     engine's own runtime (kite_gpu_copy_peak): H2D alone, D2H alone, both
-    at once on two streams; 1 GiB copies, best of 3."""
+    at once on two streams; 1 GiB copies, best of 5 after one untimed copy
+    each way."""
     out = ctx.copy_peak(1 << 30)
     log(f"pcie: H2D {out['h2d_GBps']:.1f} GB/s, D2H {out['d2h_GBps']:.1f} GB/s, "
         f"both at once {out['both_GBps']:.1f} GB/s (page-locked, 1 GiB copies)")
@@ -136,6 +142,26 @@ def size_class(nbytes: int) -> int:
     """The engine's input size classes for tuned settings (below 64 MiB,
     512 MiB, 4 GiB, above)."""
     return 0 if nbytes < 64 << 20 else (1 if nbytes < 512 << 20 else (2 if nbytes < 4 << 30 else 3))
+
+
+TUNED_KEYS = ("split_target_div", "max_digit_bits", "hash_classes", "extract_subs", "split_above_div")
+
+
+def apply_tuned(ctx, spec: str, k: int, P: int) -> dict:
+    """--tuned "CLASS:div,bits,classes,subs,above;...": set the count
+    settings of those size classes (kite_gpu_set_tuned) instead of timing
+    candidates, e.g. to profile with the choices an earlier run made."""
+    out = {}
+    for part in filter(None, (x.strip() for x in spec.split(";"))):
+        cls, vals = part.split(":")
+        chosen = dict(zip(TUNED_KEYS, (int(v) for v in vals.split(","))))
+        if len(chosen) != len(TUNED_KEYS):
+            raise SystemExit(f"--tuned: class {cls} needs {len(TUNED_KEYS)} values")
+        ctx.set_tuned(k, P, int(cls), chosen)
+        out[str(int(cls))] = {"chosen": chosen, "default_ms": None, "tuned_ms": None, "sample": None,
+                              "trials": 0, "fixed": "--tuned"}
+        log(f"tuned settings for size class {cls}: {chosen}")
+    return out
 
 
 def autotune_cohort(ctx, arrs, names, k: int, canon: bool, P: int) -> dict:
@@ -639,7 +665,9 @@ def single_only(ctx, args) -> int:
     profiling commands; not the headline configuration)."""
     from kite_amd.workloads import WORKLOADS
     w = WORKLOADS[args.single_workload]
-    single, fq = bench_single(ctx, w, args.single_steps, 3, not args.no_autotune)
+    if args.tuned:
+        apply_tuned(ctx, args.tuned, w.k, w.partitions)
+    single, fq = bench_single(ctx, w, args.single_steps, 3, not (args.no_autotune or args.tuned))
     line = {"metric": "input k-mers/s (single sample, FASTQ resident in HBM; profiling run)",
             "value": single["kmers_per_s"], "unit": "k-mers/s", "n_gpus": 1, "steps": single["steps"],
             "warmup": 3, "ms_per_step": single["ms_per_step"], "higher_is_better": True, "scaling": "weak",
@@ -664,6 +692,10 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="count with the built-in default settings")
+    ap.add_argument("--tuned", default="",
+                    help='fixed count settings per input size class instead of the autotuner, '
+                         '"CLASS:split_div,digit_bits,hash_classes,extract_subs,split_above;..." '
+                         '(e.g. the choices of an earlier run, for profiling it)')
     ap.add_argument("--no-batch", action="store_true",
                     help="profiling runs: only the HBM-resident single-sample leg, printed as the line")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -699,7 +731,9 @@ def main() -> int:
     pcie = pcie_peak(ctx)
     ws, pins = make_batch24(args.batch_scale)
     tune = None
-    if not args.no_autotune:
+    if args.tuned:
+        tune = apply_tuned(ctx, args.tuned, 31, 256)
+    elif not args.no_autotune:
         tune = autotune_cohort(ctx, [p.array for p in pins], [w_.name for w_ in ws], 31, True, 256)
     batch, tables, kern, kmers = bench_batch24(ctx, pins, args.steps, args.warmup, args.plain_reps)
     wall = batch["wall_s"]
@@ -710,13 +744,17 @@ def main() -> int:
     kernels = kernel_table(kern, args.steps)
     log("batch24 kernels: " + json.dumps({n: round(v["ms_per_step"], 3) for n, v in kernels.items()}))
     moved = batch["fastq_bytes"] + batch["chunk_bytes"]
+    achieved = moved / (wall / args.steps) / 1e9
     pcie_line = {"h2d_bytes": batch["fastq_bytes"], "d2h_bytes": batch["chunk_bytes"],
-                 "achieved_GBps": moved / (wall / args.steps) / 1e9, "peak_GBps": pcie["both_GBps"],
-                 "frac": moved / (wall / args.steps) / 1e9 / pcie["both_GBps"],
-                 "peak_source": "kite_gpu_copy_peak in this run: page-locked 1 GiB copies, H2D and D2H at once",
+                 "achieved_GBps": achieved, "peak_GBps": PCIE5_X16_BOTH_GBPS,
+                 "frac": achieved / PCIE5_X16_BOTH_GBPS,
+                 "peak_source": "PCIe 5.0 x16, both directions (2 x 63.0 GB/s)",
+                 "measured_frac": achieved / pcie["both_GBps"],
+                 "measured_source": "kite_gpu_copy_peak in this run: page-locked 1 GiB copies, H2D and D2H at once",
                  **{f"measured_{k}": v for k, v in pcie.items()}}
-    log(f"batch24 PCIe: {pcie_line['achieved_GBps']:.1f} GB/s of {pcie['both_GBps']:.1f} "
-        f"({pcie_line['frac']:.2f}); plain {batch['plain']['kmers_per_s'] / 1e9:.2f} G k-mers/s, "
+    log(f"batch24 PCIe: {achieved:.1f} GB/s = {pcie_line['frac']:.2f} of the link's {PCIE5_X16_BOTH_GBPS:.1f}, "
+        f"{pcie_line['measured_frac']:.2f} of the {pcie['both_GBps']:.1f} copies measured; "
+        f"plain {batch['plain']['kmers_per_s'] / 1e9:.2f} G k-mers/s, "
         f"batched/plain {batch['batched_vs_plain']:.2f}x")
 
     analyze = None
@@ -737,7 +775,7 @@ def main() -> int:
     single = fq = None
     w = WORKLOADS[args.single_workload]
     if not args.no_single:
-        single, fq = bench_single(ctx, w, args.single_steps, 3, not args.no_autotune)
+        single, fq = bench_single(ctx, w, args.single_steps, 3, not (args.no_autotune or args.tuned))
     if not args.no_dump:
         fq = w.fastq() if fq is None else fq
         single = single or {}

@@ -211,10 +211,22 @@ int kite_gpu_count_batch_chunks(kite_gpu_ctx *ctx, uint32_t k, int canonical, ui
 int kite_gpu_autotune(kite_gpu_ctx *ctx, uint32_t k, int canonical, uint32_t partitions, const uint8_t *fastq,
                       size_t len, char *report, size_t cap);
 
+/* Sets the count settings of one (key width of k, partition count, input
+ * size class 0-3) without timing anything, as if kite_gpu_autotune had
+ * chosen them: a caller re-applies choices kept from an earlier run.  The
+ * values are the autotuner's candidates (split target 1, 2, 4, 8; digit
+ * bits 8-12; hash classes 2-4; extract sub-tiles 1, 2, 4, 8; split-above
+ * divisor 1 or 2); others are rejected.  Counterpart of passing the
+ * reference planner's derived settings back in by hand (its --partitions,
+ * -t and --memory flags, src/count/mod.rs:923-1016). */
+int kite_gpu_set_tuned(kite_gpu_ctx *ctx, uint32_t k, uint32_t partitions, int size_class, int split_target_div,
+                       int max_digit_bits, int hash_classes, int extract_subs, int split_above_div);
+
 /* Host-link copy rates of this device: page-locked buffers of `bytes`,
- * H2D alone, D2H alone, and both at once on two streams (GB/s, best of 3).
- * The bound of the batch above, whose H2D and D2H run at once.  No
- * counterpart in the reference (it has no device link). */
+ * H2D alone, D2H alone, and both at once on two streams (GB/s; one untimed
+ * copy each way, then the best of 5).  The bound of the batch above, whose
+ * H2D and D2H run at once.  No counterpart in the reference (it has no
+ * device link). */
 int kite_gpu_copy_peak(kite_gpu_ctx *ctx, uint64_t bytes, double *h2d_gbps, double *d2h_gbps, double *both_gbps);
 
 /* Page-locked host memory for FASTQ buffers and results (hipHostMalloc). */

@@ -151,6 +151,8 @@ GPU_SYMBOLS = {
                                  C.c_uint32, C.c_int, C.c_uint32, HISTO_FN, C.c_void_p, C.POINTER(DumpStats)]),
     "kite_gpu_autotune": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_uint32, C.c_void_p, C.c_size_t,
                                     C.c_char_p, C.c_size_t]),
+    "kite_gpu_set_tuned": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.c_int, C.c_int]),
     "kite_gpu_copy_peak": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                      C.POINTER(C.c_double)]),
     "kite_gpu_host_alloc": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
@@ -614,6 +616,13 @@ class GpuContext:
         _check(self._lib.kite_gpu_autotune(self._h, k, int(bool(canonical)), partitions, arr.ctypes.data, arr.size,
                                            buf, len(buf)))
         return json.loads(buf.value.decode())
+
+    def set_tuned(self, k: int, partitions: int, size_class: int, chosen: dict) -> None:
+        """kite_gpu_set_tuned: re-apply settings an autotune run chose
+        (`chosen` as in its report) for one size class."""
+        _check(self._lib.kite_gpu_set_tuned(self._h, k, partitions, size_class, int(chosen["split_target_div"]),
+                                            int(chosen["max_digit_bits"]), int(chosen["hash_classes"]),
+                                            int(chosen["extract_subs"]), int(chosen["split_above_div"])))
 
     def copy_peak(self, nbytes: int = 1 << 30) -> dict:
         """kite_gpu_copy_peak: page-locked H2D, D2H and both-at-once GB/s."""

@@ -1425,6 +1425,34 @@ int kite_gpu_autotune(kite_gpu_ctx *ctx, uint32_t k, int canonical, uint32_t par
     });
 }
 
+int kite_gpu_set_tuned(kite_gpu_ctx *ctx, uint32_t k, uint32_t partitions, int size_class, int split_target_div,
+                       int max_digit_bits, int hash_classes, int extract_subs, int split_above_div) {
+    if (!ctx) return set_error(KITE_E_INVALID, "invalid argument: ctx is NULL");
+    if (k < 1 || k > 128) return set_error(KITE_E_INVALID, "invalid k-mer length: k must be between 1 and 128");
+    if (partitions == 0 || (partitions & (partitions - 1)))
+        return set_error(KITE_E_INVALID, "invalid partition count: must be a power of 2");
+    auto one_of = [](int v, std::initializer_list<int> ok) {
+        for (int o : ok)
+            if (v == o) return true;
+        return false;
+    };
+    if (size_class < 0 || size_class > 3 || !one_of(split_target_div, {1, 2, 4, 8}) || max_digit_bits < 8 ||
+        max_digit_bits > 12 || hash_classes < 2 || hash_classes > 4 || !one_of(extract_subs, {1, 2, 4, 8}) ||
+        !one_of(split_above_div, {1, 2}))
+        return set_error(KITE_E_INVALID, "invalid count settings: outside the autotuner's candidates");
+    return guarded([&]() -> int {
+        kite::CountTunables t;
+        t.split_target_div = split_target_div;
+        t.max_digit_bits = max_digit_bits;
+        t.hash_classes = hash_classes;
+        t.extract_subs = extract_subs;
+        t.split_above_div = split_above_div;
+        const int W = k <= 31 ? 1 : (k <= 64 ? 2 : 4);
+        ctx->tuned[std::make_tuple(W, partitions, size_class)] = t;
+        return KITE_OK;
+    });
+}
+
 int kite_gpu_copy_peak(kite_gpu_ctx *ctx, uint64_t bytes, double *h2d_gbps, double *d2h_gbps, double *both_gbps) {
     if (!ctx || bytes == 0) return set_error(KITE_E_INVALID, "invalid argument: ctx is NULL or bytes is 0");
     return guarded([&]() -> int {
@@ -1445,8 +1473,14 @@ int kite_gpu_copy_peak(kite_gpu_ctx *ctx, uint64_t bytes, double *h2d_gbps, doub
                 (void)hipStreamDestroy(y);
             }
         } guard{a, b};
+        // one untimed copy each way first (first use of the buffers and DMA
+        // queues), then the best of five of each mode
+        KITE_HIP(hipMemcpyAsync(ddst, hsrc, bytes, hipMemcpyHostToDevice, a));
+        KITE_HIP(hipMemcpyAsync(hdst, dsrc, bytes, hipMemcpyDeviceToHost, b));
+        KITE_HIP(hipStreamSynchronize(a));
+        KITE_HIP(hipStreamSynchronize(b));
         double best[3] = {0, 0, 0};
-        for (int rep = 0; rep < 3; rep++)
+        for (int rep = 0; rep < 5; rep++)
             for (int mode = 1; mode <= 3; mode++) {
                 KITE_HIP(hipDeviceSynchronize());
                 const auto t0 = std::chrono::steady_clock::now();

@@ -990,11 +990,9 @@ __global__ __launch_bounds__(NT, hash_sort_waves(IPT)) void k_bucket_hash_sort(u
     const uint32_t m = (uint32_t)bk.size;
     int ipt = (int)((m + NT - 1) / NT);
     uint32_t wbase = (uint32_t)wave * 64u * (uint32_t)ipt;
-    for (int j = tid; j < HT; j += NT) ht[j] = 0;
-    for (int j = tid; j < CAP / 2; j += NT) scnt32[j] = 0;
     Key<W> key[IPT];
     // all loads first (positions past the bucket read its last key), then
-    // the LDS copies: the loads are in flight together
+    // the table clears while they are in flight, then the LDS copies
 #pragma unroll
     for (int it = 0; it < IPT; it++) {
         const uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;
@@ -1002,6 +1000,8 @@ __global__ __launch_bounds__(NT, hash_sort_waves(IPT)) void k_bucket_hash_sort(u
 #pragma unroll
         for (int r = 0; r < W; r++) key[it].w[r] = p[r];
     }
+    for (int j = tid; j < HT; j += NT) ht[j] = 0;
+    for (int j = tid; j < CAP / 2; j += NT) scnt32[j] = 0;
 #pragma unroll
     for (int it = 0; it < IPT; it++) {
         const uint32_t pos = wbase + (uint32_t)it * 64u + (uint32_t)lane;

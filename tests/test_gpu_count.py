@@ -296,3 +296,26 @@ def test_autotune_choice_keeps_parity(gpu_ctx):
             assert rep["chosen"]["max_digit_bits"] in (8, 10, 11, 12)
             res, tab, _ = run_both(ctx, fq, w.k, w.canonical, w.partitions)
             check_same(res, tab, f"{w.name} after autotune {rep['chosen']}")
+
+
+def test_set_tuned_settings_keep_parity():
+    """kite_gpu_set_tuned applies fixed settings to one size class without
+    timing: counts of that class must still give the oracle's table, here
+    with every setting away from the built-in one; values outside the
+    autotuner's candidates are rejected."""
+    from kite_amd.workloads import K31_CANON_50MBP
+    from kite_amd import GpuContext, KiteError
+    w = K31_CANON_50MBP
+    fq = w.fastq(n_reads=120000)
+    with GpuContext(0) as ctx:
+        chosen = {"split_target_div": 8, "max_digit_bits": 8, "hash_classes": 2, "extract_subs": 1,
+                  "split_above_div": 2}
+        ctx.set_tuned(w.k, w.partitions, 0, chosen)
+        res, tab, _ = run_both(ctx, fq, w.k, w.canonical, w.partitions)
+        check_same(res, tab, f"{w.name} with fixed settings {chosen}")
+        for bad in ({**chosen, "split_target_div": 3}, {**chosen, "max_digit_bits": 13},
+                    {**chosen, "extract_subs": 0}):
+            with pytest.raises(KiteError, match="invalid count settings"):
+                ctx.set_tuned(w.k, w.partitions, 0, bad)
+        with pytest.raises(KiteError, match="invalid count settings"):
+            ctx.set_tuned(w.k, w.partitions, 4, chosen)
