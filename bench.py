@@ -43,10 +43,13 @@ def log(msg: str) -> None:
 
 
 def cpu_baseline(arrs, names, threads: int, reps: int = 3) -> dict:
-    """Time the oracle -- a C restatement of the reference `kite count`
-    (parse, k-mers, partition, sort, run-length counts; output /dev/null) --
-    on a bounded sample of the cohort: whole samples, each its own count run
-    as the reference would do them, summed; median of `reps` passes."""
+    """Time the reference's count loop restated in C (`kite_oracle
+    count-ref`: workers taking record-aligned chunks in turn, inline rolling
+    k-mers into 64-entry write-combining partition slots, then per worker and
+    partition one run -- MSD/LSD byte radix, RLE, DVLE encode; output
+    discarded) on a bounded sample of the cohort: whole samples, each its own
+    count run as the reference would do them, summed; median of `reps`
+    passes."""
     exe = ROOT / "oracle" / "build" / "kite_oracle"
     if not exe.exists():
         return {"value": None, "unit": "k-mers/s", "cores": 0, "kind": "port", "sample": "oracle binary missing"}
@@ -62,7 +65,7 @@ def cpu_baseline(arrs, names, threads: int, reps: int = 3) -> dict:
             secs, kmers = 0.0, 0
             for p in paths:
                 # chunk size as the reference planner picks it for a large budget: 64 MiB
-                cmd = [str(exe), "count", str(p), "/dev/null", "31", "1", "256", str(64 << 20), str(threads)]
+                cmd = [str(exe), "count-ref", str(p), "/dev/null", "31", "1", "256", str(64 << 20), str(threads)]
                 r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
                 if r.returncode != 0:
                     raise RuntimeError(r.stderr)
@@ -74,9 +77,11 @@ def cpu_baseline(arrs, names, threads: int, reps: int = 3) -> dict:
                 "host_nproc": os.cpu_count(),
                 "sample": f"cohort samples {', '.join(names)} ({kmers} k-mers), each its own count run, "
                           f"median of {reps} passes, {threads} threads, output /dev/null",
-                "note": "simplified restatement of the reference count (per-thread partition vectors, then "
-                        "sort + run-length counts per partition); it has neither the reference's "
-                        "write-combined scatter nor its MSD/LSD radix sorter",
+                "note": "the reference's count loop restated in C (count/mod.rs:512-540, worker.rs:56-449, "
+                        "sorter.rs:67-272, 514-530): write-combined partition scatter, MSD/LSD byte radix "
+                        "with comparison-sort leaves, saturating RLE and DVLE encode per worker run; without "
+                        "its AVX-512 histogram; 64 MiB chunks, the reference planner's choice on a "
+                        "large-memory host",
                 "rates": rates}
     finally:
         for p in paths:

@@ -514,6 +514,297 @@ int ok_count_fastq(const uint8_t *data, size_t len, int k, int canonical, uint32
     return 0;
 }
 
+/* ---- the reference's count hot path, for the CPU baseline ---------------
+ * ok_count_fastq above is the checker: per-thread partition vectors, then
+ * one sort of each whole partition.  ok_count_ref restates the reference's
+ * own count loop for one-word keys (k <= 31) so that the CPU baseline times
+ * the reference's algorithm:
+ *   - workers take record-aligned chunks in turn (the bounded channel of
+ *     count/mod.rs:512-540), parse them (count/parser.rs:19-65) and roll
+ *     k-mers inline (count/kmer.rs:203-268);
+ *   - each k-mer goes to a 64-entry write-combining slot of its partition,
+ *     which is appended to the partition's local vector when full
+ *     (worker.rs:15, 56-175);
+ *   - at the end each worker drains its slots and flushes every partition
+ *     as one run (worker.rs:273-449): radix_sort_u64_kmer (sorter.rs:67-93:
+ *     MSD byte radix from byte ceil(2k/8) - 1 down for n >= 131072 with a
+ *     comparison-sort leaf at <= 1024, else LSD over ceil(2k/8) bytes, a
+ *     comparison sort below 1024), RLE with u32 saturation (sorter.rs:514-530),
+ *     DVLE encode (run_block.rs:9-35).
+ * The reference's AVX-512 histogram and its zstd step (not used in fast
+ * mode) are left out.  Runs stay per worker, as the reference writes them;
+ * with `out` the runs are merged into a table for the tests. */
+#define REF_WCOM 64
+#define REF_MSD_THRESHOLD 131072
+#define REF_MSD_LEAF 1024
+
+static void ref_isort(uint64_t *d, size_t n) {
+    for (size_t i = 1; i < n; i++) {
+        uint64_t v = d[i];
+        size_t j = i;
+        while (j > 0 && d[j - 1] > v) { d[j] = d[j - 1]; j--; }
+        d[j] = v;
+    }
+}
+
+/* comparison sort of a small array: median-of-three quicksort, insertion
+ * sort below 24 elements */
+static void ref_qsort(uint64_t *d, size_t n) {
+    while (n > 24) {
+        uint64_t a = d[0], b = d[n / 2], c = d[n - 1];
+        uint64_t piv = a < b ? (b < c ? b : (a < c ? c : a)) : (a < c ? a : (b < c ? c : b));
+        size_t i = 0, j = n - 1;
+        for (;;) {
+            while (d[i] < piv) i++;
+            while (d[j] > piv) j--;
+            if (i >= j) break;
+            uint64_t t = d[i]; d[i] = d[j]; d[j] = t;
+            i++; j--;
+        }
+        /* recurse into the smaller side */
+        if (j + 1 < n - j - 1) { ref_qsort(d, j + 1); d += j + 1; n -= j + 1; }
+        else { ref_qsort(d + j + 1, n - j - 1); n = j + 1; }
+    }
+    ref_isort(d, n);
+}
+
+static void ref_msd(uint64_t *d, size_t n, int byte_hi, int byte_lo, uint64_t *tmp) {
+    if (n <= REF_MSD_LEAF) { ref_qsort(d, n); return; }
+    const unsigned sh = (unsigned)byte_hi * 8u;
+    size_t hist[256] = {0}, pre[256];
+    for (size_t i = 0; i < n; i++) hist[(d[i] >> sh) & 255]++;
+    size_t acc = 0;
+    for (int b = 0; b < 256; b++) { pre[b] = acc; acc += hist[b]; }
+    for (size_t i = 0; i < n; i++) tmp[pre[(d[i] >> sh) & 255]++] = d[i];
+    memcpy(d, tmp, n * 8);
+    if (byte_hi > byte_lo) {
+        size_t off = 0;
+        for (int b = 0; b < 256; b++) {
+            if (hist[b] > 1) ref_msd(d + off, hist[b], byte_hi - 1, byte_lo, tmp);
+            off += hist[b];
+        }
+    }
+}
+
+static void ref_lsd(uint64_t *d, size_t n, int passes, uint64_t *tmp) {
+    if (n < 2) return;
+    if (n < 1024) { ref_qsort(d, n); return; }
+    uint64_t *src = d, *dst = tmp;
+    for (int p = 0; p < passes; p++) {
+        const unsigned sh = (unsigned)p * 8u;
+        size_t hist[256] = {0};
+        for (size_t i = 0; i < n; i++) hist[(src[i] >> sh) & 255]++;
+        size_t acc = 0;
+        for (int b = 0; b < 256; b++) { size_t c = hist[b]; hist[b] = acc; acc += c; }
+        for (size_t i = 0; i < n; i++) dst[hist[(src[i] >> sh) & 255]++] = src[i];
+        uint64_t *t = src; src = dst; dst = t;
+    }
+    if (src != d) memcpy(d, src, n * 8);
+}
+
+typedef struct {
+    const uint8_t *data;
+    const uint64_t *chunks;
+    size_t n_chunks;
+    size_t *next;          /* shared chunk cursor */
+    int k, canon, bits, keep;
+    uint32_t P;
+    uint64_t **bk;         /* per-partition local vectors */
+    size_t *bn, *bcap;
+    uint64_t *wc;          /* P x REF_WCOM staging slots */
+    uint8_t *wl;
+    uint64_t kmers, reads, rows, bytes;
+    uint64_t **rk, **rc;   /* kept runs (tests) */
+    size_t *rn;
+} ref_worker;
+
+static void ref_append(ref_worker *w, uint32_t b, const uint64_t *src, size_t cnt) {
+    if (w->bn[b] + cnt > w->bcap[b]) {
+        size_t nc = w->bcap[b] ? w->bcap[b] * 2 : 512;
+        while (nc < w->bn[b] + cnt) nc *= 2;
+        w->bk[b] = (uint64_t *)realloc(w->bk[b], nc * 8);
+        w->bcap[b] = nc;
+    }
+    memcpy(w->bk[b] + w->bn[b], src, cnt * 8);
+    w->bn[b] += cnt;
+}
+
+static void *ref_worker_main(void *arg) {
+    ref_worker *w = (ref_worker *)arg;
+    const int k = w->k;
+    const uint64_t mask = (1ull << (2 * k)) - 1;
+    const unsigned rsh = 2u * (unsigned)(k - 1);
+    const unsigned bsh = 64u - (unsigned)w->bits;
+    for (;;) {
+        const size_t c = __atomic_fetch_add(w->next, 1, __ATOMIC_RELAXED);
+        if (c >= w->n_chunks) break;
+        const uint8_t *d = w->data + w->chunks[2 * c];
+        const size_t len = w->chunks[2 * c + 1];
+        if (len == 0 || d[0] != '@') continue;  /* worker.rs:217 */
+        int state = 0;
+        size_t s0 = 0;
+        for (size_t i = 0; i <= len; i++) {
+            /* the end of the chunk closes an unterminated sequence line */
+            if (i < len && d[i] != '\n') continue;
+            if (state == 1 && i > s0) {
+                w->reads++;
+                uint64_t fwd = 0, rev = 0;
+                uint32_t valid = 0;
+                for (size_t q = s0; q < i; q++) {
+                    const uint8_t e = ENC[d[q]];
+                    if (e == 255) { fwd = rev = 0; valid = 0; continue; }
+                    fwd = ((fwd << 2) | e) & mask;
+                    rev = (rev >> 2) | ((uint64_t)(e ^ 3) << rsh);
+                    if (++valid >= (uint32_t)k) {
+                        const uint64_t key = (w->canon && rev < fwd) ? rev : fwd;
+                        const uint32_t b = w->bits ? (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> bsh) : 0u;
+                        uint8_t l = w->wl[b];
+                        w->wc[(size_t)b * REF_WCOM + l] = key;
+                        if (++l == REF_WCOM) { ref_append(w, b, w->wc + (size_t)b * REF_WCOM, REF_WCOM); l = 0; }
+                        w->wl[b] = l;
+                        w->kmers++;
+                    }
+                }
+            }
+            if (i == len) break;
+            if (state == 0) { state = 1; s0 = i + 1; }
+            else if (state == 1) state = 2;
+            else if (state == 2) state = 3;
+            else state = 0;
+        }
+    }
+    /* flush_all: drain each slot, sort, RLE, encode */
+    const int passes = (2 * k + 7) / 8 < 8 ? (2 * k + 7) / 8 : 8;
+    uint64_t *tmp = NULL;
+    size_t tcap = 0;
+    uint8_t *enc = NULL;
+    size_t ecap = 0;
+    uint32_t *cnt = NULL;
+    size_t ccap = 0;
+    for (uint32_t b = 0; b < w->P; b++) {
+        if (w->wl[b]) { ref_append(w, b, w->wc + (size_t)b * REF_WCOM, w->wl[b]); w->wl[b] = 0; }
+        const size_t n = w->bn[b];
+        if (!n) continue;
+        uint64_t *keys = w->bk[b];
+        if (n > tcap) { free(tmp); tcap = n; tmp = (uint64_t *)malloc(tcap * 8); }
+        if (n >= REF_MSD_THRESHOLD) ref_msd(keys, n, passes - 1, 0, tmp);
+        else ref_lsd(keys, n, passes, tmp);
+        if (n > ccap) { free(cnt); ccap = n; cnt = (uint32_t *)malloc(ccap * 4); }
+        size_t u = 0;
+        uint32_t run = 1;
+        for (size_t i = 1; i <= n; i++) {
+            if (i < n && keys[i] == keys[i - 1]) { if (run != 0xffffffffu) run++; continue; }
+            keys[u] = keys[i - 1];
+            cnt[u++] = run;
+            run = 1;
+        }
+        if (n * 15 + 16 > ecap) { free(enc); ecap = n * 15 + 16; enc = (uint8_t *)malloc(ecap); }
+        w->bytes += ok_dvle_encode(keys, cnt, u, 1, enc);
+        w->rows += u;
+        if (w->keep) {
+            w->rk[b] = (uint64_t *)malloc(u * 8);
+            w->rc[b] = (uint64_t *)malloc(u * 8);
+            memcpy(w->rk[b], keys, u * 8);
+            for (size_t i = 0; i < u; i++) w->rc[b][i] = cnt[i];
+            w->rn[b] = u;
+        }
+        free(w->bk[b]);
+        w->bk[b] = NULL;
+        w->bn[b] = w->bcap[b] = 0;
+    }
+    free(tmp);
+    free(enc);
+    free(cnt);
+    return NULL;
+}
+
+typedef struct { uint64_t key, count; } ref_pair;
+static int ref_pair_cmp(const void *a, const void *b) {
+    const uint64_t x = ((const ref_pair *)a)->key, y = ((const ref_pair *)b)->key;
+    return x < y ? -1 : (x > y);
+}
+
+int ok_count_ref(const uint8_t *data, size_t len, int k, int canonical, uint32_t partitions, size_t chunk_size,
+                 int threads, ok_table *out, ok_ref_stats *st) {
+    enc_init();
+    if (st) memset(st, 0, sizeof *st);
+    if (out) memset(out, 0, sizeof *out);
+    if (k < 1 || k > 31) return fail("ok_count_ref: one-word keys only (k between 1 and 31)");
+    if (partitions == 0 || (partitions & (partitions - 1)))
+        return fail("invalid partition count: must be a power of 2");
+    if (threads < 1) threads = 1;
+    const size_t cap = len / (chunk_size ? chunk_size : 1) + 4;
+    uint64_t *chunks = (uint64_t *)malloc(cap * 16);
+    const size_t nch = ok_plain_chunks(data, len, chunk_size, chunks, cap);
+    if (nch > cap) { free(chunks); return fail("chunk table overflow"); }
+    size_t next = 0;
+    ref_worker *ws = (ref_worker *)calloc((size_t)threads, sizeof *ws);
+    pthread_t *th = (pthread_t *)calloc((size_t)threads, sizeof *th);
+    for (int t = 0; t < threads; t++) {
+        ref_worker *w = &ws[t];
+        w->data = data; w->chunks = chunks; w->n_chunks = nch; w->next = &next;
+        w->k = k; w->canon = canonical; w->bits = __builtin_ctz(partitions); w->P = partitions;
+        w->keep = out != NULL;
+        w->bk = (uint64_t **)calloc(partitions, sizeof(uint64_t *));
+        w->bn = (size_t *)calloc(partitions, sizeof(size_t));
+        w->bcap = (size_t *)calloc(partitions, sizeof(size_t));
+        w->wc = (uint64_t *)malloc((size_t)partitions * REF_WCOM * 8);
+        w->wl = (uint8_t *)calloc(partitions, 1);
+        if (out) {
+            w->rk = (uint64_t **)calloc(partitions, sizeof(uint64_t *));
+            w->rc = (uint64_t **)calloc(partitions, sizeof(uint64_t *));
+            w->rn = (size_t *)calloc(partitions, sizeof(size_t));
+        }
+        pthread_create(&th[t], NULL, ref_worker_main, w);
+    }
+    for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
+    for (int t = 0; t < threads; t++) {
+        if (st) {
+            st->reads += ws[t].reads;
+            st->kmers += ws[t].kmers;
+            st->run_rows += ws[t].rows;
+            st->encoded_bytes += ws[t].bytes;
+        }
+    }
+    if (out) {
+        /* tests: merge the workers' runs of each partition into totals */
+        out->k = k; out->canonical = canonical; out->words = 1; out->partitions = partitions;
+        out->part_off = (uint64_t *)malloc(((size_t)partitions + 1) * 8);
+        size_t total = 0;
+        for (int t = 0; t < threads; t++)
+            for (uint32_t p = 0; p < partitions; p++) total += ws[t].rn[p];
+        out->keys = (uint64_t *)malloc((total ? total : 1) * 8);
+        out->counts = (uint64_t *)malloc((total ? total : 1) * 8);
+        ref_pair *pairs = (ref_pair *)malloc((total ? total : 1) * sizeof(ref_pair));
+        uint64_t o = 0;
+        for (uint32_t p = 0; p < partitions; p++) {
+            size_t m = 0;
+            for (int t = 0; t < threads; t++)
+                for (size_t i = 0; i < ws[t].rn[p]; i++) pairs[m++] = (ref_pair){ws[t].rk[p][i], ws[t].rc[p][i]};
+            qsort(pairs, m, sizeof *pairs, ref_pair_cmp);
+            out->part_off[p] = o;
+            for (size_t i = 0; i < m; i++) {
+                if (i && pairs[i].key == pairs[i - 1].key) { out->counts[o - 1] += pairs[i].count; continue; }
+                out->keys[o] = pairs[i].key;
+                out->counts[o++] = pairs[i].count;
+            }
+        }
+        out->part_off[partitions] = o;
+        out->n_unique = o;
+        free(pairs);
+        for (int t = 0; t < threads; t++) { out->n_kmers += ws[t].kmers; out->n_reads += ws[t].reads; }
+        for (int t = 0; t < threads; t++) {
+            for (uint32_t p = 0; p < partitions; p++) { free(ws[t].rk[p]); free(ws[t].rc[p]); }
+            free(ws[t].rk); free(ws[t].rc); free(ws[t].rn);
+        }
+    }
+    for (int t = 0; t < threads; t++) {
+        free(ws[t].bk); free(ws[t].bn); free(ws[t].bcap); free(ws[t].wc); free(ws[t].wl);
+    }
+    free(ws); free(th); free(chunks);
+    return 0;
+}
+
 void ok_table_free(ok_table *t) {
     free(t->part_off); free(t->keys); free(t->counts);
     memset(t, 0, sizeof *t);
@@ -1230,6 +1521,7 @@ static double now_s(void) {
 static int usage(void) {
     fprintf(stderr,
             "kite_oracle count IN.fastq OUT.kite K CANON PARTITIONS CHUNK_BYTES THREADS [MIN|-] [MAX|-]\n"
+            "kite_oracle count-ref IN.fastq /dev/null K CANON PARTITIONS CHUNK_BYTES THREADS  (k <= 31)\n"
             "kite_oracle dump IN.kite OUT.tsv|-\n"
             "kite_oracle histo OUT.tsv IN.kite...\n"
             "kite_oracle merge OUT.kite LANES IN.kite...\n");
@@ -1268,6 +1560,27 @@ int main(int argc, char **argv) {
                (unsigned long long)t.n_unique, t1 - t0);
         ok_table_free(&t);
         if (rc) { fprintf(stderr, "error: %s\n", ok_last_error()); return 1; }
+        return 0;
+    }
+    if (!strcmp(argv[1], "count-ref")) {
+        if (argc < 9) return usage();
+        int fd = open(argv[2], O_RDONLY);
+        if (fd < 0) { perror("open"); return 1; }
+        struct stat st;
+        fstat(fd, &st);
+        size_t len = (size_t)st.st_size;
+        const uint8_t *img = len ? (const uint8_t *)mmap(NULL, len, PROT_READ, MAP_PRIVATE, fd, 0) : (const uint8_t *)"";
+        ok_ref_stats rs;
+        double t0 = now_s();
+        if (ok_count_ref(img, len, atoi(argv[4]), atoi(argv[5]), (uint32_t)strtoul(argv[6], 0, 10),
+                         (size_t)strtoull(argv[7], 0, 10), atoi(argv[8]), NULL, &rs)) {
+            fprintf(stderr, "error: %s\n", ok_last_error());
+            return 1;
+        }
+        double t1 = now_s();
+        printf("{\"reads\": %llu, \"kmers\": %llu, \"run_rows\": %llu, \"encoded_bytes\": %llu, \"count_s\": %.6f}\n",
+               (unsigned long long)rs.reads, (unsigned long long)rs.kmers, (unsigned long long)rs.run_rows,
+               (unsigned long long)rs.encoded_bytes, t1 - t0);
         return 0;
     }
     if (!strcmp(argv[1], "dump")) {

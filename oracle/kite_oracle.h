@@ -68,6 +68,20 @@ int ok_count_fastq(const uint8_t *data, size_t len, int k, int canonical,
                    uint32_t partitions, size_t chunk_size, int threads, ok_table *out);
 void ok_table_free(ok_table *t);
 
+/* The reference's count loop for one-word keys (k <= 31), for the CPU
+ * baseline: chunks taken in turn by `threads` workers, inline rolling
+ * k-mers into 64-entry write-combining partition slots, then per worker and
+ * partition one run: MSD/LSD byte radix sort, saturating RLE, DVLE encode
+ * (count/mod.rs:512-540, worker.rs:56-449, sorter.rs:67-272, 514-530).
+ * `out` (may be NULL) receives the runs merged into totals (tests). */
+typedef struct ok_ref_stats {
+    uint64_t reads, kmers;
+    uint64_t run_rows;       /* rows over every worker's runs */
+    uint64_t encoded_bytes;  /* DVLE bytes over every run */
+} ok_ref_stats;
+int ok_count_ref(const uint8_t *data, size_t len, int k, int canonical, uint32_t partitions,
+                 size_t chunk_size, int threads, ok_table *out, ok_ref_stats *st);
+
 /* Radix sort + saturating RLE of raw keys (count/sorter.rs:514-530):
  * sorts `n` keys of `words` words in place, writes unique keys (in place,
  * compacted) and u32 saturating run lengths; returns the unique count. */

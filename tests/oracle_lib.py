@@ -51,6 +51,9 @@ def lib() -> C.CDLL:
         L.ok_count_fastq.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_uint32,
                                      C.c_size_t, C.c_int, C.POINTER(OkTable)]
         L.ok_table_free.argtypes = [C.POINTER(OkTable)]
+        L.ok_count_ref.restype = C.c_int
+        L.ok_count_ref.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_uint32, C.c_size_t, C.c_int,
+                                   C.POINTER(OkTable), C.POINTER(OkRefStats)]
         L.ok_sort_rle.restype = C.c_size_t
         L.ok_sort_rle.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_void_p]
         L.ok_dvle_encode.restype = C.c_size_t
@@ -125,6 +128,11 @@ def plain_chunks(data: bytes | np.ndarray, chunk_size: int) -> list[tuple[int, i
     return [(int(out[2 * i]), int(out[2 * i + 1])) for i in range(n)]
 
 
+class OkRefStats(C.Structure):
+    _fields_ = [("reads", C.c_uint64), ("kmers", C.c_uint64), ("run_rows", C.c_uint64),
+                ("encoded_bytes", C.c_uint64)]
+
+
 class Table:
     """Oracle count table copied into numpy arrays."""
 
@@ -151,6 +159,23 @@ def count_fastq(data: bytes | np.ndarray, k: int, canonical: bool, partitions: i
     out = Table(t)
     lib().ok_table_free(C.byref(t))
     return out
+
+
+def count_ref(data: bytes | np.ndarray, k: int, canonical: bool, partitions: int,
+              chunk_size: int = 8 << 20, threads: int = 4) -> tuple[Table, dict]:
+    """ok_count_ref (the reference-shaped count of the CPU baseline): its
+    per-worker runs merged into a table, and its run statistics."""
+    arr = np.frombuffer(data, dtype=np.uint8) if isinstance(data, (bytes, bytearray)) else data
+    arr = np.ascontiguousarray(arr)
+    t, st = OkTable(), OkRefStats()
+    rc = lib().ok_count_ref(arr.ctypes.data if arr.size else None, arr.size, k, int(canonical), partitions,
+                            chunk_size, threads, C.byref(t), C.byref(st))
+    if rc:
+        raise RuntimeError(_err())
+    out = Table(t)
+    lib().ok_table_free(C.byref(t))
+    return out, {"reads": st.reads, "kmers": st.kmers, "run_rows": st.run_rows,
+                 "encoded_bytes": st.encoded_bytes}
 
 
 def sort_rle(keys: np.ndarray, words: int, k: int) -> tuple[np.ndarray, np.ndarray]:

@@ -141,3 +141,48 @@ def test_exact_windows_counts():
     for k in (1, 5, 21, 31):
         exp = sum(len(brute.kmers(s, k, False)) for s in brute.fastq_sequences(text.encode()))
         assert brute.exact_windows(arr, k, reads_per_block=37) == exp, k
+
+
+def test_reference_shaped_count_matches_checker():
+    """ok_count_ref (the CPU baseline's restatement of the reference count
+    loop: write-combined partition slots, MSD/LSD byte radix per worker run,
+    RLE, DVLE) must give the checker's totals once its per-worker runs are
+    merged: on the brute-force records, with chunks that do not start with
+    '@', and on partitions large enough for the MSD path (>= 131072 keys in
+    one worker's run)."""
+    rng = random.Random(7)
+    recs = []
+    for i in range(300):
+        L = rng.randint(0, 80)
+        recs.append(f"@r{i}\n{rand_seq(rng, L, 'AAACCGGTTTN')}\n+\n{'#' * L}\n")
+    fq = "".join(recs).encode()
+    for k, canon, P, chunk, threads in [(4, False, 2, 512, 2), (11, True, 4, 300, 3), (21, False, 1, 1 << 20, 1),
+                                        (31, True, 8, 700, 4)]:
+        ref, st = ol.count_ref(fq, k, canon, P, chunk_size=chunk, threads=threads)
+        chk = ol.count_fastq(fq, k, canon, P, chunk_size=chunk, threads=threads)
+        assert np.array_equal(ref.part_off, chk.part_off), (k, P)
+        assert np.array_equal(ref.keys, chk.keys) and np.array_equal(ref.counts, chk.counts), (k, P)
+        assert st["kmers"] == chk.n_kmers and st["reads"] == chk.n_reads
+        assert st["run_rows"] >= chk.n_unique and st["encoded_bytes"] > 0
+    good = b"@a\nACGTACGTAC\n+\nIIIIIIIIII\n"
+    bad = b"ja\nACGTACGTAC\n+\nIIIIIIIIII\n"
+    fq2 = good * 2 + bad + good
+    ref, st = ol.count_ref(fq2, 5, False, 1, chunk_size=54, threads=2)
+    assert st["reads"] == 2 and ref.n_reads == 2
+    # one partition of 400 k random 31-mers from a 300 k-base genome: the
+    # worker's run takes the MSD path with leaves below 1024 keys
+    g = "".join(rng.choice("ACGT") for _ in range(300_000))
+    reads = []
+    for i in range(4000):
+        s = rng.randrange(0, len(g) - 150)
+        reads.append(f"@m{i}\n{g[s:s + 150]}\n+\n{'I' * 150}\n")
+    fq3 = "".join(reads).encode()
+    ref, st = ol.count_ref(fq3, 31, True, 1, chunk_size=1 << 24, threads=1)
+    chk = ol.count_fastq(fq3, 31, True, 1, chunk_size=1 << 24, threads=1)
+    assert st["kmers"] == 4000 * 120 >= 131072
+    assert np.array_equal(ref.keys, chk.keys) and np.array_equal(ref.counts, chk.counts)
+    # the run holds the unique keys of the one worker, DVLE-encoded
+    assert st["run_rows"] == chk.n_unique
+    assert st["encoded_bytes"] == len(ol.dvle_encode(chk.keys, chk.counts.astype(np.uint32), 1))
+    with pytest.raises(RuntimeError, match="one-word keys"):
+        ol.count_ref(fq, 32, True, 2)
