@@ -23,13 +23,26 @@ namespace kite {
 
 namespace {
 
-// RFC 1951 3.2.5: length codes 257..285 and distance codes 0..29
-__constant__ uint16_t c_lbase[29] = {3,  4,  5,  6,  7,  8,  9,  10, 11,  13,  15,  17,  19,  23, 27,
-                                     31, 35, 43, 51, 59, 67, 83, 99, 115, 131, 163, 195, 227, 258};
-__constant__ uint8_t c_lext[29] = {0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 2, 3, 3, 3, 3, 4, 4, 4, 4, 5, 5, 5, 5, 0};
-__constant__ uint16_t c_dbase[30] = {1,   2,   3,   4,   5,   7,    9,    13,   17,   25,   33,   49,   65,    97,    129,
-                                     193, 257, 385, 513, 769, 1025, 1537, 2049, 3073, 4097, 6145, 8193, 12289, 16385, 24577};
-__constant__ uint8_t c_dext[30] = {0, 0, 0, 0, 1, 1, 2, 2, 3, 3, 4, 4, 5, 5, 6, 6, 7, 7, 8, 8, 9, 9, 10, 10, 11, 11, 12, 12, 13, 13};
+// RFC 1951 3.2.5: the base and extra bits of length code 257 + s (s < 29)
+// and of distance code d (d < 30), from the closed forms the tables follow:
+// after the first 8 lengths (3..10) each group of four codes doubles the step
+// and adds an extra bit, up to code 284; code 285 is 258.  Distances 1..4
+// have no extra bits, then each pair of codes does the same.  (Computed, not
+// looked up: a table in device memory costs the serial decoder a load round
+// trip per match.)
+__device__ __forceinline__ uint32_t len_base(uint32_t s, int *extra) {
+    if (s < 8) { *extra = 0; return s + 3; }
+    if (s >= 28) { *extra = 0; return 258; }
+    const int e = (int)(s >> 2) - 1;
+    *extra = e;
+    return ((4u + (s & 3u)) << e) + 3u;
+}
+__device__ __forceinline__ uint32_t dist_base(uint32_t d, int *extra) {
+    if (d < 4) { *extra = 0; return d + 1; }
+    const int e = (int)(d >> 1) - 1;
+    *extra = e;
+    return ((2u + (d & 1u)) << e) + 1u;
+}
 // RFC 1951 3.2.7: order of the code length code lengths
 __constant__ uint8_t c_clorder[19] = {16, 17, 18, 0, 8, 7, 9, 6, 10, 5, 11, 4, 12, 3, 13, 2, 14, 1, 15};
 
@@ -308,13 +321,17 @@ __global__ __launch_bounds__(64, 4) void k_inflate(const uint8_t *__restrict__ c
                     e = E_STREAM;
                     break;
                 }
-                const uint32_t len = c_lbase[sym] + b.get(c_lext[sym]);
+                int lx;
+                const uint32_t lb = len_base((uint32_t)sym, &lx);
+                const uint32_t len = lb + b.get(lx);
                 const int ds = decode_fast(b, t.lut_d, DB, t.dcount, t.dsym);
                 if (ds < 0 || ds >= 30) {
                     e = E_STREAM;
                     break;
                 }
-                const uint32_t dist = c_dbase[ds] + b.get(c_dext[ds]);
+                int dx;
+                const uint32_t db = dist_base((uint32_t)ds, &dx);
+                const uint32_t dist = db + b.get(dx);
                 if (dist > produced) {
                     e = E_STREAM;
                     break;
