@@ -9,13 +9,14 @@ D2H of the chunk payloads a .kite file holds.  `value` = input k-mers of
 all 24 samples per step / time per step.  Beside it: the plain per-sample
 path on the same cohort ("batched vs plain"), the dominant kernel's
 roofline (HIP events over the timed steps) and the PCIe rate against the
-pinned-copy peak measured in the same run.
+PCIe 5.0 x16 link (and the pinned-copy rate measured in the same run).
 
 More measurements ride on the same line: Pearson + Spearman of the
 cohort's union at --min 2 (split-bf16 MFMA Gram against the FP64 Gram and
 the FP64 oracle), the single-sample k=31 count with its FASTQ resident in
 HBM (the count kernels' own rate), `kite dump`, BGZF inflate, and the CPU
-baseline (the oracle's count on a bounded sample of the cohort).
+baseline (the reference's count loop restated in C, on a bounded sample of
+the cohort).
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -770,7 +771,7 @@ def main() -> int:
     cpu = None
     if not args.no_cpu_baseline:
         idx = [int(x) for x in args.cpu_samples.split(",") if x != ""]
-        log(f"cpu baseline: oracle count on cohort samples {idx}, {threads} threads")
+        log(f"cpu baseline: the reference's count loop in C on cohort samples {idx}, {threads} threads")
         cpu = cpu_baseline([pins[i].array for i in idx], [ws[i].name for i in idx], threads)
         log(f"cpu baseline: {cpu['value'] / 1e6 if cpu['value'] else 0:.1f} M k-mers/s")
     for p in pins:
