@@ -388,6 +388,84 @@ def bench_analyze(ctx, tables: list, min_count: int, cpu_rows_frac_parts: int) -
     return out
 
 
+def merge_inputs(dirpath: Path, reads: int) -> list[Path]:
+    """Two k=63 canonical samples of the 100 Mbp genome (P=1024) with
+    different read seeds, each counted by `kite count` on the GPU into
+    dirpath/s0.kite and dirpath/s1.kite."""
+    import dataclasses
+    import subprocess
+    from kite_amd.workloads import K63_CANON_100MBP
+    base = K63_CANON_100MBP.scaled(reads)
+    samples = [dataclasses.replace(base, seed=3003, genome_seed=3003),
+               dataclasses.replace(base, seed=3004, genome_seed=3003, name=base.name + "-second")]
+    genome = samples[0].genome()
+    kite_bin = ROOT / "kite_amd" / "lib" / "kite"
+    out = []
+    for i, w in enumerate(samples):
+        fq_path = dirpath / f"s{i}.fq"
+        w.fastq(genome=genome).tofile(fq_path)
+        kp = dirpath / f"s{i}.kite"
+        r = subprocess.run([str(kite_bin), "count", "-i", str(fq_path), "-o", str(kp), "-k", "63", "--canonical",
+                            "true", "--partitions", "1024"], capture_output=True, text=True, timeout=600)
+        if r.returncode:
+            raise RuntimeError(f"kite count failed: {r.stderr[-2000:]}")
+        fq_path.unlink()
+        out.append(kp)
+    return out
+
+
+def bench_merge(reads: int = 1_000_000) -> dict:
+    """`kite merge` of two k=63 samples (merge_inputs): the device engine
+    (sorted union with u64 sums in one merge-path launch per partition
+    batch) against the host engine (`--engine cpu`: the same union on this
+    host's worker threads), byte-identical outputs.  Both engines share the
+    file reading, partition decode and chunk encode of the CLI; `merge_s`
+    is the union itself (device: H2D of the decoded rows, kernels, D2H)."""
+    import re
+    import subprocess
+    import tempfile
+    kite_bin = ROOT / "kite_amd" / "lib" / "kite"
+    tmp = Path(tempfile.mkdtemp(prefix="kite_merge_", dir="/dev/shm" if os.path.isdir("/dev/shm") else None))
+    try:
+        t0 = time.perf_counter()
+        ins = merge_inputs(tmp, reads)
+        log(f"merge inputs: 2 x {reads} reads (k=63, P=1024) counted in {time.perf_counter() - t0:.1f} s, "
+            f"{sum(p.stat().st_size for p in ins) / 1e9:.2f} GB .kite")
+        res = {}
+        for engine in ("gpu", "cpu"):
+            best = None
+            for _ in range(2):
+                outp = tmp / f"m_{engine}.kite"
+                t = time.perf_counter()
+                r = subprocess.run([str(kite_bin), "merge", "--engine", engine, "-i", *map(str, ins), "-o", str(outp)],
+                                   capture_output=True, text=True, timeout=600)
+                wall = time.perf_counter() - t
+                if r.returncode:
+                    raise RuntimeError(f"kite merge failed: {r.stderr[-2000:]}")
+                m = re.search(r"wrote (\d+) rows .* merge ([\d.]+)s", r.stderr)
+                cur = {"wall_s": wall, "merge_s": float(m.group(2)), "rows": int(m.group(1))}
+                if best is None or cur["wall_s"] < best["wall_s"]:
+                    best = cur
+            res[engine] = best
+            log(f"kite merge --engine {engine}: {best['wall_s']:.2f} s wall, union {best['merge_s'] * 1e3:.1f} ms, "
+                f"{best['rows']} rows")
+        same = (tmp / "m_gpu.kite").read_bytes() == (tmp / "m_cpu.kite").read_bytes()
+        in_rows = []
+        for p in ins:
+            r = subprocess.run([str(kite_bin), "info", "-i", str(p)], capture_output=True, text=True, timeout=120)
+            mm = re.search(r"Total Rows:\s*(\d+)", r.stdout)
+            in_rows.append(int(mm.group(1)) if mm else None)
+        return {"workload": f"two k63-canon-100mbp samples of {reads} reads (seeds 3003, 3004), P=1024",
+                "input_kite_bytes": [p.stat().st_size for p in ins], "input_rows": in_rows,
+                "gpu": res["gpu"], "cpu": res["cpu"], "bit_exact_gpu_vs_cpu": same,
+                "union_speedup": res["cpu"]["merge_s"] / res["gpu"]["merge_s"],
+                "includes": "wall: process start, .kite reads, partition decode, union, chunk encode, .kite write "
+                            "(/dev/shm); merge_s: the union only"}
+    finally:
+        import shutil
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def bench_dump(w, fq) -> dict:
     """`kite dump` of the workload's .kite (written by `kite count` on the
     GPU): the device engine against the host engine (partition workers on
@@ -711,6 +789,10 @@ def main() -> int:
     ap.add_argument("--no-single", action="store_true", help="skip the HBM-resident single-sample leg")
     ap.add_argument("--no-dump", action="store_true", help="skip the kite dump leg")
     ap.add_argument("--no-bgzf", action="store_true", help="skip the BGZF inflate leg")
+    ap.add_argument("--no-merge", action="store_true", help="skip the kite merge leg")
+    ap.add_argument("--merge-reads", type=int, default=1_000_000, help="reads of each k=63 merge input")
+    ap.add_argument("--merge-inputs", default="",
+                    help="profiling runs: only write the two k=63 merge inputs into this directory and exit")
     ap.add_argument("--analyze-cpu-parts", type=int, default=8,
                     help="partitions (of 256) whose union rows the CPU oracle sample covers")
     args = ap.parse_args()
@@ -730,6 +812,13 @@ def main() -> int:
     except KiteError as e:
         print(f"bench.py needs a HIP device: {e}", file=sys.stderr)
         return 1
+    if args.merge_inputs:
+        d = Path(args.merge_inputs)
+        d.mkdir(parents=True, exist_ok=True)
+        ctx.close()
+        for p in merge_inputs(d, args.merge_reads):
+            log(f"wrote {p} ({p.stat().st_size / 1e9:.2f} GB)")
+        return 0
     if args.calibrate_copy:
         calibrate_copy()
     if args.no_batch:
@@ -786,6 +875,9 @@ def main() -> int:
         fq = w.fastq() if fq is None else fq
         single = single or {}
         single["dump"] = bench_dump(w, fq)
+    merge = None
+    if not args.no_merge:
+        merge = bench_merge(args.merge_reads)
     if not args.no_bgzf:
         fq = w.fastq() if fq is None else fq
         bgz = bench_bgzf(ctx, w, fq, threads)
@@ -827,6 +919,7 @@ def main() -> int:
         "autotune": tune,
         "analyze": analyze,
         "single_sample_hbm": single,
+        "merge": merge,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

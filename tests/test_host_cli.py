@@ -156,3 +156,27 @@ def test_memory_spec_errors(tmp_path):
     for spec, msg in [("12X", "suffix must be M, G, or T"), ("0", "greater than zero"), ("G", "expected formats")]:
         r = kite("count", "-i", fq, "-o", tmp_path / "x.kite", "--memory", spec, check=False)
         assert r.returncode == 1 and msg in r.stderr, r.stderr
+
+
+@pytest.mark.parametrize("k,canon,P,n_inputs", [(21, True, 16, 2), (63, False, 64, 3), (77, True, 4, 2)])
+def test_cli_merge_cpu_engine_matches_oracle(tmp_path, k, canon, P, n_inputs):
+    """`kite merge --engine cpu` (the host engine: per-partition union with
+    u64 sums on the worker threads) writes the same bytes as the oracle's
+    merge, header filters and markers included; no device is opened."""
+    rng = random.Random(k + n_inputs)
+    paths = []
+    for i in range(n_inputs):
+        recs = []
+        for r in range(1500):
+            L = rng.randint(60, 180)
+            s = "".join(rng.choice("ACGT") for _ in range(L))
+            recs.append(f"@r{r}\n{s}\n+\n{'I' * L}\n")
+        t = ol.count_fastq(("".join(recs) * 2).encode(), k, canon, P)
+        p = tmp_path / f"in{i}.kite"
+        ol.write_table(p, t, min_count=2 if i == 0 else None)
+        paths.append(p)
+    lanes = min(6, max(2, ((os.cpu_count() or 1) + 7) // 8))
+    r = kite("merge", "--engine", "cpu", "-i", *paths, "-o", tmp_path / "c.kite")
+    assert "engine=cpu" in r.stderr
+    ol.kite_merge(paths, tmp_path / "o.kite", lanes=lanes)
+    assert (tmp_path / "c.kite").read_bytes() == (tmp_path / "o.kite").read_bytes()
