@@ -393,6 +393,42 @@ int kite_gpu_histo(kite_gpu_ctx *ctx, uint32_t key_width, uint32_t partitions, c
                    uint64_t n_runs, int has_min, uint32_t min_count, int has_max, uint32_t max_count,
                    kite_histo_fn fn, void *user, kite_dump_stats *stats);
 
+/* ---- merge of stored runs ------------------------------------------------------
+ * `kite merge` on the device, partition by partition as
+ * build_merge_partition_chunk does (src/merge/mod.rs:198-214): each input's
+ * stored runs of the partition (as for kite_gpu_dump) are decoded and summed
+ * per key with u64 saturation (load_partition_counts, src/kiteops/mod.rs:
+ * 433-473), that input's header filter is applied to each total
+ * (apply_reader_filter_to_rows, :103-108), the inputs are merged with u64
+ * saturating sums (merge_vectors_total, :587-724), and the partition becomes
+ * one chunk (serialize_partition_chunk, :804-859): DVLE for 64- and 128-bit
+ * keys, RAW for 256-bit keys.  `fn` receives the chunks of the non-empty
+ * partitions in partition order, as the merge hands them to append_chunk
+ * (src/merge/mod.rs:134-151).  A total above u32::MAX fails with
+ * KITE_E_OVERFLOW ("merged count exceeds u32 for key <key>"), before any
+ * chunk of the partition group that holds it is passed on.  Only payload
+ * bytes cross the host link: the stored payloads in, the chunks out. */
+typedef struct kite_merge_input {
+    const kite_dump_run *runs;  /* the input's stored runs, partitions non-decreasing */
+    uint64_t n_runs;
+    int has_min;                /* the input header's count filter */
+    uint32_t min_count;
+    int has_max;
+    uint32_t max_count;
+} kite_merge_input;
+
+typedef struct kite_merge_stats {
+    uint64_t runs, rows_in;   /* stored runs and rows read */
+    uint64_t rows_kept;       /* rows of all inputs after the header filters */
+    uint64_t rows, bytes;     /* merged rows, chunk payload bytes */
+    uint64_t groups;          /* partition groups through the device */
+    double decode_ms, merge_ms, encode_ms; /* device time per stage */
+    double in_ms, out_ms;     /* host wall time: payloads staged and copied in; chunks copied out and handed to fn */
+} kite_merge_stats;
+
+int kite_gpu_merge_runs(kite_gpu_ctx *ctx, uint32_t key_width, uint32_t partitions, const kite_merge_input *inputs,
+                        uint32_t n_inputs, kite_append_chunk_fn fn, void *user, kite_merge_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -99,6 +99,21 @@ class DumpStats(C.Structure):
 
 
 DUMP_WRITE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_char), C.c_uint64)
+
+
+class MergeInput(C.Structure):
+    _fields_ = [("runs", C.POINTER(DumpRun)), ("n_runs", C.c_uint64), ("has_min", C.c_int),
+                ("min_count", C.c_uint32), ("has_max", C.c_int), ("max_count", C.c_uint32)]
+
+
+class MergeStats(C.Structure):
+    _fields_ = [("runs", C.c_uint64), ("rows_in", C.c_uint64), ("rows_kept", C.c_uint64), ("rows", C.c_uint64),
+                ("bytes", C.c_uint64), ("groups", C.c_uint64), ("decode_ms", C.c_double),
+                ("merge_ms", C.c_double), ("encode_ms", C.c_double), ("in_ms", C.c_double),
+                ("out_ms", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
 HISTO_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_uint64)
 
 # name -> (restype, argtypes); every symbol include/kite_gpu.h declares
@@ -149,6 +164,8 @@ GPU_SYMBOLS = {
                                 C.c_void_p, C.POINTER(DumpStats)]),
     "kite_gpu_histo": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(DumpRun), C.c_uint64, C.c_int,
                                  C.c_uint32, C.c_int, C.c_uint32, HISTO_FN, C.c_void_p, C.POINTER(DumpStats)]),
+    "kite_gpu_merge_runs": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(MergeInput), C.c_uint32,
+                                      APPEND_FN, C.c_void_p, C.POINTER(MergeStats)]),
     "kite_gpu_autotune": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_uint32, C.c_void_p, C.c_size_t,
                                     C.c_char_p, C.c_size_t]),
     "kite_gpu_set_tuned": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_int,
@@ -591,6 +608,35 @@ class GpuContext:
                                       int(min_count is not None), int(min_count or 0),
                                       int(max_count is not None), int(max_count or 0), fn, None, C.byref(st))
         del keep
+        _check(rc)
+        return out, st.as_dict()
+
+    def merge_runs(self, key_width: int, partitions: int, inputs) -> tuple[list[tuple], dict]:
+        """`kite merge` of stored runs on the device (kite_gpu_merge_runs).
+        inputs: [(runs, min_count, max_count)], runs as for dump_runs.
+        Returns the chunks (partition, codec, rows, raw_bytes, key_min[4],
+        key_max[4], payload bytes) in partition order, and the stats."""
+        arrs, keeps = [], []
+        ins = (MergeInput * len(inputs))()
+        for i, (runs, mn, mx) in enumerate(inputs):
+            arr, keep = self._runs_array(runs)
+            arrs.append(arr)
+            keeps.append(keep)
+            ins[i].runs = C.cast(arr, C.POINTER(DumpRun))
+            ins[i].n_runs = len(runs)
+            ins[i].has_min, ins[i].min_count = int(mn is not None), int(mn or 0)
+            ins[i].has_max, ins[i].max_count = int(mx is not None), int(mx or 0)
+        out: list[tuple] = []
+
+        def cb(user, part, codec, rows, raw, kmin, kmax, data, ln):
+            out.append((part, codec, rows, raw, tuple(kmin[i] for i in range(4)),
+                        tuple(kmax[i] for i in range(4)), C.string_at(data, ln)))
+            return 0
+
+        fn = APPEND_FN(cb)
+        st = MergeStats()
+        rc = self._lib.kite_gpu_merge_runs(self._h, key_width, partitions, ins, len(inputs), fn, None, C.byref(st))
+        del keeps, arrs
         _check(rc)
         return out, st.as_dict()
 

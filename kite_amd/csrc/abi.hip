@@ -184,6 +184,14 @@ struct kite_gpu_ctx {
     kite::HostPinned dump_stage, dump_text;
     kite::DevBuf dump_pay;
 
+    // merge of stored runs: per-input tables of a partition group, their
+    // union, its u32 counts and chunk encode buffers, pinned bounce buffers
+    std::vector<kite::DevTable> merge_tabs;
+    kite::DevTable merge_res;
+    kite::DevBuf merge_c32;
+    kite::EncodeBufs merge_enc;
+    kite::HostPinned merge_in[2], merge_out[2];
+
     // BGZF input inflated on the device into `staging`: the staged bytes
     // then hold whole records plus the start of the next one
     bool bgzf_feed = false;
@@ -1734,3 +1742,261 @@ int kite_gpu_histo(kite_gpu_ctx *ctx, uint32_t key_width, uint32_t partitions, c
 }
 
 }  // extern "C"
+
+namespace {
+
+// One host byte range bound for device memory: src[0, bytes) to dst.
+struct HostRange {
+    const uint8_t *src;
+    uint64_t bytes;
+    uint64_t dst;  // byte offset in the device buffer
+};
+
+// Copies the ranges (ascending, disjoint destinations) into dev[0, total)
+// through two page-locked bounce buffers of PIECE bytes: up to 16 host
+// threads fill one piece while the H2D of the other runs.  Gaps between
+// ranges are left as they are.
+void h2d_ranges(kite_gpu_ctx *ctx, uint8_t *dev, uint64_t total, const std::vector<HostRange> &rs) {
+    constexpr uint64_t PIECE = 256ull << 20;
+    hipStream_t s = ctx->stream;
+    const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    hipEvent_t done[2];
+    for (auto &e : done) KITE_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    struct Ev {
+        hipEvent_t *e;
+        ~Ev() {
+            (void)hipEventDestroy(e[0]);
+            (void)hipEventDestroy(e[1]);
+        }
+    } ev{done};
+    size_t r0 = 0;  // first range that may overlap the current piece
+    for (uint64_t q = 0, a = 0; a < total; q++, a += PIECE) {
+        const uint64_t b = std::min(total, a + PIECE);
+        uint8_t *pin = static_cast<uint8_t *>(ctx->merge_in[q & 1].ensure(PIECE));
+        if (q >= 2) KITE_HIP(hipEventSynchronize(done[q & 1]));  // its last H2D has read it
+        // the parts of the ranges inside [a, b), cut in 4 MiB copies
+        struct Seg {
+            const uint8_t *src;
+            uint64_t bytes, at;
+        };
+        std::vector<Seg> seg;
+        while (r0 < rs.size() && rs[r0].dst + rs[r0].bytes <= a) r0++;
+        for (size_t r = r0; r < rs.size() && rs[r].dst < b; r++) {
+            const uint64_t lo = std::max(a, rs[r].dst), hi = std::min(b, rs[r].dst + rs[r].bytes);
+            for (uint64_t x = lo; x < hi; x += 4ull << 20)
+                seg.push_back(Seg{rs[r].src + (x - rs[r].dst), std::min<uint64_t>(4ull << 20, hi - x), x - a});
+        }
+        std::atomic<size_t> next{0};
+        auto copy = [&] {
+            for (size_t i; (i = next.fetch_add(1)) < seg.size();) memcpy(pin + seg[i].at, seg[i].src, seg[i].bytes);
+        };
+        std::vector<std::thread> th;
+        for (unsigned t = 1; t < std::min<size_t>(nt, seg.size()); t++) th.emplace_back(copy);
+        copy();
+        for (auto &x : th) x.join();
+        KITE_HIP(hipMemcpyAsync(dev + a, pin, b - a, hipMemcpyHostToDevice, s));
+        KITE_HIP(hipEventRecord(done[q & 1], s));
+    }
+}
+
+std::string u128_decimal(uint64_t lo, uint64_t hi) {
+    unsigned __int128 v = ((unsigned __int128)hi << 64) | lo;
+    std::string d;
+    do {
+        d.push_back(char('0' + (int)(v % 10)));
+        v /= 10;
+    } while (v);
+    return std::string(d.rbegin(), d.rend());
+}
+
+}  // namespace
+
+extern "C" {
+
+int kite_gpu_merge_runs(kite_gpu_ctx *ctx, uint32_t key_width, uint32_t partitions, const kite_merge_input *inputs,
+                        uint32_t n_inputs, kite_append_chunk_fn fn, void *user, kite_merge_stats *stats) {
+    if (!ctx || !fn || (n_inputs && !inputs)) return set_error(KITE_E_INVALID, "invalid argument: NULL pointer");
+    if (n_inputs == 0) return set_error(KITE_E_INVALID, "invalid argument: no inputs");
+    if (partitions == 0) return set_error(KITE_E_INVALID, "invalid partition count 0");
+    for (uint32_t i = 0; i < n_inputs; i++)
+        if (int rc = check_runs(key_width, partitions, inputs[i].runs, inputs[i].n_runs)) return rc;
+    const int W = (int)(key_width / 64);
+    return guarded([&]() -> int {
+        bind(ctx);
+        hipStream_t s = ctx->stream;
+        kite_merge_stats st{};
+        // first run of every partition, per input
+        std::vector<std::vector<uint64_t>> first(n_inputs, std::vector<uint64_t>((size_t)partitions + 1, 0));
+        for (uint32_t i = 0; i < n_inputs; i++) {
+            const kite_merge_input &in = inputs[i];
+            uint64_t r = 0;
+            for (uint32_t p = 0; p <= partitions; p++) {
+                while (r < in.n_runs && in.runs[r].partition < p) r++;
+                first[i][p] = r;
+            }
+            st.runs += in.n_runs;
+        }
+        // partition groups bounded by rows over all inputs (device memory is
+        // about six copies of a row of key and u64 count, plus its chunk
+        // bytes) and by stored payload bytes
+        uint64_t row_cap = std::min<uint64_t>(384ull << 20, std::max<uint64_t>(1, ctx->budget / (6 * (8 * W + 8) + 64)));
+        if (const char *e = getenv("KITE_MERGE_GROUP_ROWS")) row_cap = std::max<uint64_t>(1, strtoull(e, nullptr, 10));
+        constexpr uint64_t PAY_CAP = 8ull << 30;
+        if (ctx->merge_tabs.size() != n_inputs) {
+            std::vector<kite::DevTable> fresh(n_inputs);  // DevTable owns device memory and does not move
+            ctx->merge_tabs.swap(fresh);
+        }
+        const uint32_t codec = W == 4 ? KITE_CODEC_RAW : KITE_CODEC_DVLE;
+        uint32_t p0 = 0;
+        while (p0 < partitions) {
+            uint32_t p1 = p0;
+            uint64_t rows = 0, pay = 0;
+            while (p1 < partitions) {
+                uint64_t pr = 0, pb = 0;
+                for (uint32_t i = 0; i < n_inputs; i++)
+                    for (uint64_t r = first[i][p1]; r < first[i][p1 + 1]; r++) {
+                        pr += inputs[i].runs[r].rows;
+                        pb += (inputs[i].runs[r].bytes + 15) & ~15ull;
+                    }
+                if (p1 > p0 && (rows + pr > row_cap || pay + pb > PAY_CAP)) break;
+                rows += pr;
+                pay += pb;
+                p1++;
+            }
+            const uint32_t np = p1 - p0;
+            // stored payloads of the group: input by input, 16-byte aligned
+            std::vector<std::vector<kite::DumpRun>> dr(n_inputs);
+            std::vector<HostRange> ranges;
+            uint64_t off = 0;
+            for (uint32_t i = 0; i < n_inputs; i++) {
+                uint32_t rank = 0;
+                for (uint64_t r = first[i][p0]; r < first[i][p1]; r++) {
+                    const kite_dump_run &x = inputs[i].runs[r];
+                    rank = (r > first[i][p0] && x.partition == inputs[i].runs[r - 1].partition) ? rank + 1 : 0;
+                    kite::DumpRun d{};
+                    d.src = off;
+                    d.bytes = x.bytes;
+                    d.rows = x.rows;
+                    d.codec = (W == 4 || x.codec == KITE_CODEC_RAW) ? kite::DUMP_CODEC_RAW : kite::DUMP_CODEC_DVLE;
+                    d.part = x.partition - p0;
+                    d.table = rank;
+                    dr[i].push_back(d);
+                    if (x.bytes) ranges.push_back(HostRange{x.data, x.bytes, off});
+                    off += (x.bytes + 15) & ~15ull;
+                }
+            }
+            const auto t_in = std::chrono::steady_clock::now();
+            uint8_t *dev = static_cast<uint8_t *>(ctx->dump_pay.ensure(off + 16));
+            h2d_ranges(ctx, dev, off + 16, ranges);
+            KITE_HIP(hipStreamSynchronize(s));
+            st.in_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_in).count();
+            st.rows_in += rows;
+
+            // every input decoded, summed per key and filtered into its table
+            for (uint32_t i = 0; i < n_inputs; i++) {
+                const uint64_t *k = nullptr, *c = nullptr, *o = nullptr;
+                kite::DumpTimes dt;
+                const uint64_t n = ctx->dump->decode_sum(dev, dr[i], np, W, &k, &c, &o, &dt);
+                st.decode_ms += dt.decode_ms + dt.merge_ms;
+                kite::DumpFilter f;
+                f.has_min = inputs[i].has_min ? 1u : 0u;
+                f.min = inputs[i].min_count;
+                f.has_max = inputs[i].has_max ? 1u : 0u;
+                f.max = inputs[i].max_count;
+                ctx->merge->filter_into(k, c, o, n, np, W, f, ctx->merge_tabs[i]);
+                st.rows_kept += ctx->merge_tabs[i].rows;
+            }
+            hipEvent_t e[4];
+            for (auto &x : e) KITE_HIP(hipEventCreate(&x));
+            struct Ev4 {
+                hipEvent_t *e;
+                ~Ev4() {
+                    for (int q = 0; q < 4; q++) (void)hipEventDestroy(e[q]);
+                }
+            } ev4{e};
+            KITE_HIP(hipEventRecord(e[0], s));
+            kite::DevTable &u = ctx->merge_res;
+            ctx->merge->merge_all(ctx->merge_tabs, np, W, u);
+            KITE_HIP(hipEventRecord(e[1], s));
+            // serialize_partition_chunk: every total must fit u32
+            uint32_t *c32 = static_cast<uint32_t *>(ctx->merge_c32.ensure(std::max<uint64_t>(u.rows, 1) * 4));
+            const uint64_t bad = ctx->merge->narrow_u32(u.counts.as<uint64_t>(), u.rows, c32);
+            if (bad < u.rows) {
+                uint64_t kw[4] = {0, 0, 0, 0};
+                KITE_HIP(hipMemcpy(kw, u.keys.as<uint64_t>() + bad * W, 8 * W, hipMemcpyDeviceToHost));
+                return set_error(KITE_E_OVERFLOW, W == 4 ? std::string("merged count exceeds u32")
+                                                         : "merged count exceeds u32 for key " +
+                                                               u128_decimal(kw[0], W == 2 ? kw[1] : 0));
+            }
+            uint64_t *hb = static_cast<uint64_t *>(ctx->merge_enc.host_byte_off.ensure(((size_t)np + 1) * 8));
+            const kite::EncodeDevice ed = kite::encode_on_device(u.keys.as<uint64_t>(), c32, u.off.as<uint64_t>(),
+                                                                 u.rows, np, W, s, ctx->merge_enc, hb);
+            KITE_HIP(hipEventRecord(e[2], s));
+            std::vector<uint64_t> boff(hb, hb + np + 1), roff((size_t)np + 1), mm((size_t)np * 8);
+            KITE_HIP(hipMemcpyAsync(roff.data(), u.off.ptr, ((size_t)np + 1) * 8, hipMemcpyDeviceToHost, s));
+            KITE_HIP(hipMemcpyAsync(mm.data(), ed.minmax, (size_t)np * 64, hipMemcpyDeviceToHost, s));
+            KITE_HIP(hipStreamSynchronize(s));
+            float ms = 0;
+            KITE_HIP(hipEventElapsedTime(&ms, e[0], e[1]));
+            st.merge_ms += ms;
+            KITE_HIP(hipEventElapsedTime(&ms, e[1], e[2]));
+            st.encode_ms += ms;
+            st.rows += u.rows;
+            st.bytes += ed.total;
+
+            // chunks out in pieces of whole partitions (a partition larger
+            // than a piece is a piece of its own), the next piece copied
+            // while fn takes the current one
+            const auto t_out = std::chrono::steady_clock::now();
+            constexpr uint64_t PIECE = 64ull << 20;
+            std::vector<std::pair<uint32_t, uint32_t>> pieces;
+            for (uint32_t a = 0; a < np;) {
+                uint32_t b = a + 1;
+                while (b < np && boff[b + 1] - boff[a] <= PIECE) b++;
+                pieces.emplace_back(a, b);
+                a = b;
+            }
+            hipEvent_t done[2];
+            for (auto &x : done) KITE_HIP(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+            struct Ev2 {
+                hipEvent_t *e;
+                ~Ev2() {
+                    (void)hipEventDestroy(e[0]);
+                    (void)hipEventDestroy(e[1]);
+                }
+            } ev2{done};
+            auto issue = [&](size_t q) {
+                const uint64_t a = boff[pieces[q].first], n = boff[pieces[q].second] - a;
+                uint8_t *pin = static_cast<uint8_t *>(ctx->merge_out[q & 1].ensure(std::max<uint64_t>(n, PIECE)));
+                if (n) KITE_HIP(hipMemcpyAsync(pin, ed.bytes + a, n, hipMemcpyDeviceToHost, s));
+                KITE_HIP(hipEventRecord(done[q & 1], s));
+            };
+            if (!pieces.empty()) issue(0);
+            for (size_t q = 0; q < pieces.size(); q++) {
+                KITE_HIP(hipEventSynchronize(done[q & 1]));
+                if (q + 1 < pieces.size()) issue(q + 1);
+                const uint8_t *pin = ctx->merge_out[q & 1].as<uint8_t>();
+                const uint64_t base = boff[pieces[q].first];
+                for (uint32_t p = pieces[q].first; p < pieces[q].second; p++) {
+                    const uint64_t nr = roff[p + 1] - roff[p];
+                    if (!nr) continue;
+                    const uint64_t len = boff[p + 1] - boff[p];
+                    if (fn(user, p0 + p, codec, nr, len, &mm[(size_t)p * 8], &mm[(size_t)p * 8 + 4],
+                           pin + (boff[p] - base), (size_t)len) != 0) {
+                        (void)hipStreamSynchronize(s);
+                        return set_error(KITE_E_CALLBACK, "merge chunk callback aborted");
+                    }
+                }
+            }
+            KITE_HIP(hipStreamSynchronize(s));
+            st.out_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_out).count();
+            st.groups++;
+            p0 = p1;
+        }
+        if (stats) *stats = st;
+        return KITE_OK;
+    });
+}
+
+}  // extern "C"
+

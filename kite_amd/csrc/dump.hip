@@ -513,7 +513,7 @@ inline dim3 grid_for(uint64_t n) { return dim3((unsigned)std::max<uint64_t>(1, c
 
 void DumpEngine::release() {
     for (DevBuf *b : {&runs_d_, &tile_run_, &tcnt_, &tbase_, &ends_, &vals_, &counts_, &scan_tmp_, &tlen_, &toff_,
-                      &text_, &err_, &tabptr_, &lines_, &loff_, &hsmall_, &hbig_d_, &ptile_, &ptrun_, &psum_})
+                      &text_, &err_, &tabptr_, &lines_, &loff_, &hsmall_, &hbig_d_, &ptile_, &ptrun_, &psum_, &poff_})
         b->release();
     tabs_.clear();
     merged_.keys.release();
@@ -724,6 +724,45 @@ uint64_t DumpEngine::run_w(const uint8_t *d_payload, const std::vector<DumpRun> 
     if (rows_out) *rows_out = rows;
     if (lines_out) *lines_out = hv[1];
     return bytes;
+}
+
+uint64_t DumpEngine::decode_sum(const uint8_t *d_payload, const std::vector<DumpRun> &runs, uint32_t np, int W,
+                                const uint64_t **keys, const uint64_t **counts, const uint64_t **off,
+                                DumpTimes *times) {
+    if (W == 1) return decode_sum_w<1>(d_payload, runs, np, keys, counts, off, times);
+    if (W == 2) return decode_sum_w<2>(d_payload, runs, np, keys, counts, off, times);
+    return decode_sum_w<4>(d_payload, runs, np, keys, counts, off, times);
+}
+
+template <int W>
+uint64_t DumpEngine::decode_sum_w(const uint8_t *d_payload, const std::vector<DumpRun> &runs, uint32_t np,
+                                  const uint64_t **keys, const uint64_t **counts, const uint64_t **off,
+                                  DumpTimes *times) {
+    hipStream_t s = stream_;
+    Events ev;
+    // stored counts are read as u64 and summed with saturation
+    const uint64_t rows = decode_merge_w<W>(d_payload, runs, np, false, keys, counts, ev.e);
+    uint32_t R = 0;
+    for (const DumpRun &r : runs)
+        if (r.rows) R = std::max(R, r.table + 1);
+    if (R > 1) {
+        *off = merged_.off.as<uint64_t>();  // the merge of the runs wrote the offsets
+    } else {
+        // one run per partition: its offsets follow from the run sizes
+        std::vector<uint64_t> h((size_t)np + 1, 0);
+        for (const DumpRun &r : runs) h[r.part + 1] += r.rows;
+        for (uint32_t p = 0; p < np; p++) h[p + 1] += h[p];
+        uint64_t *d = static_cast<uint64_t *>(poff_.ensure(((uint64_t)np + 1) * 8));
+        KITE_HIP(hipMemcpyAsync(d, h.data(), ((uint64_t)np + 1) * 8, hipMemcpyHostToDevice, s));
+        *off = d;
+    }
+    KITE_HIP(hipStreamSynchronize(s));  // (h is a local)
+    if (times) {
+        KITE_HIP(hipEventElapsedTime(&times->decode_ms, ev.e[0], ev.e[1]));
+        KITE_HIP(hipEventElapsedTime(&times->merge_ms, ev.e[1], ev.e[2]));
+        times->format_ms = 0;
+    }
+    return rows;
 }
 
 void DumpEngine::histo_begin() {

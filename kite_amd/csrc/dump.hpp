@@ -31,10 +31,6 @@ struct DumpRun {
 
 constexpr uint32_t HISTO_SMALL = 8192;  // totals below this are binned in LDS
 
-struct DumpFilter {
-    uint32_t has_min = 0, min = 0, has_max = 0, max = 0;
-};
-
 struct DumpTimes {
     float decode_ms = 0, merge_ms = 0, format_ms = 0;
 };
@@ -55,6 +51,13 @@ class DumpEngine {
                  const DumpFilter &f, uint64_t *rows_out, uint64_t *lines_out, DumpTimes *times);
     const char *text() const { return text_.as<char>(); }
 
+    // Decodes `runs` and sums the runs of each partition per key with u64
+    // saturation (load_partition_counts, src/kiteops/mod.rs:433-473): returns
+    // the rows, and device pointers to their keys, u64 counts and the np + 1
+    // partition row offsets, valid until the next call on this engine.
+    uint64_t decode_sum(const uint8_t *d_payload, const std::vector<DumpRun> &runs, uint32_t np, int W,
+                        const uint64_t **keys, const uint64_t **counts, const uint64_t **off, DumpTimes *times);
+
     // `kite histo` of one input over its partition groups: histo_begin,
     // then histo per group (decode, merge with u64 sums, filter on the
     // total, frequency of every total), then histo_end with the bins in
@@ -70,6 +73,9 @@ class DumpEngine {
     uint64_t decode_merge_w(const uint8_t *d_payload, const std::vector<DumpRun> &runs, uint32_t np, bool trunc32,
                             const uint64_t **keys, const uint64_t **counts, hipEvent_t ev[3]);
     template <int W>
+    uint64_t decode_sum_w(const uint8_t *d_payload, const std::vector<DumpRun> &runs, uint32_t np,
+                          const uint64_t **keys, const uint64_t **counts, const uint64_t **off, DumpTimes *times);
+    template <int W>
     void histo_w(const uint8_t *d_payload, const std::vector<DumpRun> &runs, uint32_t np, const DumpFilter &f,
                  uint64_t *rows_out, DumpTimes *times);
     template <int W>
@@ -79,7 +85,7 @@ class DumpEngine {
     hipStream_t stream_;
     MergeEngine *merge_;
     DevBuf runs_d_, tile_run_, tcnt_, tbase_, ends_, vals_, counts_, scan_tmp_, tlen_, toff_, text_, err_,
-        tabptr_, lines_, loff_, hsmall_, hbig_d_, ptile_, ptrun_, psum_;
+        tabptr_, lines_, loff_, hsmall_, hbig_d_, ptile_, ptrun_, psum_, poff_;
     std::map<uint64_t, uint64_t> hbig_;  // frequencies of totals >= HISTO_SMALL
     std::vector<DevTable> tabs_;
     DevTable merged_;

@@ -161,6 +161,49 @@ __global__ void k_zero_u32(uint32_t *p, uint64_t n) {
     if (i < n) p[i] = 0;
 }
 
+// 1 for every row whose count lies within the filter's bounds.
+__global__ void k_keep_flags(const uint64_t *__restrict__ counts, uint64_t n, DumpFilter f,
+                             uint32_t *__restrict__ flags) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t c = counts[i];
+    flags[i] = ((!f.has_min || c >= f.min) && (!f.has_max || c <= f.max)) ? 1u : 0u;
+}
+
+// Kept rows to their scanned positions.
+template <int W>
+__global__ void k_keep_rows(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ counts, uint64_t n,
+                            const uint32_t *__restrict__ flags, const uint64_t *__restrict__ pos,
+                            uint64_t *__restrict__ out_keys, uint64_t *__restrict__ out_counts) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !flags[i]) return;
+    const uint64_t o = pos[i];
+#pragma unroll
+    for (int r = 0; r < W; r++) out_keys[o * W + r] = keys[i * W + r];
+    out_counts[o] = counts[i];
+}
+
+// out[p] = pos[off[p]]: partition offsets after the compaction (pos holds
+// n + 1 entries, off[p] <= n).
+__global__ void k_gather_pos(const uint64_t *__restrict__ off, uint32_t m, const uint64_t *__restrict__ pos,
+                             uint64_t *__restrict__ out) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < m) out[p] = pos[off[p]];
+}
+
+__global__ void k_set_u64(unsigned long long *p, unsigned long long v) { *p = v; }
+
+// Counts narrowed to u32; the smallest row index whose count does not fit
+// goes to first_bad.
+__global__ void k_narrow_u32(const uint64_t *__restrict__ c, uint64_t n, uint32_t *__restrict__ out,
+                             unsigned long long *__restrict__ first_bad) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t v = c[i];
+    if (v > 0xFFFFFFFFull) atomicMin(first_bad, (unsigned long long)i);
+    out[i] = v > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)v;
+}
+
 // ---- host ----------------------------------------------------------------
 namespace {
 uint32_t blocks(uint64_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
@@ -317,6 +360,58 @@ void MergeEngine::download(const DevTable &t, int W, uint32_t P, uint64_t *keys,
         KITE_HIP(hipMemcpyAsync(counts, t.counts.ptr, t.rows * 8, hipMemcpyDeviceToHost, stream_));
     if (part_off) KITE_HIP(hipMemcpyAsync(part_off, t.off.ptr, (P + 1) * 8, hipMemcpyDeviceToHost, stream_));
     KITE_HIP(hipStreamSynchronize(stream_));
+}
+
+void MergeEngine::filter_into(const uint64_t *keys, const uint64_t *counts, const uint64_t *off, uint64_t n,
+                              uint32_t P, int W, const DumpFilter &f, DevTable &out) {
+    hipStream_t s = stream_;
+    uint64_t *ok = static_cast<uint64_t *>(out.keys.ensure(std::max<uint64_t>(n, 1) * 8 * W));
+    uint64_t *oc = static_cast<uint64_t *>(out.counts.ensure(std::max<uint64_t>(n, 1) * 8));
+    uint64_t *oo = static_cast<uint64_t *>(out.off.ensure(((uint64_t)P + 1) * 8));
+    if (!f.has_min && !f.has_max) {
+        if (n) {
+            KITE_HIP(hipMemcpyAsync(ok, keys, n * 8 * W, hipMemcpyDeviceToDevice, s));
+            KITE_HIP(hipMemcpyAsync(oc, counts, n * 8, hipMemcpyDeviceToDevice, s));
+        }
+        KITE_HIP(hipMemcpyAsync(oo, off, ((uint64_t)P + 1) * 8, hipMemcpyDeviceToDevice, s));
+        out.rows = n;
+        return;
+    }
+    uint32_t *flags = static_cast<uint32_t *>(head_.ensure(std::max<uint64_t>(n, 1) * 4));
+    uint64_t *pos = static_cast<uint64_t *>(pos_.ensure((n + 1) * 8));
+    uint64_t *tmp = static_cast<uint64_t *>(scan_tmp_.ensure(scan_temp_elems(n + 1) * 8));
+    if (n) {
+        k_keep_flags<<<blocks(n, 256), 256, 0, s>>>(counts, n, f, flags);
+        KITE_CHECK_LAUNCH();
+    }
+    scan_to_u64(flags, pos, n, tmp, s);
+    if (n) {
+        if (W == 1) k_keep_rows<1><<<blocks(n, 256), 256, 0, s>>>(keys, counts, n, flags, pos, ok, oc);
+        else if (W == 2) k_keep_rows<2><<<blocks(n, 256), 256, 0, s>>>(keys, counts, n, flags, pos, ok, oc);
+        else k_keep_rows<4><<<blocks(n, 256), 256, 0, s>>>(keys, counts, n, flags, pos, ok, oc);
+        KITE_CHECK_LAUNCH();
+    }
+    k_gather_pos<<<blocks(P + 1, 256), 256, 0, s>>>(off, P + 1, pos, oo);
+    KITE_CHECK_LAUNCH();
+    uint64_t kept = 0;
+    KITE_HIP(hipMemcpyAsync(&kept, pos + n, 8, hipMemcpyDeviceToHost, s));
+    KITE_HIP(hipStreamSynchronize(s));
+    out.rows = kept;
+}
+
+uint64_t MergeEngine::narrow_u32(const uint64_t *counts, uint64_t n, uint32_t *out) {
+    hipStream_t s = stream_;
+    unsigned long long *bad = static_cast<unsigned long long *>(bad_.ensure(8));
+    k_set_u64<<<1, 1, 0, s>>>(bad, (unsigned long long)n);
+    KITE_CHECK_LAUNCH();
+    if (n) {
+        k_narrow_u32<<<blocks(n, 256), 256, 0, s>>>(counts, n, out, bad);
+        KITE_CHECK_LAUNCH();
+    }
+    unsigned long long h = 0;
+    KITE_HIP(hipMemcpyAsync(&h, bad, 8, hipMemcpyDeviceToHost, s));
+    KITE_HIP(hipStreamSynchronize(s));
+    return (uint64_t)h;
 }
 
 template void MergeEngine::merge2<1>(const DevTable &, const DevTable &, DevTable &, uint32_t);

@@ -17,6 +17,12 @@ struct HostTable {
     uint64_t rows() const { return part_off.empty() ? 0 : part_off.back(); }
 };
 
+// Inclusive count bounds of a table's header (count_passes_filter,
+// src/kiteops/mod.rs:44-56), compared with u64 counts.
+struct DumpFilter {
+    uint32_t has_min = 0, min = 0, has_max = 0, max = 0;
+};
+
 struct DevTable {
     DevBuf keys, counts, off;
     uint64_t rows = 0;
@@ -35,13 +41,21 @@ class MergeEngine {
     // Largest count over the tables.
     uint64_t max_count(const std::vector<DevTable> &tabs);
     void download(const DevTable &t, int W, uint32_t P, uint64_t *keys, uint64_t *counts, uint64_t *part_off);
+    // The rows of a device table (n rows, u64 counts, P + 1 partition
+    // offsets) whose count passes `f`, copied into `out` with their offsets
+    // (apply_reader_filter_to_rows, src/kiteops/mod.rs:103-108).
+    void filter_into(const uint64_t *keys, const uint64_t *counts, const uint64_t *off, uint64_t n, uint32_t P, int W,
+                     const DumpFilter &f, DevTable &out);
+    // u64 counts narrowed to u32 into out; returns the first row whose count
+    // exceeds u32::MAX, or n when none does.
+    uint64_t narrow_u32(const uint64_t *counts, uint64_t n, uint32_t *out);
 
     template <int W>
     void merge2(const DevTable &A, const DevTable &B, DevTable &out, uint32_t P);
 
    private:
     hipStream_t stream_;
-    DevBuf mk_, mc_, head_, pos_, scan_tmp_, tile_pre_;
+    DevBuf mk_, mc_, head_, pos_, scan_tmp_, tile_pre_, bad_;
 };
 
 }  // namespace kite

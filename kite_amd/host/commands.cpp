@@ -105,32 +105,6 @@ FILE *open_out(const std::string &path) {
     return f;
 }
 
-// Every stored run of a file in partition order, as the device engine
-// takes them; zstd payloads are decoded into `decoded` first.
-void collect_runs(const Reader &r, std::vector<kite_dump_run> &runs, std::vector<std::vector<uint8_t>> &decoded) {
-    const Header &h = r.header();
-    const int W = r.words();
-    runs.reserve(r.chunk_count());
-    for (uint32_t p = 0; p < h.partitions; p++) {
-        auto [first, cnt] = r.partition_chunks(p);
-        for (uint64_t c = first; c < first + cnt; c++) {
-            const ChunkMeta &m = r.chunks()[c];
-            kite_dump_run x{};
-            x.partition = p;
-            x.rows = m.rows;
-            x.codec = (W == 4 || !(m.codec & CODEC_DVLE)) ? KITE_CODEC_RAW : KITE_CODEC_DVLE;
-            if (m.codec & CODEC_ZSTD) {
-                decoded.push_back(r.payload(m));
-                x.data = decoded.back().data();
-                x.bytes = decoded.back().size();
-            } else {
-                x.data = r.raw_payload(m);
-                x.bytes = m.encoded;
-            }
-            runs.push_back(x);
-        }
-    }
-}
 
 // Opens the device engine for a host command unless the environment
 // variable `var` says "host"; "gpu" makes a missing device an error.

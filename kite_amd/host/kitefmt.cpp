@@ -99,6 +99,7 @@ Writer::Writer(const std::string &path, const Header &h, size_t staging_bytes, i
     size_t hw = staging_bytes * (size_t)write_lanes * 4;
     high_ = std::max(hw, staging_bytes);
     low_ = high_ / 2;
+    spare_cap_ = high_ / std::max<size_t>(1, staging_bytes) + 2 * (size_t)std::max(1, write_lanes) + 4;
     st_.resize(h.partitions);
     uint8_t hdr[128];
     Header prov = h;
@@ -153,6 +154,10 @@ void Writer::worker() {
             busy_--;
             queued_bytes_ -= j.buf.size();
             if (!e.empty() && werr_.empty()) werr_ = e;
+            if (spare_.size() < spare_cap_) {
+                j.buf.clear();
+                spare_.push_back(std::move(j.buf));
+            }
         }
         cv_done_.notify_all();
     }
@@ -246,6 +251,14 @@ void Writer::append_chunk(uint32_t p, uint32_t codec, uint64_t rows, uint64_t ra
         return;
     }
     m.offset = s.buf.size();
+    if (s.buf.capacity() == 0) {
+        std::lock_guard<std::mutex> g(mu_);
+        if (!spare_.empty()) {
+            s.buf.swap(spare_.back());
+            spare_.pop_back();
+        }
+    }
+    if (s.buf.capacity() < s.buf.size() + len) s.buf.reserve(std::max(s.buf.size() + len, std::min(S_, 2 * s.buf.capacity())));
     s.buf.insert(s.buf.end(), data, data + len);
     s.pending.push_back(m);
     staged_ += len;

@@ -127,6 +127,10 @@ class Writer {
     size_t busy_ = 0, queued_bytes_ = 0;
     bool stop_ = false;
     std::string werr_;
+    // written staging buffers kept for reuse (their pages stay mapped, so a
+    // staged chunk is a plain copy rather than a copy plus page faults)
+    std::vector<std::vector<uint8_t>> spare_;
+    size_t spare_cap_ = 0;
 };
 
 // Memory-mapped reader with the validation of KiteReader::open

@@ -1,6 +1,7 @@
-// merge_host.cpp -- see merge_host.hpp.  Partitions go through in batches
-// sized to a host memory budget; each batch is one GPU union call, or with
-// the host engine one pass of the worker threads over its partitions.
+// merge_host.cpp -- see merge_host.hpp.  The device engine hands every
+// input's stored runs to kite_gpu_merge_runs; the host engine takes the
+// partitions in batches sized to a host memory budget, one pass of the
+// worker threads over each batch.
 #include "merge_host.hpp"
 
 #include <algorithm>
@@ -12,6 +13,31 @@
 #include "kite_gpu.h"
 
 namespace kh {
+
+void collect_runs(const Reader &r, std::vector<kite_dump_run> &runs, std::vector<std::vector<uint8_t>> &decoded) {
+    const Header &h = r.header();
+    const int W = r.words();
+    runs.reserve(r.chunk_count());
+    for (uint32_t p = 0; p < h.partitions; p++) {
+        auto [first, cnt] = r.partition_chunks(p);
+        for (uint64_t c = first; c < first + cnt; c++) {
+            const ChunkMeta &m = r.chunks()[c];
+            kite_dump_run x{};
+            x.partition = p;
+            x.rows = m.rows;
+            x.codec = (W == 4 || !(m.codec & CODEC_DVLE)) ? KITE_CODEC_RAW : KITE_CODEC_DVLE;
+            if (m.codec & CODEC_ZSTD) {
+                decoded.push_back(r.payload(m));
+                x.data = decoded.back().data();
+                x.bytes = decoded.back().size();
+            } else {
+                x.data = r.raw_payload(m);
+                x.bytes = m.encoded;
+            }
+            runs.push_back(x);
+        }
+    }
+}
 
 void load_filtered(const Reader &r, uint32_t p0, uint32_t p1, unsigned threads, std::vector<uint64_t> &keys,
                    std::vector<uint64_t> &counts, std::vector<uint64_t> &off) {
@@ -127,20 +153,69 @@ void merge_batch_host(const std::vector<std::vector<uint64_t>> &K, const std::ve
 
 }  // namespace
 
-MergeStats merge_files(const std::vector<std::unique_ptr<Reader>> &rs, Writer &w, unsigned threads, bool gpu) {
+namespace {
+
+// The device engine: every input's stored runs go to kite_gpu_merge_runs,
+// which returns the merged partitions as chunk payloads in partition order.
+MergeStats merge_files_gpu(const std::vector<std::unique_ptr<Reader>> &rs, Writer &w) {
     MergeStats ms;
-    ms.engine = gpu ? "gpu" : "cpu";
-    const uint32_t P = rs[0]->partitions();
-    const int W = rs[0]->words();
+    ms.engine = "gpu";
     kite_gpu_ctx *ctx = nullptr;
-    if (gpu && kite_gpu_open(0, 0, &ctx))
-        throw Error(std::string("failed to open the GPU engine: ") + kite_gpu_last_error());
+    if (kite_gpu_open(0, 0, &ctx)) throw Error(std::string("failed to open the GPU engine: ") + kite_gpu_last_error());
     struct Closer {
         kite_gpu_ctx *c;
-        ~Closer() {
-            if (c) kite_gpu_close(c);
-        }
+        ~Closer() { kite_gpu_close(c); }
     } closer{ctx};
+    const size_t n = rs.size();
+    std::vector<std::vector<kite_dump_run>> runs(n);
+    std::vector<std::vector<std::vector<uint8_t>>> decoded(n);
+    std::vector<kite_merge_input> in(n);
+    for (size_t i = 0; i < n; i++) {
+        collect_runs(*rs[i], runs[i], decoded[i]);
+        const Header &h = rs[i]->header();
+        in[i] = kite_merge_input{runs[i].data(), runs[i].size(), h.min_count.has_value() ? 1 : 0,
+                                 h.min_count.value_or(0), h.max_count.has_value() ? 1 : 0, h.max_count.value_or(0)};
+    }
+    struct Sink {
+        Writer *w;
+        std::string err;
+    } sink{&w, {}};
+    auto append = [](void *user, uint32_t partition, uint32_t codec, uint64_t rows, uint64_t raw_bytes,
+                     const uint64_t key_min[4], const uint64_t key_max[4], const uint8_t *data, size_t len) -> int {
+        Sink *s = static_cast<Sink *>(user);
+        try {
+            s->w->append_chunk(partition, codec, rows, raw_bytes, key_min, key_max, data, len);
+        } catch (const std::exception &e) {
+            s->err = e.what();
+            return 1;
+        }
+        return 0;
+    };
+    const Header &f = rs[0]->header();
+    kite_merge_stats st{};
+    const auto t0 = std::chrono::steady_clock::now();
+    const int rc = kite_gpu_merge_runs(ctx, f.key_width, f.partitions, in.data(), (uint32_t)n, append, &sink, &st);
+    ms.merge_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (!sink.err.empty()) throw Error(sink.err);
+    if (rc == KITE_E_OVERFLOW) throw Error(kite_gpu_last_error());
+    if (rc) throw Error(std::string("GPU merge failed: ") + kite_gpu_last_error());
+    fprintf(stderr,
+            "[kite merge] gpu: %llu runs, %llu rows in, %llu kept, %llu merged, %llu chunk bytes in %llu group(s); "
+            "in %.1f ms, device decode %.1f ms, merge %.1f ms, encode %.1f ms, out %.1f ms\n",
+            (unsigned long long)st.runs, (unsigned long long)st.rows_in, (unsigned long long)st.rows_kept,
+            (unsigned long long)st.rows, (unsigned long long)st.bytes, (unsigned long long)st.groups, st.in_ms,
+            st.decode_ms, st.merge_ms, st.encode_ms, st.out_ms);
+    return ms;
+}
+
+}  // namespace
+
+MergeStats merge_files(const std::vector<std::unique_ptr<Reader>> &rs, Writer &w, unsigned threads, bool gpu) {
+    if (gpu) return merge_files_gpu(rs, w);
+    MergeStats ms;
+    ms.engine = "cpu";
+    const uint32_t P = rs[0]->partitions();
+    const int W = rs[0]->words();
     // batch partitions so the decoded rows of all inputs stay near 2 GiB
     uint64_t total_rows = 0;
     for (auto &r : rs) total_rows += r->header().total_rows;
@@ -152,28 +227,10 @@ MergeStats merge_files(const std::vector<std::unique_ptr<Reader>> &rs, Writer &w
     for (uint32_t p0 = 0; p0 < P; p0 += step) {
         const uint32_t p1 = std::min(P, p0 + step), np = p1 - p0;
         std::vector<std::vector<uint64_t>> K(n), C(n), O(n);
-        uint64_t sum = 0;
-        for (size_t i = 0; i < n; i++) {
-            load_filtered(*rs[i], p0, p1, threads, K[i], C[i], O[i]);
-            sum += O[i][np];
-        }
-        std::vector<const uint64_t *> kp(n), cp(n), op(n);
-        for (size_t i = 0; i < n; i++) {
-            kp[i] = K[i].data();
-            cp[i] = C[i].data();
-            op[i] = O[i].data();
-        }
+        for (size_t i = 0; i < n; i++) load_filtered(*rs[i], p0, p1, threads, K[i], C[i], O[i]);
         std::vector<uint64_t> ok, oc, oo(np + 1);
         auto t0 = std::chrono::steady_clock::now();
-        if (gpu) {
-            ok.resize(std::max<uint64_t>(sum, 1) * W);
-            oc.resize(std::max<uint64_t>(sum, 1));
-            if (kite_gpu_merge_tables(ctx, (uint32_t)W, np, (uint32_t)n, kp.data(), cp.data(), op.data(), ok.data(),
-                                      oc.data(), oo.data()))
-                throw Error(std::string("GPU merge failed: ") + kite_gpu_last_error());
-        } else {
-            merge_batch_host(K, C, O, np, W, threads, ok, oc, oo);
-        }
+        merge_batch_host(K, C, O, np, W, threads, ok, oc, oo);
         ms.merge_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         for (auto &v : K) std::vector<uint64_t>().swap(v);
         for (auto &v : C) std::vector<uint64_t>().swap(v);

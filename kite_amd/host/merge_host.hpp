@@ -1,19 +1,26 @@
-// merge_host.hpp -- `kite merge` data path: decode each input's partitions,
-// apply its header filter, union + sum on the GPU, serialize in order.
+// merge_host.hpp -- `kite merge` data path.  The device engine takes every
+// input's stored runs and returns the merged partitions as chunk payloads;
+// the host engine decodes each input's partitions, applies its header
+// filter, merges them on worker threads and serializes them in order.
 #pragma once
 
 #include <memory>
 #include <string>
 #include <vector>
 
+#include "kite_gpu.h"
 #include "kitefmt.hpp"
 
 namespace kh {
 
 struct MergeStats {
     std::string engine;
-    double merge_s = 0;
+    double merge_s = 0;  // device engine: the kite_gpu_merge_runs call; host engine: the merges alone
 };
+
+// Every stored run of a file in partition order, as the device engines
+// take them; zstd payloads are decoded into `decoded` first.
+void collect_runs(const Reader &r, std::vector<kite_dump_run> &runs, std::vector<std::vector<uint8_t>> &decoded);
 
 MergeStats merge_files(const std::vector<std::unique_ptr<Reader>> &rs, Writer &w, unsigned threads, bool gpu);
 
