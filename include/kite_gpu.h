@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define KITE_GPU_ABI_VERSION 3
+#define KITE_GPU_ABI_VERSION 4
 
 enum {
     KITE_OK = 0,
@@ -58,6 +58,8 @@ typedef struct kite_count_stats {
     uint32_t split_levels;    /* MSD split levels of the last batch */
     uint32_t skipped_chunks;  /* fed chunks dropped for not starting with '@' */
     uint32_t words;           /* key words (1, 2 or 4) */
+    uint32_t graph;           /* batch: 1 when the sample's device work ran as one captured HIP graph launch */
+    uint32_t reserved;
     double device_ms;         /* device time of all batches (events) */
     double extract_ms, split_ms, sort_ms, compact_ms; /* last batch */
 } kite_count_stats;

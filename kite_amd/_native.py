@@ -43,7 +43,8 @@ class CountStats(C.Structure):
     _fields_ = [
         ("input_bytes", C.c_uint64), ("kmers", C.c_uint64), ("unique", C.c_uint64),
         ("buckets", C.c_uint64), ("batches", C.c_uint32), ("split_levels", C.c_uint32),
-        ("skipped_chunks", C.c_uint32), ("words", C.c_uint32), ("device_ms", C.c_double),
+        ("skipped_chunks", C.c_uint32), ("words", C.c_uint32), ("graph", C.c_uint32),
+        ("reserved", C.c_uint32), ("device_ms", C.c_double),
         ("extract_ms", C.c_double), ("split_ms", C.c_double), ("sort_ms", C.c_double),
         ("compact_ms", C.c_double),
     ]

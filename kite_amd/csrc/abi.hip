@@ -99,6 +99,7 @@ struct BatchSlot {
     hipEvent_t d2h_done = nullptr;
     hipEvent_t run_done = nullptr;  // the run that read input buffer [slot] ended
     hipEvent_t in_ready = nullptr;  // input buffer [slot] holds its sample
+    hipEvent_t enc_done = nullptr;  // graph batch: the slot's chunk payload and summary are written
 };
 
 }  // namespace
@@ -183,6 +184,24 @@ struct kite_gpu_ctx {
     // dump: staged payloads of a partition group, pinned text pieces
     kite::HostPinned dump_stage, dump_text;
     kite::DevBuf dump_pay;
+
+    // graph-captured batch (encoded chunks): one executable graph per (input
+    // size class, ring slot), valid while the shape, buffers and settings
+    // it was captured with stay the same
+    struct BatchGraph {
+        hipGraphExec_t count = nullptr, encode = nullptr;
+        kite::CountTunables tun;
+        const void *in = nullptr, *out = nullptr, *meta = nullptr, *len = nullptr;  // the slot's buffers they use
+        uint64_t max_len = 0;  // the count's grids cover inputs up to this length
+    };
+    std::map<std::pair<int, int>, BatchGraph> graphs;
+    std::tuple<uint32_t, int, uint32_t, uint64_t> graph_shape{0, -1, 0, 0};  // k, canonical, P, input capacity
+    kite::EncodeBufs graph_enc;                // scratch of the planned chunk encode
+    kite::DevBuf graph_bytes[MAX_RING];        // chunk payloads of each ring slot
+    kite::HostPinned graph_meta[MAX_RING];     // per slot: a length word (warm-up count), then the summary
+    kite::DevBuf graph_dlen;                   // per slot: the input length the count graph reads
+    kite::HostPinned graph_lens;               // the batch's input lengths, copied next to each input
+    uint64_t graph_fallbacks = 0;              // samples a planned count handed to the classic path
 
     // merge of stored runs: per-input tables of a partition group, their
     // union, its u32 counts and chunk encode buffers, pinned bounce buffers
@@ -447,6 +466,10 @@ void kite_gpu_close(kite_gpu_ctx *ctx) {
     try {
         bind(ctx);
         (void)hipStreamSynchronize(ctx->stream);
+        for (auto &g : ctx->graphs)
+            for (hipGraphExec_t e : {g.second.count, g.second.encode})
+                if (e) (void)hipGraphExecDestroy(e);
+        ctx->graphs.clear();
         ctx->count.reset();
         ctx->dump.reset();
         ctx->merge.reset();
@@ -474,7 +497,8 @@ void kite_gpu_close(kite_gpu_ctx *ctx) {
             ctx->slots[b].enc_bytes.release();
             ctx->slots[b].enc_minmax.release();
             ctx->slots[b].enc_boff.release();
-            for (hipEvent_t e : {ctx->slots[b].d2h_done, ctx->slots[b].run_done, ctx->slots[b].in_ready})
+            for (hipEvent_t e : {ctx->slots[b].d2h_done, ctx->slots[b].run_done, ctx->slots[b].in_ready,
+                                 ctx->slots[b].enc_done})
                 if (e) (void)hipEventDestroy(e);
         }
         if (ctx->h2d) (void)hipStreamDestroy(ctx->h2d);
@@ -999,13 +1023,15 @@ struct BatchSink {
     kite_batch_result_fn raw = nullptr;
     kite_batch_chunks_fn chunks = nullptr;
     void *user = nullptr;
+    uint32_t base = 0;  // the caller's index of the first sample passed in
     bool encoded() const { return chunks != nullptr; }
     int call(kite_gpu_ctx *c, BatchSlot &sl) const {
         if (chunks)
-            return chunks(user, sl.sample, &sl.st, c->words == 4 ? KITE_CODEC_RAW : KITE_CODEC_DVLE,
+            return chunks(user, base + sl.sample, &sl.st, c->words == 4 ? KITE_CODEC_RAW : KITE_CODEC_DVLE,
                           sl.enc_bytes.as<uint8_t>(), sl.enc_boff.as<uint64_t>(), sl.part_off.data(),
                           sl.enc_minmax.as<uint64_t>());
-        return raw(user, sl.sample, &sl.st, sl.keys.as<uint64_t>(), sl.counts.as<uint32_t>(), sl.part_off.data());
+        return raw(user, base + sl.sample, &sl.st, sl.keys.as<uint64_t>(), sl.counts.as<uint32_t>(),
+                   sl.part_off.data());
     }
 };
 
@@ -1130,6 +1156,14 @@ struct Consumer {
             }
             cv.notify_all();
         }
+    }
+    // Frees slot s without a callback (a sample whose result is dropped).
+    void release(int s) {
+        {
+            std::lock_guard<std::mutex> g(mu);
+            free_slot[s] = true;
+        }
+        cv.notify_all();
     }
     // Blocks until slot s is free; returns false once an error was recorded.
     bool acquire(int s) {
@@ -1286,6 +1320,333 @@ void batch_pipelined(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, 
     }
 }
 
+// Graph-captured batches unless KITE_BATCH_GRAPH=0 (the classic pipeline,
+// with its host read-backs between launches, for comparison).
+bool use_graphs() {
+    const char *e = getenv("KITE_BATCH_GRAPH");
+    return !(e && atoi(e) == 0);
+}
+
+bool same_tunables(const kite::CountTunables &a, const kite::CountTunables &b) {
+    return a.split_target_div == b.split_target_div && a.max_digit_bits == b.max_digit_bits && a.dedup == b.dedup &&
+           a.hash_classes == b.hash_classes && a.split_chunk == b.split_chunk && a.extract_subs == b.extract_subs &&
+           a.extract_staged == b.extract_staged && a.split_staged == b.split_staged &&
+           a.split_above_div == b.split_above_div;
+}
+
+void drop_graphs(kite_gpu_ctx *c) {
+    for (auto &g : c->graphs)
+        for (hipGraphExec_t e : {g.second.count, g.second.encode})
+            if (e) KITE_HIP(hipGraphExecDestroy(e));
+    c->graphs.clear();
+}
+
+// Captures what `enqueue` puts on the engine stream as an executable graph.
+template <class F>
+hipGraphExec_t capture(kite_gpu_ctx *c, F &&enqueue) {
+    hipGraph_t graph = nullptr;
+    KITE_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    try {
+        enqueue();
+    } catch (...) {
+        (void)hipStreamEndCapture(c->stream, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    KITE_HIP(hipStreamEndCapture(c->stream, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    KITE_HIP(e);
+    return exec;
+}
+
+// Bytes of chunk payload a ring slot holds for a planned count of up to n
+// k-mers: beyond it the sample takes the classic path.
+uint64_t graph_bytes_cap(uint64_t n, int W) { return n * (W == 1 ? 10 : (W == 2 ? 20 : 36)); }
+
+// The pipelined batch of encoded chunks with every sample's device work --
+// the count, then the chunk encode, each sized on the device -- as two
+// launches of captured HIP graphs (a pair per input size class and ring
+// slot, captured on first use and kept while the shape, buffers and
+// settings hold):
+//   copy stream h2d:  the H2D of sample j into buffer j % R, after the count
+//                     graph of sample j - R read that buffer;
+//   engine stream:    the count graph of sample i once its H2D ended; then,
+//                     once the D2H of sample i - R has read the slot's
+//                     payload buffer (a stream wait), the encode graph,
+//                     which writes the chunk payload into that buffer and
+//                     the sizes and partition tables into the slot's
+//                     page-locked summary;
+//   drainer thread:   after sample i's encode graph, the D2H of exactly its
+//                     payload bytes on the copy stream d2h;
+//   consumer thread:  the callback after that D2H, which frees the slot.
+// The engine thread reads nothing back between samples: it only waits, on
+// the host, for the drainer to have queued the D2H whose end the stream
+// wait names.  A sample whose data exceeds a capacity of the planned count
+// (its summary says so) stops the graphs there: the samples from it on run
+// through the classic pipeline.  Returns the first sample not delivered (n
+// when all were).
+uint32_t batch_graph(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, const BatchSink &sink) {
+    const int R = batch_ring();
+    if (!c->h2d) KITE_HIP(hipStreamCreateWithFlags(&c->h2d, hipStreamNonBlocking));
+    if (!c->d2h) KITE_HIP(hipStreamCreateWithFlags(&c->d2h, hipStreamNonBlocking));
+    for (int b = 0; b < R; b++) {
+        BatchSlot &sl = c->slots[b];
+        for (hipEvent_t *e : {&sl.d2h_done, &sl.run_done, &sl.in_ready, &sl.enc_done})
+            if (!*e) KITE_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    }
+    const bool prof = c->timeline;
+    if (prof) {
+        if (!c->tl_start) KITE_HIP(hipEventCreate(&c->tl_start));
+        while (c->tl_events.size() < n) {
+            std::array<hipEvent_t, 6> ev{};
+            for (hipEvent_t &e : ev) KITE_HIP(hipEventCreate(&e));
+            c->tl_events.push_back(ev);
+        }
+    }
+    uint64_t maxlen = 0;
+    for (uint32_t i = 0; i < n; i++) maxlen = std::max<uint64_t>(maxlen, smp[i].len);
+    const uint32_t P = c->partitions;
+    const int W = c->words;
+    // the shape the graphs hold: a larger input or another k, P re-prepares
+    // the buffers (and so drops every graph)
+    const bool same_kp = std::get<0>(c->graph_shape) == c->k && std::get<1>(c->graph_shape) == c->canonical &&
+                         std::get<2>(c->graph_shape) == P;
+    const uint64_t cap = same_kp ? std::max<uint64_t>(maxlen, std::get<3>(c->graph_shape)) : maxlen;
+    for (int b = 0; b < std::min<int>(R, (int)n); b++) c->batch_in[b].ensure(cap);
+    const auto shape = std::make_tuple(c->k, c->canonical, P, cap);
+    const bool reshape = shape != c->graph_shape;
+    if (reshape) {
+        KITE_HIP(hipStreamSynchronize(c->stream));
+        drop_graphs(c);
+        c->count->dev_prepare(cap, (int)c->k, P, c->graph_enc);
+    }
+    const uint64_t bcap = graph_bytes_cap(c->count->dev_caps().n, W);
+    for (int b = 0; b < R; b++) {
+        c->graph_bytes[b].ensure(bcap);
+        c->graph_meta[b].ensure(64 + kite::CountEngine::dev_summary_bytes(P));
+    }
+    uint64_t *dlen = static_cast<uint64_t *>(c->graph_dlen.ensure(MAX_RING * 8));
+    uint64_t *lens = static_cast<uint64_t *>(c->graph_lens.ensure(std::max<size_t>(n, 1) * 8));
+    for (uint32_t i = 0; i < n; i++) lens[i] = smp[i].len;
+    auto len_word = [&](int b) { return c->graph_meta[b].as<uint64_t>(); };
+    auto summary = [&](int b) { return reinterpret_cast<kite::DevSummary *>(c->graph_meta[b].as<uint8_t>() + 64); };
+    if (reshape) {
+        // one planned count of an empty input, outside any capture: it sets
+        // the kernels' attributes and sizes the grids the captures reuse
+        *len_word(0) = 0;
+        c->count->dev_enqueue_count(c->batch_in[0].as<uint8_t>(), len_word(0), cap, (int)c->k, c->canonical, P);
+        c->count->dev_enqueue_encode(c->graph_enc, c->graph_bytes[0].as<uint8_t>(), bcap, summary(0));
+        KITE_HIP(hipStreamSynchronize(c->stream));
+        c->graph_shape = shape;
+    }
+    // the graphs of (size class, slot), captured when missing or stale; the
+    // count's grids are sized for the longest input of the class (so a small
+    // sample does not pay for the largest one's grids)
+    std::map<int, uint64_t> class_max;
+    for (uint32_t i = 0; i < n; i++) {
+        uint64_t &m = class_max[size_class(smp[i].len)];
+        m = std::max<uint64_t>(m, smp[i].len);
+    }
+    auto graphs_for = [&](uint64_t len, int b) -> kite_gpu_ctx::BatchGraph & {
+        apply_tuned(c, len);
+        const kite::CountTunables t = c->count->resolved_tunables();
+        const int cls = size_class(len);
+        kite_gpu_ctx::BatchGraph &g = c->graphs[std::make_pair(cls, b)];
+        // a grid for the class's longest input this batch (kept when it covers it)
+        if (g.count && g.encode && same_tunables(g.tun, t) && g.in == c->batch_in[b].ptr &&
+            g.out == c->graph_bytes[b].ptr && g.meta == c->graph_meta[b].ptr && g.len == dlen + b &&
+            g.max_len >= class_max[cls] && g.max_len <= cap)
+            return g;
+        const uint64_t grid_len = class_max[cls];
+        for (hipGraphExec_t *e : {&g.count, &g.encode}) {
+            if (*e) KITE_HIP(hipGraphExecDestroy(*e));
+            *e = nullptr;
+        }
+        g.count = capture(c, [&] {
+            c->count->dev_enqueue_count(c->batch_in[b].as<uint8_t>(), dlen + b, grid_len, (int)c->k, c->canonical, P);
+        });
+        g.encode = capture(c, [&] {
+            c->count->dev_enqueue_encode(c->graph_enc, c->graph_bytes[b].as<uint8_t>(), bcap, summary(b));
+        });
+        g.tun = t;
+        g.in = c->batch_in[b].ptr;
+        g.out = c->graph_bytes[b].ptr;
+        g.meta = c->graph_meta[b].ptr;
+        g.len = dlen + b;
+        g.max_len = grid_len;
+        return g;
+    };
+    // every graph this batch needs, before any copy is queued
+    std::vector<std::pair<hipGraphExec_t, hipGraphExec_t>> exec(n);
+    for (uint32_t i = 0; i < n; i++) {
+        const kite_gpu_ctx::BatchGraph &g = graphs_for(smp[i].len, (int)(i % (uint32_t)R));
+        exec[i] = std::make_pair(g.count, g.encode);
+    }
+
+    auto h2d = [&](uint32_t j) {
+        const int b = (int)(j % (uint32_t)R);
+        if (j >= (uint32_t)R) KITE_HIP(hipStreamWaitEvent(c->h2d, c->slots[b].run_done, 0));
+        if (prof) KITE_HIP(hipEventRecord(c->tl_events[j][0], c->h2d));
+        // the length word travels with the input: the count graph of sample
+        // j - R has read both when run_done fires
+        KITE_HIP(hipMemcpyAsync(dlen + b, lens + j, 8, hipMemcpyHostToDevice, c->h2d));
+        if (smp[j].len)
+            KITE_HIP(hipMemcpyAsync(c->batch_in[b].ptr, smp[j].fastq, smp[j].len, hipMemcpyHostToDevice, c->h2d));
+        if (prof) KITE_HIP(hipEventRecord(c->tl_events[j][1], c->h2d));
+        KITE_HIP(hipEventRecord(c->slots[b].in_ready, c->h2d));
+    };
+
+    // drainer: after sample i's encode graph, the D2H of its payload
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<uint32_t> launched;
+    uint32_t drained = 0;  // samples whose D2H is queued (or that were dropped)
+    bool stop = false;
+    uint32_t overflow_at = n;  // the first sample whose planned count overflowed
+    std::string drain_err;
+    Consumer cons(c, sink);
+    auto mark_drained = [&](uint32_t i) {
+        {
+            std::lock_guard<std::mutex> g(mu);
+            drained = i + 1;
+        }
+        cv.notify_all();
+    };
+    std::thread drainer([&] {
+        for (;;) {
+            uint32_t i;
+            {
+                std::unique_lock<std::mutex> g(mu);
+                cv.wait(g, [&] { return stop || !launched.empty(); });
+                if (launched.empty()) return;
+                i = launched.front();
+                launched.pop_front();
+            }
+            const int s = (int)(i % (uint32_t)R);
+            BatchSlot &sl = c->slots[s];
+            bool taken = false;
+            try {
+                KITE_HIP(hipEventSynchronize(sl.enc_done));
+                const kite::DevSummary sm = *summary(s);
+                bool skip;
+                {
+                    std::lock_guard<std::mutex> g(mu);
+                    if (sm.overflow && overflow_at == n) overflow_at = i;
+                    skip = i >= overflow_at;
+                }
+                if (skip || !cons.acquire(s)) {
+                    mark_drained(i);
+                    continue;
+                }
+                taken = true;
+                const uint64_t *tab = reinterpret_cast<const uint64_t *>(summary(s) + 1);
+                sl.sample = i;
+                sl.st = kite_count_stats{};
+                sl.st.input_bytes = smp[i].len;
+                sl.st.kmers = sm.n_kmers;
+                sl.st.unique = sm.n_unique;
+                sl.st.buckets = sm.buckets;
+                sl.st.batches = 1;
+                sl.st.split_levels = sm.levels;
+                sl.st.skipped_chunks = (smp[i].len > 0 && smp[i].fastq[0] != '@') ? 1u : 0u;
+                sl.st.words = (uint32_t)W;
+                sl.st.graph = 1;
+                sl.part_off.assign(tab, tab + P + 1);
+                uint64_t *boff = static_cast<uint64_t *>(sl.enc_boff.ensure(((size_t)P + 1) * 8));
+                std::copy(tab + P + 1, tab + 2 * ((size_t)P + 1), boff);
+                uint64_t *mm = static_cast<uint64_t *>(sl.enc_minmax.ensure((size_t)P * 64));
+                std::copy(tab + 2 * ((size_t)P + 1), tab + 2 * ((size_t)P + 1) + 8 * (size_t)P, mm);
+                sl.enc_total = sm.bytes;
+                uint8_t *hb = static_cast<uint8_t *>(sl.enc_bytes.ensure(std::max<uint64_t>(sm.bytes, 1)));
+                if (prof) KITE_HIP(hipEventRecord(c->tl_events[i][4], c->d2h));
+                if (sm.bytes)
+                    KITE_HIP(hipMemcpyAsync(hb, c->graph_bytes[s].ptr, sm.bytes, hipMemcpyDeviceToHost, c->d2h));
+                if (prof) KITE_HIP(hipEventRecord(c->tl_events[i][5], c->d2h));
+                KITE_HIP(hipEventRecord(sl.d2h_done, c->d2h));
+                mark_drained(i);
+                taken = false;
+                cons.push(s);
+            } catch (const std::exception &e) {
+                {
+                    std::lock_guard<std::mutex> g(mu);
+                    if (drain_err.empty()) drain_err = e.what();
+                    if (overflow_at == n) overflow_at = i;
+                }
+                if (taken) cons.release(s);
+                mark_drained(i);
+            }
+        }
+    });
+    std::string failure;
+    uint32_t launched_n = 0;
+    try {
+        if (prof) KITE_HIP(hipEventRecord(c->tl_start, c->h2d));
+        for (uint32_t j = 0; j + 1 < (uint32_t)R && j < n; j++) h2d(j);
+        for (uint32_t i = 0; i < n; i++) {
+            {
+                std::lock_guard<std::mutex> g(mu);
+                if (i >= overflow_at || !cons.err().empty()) break;
+            }
+            const int s = (int)(i % (uint32_t)R);
+            BatchSlot &sl = c->slots[s];
+            if (i + R - 1 < n) h2d(i + R - 1);
+            KITE_HIP(hipStreamWaitEvent(c->stream, sl.in_ready, 0));
+            if (prof) KITE_HIP(hipEventRecord(c->tl_events[i][2], c->stream));
+            KITE_HIP(hipGraphLaunch(exec[i].first, c->stream));
+            KITE_HIP(hipEventRecord(sl.run_done, c->stream));  // input buffer s is free again
+            if (i >= (uint32_t)R) {
+                // the slot's payload buffer is read by the D2H of sample i - R:
+                // once the drainer has queued that copy, the stream waits for
+                // its end
+                std::unique_lock<std::mutex> g(mu);
+                cv.wait(g, [&] { return drained > i - R; });
+                if (i >= overflow_at) break;
+                g.unlock();
+                KITE_HIP(hipStreamWaitEvent(c->stream, sl.d2h_done, 0));
+            }
+            KITE_HIP(hipGraphLaunch(exec[i].second, c->stream));
+            if (prof) KITE_HIP(hipEventRecord(c->tl_events[i][3], c->stream));
+            KITE_HIP(hipEventRecord(sl.enc_done, c->stream));
+            launched_n = i + 1;
+            {
+                std::lock_guard<std::mutex> g(mu);
+                launched.push_back(i);
+            }
+            cv.notify_all();
+        }
+    } catch (const std::exception &e) {
+        failure = e.what();
+    }
+    {
+        std::lock_guard<std::mutex> g(mu);
+        stop = true;
+    }
+    cv.notify_all();
+    drainer.join();
+    // (after a failure a slot may stay taken: nothing more is drained)
+    if (failure.empty()) cons.drain();
+    cons.stop();
+    (void)hipStreamSynchronize(c->h2d);
+    (void)hipStreamSynchronize(c->d2h);
+    (void)hipStreamSynchronize(c->stream);
+    if (!failure.empty()) throw kite::Error(failure);
+    if (!drain_err.empty()) throw kite::Error(drain_err);
+    std::string ce = cons.err();
+    if (!ce.empty()) throw kite::Error(ce);
+    if (prof) {
+        c->tl_ms.assign((size_t)n * 6, 0.0);
+        for (uint32_t i = 0; i < std::min(launched_n, overflow_at); i++)
+            for (int q = 0; q < 6; q++) {
+                float ms = 0;
+                KITE_HIP(hipEventElapsedTime(&ms, c->tl_start, c->tl_events[i][q]));
+                c->tl_ms[(size_t)i * 6 + q] = ms;
+            }
+    }
+    return std::min(overflow_at, n);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1313,8 +1674,21 @@ int count_batch_impl(kite_gpu_ctx *ctx, uint32_t k, int canonical, uint32_t part
             if (samples[i].len > ctx->batch_cap)
                 return set_error(KITE_E_NOMEM, "sample " + std::to_string(i) +
                                                    " exceeds the device budget of one batch");
-        if (mode == KITE_BATCH_PIPELINED) batch_pipelined(ctx, samples, n_samples, sink);
-        else batch_plain(ctx, samples, n_samples, sink, mode == KITE_BATCH_SERIAL);
+        if (mode == KITE_BATCH_PIPELINED && sink.encoded() && use_graphs()) {
+            // captured graphs; samples from the first one whose data exceeds
+            // the planned count's capacities on go through the classic pipeline
+            const uint32_t done = batch_graph(ctx, samples, n_samples, sink);
+            if (done < n_samples) {
+                ctx->graph_fallbacks += n_samples - done;
+                BatchSink rest = sink;
+                rest.base = done;
+                batch_pipelined(ctx, samples + done, n_samples - done, rest);
+            }
+        } else if (mode == KITE_BATCH_PIPELINED) {
+            batch_pipelined(ctx, samples, n_samples, sink);
+        } else {
+            batch_plain(ctx, samples, n_samples, sink, mode == KITE_BATCH_SERIAL);
+        }
         return KITE_OK;
     });
     if (rc == KITE_E_DEVICE && g_last_error.find("callback aborted") != std::string::npos) rc = KITE_E_CALLBACK;

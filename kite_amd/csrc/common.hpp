@@ -55,7 +55,7 @@ struct DevBuf {
     T *as() const { return static_cast<T *>(ptr); }
 };
 
-inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+__host__ __device__ inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
 // ---- device-side scan helpers ---------------------------------------------
 
@@ -137,5 +137,11 @@ __device__ __forceinline__ T block_excl_scan(T v, T *ws, T *total) {
 void scan_to_u64(const uint32_t *in, uint64_t *out, uint64_t n, uint64_t *temp, hipStream_t s);
 void scan_to_u64(const uint64_t *in, uint64_t *out, uint64_t n, uint64_t *temp, hipStream_t s);
 size_t scan_temp_elems(uint64_t n);
+// The same with the length read from device memory (min(*dn, n_cap)), for
+// launch sequences fixed before the data is seen; temp as for n_cap.
+void scan_to_u64_dev(const uint32_t *in, uint64_t *out, const uint64_t *dn, uint64_t n_cap, uint64_t *temp,
+                     hipStream_t s);
+void scan_to_u64_dev(const uint64_t *in, uint64_t *out, const uint64_t *dn, uint64_t n_cap, uint64_t *temp,
+                     hipStream_t s);
 
 }  // namespace kite

@@ -20,8 +20,11 @@
 #include <cstdlib>
 #include <cstring>
 #include <type_traits>
+#include <tuple>
+#include <map>
 
 #include "count.hpp"
+#include "encode.hpp"
 
 namespace kite {
 
@@ -140,10 +143,9 @@ __device__ __forceinline__ uint64_t xcd_tile(uint64_t b, uint64_t n) {
 }
 
 // ---- newline count per sub-tile -------------------------------------------
-__global__ __launch_bounds__(XT) void k_nl_count(const uint8_t *__restrict__ fq, uint64_t len,
-                                                  uint32_t *__restrict__ out) {
-    __shared__ uint32_t ws[XT / 64 + 1];
-    uint64_t base = (uint64_t)blockIdx.x * SUB + (uint64_t)threadIdx.x * SEGB;
+__device__ __forceinline__ void nl_count_tile(const uint8_t *__restrict__ fq, uint64_t len, uint64_t tile,
+                                              uint32_t *__restrict__ out, uint32_t *ws) {
+    uint64_t base = tile * SUB + (uint64_t)threadIdx.x * SEGB;
     uint32_t c = 0;
     if ((int)threadIdx.x * SEGB >= SUB) {
         // past the sub-tile (its last threads)
@@ -157,7 +159,13 @@ __global__ __launch_bounds__(XT) void k_nl_count(const uint8_t *__restrict__ fq,
     }
     uint32_t total;
     (void)block_excl_scan<XT, uint32_t>(c, ws, &total);
-    if (threadIdx.x == 0) out[blockIdx.x] = total;
+    if (threadIdx.x == 0) out[tile] = total;
+}
+
+__global__ __launch_bounds__(XT) void k_nl_count(const uint8_t *__restrict__ fq, uint64_t len,
+                                                  uint32_t *__restrict__ out) {
+    __shared__ uint32_t ws[XT / 64 + 1];
+    nl_count_tile(fq, len, blockIdx.x, out, ws);
 }
 
 struct ExtractArgs {
@@ -171,6 +179,7 @@ struct ExtractArgs {
     uint32_t *hist;             // [P][nwg] counts (histogram pass)
     const uint64_t *off;        // scanned [P][nwg] (scatter pass)
     uint64_t *out;              // partition-ordered keys (scatter pass)
+    const DevPlan *pl;          // device-planned count: len and n_sub from here (else the fields above)
 };
 
 // Both passes walk identical k-mers; the scatter pass places each key at
@@ -214,9 +223,24 @@ __global__ __launch_bounds__(XT) void k_extract(ExtractArgs a) {
     uint32_t *lcnt = dyn + ((P + 1) & ~1u) + 2 * P;
     uint32_t *loff = lcnt + P;
     const int tid = threadIdx.x;
-    const uint32_t wg = (uint32_t)xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t *sb = reinterpret_cast<const uint32_t *>(sb4);
-    const bool live = a.len > 0 && a.fq[0] == '@';
+    // the input size, from the device plan when the launch was fixed before
+    // the data was seen (the grid then covers a capacity)
+    const uint64_t alen = a.pl ? a.pl->len : a.len;
+    const uint64_t an_sub = a.pl ? a.pl->n_sub : a.n_sub;
+    if constexpr (SCATTER) {
+        if (a.pl && a.pl->overflow) return;  // the keys would not fit: nothing is written
+    }
+    // the workgroups that have input are spread over the XCDs as one grid of
+    // their own number; the rest hold empty matrix columns
+    const uint64_t busy = a.pl ? ceil_div(an_sub, a.subs) : gridDim.x;
+    if (blockIdx.x >= busy) {
+        if constexpr (!SCATTER)
+            for (uint32_t p = tid; p < P; p += XT) a.hist[(uint64_t)p * a.nwg + blockIdx.x] = 0;
+        return;
+    }
+    const uint32_t wg = (uint32_t)xcd_tile(blockIdx.x, busy);
+    const bool live = alen > 0 && a.fq[0] == '@';
 
     for (uint32_t p = tid; p < P; p += XT) {
         cnt[p] = 0;
@@ -236,10 +260,10 @@ __global__ __launch_bounds__(XT) void k_extract(ExtractArgs a) {
 
     for (uint32_t si = 0; si < a.subs; si++) {
         uint64_t st = (uint64_t)wg * a.subs + si;
-        if (st >= a.n_sub) break;
+        if (st >= an_sub) break;
         const int64_t base = (int64_t)(st * SUB);
         __syncthreads();
-        if (base >= HALO && base + SUB <= (int64_t)a.len) {
+        if (base >= HALO && base + SUB <= (int64_t)alen) {
             // the whole sub-tile and its halo are inside: unconditional
             // loads, all in flight before the first LDS store
             constexpr int LPT = EXT_U4 / XT;
@@ -253,14 +277,14 @@ __global__ __launch_bounds__(XT) void k_extract(ExtractArgs a) {
         } else for (int i = tid; i < EXT_U4; i += XT) {
             int64_t gb = base - HALO + 16 * (int64_t)i;
             uint4 v = make_uint4(0, 0, 0, 0);
-            if (gb >= 0 && gb + 16 <= (int64_t)a.len) {
+            if (gb >= 0 && gb + 16 <= (int64_t)alen) {
                 v = *reinterpret_cast<const uint4 *>(a.fq + gb);
-            } else if (gb + 16 > 0 && gb < (int64_t)a.len) {
+            } else if (gb + 16 > 0 && gb < (int64_t)alen) {
                 uint8_t tmp[16];
 #pragma unroll
                 for (int q = 0; q < 16; q++) {
                     int64_t g = gb + q;
-                    tmp[q] = (g >= 0 && g < (int64_t)a.len) ? a.fq[g] : 0;
+                    tmp[q] = (g >= 0 && g < (int64_t)alen) ? a.fq[g] : 0;
                 }
                 memcpy(&v, tmp, 16);
             }
@@ -594,14 +618,13 @@ struct SplitArgs {
 // offsets from the segment start (segments below 2^32 keys).  MODE 2
 // (scatter, chunked) reserves from the stored per-chunk counts.
 template <int W, int MODE>
-__global__ __launch_bounds__(ST) void k_split(SplitArgs a) {
+__device__ __forceinline__ void split_tile(const SplitArgs &a, uint64_t b) {
     constexpr int IPT = SplitCfg<W>::IPT, TK = SplitCfg<W>::TK;
     const uint64_t TKL = (uint64_t)TK * a.chunk;
     __shared__ uint32_t hist[1 << MAX_DBITS];
     __shared__ uint32_t cbase32[MODE == 1 ? (1 << MAX_DBITS) : 1];
     __shared__ uint64_t cbase[MODE == 2 ? (1 << MAX_DBITS) : 1];
     const int tid = threadIdx.x;
-    const uint64_t b = xcd_tile(blockIdx.x, gridDim.x);
     uint32_t lo = 0, hi = a.nseg;  // largest s with tile_pre[s] <= b
     while (hi - lo > 1) {
         uint32_t mid = (lo + hi) / 2;
@@ -687,6 +710,11 @@ __global__ __launch_bounds__(ST) void k_split(SplitArgs a) {
     }
 }
 
+template <int W, int MODE>
+__global__ __launch_bounds__(ST) void k_split(SplitArgs a) {
+    split_tile<W, MODE>(a, xcd_tile(blockIdx.x, gridDim.x));
+}
+
 // Scatter pass of one tile through LDS (children up to 2^SPLIT_STAGED_BITS):
 // each key's rank among its child's keys comes from the LDS count itself,
 // the tile is regrouped by child in LDS, and the grouped tile is stored in
@@ -697,7 +725,7 @@ __global__ __launch_bounds__(ST) void k_split(SplitArgs a) {
 constexpr int SST = 512;
 constexpr int SPLIT_STAGED_BITS = 11;
 template <int W>
-__global__ __launch_bounds__(SST) void k_split_staged(SplitArgs a) {
+__device__ __forceinline__ void split_staged_tile(const SplitArgs &a, uint64_t b) {
     constexpr int TK = SplitCfg<W>::TK, IPT = TK / SST;
     constexpr int ND = 1 << SPLIT_STAGED_BITS, CPT = ND / SST;
     static_assert(TK % SST == 0 && ND % SST == 0, "tile and children split evenly over the threads");
@@ -705,7 +733,6 @@ __global__ __launch_bounds__(SST) void k_split_staged(SplitArgs a) {
     __shared__ uint32_t loff[ND];  // child counts, then each child's first LDS position
     __shared__ uint32_t gofs[ND];  // child's destination minus its LDS position (mod 2^32)
     const int tid = threadIdx.x;
-    const uint64_t b = xcd_tile(blockIdx.x, gridDim.x);
     uint32_t lo = 0, hi = a.nseg;  // largest s with tile_pre[s] <= b
     while (hi - lo > 1) {
         uint32_t mid = (lo + hi) / 2;
@@ -780,6 +807,11 @@ __global__ __launch_bounds__(SST) void k_split_staged(SplitArgs a) {
     }
 }
 
+template <int W>
+__global__ __launch_bounds__(SST) void k_split_staged(SplitArgs a) {
+    split_staged_tile<W>(a, xcd_tile(blockIdx.x, gridDim.x));
+}
+
 // ---- per-bucket LSD radix sort in LDS + run-length counts ------------------
 // One 256-thread workgroup per bucket; IPT (items per thread) sets the size
 // class, so small buckets take little LDS and many workgroups share a CU.
@@ -788,12 +820,9 @@ __global__ __launch_bounds__(SST) void k_split_staged(SplitArgs a) {
 // wave uses ballot-matched digits, processed item by item, so every pass is
 // stable in position order; waves are combined by a (digit, wave) scan.
 template <int W, int IPT>
-__global__ __launch_bounds__(BT) void k_bucket_sort(uint64_t *__restrict__ A,
-                                                     uint64_t *__restrict__ B,
-                                                     uint32_t *__restrict__ counts_tmp,
-                                                     const Bucket *__restrict__ buckets,
-                                                     const uint32_t *__restrict__ order,
-                                                     uint32_t *__restrict__ ucount, int k) {
+__device__ __forceinline__ void lsd_sort_bucket(uint64_t *__restrict__ A, uint64_t *__restrict__ B,
+                                                uint32_t *__restrict__ counts_tmp, const Bucket *__restrict__ buckets,
+                                                const uint32_t bid, uint32_t *__restrict__ ucount, int k) {
     constexpr int CAP = BT * IPT;
     __shared__ uint64_t skeys[CAP * W];
     __shared__ uint32_t whist[256 * BNW];  // [digit][wave]
@@ -801,7 +830,6 @@ __global__ __launch_bounds__(BT) void k_bucket_sort(uint64_t *__restrict__ A,
     __shared__ uint32_t ws[BNW + 1];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const uint32_t bid = order[blockIdx.x];
     const Bucket bk = buckets[bid];
     uint64_t *X = bk.array ? B : A;
     const uint64_t start = bk.start;
@@ -926,6 +954,15 @@ __global__ __launch_bounds__(BT) void k_bucket_sort(uint64_t *__restrict__ A,
     if (tid == 0) ucount[bid] = utotal;
 }
 
+template <int W, int IPT>
+__global__ __launch_bounds__(BT) void k_bucket_sort(uint64_t *__restrict__ A, uint64_t *__restrict__ B,
+                                                     uint32_t *__restrict__ counts_tmp,
+                                                     const Bucket *__restrict__ buckets,
+                                                     const uint32_t *__restrict__ order,
+                                                     uint32_t *__restrict__ ucount, int k) {
+    lsd_sort_bucket<W, IPT>(A, B, counts_tmp, buckets, order[blockIdx.x], ucount, k);
+}
+
 constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v / 2); }
 
 // Waves per SIMD of every hash-sort class: about 18 B of LDS per item (keys,
@@ -947,14 +984,11 @@ constexpr int hash_sort_waves(int ipt) { return ipt > 4 ? 4 : 8; }
 // this kernel.  Output equals k_bucket_sort's: ascending unique keys with
 // their run lengths.
 template <int W, int IPT, int NT>
-__global__ __launch_bounds__(NT, hash_sort_waves(IPT)) void k_bucket_hash_sort(uint64_t *__restrict__ A,
-                                                          uint64_t *__restrict__ B,
-                                                          uint32_t *__restrict__ counts_tmp,
-                                                          const Bucket *__restrict__ buckets,
-                                                          const uint32_t *__restrict__ order,
-                                                          uint32_t *__restrict__ ucount, int k,
-                                                          uint32_t *__restrict__ fb_list,
-                                                          uint32_t *__restrict__ fb_count) {
+__device__ __forceinline__ void hash_sort_bucket(uint64_t *__restrict__ A, uint64_t *__restrict__ B,
+                                                 uint32_t *__restrict__ counts_tmp,
+                                                 const Bucket *__restrict__ buckets, const uint32_t bid,
+                                                 uint32_t *__restrict__ ucount, int k,
+                                                 uint32_t *__restrict__ fb_list, uint32_t *__restrict__ fb_count) {
     constexpr int NW = NT / 64;
     constexpr int CAP = NT * IPT;
     constexpr int HT = 2 * CAP;  // hash slots (load factor <= 1/2), then bins
@@ -971,7 +1005,6 @@ __global__ __launch_bounds__(NT, hash_sort_waves(IPT)) void k_bucket_hash_sort(u
     uint32_t *scnt32 = reinterpret_cast<uint32_t *>(scnt);
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const uint32_t bid = order[blockIdx.x];
     const Bucket bk = buckets[bid];
     uint64_t *X = bk.array ? B : A;
     const uint64_t start = bk.start;
@@ -1162,6 +1195,18 @@ __global__ __launch_bounds__(NT, hash_sort_waves(IPT)) void k_bucket_hash_sort(u
     if (tid == 0) ucount[bid] = U;
 }
 
+template <int W, int IPT, int NT>
+__global__ __launch_bounds__(NT, hash_sort_waves(IPT)) void k_bucket_hash_sort(uint64_t *__restrict__ A,
+                                                          uint64_t *__restrict__ B,
+                                                          uint32_t *__restrict__ counts_tmp,
+                                                          const Bucket *__restrict__ buckets,
+                                                          const uint32_t *__restrict__ order,
+                                                          uint32_t *__restrict__ ucount, int k,
+                                                          uint32_t *__restrict__ fb_list,
+                                                          uint32_t *__restrict__ fb_count) {
+    hash_sort_bucket<W, IPT, NT>(A, B, counts_tmp, buckets, order[blockIdx.x], ucount, k, fb_list, fb_count);
+}
+
 template <int W>
 __global__ __launch_bounds__(256) void k_compact(const uint64_t *__restrict__ A,
                                                  const uint64_t *__restrict__ B,
@@ -1239,12 +1284,12 @@ __global__ void k_level_segs(const Bucket *__restrict__ nodes, uint32_t nn, cons
 // One workgroup per split segment: child start offsets (exclusive scan of
 // the child counts from the segment start), each child's rank among the
 // non-empty children and their number.
-__global__ __launch_bounds__(256) void k_child_scan(const Seg *__restrict__ segs, const uint32_t *__restrict__ child_hist,
-                                                    uint32_t nd, uint64_t *__restrict__ child_cur,
-                                                    uint32_t *__restrict__ nz_rank, uint32_t *__restrict__ nz_count) {
+__device__ __forceinline__ void child_scan_seg(const Seg *__restrict__ segs, const uint32_t *__restrict__ child_hist,
+                                               uint32_t nd, uint64_t *__restrict__ child_cur,
+                                               uint32_t *__restrict__ nz_rank, uint32_t *__restrict__ nz_count,
+                                               uint32_t r) {
     __shared__ uint64_t ws64[256 / 64 + 1];
     __shared__ uint32_t ws32[256 / 64 + 1];
-    const uint32_t r = blockIdx.x;
     const uint64_t base = (uint64_t)r * nd;
     const uint32_t per = (nd + 255) / 256;
     const uint32_t g0 = threadIdx.x * per;
@@ -1274,6 +1319,12 @@ __global__ __launch_bounds__(256) void k_child_scan(const Seg *__restrict__ segs
         }
     }
     if (threadIdx.x == 0) nz_count[r] = ztot;
+}
+
+__global__ __launch_bounds__(256) void k_child_scan(const Seg *__restrict__ segs, const uint32_t *__restrict__ child_hist,
+                                                    uint32_t nd, uint64_t *__restrict__ child_cur,
+                                                    uint32_t *__restrict__ nz_rank, uint32_t *__restrict__ nz_count) {
+    child_scan_seg(segs, child_hist, nd, child_cur, nz_rank, nz_count, blockIdx.x);
 }
 
 __global__ void k_level_counts(uint32_t nn, const uint32_t *__restrict__ act, const uint64_t *__restrict__ act_rank,
@@ -1412,6 +1463,430 @@ __global__ void k_class_offsets(uint32_t *__restrict__ cls_count, uint32_t *__re
     cls_off[SORT_CLASSES] = o;
 }
 
+// ---- device-planned count ----------------------------------------------------
+// The kernels of a launch sequence fixed before the data is seen (so that a
+// HIP graph can hold a whole count): every size the classic orchestration
+// reads back is kept in the DevPlan in device memory and read there, grids
+// are fixed and loop over the plan's counts, and every kernel returns at once
+// when the plan records an overflow (a capacity exceeded: the caller then
+// counts that input the classic way).  Grids that map tiles to XCDs are
+// multiples of the XCD count, so a grid-stride step keeps a tile's XCD.
+
+__global__ void k_dev_plan_init(const uint64_t *__restrict__ h_len, uint64_t cap_len, uint32_t P,
+                                DevPlan *__restrict__ pl) {
+    uint32_t *w = reinterpret_cast<uint32_t *>(pl);
+    for (uint32_t i = threadIdx.x; i < sizeof(DevPlan) / 4; i += blockDim.x) w[i] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint64_t len = *h_len;  // page-locked host word, set before the launch
+        if (len > cap_len) pl->overflow |= 1u;
+        pl->len = len > cap_len ? 0 : len;
+        pl->n_sub = pl->len == 0 ? 1 : ceil_div(pl->len, SUB);
+        pl->lv[0].nn = P;
+    }
+}
+
+__global__ __launch_bounds__(XT) void k_nl_count_dev(const uint8_t *__restrict__ fq, const DevPlan *__restrict__ pl,
+                                                      uint64_t n_sub_cap, uint32_t *__restrict__ out) {
+    __shared__ uint32_t ws[XT / 64 + 1];
+    const uint64_t len = pl->len, n_sub = pl->n_sub;
+    for (uint64_t b = blockIdx.x; b < n_sub_cap; b += gridDim.x) {
+        if (b < n_sub) nl_count_tile(fq, len, b, out, ws);
+        else if (threadIdx.x == 0) out[b] = 0;  // past the input: the scan runs over the capacity
+    }
+}
+
+// The k-mer total; beyond the key buffers, or with a partition of 2^32 keys
+// or more (the one-tile split scatters keep 32-bit offsets within a
+// partition), the count is void.
+__global__ void k_dev_kmers(const uint64_t *__restrict__ part_off, uint32_t P, uint64_t n_cap,
+                            DevPlan *__restrict__ pl) {
+    const uint64_t n = part_off[P];
+    pl->n = n;
+    if (n > n_cap) pl->overflow |= 1u;
+    for (uint32_t p = 0; p < P; p++)
+        if (part_off[p + 1] - part_off[p] > 0xFFFFFFFFull) pl->overflow |= 1u;
+}
+
+__global__ void k_dev_init_nodes(const uint64_t *__restrict__ part_off, uint32_t P, Bucket *__restrict__ nodes,
+                                 uint32_t *__restrict__ npart, const DevPlan *__restrict__ pl) {
+    if (pl->overflow) return;
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    nodes[p] = Bucket{part_off[p], part_off[p + 1] - part_off[p], 0u, 0u};
+    npart[p] = p;
+}
+
+#define KITE_GRID_STRIDE(i, n) \
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (uint64_t)gridDim.x * blockDim.x)
+
+__global__ void k_dev_level_flags(const Bucket *__restrict__ nodes, DevPlan *__restrict__ pl, int l, uint64_t cap,
+                                  int kbits, uint64_t tkl, uint32_t *__restrict__ act, uint64_t *__restrict__ tiles) {
+    if (pl->overflow) return;
+    DevLevel &lv = pl->lv[l];
+    KITE_GRID_STRIDE(i, lv.nn) {
+        const Bucket b = nodes[i];
+        const bool a = b.size > cap && (int)b.prefix < kbits;
+        act[i] = a ? 1u : 0u;
+        tiles[i] = a ? (b.size + tkl - 1) / tkl : 0;
+        if (a) {
+            atomicMax(&lv.maxsz, (unsigned long long)b.size);
+            atomicAdd(&lv.keys, (unsigned long long)b.size);
+        }
+    }
+}
+
+// The level's digit, as the classic orchestration picks it from the largest
+// node: the fewest bits (up to maxd) that bring it to `target` keys.
+__global__ void k_dev_level_plan(DevPlan *__restrict__ pl, int l, const uint64_t *__restrict__ act_rank,
+                                 const uint64_t *__restrict__ tile_off, int kbits, int maxd, uint64_t target,
+                                 int staged_ok, uint64_t nchild_cap) {
+    if (pl->overflow) return;
+    DevLevel &lv = pl->lv[l];
+    lv.nact = act_rank[lv.nn];
+    lv.tiles = tile_off[lv.nn];
+    int d = 0;
+    if (lv.nact) {
+        d = 1;
+        while (d < maxd && (lv.maxsz >> d) > target) d++;
+        d = d < kbits - lv.prefix ? d : kbits - lv.prefix;
+    }
+    lv.d = d;
+    lv.shift = kbits - lv.prefix - d;
+    lv.nchild = lv.nact << d;
+    lv.staged = staged_ok && d <= SPLIT_STAGED_BITS ? 1 : 0;
+    if (lv.nchild > nchild_cap) pl->overflow |= 8u;
+}
+
+__global__ void k_dev_level_segs(const Bucket *__restrict__ nodes, const DevPlan *__restrict__ pl, int l,
+                                 const uint32_t *__restrict__ act, const uint64_t *__restrict__ act_rank,
+                                 const uint64_t *__restrict__ tile_off, Seg *__restrict__ segs,
+                                 uint64_t *__restrict__ tile_pre, uint32_t *__restrict__ seg_node) {
+    if (pl->overflow) return;
+    const DevLevel &lv = pl->lv[l];
+    const uint32_t nd = 1u << lv.d;
+    if (blockIdx.x == 0 && threadIdx.x == 0) tile_pre[lv.nact] = lv.tiles;
+    KITE_GRID_STRIDE(i, lv.nn) {
+        if (!act[i]) continue;
+        const uint64_t r = act_rank[i];
+        const Bucket b = nodes[i];
+        segs[r] = Seg{b.start, b.size, r * nd};
+        tile_pre[r] = tile_off[i];
+        seg_node[r] = (uint32_t)i;
+    }
+}
+
+__global__ void k_dev_zero_children(uint32_t *__restrict__ ch, const DevPlan *__restrict__ pl, int l) {
+    if (pl->overflow) return;
+    KITE_GRID_STRIDE(i, pl->lv[l].nchild) ch[i] = 0;
+}
+
+// Split passes over the level's tiles.  MODE 1 (per-lane stores) runs when
+// the level's digit is too wide for the staged scatter.
+template <int W, int MODE>
+__global__ __launch_bounds__(ST) void k_split_dev(SplitArgs a, const DevPlan *__restrict__ pl, int l) {
+    if (pl->overflow) return;
+    const DevLevel &lv = pl->lv[l];
+    if (lv.nact == 0 || (MODE == 1 && lv.staged)) return;
+    a.nseg = (uint32_t)lv.nact;
+    a.shift = lv.shift;
+    a.dbits = lv.d;
+    const uint64_t T = lv.tiles;
+    for (uint64_t t = blockIdx.x; t < T; t += gridDim.x) {
+        split_tile<W, MODE>(a, xcd_tile(t, T));
+        __syncthreads();
+    }
+}
+
+template <int W>
+__global__ __launch_bounds__(SST) void k_split_staged_dev(SplitArgs a, const DevPlan *__restrict__ pl, int l) {
+    if (pl->overflow) return;
+    const DevLevel &lv = pl->lv[l];
+    if (lv.nact == 0 || !lv.staged) return;
+    a.nseg = (uint32_t)lv.nact;
+    a.shift = lv.shift;
+    a.dbits = lv.d;
+    const uint64_t T = lv.tiles;
+    for (uint64_t t = blockIdx.x; t < T; t += gridDim.x) {
+        split_staged_tile<W>(a, xcd_tile(t, T));
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_dev_child_scan(const Seg *__restrict__ segs,
+                                                        const uint32_t *__restrict__ child_hist,
+                                                        const DevPlan *__restrict__ pl, int l,
+                                                        uint64_t *__restrict__ child_cur,
+                                                        uint32_t *__restrict__ nz_rank,
+                                                        uint32_t *__restrict__ nz_count) {
+    if (pl->overflow) return;
+    const DevLevel &lv = pl->lv[l];
+    for (uint64_t r = blockIdx.x; r < lv.nact; r += gridDim.x) {
+        child_scan_seg(segs, child_hist, 1u << lv.d, child_cur, nz_rank, nz_count, (uint32_t)r);
+        __syncthreads();
+    }
+}
+
+__global__ void k_dev_level_counts(const DevPlan *__restrict__ pl, int l, const uint32_t *__restrict__ act,
+                                   const uint64_t *__restrict__ act_rank, const uint32_t *__restrict__ nz_count,
+                                   uint32_t *__restrict__ out_cnt) {
+    if (pl->overflow) return;
+    KITE_GRID_STRIDE(i, pl->lv[l].nn) out_cnt[i] = act[i] ? nz_count[act_rank[i]] : 1u;
+}
+
+__global__ void k_dev_level_next(DevPlan *__restrict__ pl, int l, const uint64_t *__restrict__ new_base,
+                                 uint64_t nn_cap) {
+    if (pl->overflow) return;
+    const DevLevel &lv = pl->lv[l];
+    const uint64_t nn = new_base[lv.nn];
+    if (nn > nn_cap) pl->overflow |= 4u;
+    pl->lv[l + 1].nn = nn > nn_cap ? 0 : nn;  // a count past the capacity is never stored
+    pl->lv[l + 1].prefix = lv.prefix + lv.d;
+}
+
+__global__ void k_dev_emit_nodes(const Bucket *__restrict__ nodes, const uint32_t *__restrict__ npart,
+                                 const DevPlan *__restrict__ pl, int l, const uint32_t *__restrict__ act,
+                                 const uint64_t *__restrict__ new_base, Bucket *__restrict__ out,
+                                 uint32_t *__restrict__ out_part) {
+    if (pl->overflow) return;
+    KITE_GRID_STRIDE(i, pl->lv[l].nn) {
+        if (act[i]) continue;
+        const uint64_t o = new_base[i];
+        out[o] = nodes[i];
+        out_part[o] = npart[i];
+    }
+}
+
+__global__ void k_dev_emit_children(const uint32_t *__restrict__ seg_node, const uint64_t *__restrict__ new_base,
+                                    const Bucket *__restrict__ nodes, const uint32_t *__restrict__ npart,
+                                    const uint32_t *__restrict__ child_hist, const uint64_t *__restrict__ child_cur,
+                                    const uint32_t *__restrict__ nz_rank, const DevPlan *__restrict__ pl, int l,
+                                    Bucket *__restrict__ out, uint32_t *__restrict__ out_part) {
+    if (pl->overflow) return;
+    const DevLevel &lv = pl->lv[l];
+    const uint32_t d = (uint32_t)lv.d;
+    KITE_GRID_STRIDE(id, lv.nchild) {
+        const uint32_t c = child_hist[id];
+        if (!c) continue;
+        const uint32_t i = seg_node[id >> d];
+        const Bucket b = nodes[i];
+        const uint64_t o = new_base[i] + nz_rank[id];
+        out[o] = Bucket{child_cur[id], c, b.prefix + d, b.array ^ 1u};
+        out_part[o] = npart[i];
+    }
+}
+
+// After the last level no node may still need a split.
+__global__ void k_dev_levels_done(DevPlan *__restrict__ pl) {
+    if (pl->lv[DEV_LEVELS].maxsz) pl->overflow |= 2u;
+}
+
+__global__ void k_dev_part_first(const uint32_t *__restrict__ npart, const DevPlan *__restrict__ pl, uint32_t P,
+                                 uint32_t *__restrict__ pfirst) {
+    if (pl->overflow) return;
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > P) return;
+    uint64_t lo = 0, hi = pl->lv[DEV_LEVELS].nn;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) / 2;
+        if (npart[mid] < p) lo = mid + 1;
+        else hi = mid;
+    }
+    pfirst[p] = (uint32_t)lo;
+}
+
+__global__ __launch_bounds__(CT) void k_dev_classify(const Bucket *__restrict__ nodes, DevPlan *__restrict__ pl,
+                                                     int kbits, ClassCaps caps, uint32_t *__restrict__ cls) {
+    __shared__ uint32_t wc[SORT_CLASSES];
+    if (pl->overflow) return;
+    const uint64_t nb = pl->lv[DEV_LEVELS].nn;
+    if (threadIdx.x < SORT_CLASSES) wc[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint64_t b0 = (uint64_t)blockIdx.x * CT; b0 < nb; b0 += (uint64_t)gridDim.x * CT) {
+        const uint64_t b = b0 + threadIdx.x;
+        int c = -1;
+        if (b < nb) {
+            const Bucket bk = nodes[b];
+            c = 0;
+            if ((int)bk.prefix < kbits && bk.size > 1)
+                while (c < SORT_CLASSES - 1 && bk.size > caps.cap[c]) c++;
+            cls[b] = (uint32_t)c;
+        }
+#pragma unroll
+        for (int q = 0; q < SORT_CLASSES; q++) {
+            const uint64_t m = __ballot(c == q);
+            if (m && __lane_id() == (unsigned)__ffsll((unsigned long long)m) - 1u)
+                atomicAdd(&wc[q], (uint32_t)__popcll(m));
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < SORT_CLASSES && wc[threadIdx.x]) atomicAdd(&pl->cls_n[threadIdx.x], wc[threadIdx.x]);
+}
+
+__global__ void k_dev_class_offsets(DevPlan *__restrict__ pl) {
+    if (pl->overflow) return;
+    uint32_t o = 0;
+    for (int c = 0; c < SORT_CLASSES; c++) {
+        pl->cls_off[c] = o;
+        o += pl->cls_n[c];
+        pl->cls_cur[c] = 0;
+    }
+    pl->cls_off[SORT_CLASSES] = o;
+}
+
+__global__ __launch_bounds__(CT) void k_dev_order_fill(const uint32_t *__restrict__ cls, DevPlan *__restrict__ pl,
+                                                       uint32_t *__restrict__ order) {
+    constexpr int NWV = CT / 64;
+    __shared__ uint32_t wofs[NWV][SORT_CLASSES];
+    __shared__ uint32_t base[SORT_CLASSES];
+    if (pl->overflow) return;
+    const uint64_t nb = pl->lv[DEV_LEVELS].nn;
+    const uint32_t lane = __lane_id(), wave = threadIdx.x / 64;
+    for (uint64_t b0 = (uint64_t)blockIdx.x * CT; b0 < nb; b0 += (uint64_t)gridDim.x * CT) {
+        const uint64_t b = b0 + threadIdx.x;
+        const int c = b < nb ? (int)cls[b] : -1;
+        uint64_t mine = 0;
+#pragma unroll
+        for (int q = 0; q < SORT_CLASSES; q++) {
+            const uint64_t m = __ballot(c == q);
+            if (c == q) mine = m;
+            if (lane == 0) wofs[wave][q] = (uint32_t)__popcll(m);
+        }
+        __syncthreads();
+        if (threadIdx.x < SORT_CLASSES) {
+            const int q = (int)threadIdx.x;
+            uint32_t run = 0;
+            for (int w = 0; w < NWV; w++) {
+                const uint32_t t = wofs[w][q];
+                wofs[w][q] = run;
+                run += t;
+            }
+            base[q] = run ? atomicAdd(&pl->cls_cur[q], run) : 0u;
+        }
+        __syncthreads();
+        if (c >= 0)
+            order[pl->cls_off[c] + base[c] + wofs[wave][c] + (uint32_t)__popcll(mine & lanemask_lt())] = (uint32_t)b;
+        __syncthreads();
+    }
+}
+
+// The buckets of class C: hash-then-bin, handing skewed buckets to the
+// class's fallback list.
+// Persistent workgroups take buckets from a cursor (the first gridDim.x by
+// their index); the next bucket is claimed when a bucket starts, so the
+// atomic's round trip overlaps the sort.
+template <class F>
+__device__ __forceinline__ void for_each_claimed(uint32_t cnt, uint32_t *cursor, F &&body) {
+    __shared__ uint32_t s_next;
+    uint32_t j = blockIdx.x;
+    while (j < cnt) {
+        uint32_t nj = 0;
+        if (threadIdx.x == 0) nj = gridDim.x + atomicAdd(cursor, 1u);
+        body(j);
+        if (threadIdx.x == 0) s_next = nj;
+        __syncthreads();
+        j = s_next;
+        __syncthreads();
+    }
+}
+
+template <int W, int IPT, int NT, int C>
+__global__ __launch_bounds__(NT, hash_sort_waves(IPT)) void k_bucket_hash_sort_dev(
+    uint64_t *__restrict__ A, uint64_t *__restrict__ B, uint32_t *__restrict__ counts_tmp,
+    const Bucket *__restrict__ buckets, const uint32_t *__restrict__ order, uint32_t *__restrict__ ucount, int k,
+    uint32_t *__restrict__ fb, DevPlan *__restrict__ pl) {
+    if (pl->overflow) return;
+    const uint32_t cnt = pl->cls_n[C], first = pl->cls_off[C];
+    for_each_claimed(cnt, &pl->sort_cur[C], [&](uint32_t j) {
+        hash_sort_bucket<W, IPT, NT>(A, B, counts_tmp, buckets, order[first + j], ucount, k, fb + first,
+                                     &pl->fb_n[C]);
+    });
+}
+
+// The buckets of class C by LSD passes: the whole class (FB false) or the
+// buckets its hash pass handed back (FB true).
+template <int W, int IPT, int C, bool FB>
+__global__ __launch_bounds__(BT) void k_bucket_sort_dev(uint64_t *__restrict__ A, uint64_t *__restrict__ B,
+                                                         uint32_t *__restrict__ counts_tmp,
+                                                         const Bucket *__restrict__ buckets,
+                                                         const uint32_t *__restrict__ list,
+                                                         uint32_t *__restrict__ ucount, int k,
+                                                         DevPlan *__restrict__ pl) {
+    if (pl->overflow) return;
+    const uint32_t cnt = FB ? pl->fb_n[C] : pl->cls_n[C], first = pl->cls_off[C];
+    for_each_claimed(cnt, &pl->sort_cur[SORT_CLASSES * (FB ? 2 : 1) + C], [&](uint32_t j) {
+        lsd_sort_bucket<W, IPT>(A, B, counts_tmp, buckets, list[first + j], ucount, k);
+    });
+}
+
+__global__ void k_dev_unique(DevPlan *__restrict__ pl, const uint64_t *__restrict__ uoff) {
+    if (pl->overflow) return;
+    pl->n_unique = uoff[pl->lv[DEV_LEVELS].nn];
+}
+
+template <int W>
+__global__ __launch_bounds__(256) void k_compact_dev(const uint64_t *__restrict__ A, const uint64_t *__restrict__ B,
+                                                     const uint32_t *__restrict__ counts_tmp,
+                                                     const Bucket *__restrict__ buckets,
+                                                     const uint64_t *__restrict__ uoff, const DevPlan *__restrict__ pl,
+                                                     uint64_t *__restrict__ out_keys,
+                                                     uint32_t *__restrict__ out_counts) {
+    if (pl->overflow) return;
+    // one bucket per wave: a fixed grid keeps many buckets' row copies in
+    // flight (a workgroup per bucket in turn would wait on each bucket's
+    // offsets before its copy)
+    const uint64_t nb = pl->lv[DEV_LEVELS].nn;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t waves = (uint64_t)gridDim.x * (256 / 64);
+    for (uint64_t b = (uint64_t)blockIdx.x * (256 / 64) + threadIdx.x / 64; b < nb; b += waves) {
+        const Bucket bk = buckets[b];
+        const uint64_t *X = bk.array ? B : A;
+        const uint64_t o = uoff[b];
+        const uint64_t u = uoff[b + 1] - o;
+        for (uint64_t i = lane; i < u; i += 64) {
+#pragma unroll
+            for (int r = 0; r < W; r++) out_keys[(o + i) * W + r] = X[(bk.start + i) * W + r];
+            out_counts[o + i] = counts_tmp[bk.start + i];
+        }
+    }
+}
+
+__global__ void k_dev_part_rows(const uint64_t *__restrict__ uoff, const uint32_t *__restrict__ pfirst, uint32_t P,
+                                const DevPlan *__restrict__ pl, uint64_t *__restrict__ pu) {
+    if (pl->overflow) return;
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p <= P) pu[p] = uoff[pfirst[p]];
+}
+
+// The summary and the partition tables to page-locked host memory.
+__global__ void k_dev_summary(const DevPlan *__restrict__ pl, uint32_t P, const uint64_t *__restrict__ pu,
+                              const uint64_t *__restrict__ byte_off, const uint64_t *__restrict__ minmax,
+                              DevSummary *__restrict__ h) {
+    uint64_t *tab = reinterpret_cast<uint64_t *>(h + 1);
+    const bool ok = pl->overflow == 0;
+    KITE_GRID_STRIDE(i, 2ull * (P + 1) + 8ull * P) {
+        uint64_t v = 0;
+        if (ok) v = i <= P ? pu[i] : (i <= 2ull * P + 1 ? byte_off[i - (P + 1)] : minmax[i - 2ull * (P + 1)]);
+        tab[i] = v;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        DevSummary sm{};
+        sm.len = pl->len;
+        sm.overflow = pl->overflow;
+        if (ok) {
+            sm.n_kmers = pl->n;
+            sm.n_unique = pl->n_unique;
+            sm.bytes = pl->bytes;
+            sm.buckets = pl->lv[DEV_LEVELS].nn;
+            uint32_t lv = 0;
+            for (int l = 0; l < DEV_LEVELS; l++) lv += pl->lv[l].nact ? 1u : 0u;
+            sm.levels = lv;
+            sm.fallback = (uint64_t)pl->fb_n[0] + pl->fb_n[1] + pl->fb_n[2] + pl->fb_n[3];
+        }
+        *h = sm;
+    }
+}
+
 // ---- host orchestration -----------------------------------------------------
 namespace {
 struct EventTimer {
@@ -1532,6 +2007,22 @@ uint64_t CountEngine::bytes_needed(uint64_t n, int words, uint64_t fastq, uint32
     return n * (3 * kb + 8) + mat * 12 + n_sub * 12 + (n / 256 + 1) * 64 + (64ull << 20);
 }
 
+// The settings of the next count: the base settings, overridden by the
+// environment.
+CountTunables CountEngine::resolved_tunables() const {
+    CountTunables t = base_;
+    if (const char *e = getenv("KITE_SPLIT_DIV")) t.split_target_div = std::max(1, atoi(e));
+    if (const char *e = getenv("KITE_SORT_DEDUP")) t.dedup = atoi(e) != 0;
+    if (const char *e = getenv("KITE_MAX_DBITS")) t.max_digit_bits = std::min(MAX_DBITS, std::max(1, atoi(e)));
+    if (const char *e = getenv("KITE_HASH_CLASSES")) t.hash_classes = std::min(SORT_CLASSES, std::max(0, atoi(e)));
+    if (const char *e = getenv("KITE_SPLIT_CHUNK")) t.split_chunk = std::min(64, std::max(1, atoi(e)));
+    if (const char *e = getenv("KITE_EXTRACT_SUBS")) t.extract_subs = std::min(64, std::max(1, atoi(e)));
+    if (const char *e = getenv("KITE_EXTRACT_STAGED")) t.extract_staged = atoi(e) != 0;
+    if (const char *e = getenv("KITE_SPLIT_STAGED")) t.split_staged = atoi(e) != 0 ? 1 : 0;
+    if (const char *e = getenv("KITE_SPLIT_ABOVE_DIV")) t.split_above_div = std::min(4, std::max(1, atoi(e)));
+    return t;
+}
+
 void CountEngine::run(const uint8_t *d_fq, uint64_t len, int k, int canonical, uint32_t partitions) {
     if (k < 1 || k > 128) throw Error("invalid k-mer length: k must be between 1 and 128");
     if (partitions == 0 || (partitions & (partitions - 1)))
@@ -1541,17 +2032,7 @@ void CountEngine::run(const uint8_t *d_fq, uint64_t len, int k, int canonical, u
                     " are not supported by the GPU count engine");
     int pbits = __builtin_ctz(partitions);
     words_ = k <= 31 ? 1 : (k <= 64 ? 2 : 4);
-    // tunables: built-in defaults, overridden per run by the environment
-    tun = base_;
-    if (const char *e = getenv("KITE_SPLIT_DIV")) tun.split_target_div = std::max(1, atoi(e));
-    if (const char *e = getenv("KITE_SORT_DEDUP")) tun.dedup = atoi(e) != 0;
-    if (const char *e = getenv("KITE_MAX_DBITS")) tun.max_digit_bits = std::min(MAX_DBITS, std::max(1, atoi(e)));
-    if (const char *e = getenv("KITE_HASH_CLASSES")) tun.hash_classes = std::min(SORT_CLASSES, std::max(0, atoi(e)));
-    if (const char *e = getenv("KITE_SPLIT_CHUNK")) tun.split_chunk = std::min(64, std::max(1, atoi(e)));
-    if (const char *e = getenv("KITE_EXTRACT_SUBS")) tun.extract_subs = std::min(64, std::max(1, atoi(e)));
-    if (const char *e = getenv("KITE_EXTRACT_STAGED")) tun.extract_staged = atoi(e) != 0;
-    if (const char *e = getenv("KITE_SPLIT_STAGED")) tun.split_staged = atoi(e) != 0 ? 1 : 0;
-    if (const char *e = getenv("KITE_SPLIT_ABOVE_DIV")) tun.split_above_div = std::min(4, std::max(1, atoi(e)));
+    tun = resolved_tunables();
     if (words_ == 1) run_w<1>(d_fq, len, k, canonical, pbits);
     else if (words_ == 2) run_w<2>(d_fq, len, k, canonical, pbits);
     else run_w<4>(d_fq, len, k, canonical, pbits);
@@ -1900,5 +2381,323 @@ void CountEngine::run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical,
 template void CountEngine::run_w<1>(const uint8_t *, uint64_t, int, int, int);
 template void CountEngine::run_w<2>(const uint8_t *, uint64_t, int, int, int);
 template void CountEngine::run_w<4>(const uint8_t *, uint64_t, int, int, int);
+
+// ---- device-planned count: host side ---------------------------------------------
+
+namespace {
+
+// Workgroups of `f` resident on the whole device at once (a loop over work
+// items spread over exactly these keeps every workgroup busy to the end).
+int resident_grid(const void *f, int threads, size_t dyn_lds) {
+    static std::map<std::tuple<const void *, int, size_t>, int> cache;
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        KITE_HIP(hipGetDevice(&dev));
+        KITE_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    const auto key = std::make_tuple(f, threads, dyn_lds);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int per = 0;
+    KITE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, f, threads, dyn_lds));
+    const int g = std::max(1, per) * cus;
+    cache[key] = g;
+    return g;
+}
+
+dim3 capped_grid(uint64_t items, uint32_t per_wg, uint32_t cap) {
+    return dim3((unsigned)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(std::max<uint64_t>(items, 1), per_wg), cap)));
+}
+
+}  // namespace
+
+void CountEngine::dev_prepare(uint64_t cap_len, int k, uint32_t partitions, EncodeBufs &enc) {
+    if (k < 1 || k > 128) throw Error("invalid k-mer length: k must be between 1 and 128");
+    if (partitions == 0 || (partitions & (partitions - 1)) || partitions > MAX_PARTITIONS)
+        throw Error("invalid partition count for the device-planned count");
+    const uint32_t P = partitions;
+    const int W = k <= 31 ? 1 : (k <= 64 ? 2 : 4);
+    DevCaps c;
+    c.len = cap_len;
+    c.n_sub = std::max<uint64_t>(1, ceil_div(cap_len, SUB));
+    c.nwg = c.n_sub;  // one sub-tile per workgroup: the widest matrix any setting makes
+    // a k-mer ends on a sequence byte, and a record's quality line is as
+    // long as its sequence line
+    c.n = cap_len / 2 + 4096;
+    c.nodes = (uint64_t)P + c.n / 256 + 65536;
+    // (KITE_PLAN_NODES lowers the node capacity, so that tests reach the
+    // classic path a planned count hands its input to)
+    if (const char *e = getenv("KITE_PLAN_NODES"))
+        c.nodes = std::max<uint64_t>((uint64_t)P + 1, std::min<uint64_t>(c.nodes, strtoull(e, nullptr, 10)));
+    c.children = (uint64_t)std::max<uint32_t>(P, 2048) * (1u << MAX_DBITS);
+    dcap_ = c;
+    dP_ = P;
+    dW_ = W;
+    const uint64_t mat = (uint64_t)P * c.nwg, kb = 8ull * W;
+    dv_.plan.ensure(sizeof(DevPlan));
+    dv_.nl.ensure(c.n_sub * 4);
+    dv_.line0.ensure((c.n_sub + 1) * 8);
+    dv_.stmp.ensure(scan_temp_elems(std::max<uint64_t>(std::max<uint64_t>(mat, c.n_sub), c.nodes) + 1) * 8);
+    dv_.hist.ensure(mat * 4);
+    dv_.off.ensure((mat + 1) * 8);
+    dv_.part_off.ensure(((uint64_t)P + 1) * 8);
+    dv_.a.ensure(c.n * kb);
+    dv_.b.ensure(c.n * kb);
+    dv_.ctmp.ensure(c.n * 4);
+    for (int q = 0; q < 2; q++) {
+        dv_.nodes[q].ensure(c.nodes * sizeof(Bucket));
+        dv_.npart[q].ensure(c.nodes * 4);
+    }
+    dv_.act.ensure(c.nodes * 4);
+    dv_.tiles.ensure(c.nodes * 8);
+    dv_.act_rank.ensure((c.nodes + 1) * 8);
+    dv_.tile_off.ensure((c.nodes + 1) * 8);
+    dv_.segs.ensure(c.nodes * sizeof(Seg));
+    dv_.tpre.ensure((c.nodes + 1) * 8);
+    dv_.seg_node.ensure(c.nodes * 4);
+    dv_.nzc.ensure(c.nodes * 4);
+    dv_.ch.ensure(c.children * 4);
+    dv_.ccur.ensure(c.children * 8);
+    dv_.nzr.ensure(c.children * 4);
+    dv_.ncnt.ensure(c.nodes * 4);
+    dv_.nbase.ensure((c.nodes + 1) * 8);
+    dv_.cls.ensure(c.nodes * 4);
+    dv_.order.ensure(c.nodes * 4);
+    dv_.fb.ensure(c.nodes * 4);
+    dv_.ucount.ensure(c.nodes * 4);
+    dv_.uoff.ensure((c.nodes + 1) * 8);
+    dv_.pfirst.ensure(((uint64_t)P + 1) * 4);
+    dv_.pu.ensure(((uint64_t)P + 1) * 8);
+    dv_.keys.ensure(c.n * kb);
+    dv_.counts.ensure(c.n * 4);
+    encode_prepare_dev(c.n, P, enc);
+}
+
+void CountEngine::dev_enqueue_count(const uint8_t *d_fq, const uint64_t *len, uint64_t max_len, int k, int canonical,
+                                    uint32_t partitions) {
+    const int W = k <= 31 ? 1 : (k <= 64 ? 2 : 4);
+    if (partitions != dP_ || W != dW_) throw Error("internal: device-planned count prepared for another shape");
+    if (max_len > dcap_.len) throw Error("internal: device-planned count above its prepared capacity");
+    const int pbits = __builtin_ctz(partitions);
+    if (W == 1) dev_enqueue_w<1>(d_fq, len, max_len, k, canonical, pbits);
+    else if (W == 2) dev_enqueue_w<2>(d_fq, len, max_len, k, canonical, pbits);
+    else dev_enqueue_w<4>(d_fq, len, max_len, k, canonical, pbits);
+}
+
+template <int W>
+void CountEngine::dev_enqueue_w(const uint8_t *d_fq, const uint64_t *h_len, uint64_t max_len, int k, int canonical,
+                                int pbits) {
+    hipStream_t s = stream_;
+    // buffers for the prepared capacity, grids for inputs up to max_len
+    DevCaps c = dcap_;
+    c.len = max_len;
+    c.n_sub = std::max<uint64_t>(1, ceil_div(max_len, SUB));
+    const uint32_t P = 1u << pbits;
+    const int kbits = 2 * k;
+    const CountTunables t = resolved_tunables();
+    const int CAP = sort_cap(W);
+    const int split_div = t.split_target_div > 0 ? t.split_target_div : (W == 1 ? 2 : 4);
+    const int hash_classes = t.hash_classes >= 0 ? t.hash_classes : (W == 1 ? SORT_CLASSES : 3);
+    const bool split_staged = t.split_staged >= 0 ? t.split_staged != 0 : W == 1;
+    const uint32_t subs = (uint32_t)std::max(1, t.extract_subs);
+    const uint64_t nwg = ceil_div(c.n_sub, subs);
+    const uint64_t mat = (uint64_t)P * nwg;
+    DevPlan *pl = dv_.plan.as<DevPlan>();
+    uint64_t *stmp = dv_.stmp.as<uint64_t>();
+    uint64_t *A = dv_.a.as<uint64_t>(), *B = dv_.b.as<uint64_t>();
+    uint32_t *ctmp = dv_.ctmp.as<uint32_t>();
+    auto chk = [] { KITE_CHECK_LAUNCH(); };
+
+    // 1. plan, line index over the capacity (zero past the input)
+    k_dev_plan_init<<<1, 256, 0, s>>>(h_len, c.len, P, pl);
+    chk();
+    k_nl_count_dev<<<capped_grid(c.n_sub, 1, 4096), XT, 0, s>>>(d_fq, pl, c.n_sub, dv_.nl.as<uint32_t>());
+    chk();
+    uint64_t *line0 = dv_.line0.as<uint64_t>();
+    scan_to_u64(dv_.nl.as<uint32_t>(), line0, c.n_sub, stmp, s);
+
+    // 2. partition histogram per extract workgroup, 3. scatter
+    ExtractArgs ea{};
+    ea.fq = d_fq;
+    ea.sub_line0 = line0;
+    ea.nwg = (uint32_t)nwg;
+    ea.subs = subs;
+    ea.k = k;
+    ea.canonical = canonical;
+    ea.pbits = pbits;
+    ea.hist = dv_.hist.as<uint32_t>();
+    ea.pl = pl;
+    size_t dyn_hist = (size_t)P * 4;
+    size_t dyn_scat = (size_t)((P + 1) & ~1u) * 4 + (size_t)P * 8;
+    const bool staged = t.extract_staged && 2 * P <= (uint32_t)StageCfg<W>::SC;
+    if (staged) dyn_scat += (size_t)P * 8;
+    {
+        const int max_dyn = (int)(((MAX_PARTITIONS + 1) & ~1u) * 4 + MAX_PARTITIONS * 8);
+        KITE_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_extract<W, 0>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, max_dyn));
+        KITE_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_extract<W, 1>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, max_dyn));
+    }
+    k_extract<W, 0><<<dim3((unsigned)nwg), dim3(XT), dyn_hist, s>>>(ea);
+    chk();
+    uint64_t *off = dv_.off.as<uint64_t>();
+    scan_to_u64(ea.hist, off, mat, stmp, s);
+    uint64_t *part_off = dv_.part_off.as<uint64_t>();
+    k_part_gather<<<dim3((P + 1 + 255) / 256), dim3(256), 0, s>>>(off, (uint32_t)nwg, P, part_off);
+    chk();
+    k_dev_kmers<<<1, 1, 0, s>>>(part_off, P, c.n, pl);
+    chk();
+    ea.off = off;
+    ea.out = A;
+    if (staged) k_extract<W, 2><<<dim3((unsigned)nwg), dim3(XT), dyn_scat, s>>>(ea);
+    else k_extract<W, 1><<<dim3((unsigned)nwg), dim3(XT), dyn_scat, s>>>(ea);
+    chk();
+
+    // 4. MSD split levels, every one launched; a level with no node to split
+    // copies the node list as it is
+    Bucket *nodes[2] = {dv_.nodes[0].as<Bucket>(), dv_.nodes[1].as<Bucket>()};
+    uint32_t *npart[2] = {dv_.npart[0].as<uint32_t>(), dv_.npart[1].as<uint32_t>()};
+    k_dev_init_nodes<<<dim3((P + 255) / 256), 256, 0, s>>>(part_off, P, nodes[0], npart[0], pl);
+    chk();
+    uint32_t *act = dv_.act.as<uint32_t>(), *nzc = dv_.nzc.as<uint32_t>(), *ncnt = dv_.ncnt.as<uint32_t>();
+    uint64_t *tiles = dv_.tiles.as<uint64_t>(), *act_rank = dv_.act_rank.as<uint64_t>();
+    uint64_t *tile_off = dv_.tile_off.as<uint64_t>(), *tpre = dv_.tpre.as<uint64_t>(), *nbase = dv_.nbase.as<uint64_t>();
+    Seg *segs = dv_.segs.as<Seg>();
+    uint32_t *seg_node = dv_.seg_node.as<uint32_t>(), *ch = dv_.ch.as<uint32_t>(), *nzr = dv_.nzr.as<uint32_t>();
+    uint64_t *ccur = dv_.ccur.as<uint64_t>();
+    const uint64_t tk = (uint64_t)split_tk(W);
+    const uint64_t cap_above = (uint64_t)CAP / (uint64_t)std::max(1, t.split_above_div);
+    const dim3 gnode = capped_grid(c.nodes, 256, 2048), gchild = capped_grid(c.children, 256, 4096);
+    const int g_hist = resident_grid(reinterpret_cast<const void *>(&k_split_dev<W, 0>), ST, 0);
+    const int g_scat = resident_grid(reinterpret_cast<const void *>(&k_split_dev<W, 1>), ST, 0);
+    const int g_stag = resident_grid(reinterpret_cast<const void *>(&k_split_staged_dev<W>), SST, 0);
+    const int g_cscan = resident_grid(reinterpret_cast<const void *>(&k_dev_child_scan), 256, 0);
+    for (int l = 0; l < DEV_LEVELS; l++) {
+        const int cur = l & 1, nxt = cur ^ 1;
+        k_dev_level_flags<<<gnode, 256, 0, s>>>(nodes[cur], pl, l, cap_above, kbits, tk, act, tiles);
+        chk();
+        scan_to_u64_dev(act, act_rank, &pl->lv[l].nn, c.nodes, stmp, s);
+        scan_to_u64_dev(tiles, tile_off, &pl->lv[l].nn, c.nodes, stmp, s);
+        k_dev_level_plan<<<1, 1, 0, s>>>(pl, l, act_rank, tile_off, kbits, t.max_digit_bits,
+                                         (uint64_t)(CAP / split_div), split_staged ? 1 : 0, c.children);
+        chk();
+        k_dev_level_segs<<<gnode, 256, 0, s>>>(nodes[cur], pl, l, act, act_rank, tile_off, segs, tpre, seg_node);
+        chk();
+        k_dev_zero_children<<<gchild, 256, 0, s>>>(ch, pl, l);
+        chk();
+        SplitArgs sa{};
+        sa.src = cur ? B : A;
+        sa.dst = cur ? A : B;
+        sa.segs = segs;
+        sa.tile_pre = tpre;
+        sa.chunk = 1;
+        sa.child_hist = ch;
+        k_split_dev<W, 0><<<dim3((unsigned)g_hist), dim3(ST), 0, s>>>(sa, pl, l);
+        chk();
+        k_dev_child_scan<<<dim3((unsigned)g_cscan), 256, 0, s>>>(segs, ch, pl, l, ccur, nzr, nzc);
+        chk();
+        k_dev_level_counts<<<gnode, 256, 0, s>>>(pl, l, act, act_rank, nzc, ncnt);
+        chk();
+        scan_to_u64_dev(ncnt, nbase, &pl->lv[l].nn, c.nodes, stmp, s);
+        k_dev_level_next<<<1, 1, 0, s>>>(pl, l, nbase, c.nodes);
+        chk();
+        k_dev_emit_nodes<<<gnode, 256, 0, s>>>(nodes[cur], npart[cur], pl, l, act, nbase, nodes[nxt], npart[nxt]);
+        chk();
+        k_dev_emit_children<<<gchild, 256, 0, s>>>(seg_node, nbase, nodes[cur], npart[cur], ch, ccur, nzr, pl, l,
+                                                   nodes[nxt], npart[nxt]);
+        chk();
+        sa.child_cur = ccur;
+        k_split_staged_dev<W><<<dim3((unsigned)g_stag), dim3(SST), 0, s>>>(sa, pl, l);
+        chk();
+        k_split_dev<W, 1><<<dim3((unsigned)g_scat), dim3(ST), 0, s>>>(sa, pl, l);
+        chk();
+    }
+    const int fin = DEV_LEVELS & 1;
+    Bucket *bkts = nodes[fin];
+    // no node may still need a split
+    k_dev_level_flags<<<gnode, 256, 0, s>>>(bkts, pl, DEV_LEVELS, cap_above, kbits, tk, act, tiles);
+    chk();
+    k_dev_levels_done<<<1, 1, 0, s>>>(pl);
+    chk();
+
+    // 5. sort + run-length count every bucket, by size class
+    uint32_t *pfirst = dv_.pfirst.as<uint32_t>(), *cls = dv_.cls.as<uint32_t>(), *ord = dv_.order.as<uint32_t>();
+    uint32_t *fb = dv_.fb.as<uint32_t>(), *ucount = dv_.ucount.as<uint32_t>();
+    k_dev_part_first<<<dim3((P + 1 + 255) / 256), 256, 0, s>>>(npart[fin], pl, P, pfirst);
+    chk();
+    ClassCaps caps;
+    for (int q = 0; q < SORT_CLASSES; q++) caps.cap[q] = (uint64_t)class_cap(W, q);
+    const dim3 gcls = capped_grid(c.nodes, CT, 512);
+    k_dev_classify<<<gcls, CT, 0, s>>>(bkts, pl, kbits, caps, cls);
+    chk();
+    k_dev_class_offsets<<<1, 1, 0, s>>>(pl);
+    chk();
+    k_dev_order_fill<<<gcls, CT, 0, s>>>(cls, pl, ord);
+    chk();
+    bool hashed[SORT_CLASSES] = {false, false, false, false};
+    auto sort_class = [&](auto cls_c) {
+        constexpr int C = decltype(cls_c)::value;
+        constexpr int IPT = SortClass<W, C>::IPT;
+        if (t.dedup && C < hash_classes) {
+            constexpr int IPTH = (C < 3 ? 4 : 8) / W > 0 ? (C < 3 ? 4 : 8) / W : 1;
+            constexpr int NTH = SortClass<W, C>::CAP / IPTH;
+            const void *f = reinterpret_cast<const void *>(&k_bucket_hash_sort_dev<W, IPTH, NTH, C>);
+            k_bucket_hash_sort_dev<W, IPTH, NTH, C><<<dim3((unsigned)resident_grid(f, NTH, 0)), dim3(NTH), 0, s>>>(
+                A, B, ctmp, bkts, ord, ucount, k, fb, pl);
+            hashed[C] = true;
+        } else {
+            const void *f = reinterpret_cast<const void *>(&k_bucket_sort_dev<W, IPT, C, false>);
+            k_bucket_sort_dev<W, IPT, C, false><<<dim3((unsigned)resident_grid(f, BT, 0)), dim3(BT), 0, s>>>(
+                A, B, ctmp, bkts, ord, ucount, k, pl);
+        }
+        chk();
+    };
+    auto fallback_class = [&](auto cls_c) {
+        constexpr int C = decltype(cls_c)::value;
+        constexpr int IPT = SortClass<W, C>::IPT;
+        if (!hashed[C]) return;
+        const void *f = reinterpret_cast<const void *>(&k_bucket_sort_dev<W, IPT, C, true>);
+        k_bucket_sort_dev<W, IPT, C, true><<<dim3((unsigned)resident_grid(f, BT, 0)), dim3(BT), 0, s>>>(
+            A, B, ctmp, bkts, fb, ucount, k, pl);
+        chk();
+    };
+    sort_class(std::integral_constant<int, 0>{});
+    sort_class(std::integral_constant<int, 1>{});
+    sort_class(std::integral_constant<int, 2>{});
+    sort_class(std::integral_constant<int, 3>{});
+    fallback_class(std::integral_constant<int, 0>{});
+    fallback_class(std::integral_constant<int, 1>{});
+    fallback_class(std::integral_constant<int, 2>{});
+    fallback_class(std::integral_constant<int, 3>{});
+
+    // 6. contiguous per-partition output, 7. chunk encode, 8. summary
+    uint64_t *uoff = dv_.uoff.as<uint64_t>(), *pu = dv_.pu.as<uint64_t>();
+    scan_to_u64_dev(ucount, uoff, &pl->lv[DEV_LEVELS].nn, c.nodes, stmp, s);
+    k_dev_unique<<<1, 1, 0, s>>>(pl, uoff);
+    chk();
+    uint64_t *okeys = dv_.keys.as<uint64_t>();
+    uint32_t *ocounts = dv_.counts.as<uint32_t>();
+    k_compact_dev<W><<<capped_grid(c.nodes, 1, 4096), 256, 0, s>>>(A, B, ctmp, bkts, uoff, pl, okeys, ocounts);
+    chk();
+    k_dev_part_rows<<<dim3((P + 1 + 255) / 256), 256, 0, s>>>(uoff, pfirst, P, pl, pu);
+    chk();
+}
+
+void CountEngine::dev_enqueue_encode(EncodeBufs &enc, uint8_t *bytes, uint64_t bytes_cap, void *h_summary) {
+    hipStream_t s = stream_;
+    const uint32_t P = dP_;
+    DevPlan *pl = dv_.plan.as<DevPlan>();
+    uint64_t *pu = dv_.pu.as<uint64_t>();
+    encode_dev(dv_.keys.as<uint64_t>(), dv_.counts.as<uint32_t>(), pu, &pl->n_unique, dcap_.n, P, dW_, s, enc, bytes,
+               bytes_cap, &pl->bytes, &pl->overflow);
+    k_dev_summary<<<capped_grid(2ull * (P + 1) + 8ull * P, 256, 64), 256, 0, s>>>(
+        pl, P, pu, enc.byte_off.as<uint64_t>(), enc.minmax.as<uint64_t>(), static_cast<DevSummary *>(h_summary));
+    KITE_CHECK_LAUNCH();
+}
+
+template void CountEngine::dev_enqueue_w<1>(const uint8_t *, const uint64_t *, uint64_t, int, int, int);
+template void CountEngine::dev_enqueue_w<2>(const uint8_t *, const uint64_t *, uint64_t, int, int, int);
+template void CountEngine::dev_enqueue_w<4>(const uint8_t *, const uint64_t *, uint64_t, int, int, int);
 
 }  // namespace kite

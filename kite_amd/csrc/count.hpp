@@ -24,11 +24,53 @@
 
 namespace kite {
 
+struct EncodeBufs;
+
 struct Bucket {
     uint64_t start;   // first key (index into the key array)
     uint64_t size;    // keys in the bucket
     uint32_t prefix;  // key bits (from the top of the 2k-bit key) shared by all keys
     uint32_t array;   // 0: keys live in A, 1: in B
+};
+
+// ---- device-planned count (a launch sequence fixed before the data) -----------
+// Every size the classic host orchestration reads back is kept here, in
+// device memory, and the kernels read it there; buffers are sized for a
+// capacity.  A count whose data would exceed a capacity sets `overflow` and
+// writes nothing further; its caller runs the classic count instead.
+constexpr int DEV_LEVELS = 3;  // MSD split levels within one planned count
+
+struct DevLevel {
+    uint64_t nn;                     // nodes in the list
+    uint64_t nact;                   // nodes split at this level
+    uint64_t tiles;                  // split workgroup tiles
+    unsigned long long maxsz, keys;  // largest split node, keys in split nodes
+    uint64_t nchild;                 // nact << d
+    int32_t d, shift, prefix, staged;  // digit bits and shift; key bits above; staged scatter
+};
+
+struct DevPlan {
+    uint64_t len, n_sub, n;       // input bytes, sub-tiles, k-mers
+    DevLevel lv[DEV_LEVELS + 1];  // lv[DEV_LEVELS].nn: the buckets
+    uint32_t cls_n[4], cls_cur[4], cls_off[5], fb_n[4];
+    uint32_t sort_cur[12];        // bucket cursors of the sort launches (hash, LSD, LSD fallback per class)
+    uint64_t n_unique, bytes;     // unique keys, encoded chunk bytes
+    uint32_t overflow;            // bit 0 keys, 1 split levels, 2 nodes, 3 children, 4 chunk bytes
+};
+
+// Host view of a planned count's result, written to page-locked memory by
+// its last kernel: the summary, then part_off (P + 1 row offsets), byte_off
+// (P + 1 chunk byte offsets) and minmax (8 P words: first and last key of
+// every partition, 4 words each).
+struct DevSummary {
+    uint64_t n_kmers, n_unique, bytes, buckets, fallback, len;
+    uint32_t overflow, levels;
+};
+
+// Capacities of a planned count (all from the input capacity in bytes).
+struct DevCaps {
+    uint64_t len = 0, n_sub = 0, nwg = 0, n = 0, nodes = 0, children = 0;
+    uint32_t subs = 0;
 };
 
 struct StageTimes {
@@ -100,9 +142,36 @@ class CountEngine {
     // built-in defaults); KITE_* environment overrides still apply on top.
     void set_base_tunables(const CountTunables &t) { base_ = t; }
 
+    // Device-planned count.  dev_prepare sizes every buffer for inputs up to
+    // cap_len bytes.  dev_enqueue_count then puts the whole count of
+    // d_fq[0, *len) on the engine stream, with the settings current at that
+    // moment, and no host read and no allocation (so that a HIP graph can
+    // capture it): `len` is a word the first kernel reads (device or
+    // page-locked host memory), the grids are sized for inputs up to
+    // max_len (at most cap_len; a longer input is an overflow).
+    // dev_enqueue_encode appends the chunk encode of that count's
+    // result into `bytes` (bytes_cap bytes, DVLE or RAW per partition, as
+    // kite_gpu_count_emit) and its summary into the page-locked `h_summary`
+    // (DevSummary, then part_off, byte_off and minmax).  Buffers of the
+    // encode live in `enc`.
+    void dev_prepare(uint64_t cap_len, int k, uint32_t partitions, EncodeBufs &enc);
+    void dev_enqueue_count(const uint8_t *d_fq, const uint64_t *len, uint64_t max_len, int k, int canonical,
+                           uint32_t partitions);
+    void dev_enqueue_encode(EncodeBufs &enc, uint8_t *bytes, uint64_t bytes_cap, void *h_summary);
+    const DevCaps &dev_caps() const { return dcap_; }
+    // The settings the next count uses: the base settings with the KITE_*
+    // environment overrides applied.
+    CountTunables resolved_tunables() const;
+    // bytes of page-locked memory a summary of `partitions` takes
+    static size_t dev_summary_bytes(uint32_t partitions) {
+        return sizeof(DevSummary) + 8ull * (2ull * (partitions + 1) + 8ull * partitions);
+    }
+
    private:
     template <int W>
     void run_w(const uint8_t *d_fq, uint64_t len, int k, int canonical, int pbits);
+    template <int W>
+    void dev_enqueue_w(const uint8_t *d_fq, const uint64_t *len, uint64_t max_len, int k, int canonical, int pbits);
 
     // records events around one launch when kernel profiling is on
     template <class F>
@@ -128,6 +197,16 @@ class CountEngine {
     // device bucket plan: node lists (ping-pong), per-level scratch, classes
     DevBuf nodes_[2], npart_[2], act_, tiles_, act_rank_, tile_off_, lstats_, seg_node_;
     DevBuf nz_rank_, nz_count_, new_cnt_, new_base_, cls_, cls_count_;
+    // device-planned count: its own buffers (a captured graph keeps their
+    // addresses, so the classic path never reallocates them) and capacities
+    struct DevBufs {
+        DevBuf plan, nl, line0, stmp, hist, off, part_off, a, b, ctmp, nodes[2], npart[2], act, tiles, act_rank,
+            tile_off, segs, tpre, seg_node, ch, ccur, nzr, nzc, ncnt, nbase, cls, order, fb, ucount, uoff, pfirst,
+            pu, keys, counts;
+    } dv_;
+    DevCaps dcap_;
+    uint32_t dP_ = 0;
+    int dW_ = 0;
     void *pin_ = nullptr;  // page-locked scratch for small read-backs
     size_t pin_bytes_ = 0;
     uint64_t *pinned_host(size_t bytes);

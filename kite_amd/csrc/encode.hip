@@ -61,10 +61,9 @@ __global__ void k_mark_heads(const uint64_t *__restrict__ part_off, uint32_t P, 
 }
 
 template <int W>
-__global__ void k_entry_len(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ counts,
-                            const uint8_t *__restrict__ head, uint64_t n, uint32_t *__restrict__ len) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void entry_len_at(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ counts,
+                                             const uint8_t *__restrict__ head, uint64_t i,
+                                             uint32_t *__restrict__ len) {
     if constexpr (W == 4) {
         len[i] = 36;
     } else {
@@ -79,11 +78,16 @@ __global__ void k_entry_len(const uint64_t *__restrict__ keys, const uint32_t *_
 }
 
 template <int W>
-__global__ void k_entry_write(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ counts,
-                              const uint8_t *__restrict__ head, const uint64_t *__restrict__ off, uint64_t n,
-                              uint8_t *__restrict__ out) {
+__global__ void k_entry_len(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ counts,
+                            const uint8_t *__restrict__ head, uint64_t n, uint32_t *__restrict__ len) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i < n) entry_len_at<W>(keys, counts, head, i, len);
+}
+
+template <int W>
+__device__ __forceinline__ void entry_write_at(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ counts,
+                                               const uint8_t *__restrict__ head, const uint64_t *__restrict__ off,
+                                               uint64_t i, uint8_t *__restrict__ out) {
     uint8_t *o = out + off[i];
     if (W == 4 || head[i]) {
         // raw row: key words little-endian, then the count
@@ -106,13 +110,65 @@ __global__ void k_entry_write(const uint64_t *__restrict__ keys, const uint32_t 
     }
 }
 
+template <int W>
+__global__ void k_entry_write(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ counts,
+                              const uint8_t *__restrict__ head, const uint64_t *__restrict__ off, uint64_t n,
+                              uint8_t *__restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) entry_write_at<W>(keys, counts, head, off, i, out);
+}
+
+// ---- the same with the row count read from device memory (fixed grids) ------
+__global__ void k_zero_u8_dev(uint8_t *__restrict__ p, const uint64_t *__restrict__ dn,
+                              const uint32_t *__restrict__ overflow) {
+    if (*overflow) return;
+    const uint64_t n = *dn;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+template <int W>
+__global__ void k_entry_len_dev(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ counts,
+                                const uint8_t *__restrict__ head, const uint64_t *__restrict__ dn,
+                                const uint32_t *__restrict__ overflow, uint32_t *__restrict__ len) {
+    if (*overflow) return;
+    const uint64_t n = *dn;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        entry_len_at<W>(keys, counts, head, i, len);
+}
+
+__global__ void k_mark_heads_dev(const uint64_t *__restrict__ part_off, uint32_t P, uint8_t *__restrict__ head,
+                                 const uint32_t *__restrict__ overflow) {
+    if (*overflow) return;
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < P && part_off[p] < part_off[p + 1]) head[part_off[p]] = 1;
+}
+
+// The payload total; beyond the output's capacity the count is void.
+__global__ void k_bytes_total(const uint64_t *__restrict__ off, const uint64_t *__restrict__ dn, uint64_t cap,
+                              uint64_t *__restrict__ total, uint32_t *__restrict__ overflow) {
+    if (*overflow) return;
+    const uint64_t t = off[*dn];
+    *total = t;
+    if (t > cap) *overflow |= 16u;
+}
+
+template <int W>
+__global__ void k_entry_write_dev(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ counts,
+                                  const uint8_t *__restrict__ head, const uint64_t *__restrict__ off,
+                                  const uint64_t *__restrict__ dn, const uint32_t *__restrict__ overflow,
+                                  uint8_t *__restrict__ out) {
+    if (*overflow) return;
+    const uint64_t n = *dn;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        entry_write_at<W>(keys, counts, head, off, i, out);
+}
+
 // Per partition: byte offset of its chunk and its first and last keys.
 template <int W>
-__global__ void k_chunk_meta(const uint64_t *__restrict__ part_off, uint32_t P, const uint64_t *__restrict__ off,
-                             const uint64_t *__restrict__ keys, uint64_t *__restrict__ byte_off,
-                             uint64_t *__restrict__ minmax) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p > P) return;
+__device__ __forceinline__ void chunk_meta_at(const uint64_t *__restrict__ part_off, uint32_t P, uint32_t p,
+                                              const uint64_t *__restrict__ off, const uint64_t *__restrict__ keys,
+                                              uint64_t *__restrict__ byte_off, uint64_t *__restrict__ minmax) {
     const uint64_t a = part_off[p];
     byte_off[p] = off[a];  // off has n + 1 entries: off[n] is the total
     if (p == P) return;
@@ -122,6 +178,23 @@ __global__ void k_chunk_meta(const uint64_t *__restrict__ part_off, uint32_t P, 
         minmax[(uint64_t)p * 8 + r] = (a < b && r < W) ? keys[a * W + r] : 0;
         minmax[(uint64_t)p * 8 + 4 + r] = (a < b && r < W) ? keys[(b - 1) * W + r] : 0;
     }
+}
+
+template <int W>
+__global__ void k_chunk_meta(const uint64_t *__restrict__ part_off, uint32_t P, const uint64_t *__restrict__ off,
+                             const uint64_t *__restrict__ keys, uint64_t *__restrict__ byte_off,
+                             uint64_t *__restrict__ minmax) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p <= P) chunk_meta_at<W>(part_off, P, p, off, keys, byte_off, minmax);
+}
+
+template <int W>
+__global__ void k_chunk_meta_dev(const uint64_t *__restrict__ part_off, uint32_t P, const uint64_t *__restrict__ off,
+                                 const uint64_t *__restrict__ keys, uint64_t *__restrict__ byte_off,
+                                 uint64_t *__restrict__ minmax, const uint32_t *__restrict__ overflow) {
+    if (*overflow) return;
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p <= P) chunk_meta_at<W>(part_off, P, p, off, keys, byte_off, minmax);
 }
 
 __global__ void k_put_host_u64(const uint64_t *__restrict__ src, uint32_t n, uint64_t *__restrict__ dst) {
@@ -167,7 +240,51 @@ EncodeDevice encode_w(const uint64_t *keys, const uint32_t *counts, const uint64
     return ed;
 }
 
+template <int W>
+void encode_dev_w(const uint64_t *keys, const uint32_t *counts, const uint64_t *part_off, const uint64_t *dn,
+                  uint64_t n_cap, uint32_t P, hipStream_t s, EncodeBufs &bufs, uint8_t *bytes, uint64_t bytes_cap,
+                  uint64_t *total, uint32_t *overflow) {
+    const unsigned T = 256;
+    const dim3 g((unsigned)std::min<uint64_t>(std::max<uint64_t>(1, ceil_div(n_cap, T)), 4096));
+    const dim3 gp((unsigned)ceil_div((uint64_t)P + 1, T));
+    uint8_t *head = bufs.head.as<uint8_t>();
+    uint32_t *len = bufs.len.as<uint32_t>();
+    uint64_t *off = bufs.off.as<uint64_t>();
+    k_zero_u8_dev<<<g, T, 0, s>>>(head, dn, overflow);
+    KITE_CHECK_LAUNCH();
+    k_mark_heads_dev<<<gp, T, 0, s>>>(part_off, P, head, overflow);
+    KITE_CHECK_LAUNCH();
+    k_entry_len_dev<W><<<g, T, 0, s>>>(keys, counts, head, dn, overflow, len);
+    KITE_CHECK_LAUNCH();
+    scan_to_u64_dev(len, off, dn, n_cap, bufs.tmp.as<uint64_t>(), s);
+    k_chunk_meta_dev<W><<<gp, T, 0, s>>>(part_off, P, off, keys, bufs.byte_off.as<uint64_t>(),
+                                         bufs.minmax.as<uint64_t>(), overflow);
+    KITE_CHECK_LAUNCH();
+    k_bytes_total<<<1, 1, 0, s>>>(off, dn, bytes_cap, total, overflow);
+    KITE_CHECK_LAUNCH();
+    k_entry_write_dev<W><<<g, T, 0, s>>>(keys, counts, head, off, dn, overflow, bytes);
+    KITE_CHECK_LAUNCH();
+}
+
 }  // namespace
+
+void encode_prepare_dev(uint64_t n_cap, uint32_t P, EncodeBufs &bufs) {
+    bufs.head.ensure(std::max<uint64_t>(n_cap, 1));
+    bufs.len.ensure(std::max<uint64_t>(n_cap, 1) * 4);
+    bufs.off.ensure((n_cap + 1) * 8);
+    bufs.tmp.ensure(scan_temp_elems(n_cap + 1) * 8);
+    bufs.byte_off.ensure(((uint64_t)P + 1) * 8);
+    bufs.minmax.ensure((uint64_t)P * 64);
+}
+
+void encode_dev(const uint64_t *d_keys, const uint32_t *d_counts, const uint64_t *d_part_off, const uint64_t *dn,
+                uint64_t n_cap, uint32_t P, int W, hipStream_t s, EncodeBufs &bufs, uint8_t *bytes,
+                uint64_t bytes_cap, uint64_t *total, uint32_t *overflow) {
+    if (W == 1) encode_dev_w<1>(d_keys, d_counts, d_part_off, dn, n_cap, P, s, bufs, bytes, bytes_cap, total, overflow);
+    else if (W == 2)
+        encode_dev_w<2>(d_keys, d_counts, d_part_off, dn, n_cap, P, s, bufs, bytes, bytes_cap, total, overflow);
+    else encode_dev_w<4>(d_keys, d_counts, d_part_off, dn, n_cap, P, s, bufs, bytes, bytes_cap, total, overflow);
+}
 
 EncodeDevice encode_on_device(const uint64_t *d_keys, const uint32_t *d_counts, const uint64_t *d_part_off, uint64_t n,
                               uint32_t P, int W, hipStream_t s, EncodeBufs &bufs, uint64_t *host_byte_off) {

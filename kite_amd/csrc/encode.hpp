@@ -90,6 +90,17 @@ struct EncodedRuns {
 EncodeDevice encode_on_device(const uint64_t *d_keys, const uint32_t *d_counts, const uint64_t *d_part_off, uint64_t n,
                               uint32_t P, int W, hipStream_t s, EncodeBufs &bufs, uint64_t *host_byte_off);
 
+// The encode of a launch sequence fixed before the data is seen: the row
+// count is read from *dn (at most n_cap; encode_prepare_dev sizes the
+// scratch for it), the payload goes to bytes (bytes_cap), its size to
+// *total, and nothing is written when *overflow is set (a total above
+// bytes_cap sets bit 4).  Byte offsets and key ranges stay in
+// bufs.byte_off / bufs.minmax on the device.
+void encode_prepare_dev(uint64_t n_cap, uint32_t P, EncodeBufs &bufs);
+void encode_dev(const uint64_t *d_keys, const uint32_t *d_counts, const uint64_t *d_part_off, const uint64_t *dn,
+                uint64_t n_cap, uint32_t P, int W, hipStream_t s, EncodeBufs &bufs, uint8_t *bytes,
+                uint64_t bytes_cap, uint64_t *total, uint32_t *overflow);
+
 // d_keys / d_counts: n reduced rows on the device, partition-ordered;
 // d_part_off: the partitions + 1 row offsets on the device (part_off the same
 // on the host).  Downloads only the encoded bytes and the chunk metadata.

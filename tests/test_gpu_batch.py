@@ -173,3 +173,61 @@ def test_batch_stage_timeline_order(gpu_ctx):
         assert d0 >= r1 - eps, i
         if i:
             assert d0 >= tl[i - 1][5] - eps and h0 >= tl[i - 1][1] - eps, i
+
+
+def test_batch_graph_path_matches_classic(gpu_ctx, monkeypatch):
+    """The pipelined batch of encoded chunks runs each sample's count and
+    encode as one captured HIP graph launch; its chunks, row offsets, key
+    ranges and counts equal the classic pipeline's (KITE_BATCH_GRAPH=0,
+    kernels launched one by one after host read-backs), for one- and
+    two-word keys, across repeated batches (the graphs are kept) and after
+    the settings change (the graphs are captured again)."""
+    ws, pins = _samples(0.003)
+    arrs = [p.array for p in pins] + [np.zeros(0, dtype=np.uint8)]
+    for k, P in ((31, 256), (45, 64)):
+        monkeypatch.setenv("KITE_BATCH_GRAPH", "0")
+        ref = gpu_ctx.count_batch(arrs, k, True, P, mode=BATCH_PIPELINED, encoded=True)
+        monkeypatch.delenv("KITE_BATCH_GRAPH")
+        for rep in range(3):
+            if rep == 2:
+                monkeypatch.setenv("KITE_SPLIT_DIV", "8")  # other settings: captured again
+            got = gpu_ctx.count_batch(arrs, k, True, P, mode=BATCH_PIPELINED, encoded=True)
+            for i, (g, r) in enumerate(zip(got, ref)):
+                assert g[5]["graph"] == 1 and r[5]["graph"] == 0, (k, rep, i)
+                assert g[0] == r[0] and g[1] == r[1], (k, rep, i)
+                for q in (2, 3, 4):
+                    assert np.array_equal(g[q], r[q]), (k, rep, i, q)
+                for f in ("kmers", "unique", "input_bytes", "skipped_chunks"):
+                    assert g[5][f] == r[5][f], (k, rep, i, f)
+        monkeypatch.delenv("KITE_SPLIT_DIV")
+    for p in pins:
+        p.free()
+
+
+def test_batch_graph_falls_back_past_its_capacity(monkeypatch):
+    """A sample whose node list would exceed the planned count's capacity
+    (lowered here to one node past the partitions) stops the graphs there:
+    that sample and the ones after it run through the classic pipeline, in
+    order, with the same results; the small samples before it (no partition
+    large enough to split) ran as graphs."""
+    from kite_amd import GpuContext
+    ws, pins = _samples(0.01)
+    big = sorted((p.array for p in pins), key=lambda a: a.size)[-2:]  # about 60 k reads: partitions split
+    assert big[0].size > 10_000_000
+    # the first 400 records: no partition large enough to split
+    small = [a[:int(np.flatnonzero(a == 10)[4 * 400 - 1]) + 1] for a in (pins[0].array, pins[1].array, big[0])]
+    arrs = small[:2] + [big[0], small[2], big[1]]
+    monkeypatch.setenv("KITE_PLAN_NODES", "257")
+    ctx = GpuContext(0)
+    try:
+        got = ctx.count_batch(arrs, 31, True, 256, mode=BATCH_PIPELINED, encoded=True)
+        monkeypatch.setenv("KITE_BATCH_GRAPH", "0")
+        ref = ctx.count_batch(arrs, 31, True, 256, mode=BATCH_PIPELINED, encoded=True)
+    finally:
+        ctx.close()
+    assert [g[5]["graph"] for g in got] == [1, 1, 0, 0, 0]
+    for i, (g, r) in enumerate(zip(got, ref)):
+        assert g[1] == r[1] and np.array_equal(g[2], r[2]) and np.array_equal(g[3], r[3]), i
+        assert g[5]["unique"] == r[5]["unique"] and g[5]["kmers"] == r[5]["kmers"], i
+    for p in pins:
+        p.free()
