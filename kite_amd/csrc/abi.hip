@@ -1478,12 +1478,20 @@ uint32_t batch_graph(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, 
         g.max_len = grid_len;
         return g;
     };
-    // every graph this batch needs, before any copy is queued
+    // every graph this batch needs, before any copy is queued; with kernel
+    // profiling on, the same launches are made one by one instead, each
+    // between two HIP events (a captured graph holds no timing)
+    const bool eager = c->profile;
     std::vector<std::pair<hipGraphExec_t, hipGraphExec_t>> exec(n);
-    for (uint32_t i = 0; i < n; i++) {
-        const kite_gpu_ctx::BatchGraph &g = graphs_for(smp[i].len, (int)(i % (uint32_t)R));
-        exec[i] = std::make_pair(g.count, g.encode);
+    if (eager) {
+        c->count->begin_kernel_stats();
+    } else {
+        for (uint32_t i = 0; i < n; i++) {
+            const kite_gpu_ctx::BatchGraph &g = graphs_for(smp[i].len, (int)(i % (uint32_t)R));
+            exec[i] = std::make_pair(g.count, g.encode);
+        }
     }
+    std::vector<kite::DevSummary> sums(n);  // each sample's summary (the profiled bytes)
 
     auto h2d = [&](uint32_t j) {
         const int b = (int)(j % (uint32_t)R);
@@ -1530,6 +1538,7 @@ uint32_t batch_graph(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, 
             try {
                 KITE_HIP(hipEventSynchronize(sl.enc_done));
                 const kite::DevSummary sm = *summary(s);
+                sums[i] = sm;
                 bool skip;
                 {
                     std::lock_guard<std::mutex> g(mu);
@@ -1552,7 +1561,7 @@ uint32_t batch_graph(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, 
                 sl.st.split_levels = sm.levels;
                 sl.st.skipped_chunks = (smp[i].len > 0 && smp[i].fastq[0] != '@') ? 1u : 0u;
                 sl.st.words = (uint32_t)W;
-                sl.st.graph = 1;
+                sl.st.graph = eager ? 0u : 1u;
                 sl.part_off.assign(tab, tab + P + 1);
                 uint64_t *boff = static_cast<uint64_t *>(sl.enc_boff.ensure(((size_t)P + 1) * 8));
                 std::copy(tab + P + 1, tab + 2 * ((size_t)P + 1), boff);
@@ -1594,7 +1603,13 @@ uint32_t batch_graph(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, 
             if (i + R - 1 < n) h2d(i + R - 1);
             KITE_HIP(hipStreamWaitEvent(c->stream, sl.in_ready, 0));
             if (prof) KITE_HIP(hipEventRecord(c->tl_events[i][2], c->stream));
-            KITE_HIP(hipGraphLaunch(exec[i].first, c->stream));
+            if (eager) {
+                apply_tuned(c, smp[i].len);
+                c->count->dev_enqueue_count(c->batch_in[s].as<uint8_t>(), dlen + s,
+                                            class_max[size_class(smp[i].len)], (int)c->k, c->canonical, P);
+            } else {
+                KITE_HIP(hipGraphLaunch(exec[i].first, c->stream));
+            }
             KITE_HIP(hipEventRecord(sl.run_done, c->stream));  // input buffer s is free again
             if (i >= (uint32_t)R) {
                 // the slot's payload buffer is read by the D2H of sample i - R:
@@ -1606,7 +1621,8 @@ uint32_t batch_graph(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, 
                 g.unlock();
                 KITE_HIP(hipStreamWaitEvent(c->stream, sl.d2h_done, 0));
             }
-            KITE_HIP(hipGraphLaunch(exec[i].second, c->stream));
+            if (eager) c->count->dev_enqueue_encode(c->graph_enc, c->graph_bytes[s].as<uint8_t>(), bcap, summary(s));
+            else KITE_HIP(hipGraphLaunch(exec[i].second, c->stream));
             if (prof) KITE_HIP(hipEventRecord(c->tl_events[i][3], c->stream));
             KITE_HIP(hipEventRecord(sl.enc_done, c->stream));
             launched_n = i + 1;
@@ -1635,6 +1651,27 @@ uint32_t batch_graph(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, 
     if (!drain_err.empty()) throw kite::Error(drain_err);
     std::string ce = cons.err();
     if (!ce.empty()) throw kite::Error(ce);
+    if (eager) {
+        // per-kernel device time of the planned launches, with each kernel's
+        // algorithmic bytes from the samples' summaries
+        c->count->collect_kernel_stats();
+        const uint64_t kb = 8ull * W;
+        for (kite::KernelStat &ks : c->count->kernel_stats_mut()) {
+            ks.bytes = 0;
+            for (uint32_t i = 0; i < std::min(launched_n, overflow_at); i++) {
+                const kite::DevSummary &m = sums[i];
+                const uint64_t mat = (uint64_t)P * m.nwg;
+                if (ks.name == "k_nl_count") ks.bytes += m.len;
+                else if (ks.name == "k_extract_hist") ks.bytes += m.len + mat * 4;
+                else if (ks.name == "k_extract_scatter") ks.bytes += m.len + mat * 8 + m.n_kmers * kb;
+                else if (ks.name == "k_split_hist") ks.bytes += m.split_keys * kb;
+                else if (ks.name == "k_split_scatter") ks.bytes += m.split_keys * 2 * kb;
+                else if (ks.name == "k_bucket_sort") ks.bytes += m.n_kmers * kb + m.n_unique * (kb + 4);
+                else if (ks.name == "k_compact") ks.bytes += 2 * m.n_unique * (kb + 4);
+            }
+        }
+        c->batch_kstats = c->count->kernel_stats();
+    }
     if (prof) {
         c->tl_ms.assign((size_t)n * 6, 0.0);
         for (uint32_t i = 0; i < std::min(launched_n, overflow_at); i++)

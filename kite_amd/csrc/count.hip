@@ -1472,7 +1472,7 @@ __global__ void k_class_offsets(uint32_t *__restrict__ cls_count, uint32_t *__re
 // counts that input the classic way).  Grids that map tiles to XCDs are
 // multiples of the XCD count, so a grid-stride step keeps a tile's XCD.
 
-__global__ void k_dev_plan_init(const uint64_t *__restrict__ h_len, uint64_t cap_len, uint32_t P,
+__global__ void k_dev_plan_init(const uint64_t *__restrict__ h_len, uint64_t cap_len, uint32_t P, uint32_t subs,
                                 DevPlan *__restrict__ pl) {
     uint32_t *w = reinterpret_cast<uint32_t *>(pl);
     for (uint32_t i = threadIdx.x; i < sizeof(DevPlan) / 4; i += blockDim.x) w[i] = 0;
@@ -1482,6 +1482,7 @@ __global__ void k_dev_plan_init(const uint64_t *__restrict__ h_len, uint64_t cap
         if (len > cap_len) pl->overflow |= 1u;
         pl->len = len > cap_len ? 0 : len;
         pl->n_sub = pl->len == 0 ? 1 : ceil_div(pl->len, SUB);
+        pl->nwg = ceil_div(pl->n_sub, subs);
         pl->lv[0].nn = P;
     }
 }
@@ -1882,6 +1883,8 @@ __global__ void k_dev_summary(const DevPlan *__restrict__ pl, uint32_t P, const 
             for (int l = 0; l < DEV_LEVELS; l++) lv += pl->lv[l].nact ? 1u : 0u;
             sm.levels = lv;
             sm.fallback = (uint64_t)pl->fb_n[0] + pl->fb_n[1] + pl->fb_n[2] + pl->fb_n[3];
+            sm.nwg = pl->nwg;
+            for (int l = 0; l < DEV_LEVELS; l++) sm.split_keys += pl->lv[l].keys;
         }
         *h = sm;
     }
@@ -1949,6 +1952,15 @@ void CountEngine::launch(const char *name, uint64_t bytes, F &&f) {
     KITE_CHECK_LAUNCH();
     KITE_HIP(hipEventRecord(b, stream_));
     pending_.push_back(PendingEv{idx, a, b});
+}
+
+void CountEngine::begin_kernel_stats() {
+    kstats_.clear();
+    for (auto &p : pending_) {
+        ev_pool_.push_back(p.a);
+        ev_pool_.push_back(p.b);
+    }
+    pending_.clear();
 }
 
 void CountEngine::collect_kernel_stats() {
@@ -2510,10 +2522,11 @@ void CountEngine::dev_enqueue_w(const uint8_t *d_fq, const uint64_t *h_len, uint
     auto chk = [] { KITE_CHECK_LAUNCH(); };
 
     // 1. plan, line index over the capacity (zero past the input)
-    k_dev_plan_init<<<1, 256, 0, s>>>(h_len, c.len, P, pl);
+    k_dev_plan_init<<<1, 256, 0, s>>>(h_len, c.len, P, subs, pl);
     chk();
-    k_nl_count_dev<<<capped_grid(c.n_sub, 1, 4096), XT, 0, s>>>(d_fq, pl, c.n_sub, dv_.nl.as<uint32_t>());
-    chk();
+    launch("k_nl_count", 0, [&] {
+        k_nl_count_dev<<<capped_grid(c.n_sub, 1, 4096), XT, 0, s>>>(d_fq, pl, c.n_sub, dv_.nl.as<uint32_t>());
+    });
     uint64_t *line0 = dv_.line0.as<uint64_t>();
     scan_to_u64(dv_.nl.as<uint32_t>(), line0, c.n_sub, stmp, s);
 
@@ -2539,8 +2552,7 @@ void CountEngine::dev_enqueue_w(const uint8_t *d_fq, const uint64_t *h_len, uint
         KITE_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_extract<W, 1>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, max_dyn));
     }
-    k_extract<W, 0><<<dim3((unsigned)nwg), dim3(XT), dyn_hist, s>>>(ea);
-    chk();
+    launch("k_extract_hist", 0, [&] { k_extract<W, 0><<<dim3((unsigned)nwg), dim3(XT), dyn_hist, s>>>(ea); });
     uint64_t *off = dv_.off.as<uint64_t>();
     scan_to_u64(ea.hist, off, mat, stmp, s);
     uint64_t *part_off = dv_.part_off.as<uint64_t>();
@@ -2550,9 +2562,10 @@ void CountEngine::dev_enqueue_w(const uint8_t *d_fq, const uint64_t *h_len, uint
     chk();
     ea.off = off;
     ea.out = A;
-    if (staged) k_extract<W, 2><<<dim3((unsigned)nwg), dim3(XT), dyn_scat, s>>>(ea);
-    else k_extract<W, 1><<<dim3((unsigned)nwg), dim3(XT), dyn_scat, s>>>(ea);
-    chk();
+    launch("k_extract_scatter", 0, [&] {
+        if (staged) k_extract<W, 2><<<dim3((unsigned)nwg), dim3(XT), dyn_scat, s>>>(ea);
+        else k_extract<W, 1><<<dim3((unsigned)nwg), dim3(XT), dyn_scat, s>>>(ea);
+    });
 
     // 4. MSD split levels, every one launched; a level with no node to split
     // copies the node list as it is
@@ -2593,8 +2606,7 @@ void CountEngine::dev_enqueue_w(const uint8_t *d_fq, const uint64_t *h_len, uint
         sa.tile_pre = tpre;
         sa.chunk = 1;
         sa.child_hist = ch;
-        k_split_dev<W, 0><<<dim3((unsigned)g_hist), dim3(ST), 0, s>>>(sa, pl, l);
-        chk();
+        launch("k_split_hist", 0, [&] { k_split_dev<W, 0><<<dim3((unsigned)g_hist), dim3(ST), 0, s>>>(sa, pl, l); });
         k_dev_child_scan<<<dim3((unsigned)g_cscan), 256, 0, s>>>(segs, ch, pl, l, ccur, nzr, nzc);
         chk();
         k_dev_level_counts<<<gnode, 256, 0, s>>>(pl, l, act, act_rank, nzc, ncnt);
@@ -2608,10 +2620,10 @@ void CountEngine::dev_enqueue_w(const uint8_t *d_fq, const uint64_t *h_len, uint
                                                    nodes[nxt], npart[nxt]);
         chk();
         sa.child_cur = ccur;
-        k_split_staged_dev<W><<<dim3((unsigned)g_stag), dim3(SST), 0, s>>>(sa, pl, l);
-        chk();
-        k_split_dev<W, 1><<<dim3((unsigned)g_scat), dim3(ST), 0, s>>>(sa, pl, l);
-        chk();
+        launch("k_split_scatter", 0, [&] {
+            k_split_staged_dev<W><<<dim3((unsigned)g_stag), dim3(SST), 0, s>>>(sa, pl, l);
+            k_split_dev<W, 1><<<dim3((unsigned)g_scat), dim3(ST), 0, s>>>(sa, pl, l);
+        });
     }
     const int fin = DEV_LEVELS & 1;
     Bucket *bkts = nodes[fin];
@@ -2643,24 +2655,29 @@ void CountEngine::dev_enqueue_w(const uint8_t *d_fq, const uint64_t *h_len, uint
             constexpr int IPTH = (C < 3 ? 4 : 8) / W > 0 ? (C < 3 ? 4 : 8) / W : 1;
             constexpr int NTH = SortClass<W, C>::CAP / IPTH;
             const void *f = reinterpret_cast<const void *>(&k_bucket_hash_sort_dev<W, IPTH, NTH, C>);
-            k_bucket_hash_sort_dev<W, IPTH, NTH, C><<<dim3((unsigned)resident_grid(f, NTH, 0)), dim3(NTH), 0, s>>>(
-                A, B, ctmp, bkts, ord, ucount, k, fb, pl);
+            const dim3 g((unsigned)resident_grid(f, NTH, 0));
+            launch("k_bucket_sort", 0, [&] {
+                k_bucket_hash_sort_dev<W, IPTH, NTH, C><<<g, dim3(NTH), 0, s>>>(A, B, ctmp, bkts, ord, ucount, k, fb,
+                                                                                 pl);
+            });
             hashed[C] = true;
         } else {
             const void *f = reinterpret_cast<const void *>(&k_bucket_sort_dev<W, IPT, C, false>);
-            k_bucket_sort_dev<W, IPT, C, false><<<dim3((unsigned)resident_grid(f, BT, 0)), dim3(BT), 0, s>>>(
-                A, B, ctmp, bkts, ord, ucount, k, pl);
+            const dim3 g((unsigned)resident_grid(f, BT, 0));
+            launch("k_bucket_sort", 0, [&] {
+                k_bucket_sort_dev<W, IPT, C, false><<<g, dim3(BT), 0, s>>>(A, B, ctmp, bkts, ord, ucount, k, pl);
+            });
         }
-        chk();
     };
     auto fallback_class = [&](auto cls_c) {
         constexpr int C = decltype(cls_c)::value;
         constexpr int IPT = SortClass<W, C>::IPT;
         if (!hashed[C]) return;
         const void *f = reinterpret_cast<const void *>(&k_bucket_sort_dev<W, IPT, C, true>);
-        k_bucket_sort_dev<W, IPT, C, true><<<dim3((unsigned)resident_grid(f, BT, 0)), dim3(BT), 0, s>>>(
-            A, B, ctmp, bkts, fb, ucount, k, pl);
-        chk();
+        const dim3 g((unsigned)resident_grid(f, BT, 0));
+        launch("k_bucket_sort", 0, [&] {
+            k_bucket_sort_dev<W, IPT, C, true><<<g, dim3(BT), 0, s>>>(A, B, ctmp, bkts, fb, ucount, k, pl);
+        });
     };
     sort_class(std::integral_constant<int, 0>{});
     sort_class(std::integral_constant<int, 1>{});
@@ -2678,8 +2695,9 @@ void CountEngine::dev_enqueue_w(const uint8_t *d_fq, const uint64_t *h_len, uint
     chk();
     uint64_t *okeys = dv_.keys.as<uint64_t>();
     uint32_t *ocounts = dv_.counts.as<uint32_t>();
-    k_compact_dev<W><<<capped_grid(c.nodes, 1, 4096), 256, 0, s>>>(A, B, ctmp, bkts, uoff, pl, okeys, ocounts);
-    chk();
+    launch("k_compact", 0, [&] {
+        k_compact_dev<W><<<capped_grid(c.nodes, 1, 4096), 256, 0, s>>>(A, B, ctmp, bkts, uoff, pl, okeys, ocounts);
+    });
     k_dev_part_rows<<<dim3((P + 1 + 255) / 256), 256, 0, s>>>(uoff, pfirst, P, pl, pu);
     chk();
 }

@@ -51,6 +51,7 @@ struct DevLevel {
 
 struct DevPlan {
     uint64_t len, n_sub, n;       // input bytes, sub-tiles, k-mers
+    uint64_t nwg;                 // extract workgroups with input
     DevLevel lv[DEV_LEVELS + 1];  // lv[DEV_LEVELS].nn: the buckets
     uint32_t cls_n[4], cls_cur[4], cls_off[5], fb_n[4];
     uint32_t sort_cur[12];        // bucket cursors of the sort launches (hash, LSD, LSD fallback per class)
@@ -64,6 +65,7 @@ struct DevPlan {
 // every partition, 4 words each).
 struct DevSummary {
     uint64_t n_kmers, n_unique, bytes, buckets, fallback, len;
+    uint64_t nwg, split_keys;  // extract workgroups with input; keys through the split passes
     uint32_t overflow, levels;
 };
 
@@ -132,6 +134,12 @@ class CountEngine {
     void set_timing(bool on) { timing_ = on; }
     void set_kernel_profile(bool on) { kprof_ = on; }
     const std::vector<KernelStat> &kernel_stats() const { return kstats_; }
+    // Device-planned counts run eagerly with kernel profiling on: clear the
+    // stats before the first, collect them (a sync on the events) after the
+    // last; the caller fills in the algorithmic bytes.
+    void begin_kernel_stats();
+    void collect_kernel_stats();
+    std::vector<KernelStat> &kernel_stats_mut() { return kstats_; }
     CountTunables tun;
 
     // Bytes of device memory the pipeline needs for n k-mers of `words` words
@@ -176,7 +184,6 @@ class CountEngine {
     // records events around one launch when kernel profiling is on
     template <class F>
     void launch(const char *name, uint64_t bytes, F &&f);
-    void collect_kernel_stats();
 
     hipStream_t stream_;
     CountTunables base_;

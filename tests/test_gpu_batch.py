@@ -231,3 +231,29 @@ def test_batch_graph_falls_back_past_its_capacity(monkeypatch):
         assert g[5]["unique"] == r[5]["unique"] and g[5]["kmers"] == r[5]["kmers"], i
     for p in pins:
         p.free()
+
+
+def test_batch_profiled_planned_launches(gpu_ctx):
+    """With kernel profiling on, the batch makes the graphs' launches one by
+    one between HIP events: the same chunks, and per-kernel device time with
+    each kernel's algorithmic bytes."""
+    ws, pins = _samples(0.002)
+    arrs = [p.array for p in pins[:5]]
+    ref = gpu_ctx.count_batch(arrs, 31, True, 256, mode=BATCH_PIPELINED, encoded=True)
+    gpu_ctx.count_profile(True)
+    try:
+        got = gpu_ctx.count_batch(arrs, 31, True, 256, mode=BATCH_PIPELINED, encoded=True)
+        ks = gpu_ctx.count_kernel_stats()
+    finally:
+        gpu_ctx.count_profile(False)
+    for i, (g, r) in enumerate(zip(got, ref)):
+        assert r[5]["graph"] == 1 and g[5]["graph"] == 0, i
+        assert g[1] == r[1] and np.array_equal(g[2], r[2]) and np.array_equal(g[3], r[3]), i
+    kmers = sum(r[5]["kmers"] for r in ref)
+    unique = sum(r[5]["unique"] for r in ref)
+    assert ks["k_bucket_sort"]["bytes"] == kmers * 8 + unique * 12
+    for name in ("k_nl_count", "k_extract_hist", "k_extract_scatter", "k_split_hist", "k_split_scatter",
+                 "k_bucket_sort", "k_compact"):
+        assert ks[name]["ms"] > 0 and ks[name]["launches"] >= len(arrs), name
+    for p in pins:
+        p.free()
