@@ -144,6 +144,30 @@ def pcie_peak(ctx) -> dict:
     return out
 
 
+def settle_link(ctx, limit_s: float = 90.0) -> dict:
+    """Waits, before the timed passes, until device-to-host copies run at
+    nearly the host-to-device rate (256 MiB page-locked copies, checked
+    every half second, at most limit_s).  Right after a process frees many
+    GB of device memory -- this one's plain-mode passes, or an earlier
+    process on the box -- D2H copies run at about half rate for several
+    seconds while H2D copies do not."""
+    t0 = time.perf_counter()
+    checks = []
+    while True:
+        o = ctx.copy_peak(256 << 20)
+        checks.append([round(o["h2d_GBps"], 1), round(o["d2h_GBps"], 1)])
+        ok = o["d2h_GBps"] >= 0.9 * o["h2d_GBps"]
+        if ok or time.perf_counter() - t0 > limit_s:
+            break
+        time.sleep(0.5)
+    waited = time.perf_counter() - t0
+    if len(checks) > 1 or not ok:
+        log(f"host link settled={ok} after {waited:.1f} s ({len(checks)} checks, first H2D/D2H GB/s "
+            f"{checks[0]}, last {checks[-1]})")
+    return {"settled": ok, "waited_s": waited, "checks": len(checks), "first_gbps": checks[0],
+            "last_gbps": checks[-1]}
+
+
 def size_class(nbytes: int) -> int:
     """The engine's input size classes for tuned settings (below 64 MiB,
     512 MiB, 4 GiB, above)."""
@@ -213,8 +237,6 @@ def bench_batch24(ctx, pins, steps: int, warmup: int, plain_reps: int) -> tuple[
         return digest(i, st, keys, counts, off)
 
     d_pipe = ctx.count_batch(arrs, k, canon, P, mode=BATCH_PIPELINED, on_result=keep)
-    d_plain = ctx.count_batch(arrs, k, canon, P, mode=BATCH_PLAIN, on_result=digest)
-    assert d_pipe == d_plain, "plain and pipelined batch tables differ"
     kmers = sum(d[0] for d in d_pipe)
     unique = sum(d[1] for d in d_pipe)
 
@@ -226,8 +248,6 @@ def bench_batch24(ctx, pins, steps: int, warmup: int, plain_reps: int) -> tuple[
                 int(np.bitwise_xor.reduce(mm.ravel())))
 
     e_pipe = ctx.count_batch(arrs, k, canon, P, mode=BATCH_PIPELINED, on_result=cdigest, encoded=True)
-    e_plain = ctx.count_batch(arrs, k, canon, P, mode=BATCH_PLAIN, on_result=cdigest, encoded=True)
-    assert e_pipe == e_plain, "plain and pipelined encoded chunks differ"
     enc_bytes = sum(d[0] for d in e_pipe)
     fastq_bytes = sum(a.size for a in arrs)
 
@@ -240,16 +260,15 @@ def bench_batch24(ctx, pins, steps: int, warmup: int, plain_reps: int) -> tuple[
     def graph_flag(i, st, codec, data, boff, rows, mm):
         return st["graph"]
 
+    link = settle_link(ctx)
     warm = []
     for i in range(warmup):
         t = time.perf_counter()
         ctx.count_batch(arrs, k, canon, P, mode=BATCH_PIPELINED, on_result=nothing_enc, encoded=True)
         warm.append(time.perf_counter() - t)
         log(f"batch24 warmup {i + 1}/{warmup}: {warm[-1] * 1e3:.1f} ms")
-    # the host link runs slower for the first seconds after the cohort's
-    # 19 GB of page-locked FASTQ are written (both copy directions, every
-    # batch mode): more untimed passes until two in a row agree within 3%
-    # of the fastest so far, at most 30
+    # more untimed passes until two in a row agree within 3% of the fastest
+    # so far, at most 30
     extra = 0
     while warmup and extra < 30 and (len(warm) < 2 or max(warm[-2:]) > 1.03 * min(warm)):
         t = time.perf_counter()
@@ -314,6 +333,14 @@ def bench_batch24(ctx, pins, steps: int, warmup: int, plain_reps: int) -> tuple[
     bare_pass = time.perf_counter() - t
     log(f"batch24 untimed passes: with stage events {timeline_pass * 1e3:.1f} ms, no events {bare_pass * 1e3:.1f} ms")
 
+    # the per-sample plain path gives the same tables and chunks (checked
+    # after the timed steps: its per-sample device allocations and frees
+    # slow the host link for seconds after)
+    d_plain = ctx.count_batch(arrs, k, canon, P, mode=BATCH_PLAIN, on_result=digest)
+    assert d_pipe == d_plain, "plain and pipelined batch tables differ"
+    e_plain = ctx.count_batch(arrs, k, canon, P, mode=BATCH_PLAIN, on_result=cdigest, encoded=True)
+    assert e_pipe == e_plain, "plain and pipelined encoded chunks differ"
+
     # the plain per-sample path, median of plain_reps
     plain = []
     for _ in range(plain_reps):
@@ -362,6 +389,7 @@ def bench_batch24(ctx, pins, steps: int, warmup: int, plain_reps: int) -> tuple[
                           "with HIP events; classic: the pipeline with kernels launched after host read-backs "
                           "(KITE_BATCH_GRAPH=0)"},
         "kernel_timing_passes": kern_passes,
+        "link_settle": link,
         "warmup_ms": [x * 1e3 for x in warm],
         "raw_tables": {**raw, "speedup": raw["plain"]["wall_s"] / raw["pipelined"]["wall_s"],
                        "bytes_d2h": int(unique * 12)},

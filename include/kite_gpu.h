@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define KITE_GPU_ABI_VERSION 4
+#define KITE_GPU_ABI_VERSION 5
 
 enum {
     KITE_OK = 0,
@@ -59,7 +59,8 @@ typedef struct kite_count_stats {
     uint32_t skipped_chunks;  /* fed chunks dropped for not starting with '@' */
     uint32_t words;           /* key words (1, 2 or 4) */
     uint32_t graph;           /* batch: 1 when the sample's device work ran as one captured HIP graph launch */
-    uint32_t reserved;
+    uint32_t d2h_engine;      /* batch: the SDMA engine (hsa_amd_sdma_engine_id_t bit) the library chose for the
+                                 sample's payload D2H; 0 when the HIP runtime chose it */
     double device_ms;         /* device time of all batches (events) */
     double extract_ms, split_ms, sort_ms, compact_ms; /* last batch */
 } kite_count_stats;

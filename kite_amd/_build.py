@@ -30,7 +30,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
-GPU_SOURCES = ["scan.hip", "count.hip", "merge.hip", "corr.hip", "encode.hip", "dump.hip", "inflate.hip", "abi.hip"]
+GPU_SOURCES = ["scan.hip", "count.hip", "merge.hip", "corr.hip", "encode.hip", "dump.hip", "inflate.hip", "hostlink.hip",
+               "abi.hip"]
 
 
 def _run(cmd: list[str]) -> None:
@@ -56,7 +57,7 @@ def build_gpu() -> Path:
     out = LIB / "libkite_gpu.so"
     tmp = LIB / "libkite_gpu.so.tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs),
-          "-lpthread"])
+          "-lhsa-runtime64", "-lpthread"])
     os.replace(tmp, out)
     return out
 

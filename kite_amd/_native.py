@@ -44,7 +44,7 @@ class CountStats(C.Structure):
         ("input_bytes", C.c_uint64), ("kmers", C.c_uint64), ("unique", C.c_uint64),
         ("buckets", C.c_uint64), ("batches", C.c_uint32), ("split_levels", C.c_uint32),
         ("skipped_chunks", C.c_uint32), ("words", C.c_uint32), ("graph", C.c_uint32),
-        ("reserved", C.c_uint32), ("device_ms", C.c_double),
+        ("d2h_engine", C.c_uint32), ("device_ms", C.c_double),
         ("extract_ms", C.c_double), ("split_ms", C.c_double), ("sort_ms", C.c_double),
         ("compact_ms", C.c_double),
     ]

@@ -21,6 +21,7 @@
 #include "corr.hpp"
 #include "dump.hpp"
 #include "encode.hpp"
+#include "hostlink.hpp"
 #include "inflate.hpp"
 #include "merge.hpp"
 
@@ -200,7 +201,8 @@ struct kite_gpu_ctx {
     kite::DevBuf graph_bytes[MAX_RING];        // chunk payloads of each ring slot
     kite::HostPinned graph_meta[MAX_RING];     // per slot: a length word (warm-up count), then the summary
     kite::DevBuf graph_dlen;                   // per slot: the input length the count graph reads
-    kite::HostPinned graph_lens;               // the batch's input lengths, copied next to each input
+    std::unique_ptr<kite::HostLink> link;      // D2H of the payloads on SDMA engines picked here
+    bool link_tried = false;
     uint64_t graph_fallbacks = 0;              // samples a planned count handed to the classic path
 
     // merge of stored runs: per-input tables of a partition group, their
@@ -466,6 +468,7 @@ void kite_gpu_close(kite_gpu_ctx *ctx) {
     try {
         bind(ctx);
         (void)hipStreamSynchronize(ctx->stream);
+        ctx->link.reset();
         for (auto &g : ctx->graphs)
             for (hipGraphExec_t e : {g.second.count, g.second.encode})
                 if (e) (void)hipGraphExecDestroy(e);
@@ -1112,6 +1115,9 @@ void batch_plain(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, cons
 struct Consumer {
     kite_gpu_ctx *c;
     BatchSink sink;
+    // set when the slots' D2H copies are queued through the context's
+    // HostLink: their ends are its completion signals, not d2h_done
+    const kite::HostLink *link = nullptr;
     std::mutex mu;
     std::condition_variable cv;
     std::deque<int> queue;
@@ -1145,10 +1151,17 @@ struct Consumer {
             }
             BatchSlot &sl = c->slots[s];
             std::string err;
-            hipError_t e = hipEventSynchronize(sl.d2h_done);
-            if (e != hipSuccess) err = std::string("HIP error: ") + hipGetErrorString(e);
-            else if (sink.call(c, sl) != 0)
-                err = "callback aborted the batch";
+            if (link) {
+                try {
+                    link->wait(s);
+                } catch (const std::exception &x) {
+                    err = x.what();
+                }
+            } else {
+                hipError_t e = hipEventSynchronize(sl.d2h_done);
+                if (e != hipSuccess) err = std::string("HIP error: ") + hipGetErrorString(e);
+            }
+            if (err.empty() && sink.call(c, sl) != 0) err = "callback aborted the batch";
             {
                 std::lock_guard<std::mutex> g(mu);
                 if (!err.empty() && error.empty()) error = err;
@@ -1370,23 +1383,36 @@ uint64_t graph_bytes_cap(uint64_t n, int W) { return n * (W == 1 ? 10 : (W == 2 
 // launches of captured HIP graphs (a pair per input size class and ring
 // slot, captured on first use and kept while the shape, buffers and
 // settings hold):
-//   copy stream h2d:  the H2D of sample j into buffer j % R, after the count
-//                     graph of sample j - R read that buffer;
-//   engine stream:    the count graph of sample i once its H2D ended; then,
-//                     once the D2H of sample i - R has read the slot's
-//                     payload buffer (a stream wait), the encode graph,
-//                     which writes the chunk payload into that buffer and
-//                     the sizes and partition tables into the slot's
+//   uploader thread:  the H2D of sample j into buffer j % R on the copy
+//                     stream h2d, once the count graph of sample j - R
+//                     (which read that buffer) is queued, behind a stream
+//                     wait for its end, and once the H2D of sample j - 1 has
+//                     ended: one input copy in flight at a time;
+//   engine stream:    the input length as a kernel argument into the
+//                     slot's length word, and the count graph of sample i
+//                     once its H2D ended; then, once the D2H of sample
+//                     i - R has read the slot's payload buffer, the encode
+//                     graph, which writes the chunk payload into that buffer
+//                     and the sizes and partition tables into the slot's
 //                     page-locked summary;
 //   drainer thread:   after sample i's encode graph, the D2H of exactly its
-//                     payload bytes on the copy stream d2h;
+//                     payload bytes, on an SDMA engine the context's
+//                     HostLink picks (on the copy stream d2h, engine left to
+//                     the runtime, when stage events are on, when
+//                     KITE_D2H_ENGINES=0 or when no engine qualifies);
 //   consumer thread:  the callback after that D2H, which frees the slot.
-// The engine thread reads nothing back between samples: it only waits, on
-// the host, for the drainer to have queued the D2H whose end the stream
-// wait names.  A sample whose data exceeds a capacity of the planned count
-// (its summary says so) stops the graphs there: the samples from it on run
-// through the classic pipeline.  Returns the first sample not delivered (n
-// when all were).
+// The copy engines: the runtime puts a lone host-to-device copy on its
+// preferred engine, but spreads copies queued together over more engines,
+// some of which carry host traffic at a fraction of the rate or share the
+// engine of the other direction; one H2D in flight, and the D2H on an
+// engine of its own, keep both directions of the link busy at once.
+// The engine thread reads nothing back between samples.  Before the encode
+// graph of sample i it waits, on the host, for the drainer to have queued
+// the D2H of sample i - R, then for that copy's end: on the host for a
+// HostLink copy, with a stream wait on its event otherwise.  A sample whose
+// data exceeds a capacity of the planned count (its summary says so) stops
+// the graphs there: the samples from it on run through the classic
+// pipeline.  Returns the first sample not delivered (n when all were).
 uint32_t batch_graph(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, const BatchSink &sink) {
     const int R = batch_ring();
     if (!c->h2d) KITE_HIP(hipStreamCreateWithFlags(&c->h2d, hipStreamNonBlocking));
@@ -1397,6 +1423,14 @@ uint32_t batch_graph(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, 
             if (!*e) KITE_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
     }
     const bool prof = c->timeline;
+    // payload D2H on engines picked here (stage events need the d2h stream)
+    const char *eng_env = getenv("KITE_D2H_ENGINES");
+    const bool want_link = !prof && !(eng_env && strcmp(eng_env, "0") == 0);
+    if (want_link && !c->link_tried) {
+        c->link_tried = true;
+        c->link = std::make_unique<kite::HostLink>(c->device, MAX_RING);
+    }
+    kite::HostLink *link = want_link && c->link && c->link->usable() ? c->link.get() : nullptr;
     if (prof) {
         if (!c->tl_start) KITE_HIP(hipEventCreate(&c->tl_start));
         while (c->tl_events.size() < n) {
@@ -1428,8 +1462,6 @@ uint32_t batch_graph(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, 
         c->graph_meta[b].ensure(64 + kite::CountEngine::dev_summary_bytes(P));
     }
     uint64_t *dlen = static_cast<uint64_t *>(c->graph_dlen.ensure(MAX_RING * 8));
-    uint64_t *lens = static_cast<uint64_t *>(c->graph_lens.ensure(std::max<size_t>(n, 1) * 8));
-    for (uint32_t i = 0; i < n; i++) lens[i] = smp[i].len;
     auto len_word = [&](int b) { return c->graph_meta[b].as<uint64_t>(); };
     auto summary = [&](int b) { return reinterpret_cast<kite::DevSummary *>(c->graph_meta[b].as<uint8_t>() + 64); };
     if (reshape) {
@@ -1493,28 +1525,19 @@ uint32_t batch_graph(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, 
     }
     std::vector<kite::DevSummary> sums(n);  // each sample's summary (the profiled bytes)
 
-    auto h2d = [&](uint32_t j) {
-        const int b = (int)(j % (uint32_t)R);
-        if (j >= (uint32_t)R) KITE_HIP(hipStreamWaitEvent(c->h2d, c->slots[b].run_done, 0));
-        if (prof) KITE_HIP(hipEventRecord(c->tl_events[j][0], c->h2d));
-        // the length word travels with the input: the count graph of sample
-        // j - R has read both when run_done fires
-        KITE_HIP(hipMemcpyAsync(dlen + b, lens + j, 8, hipMemcpyHostToDevice, c->h2d));
-        if (smp[j].len)
-            KITE_HIP(hipMemcpyAsync(c->batch_in[b].ptr, smp[j].fastq, smp[j].len, hipMemcpyHostToDevice, c->h2d));
-        if (prof) KITE_HIP(hipEventRecord(c->tl_events[j][1], c->h2d));
-        KITE_HIP(hipEventRecord(c->slots[b].in_ready, c->h2d));
-    };
-
     // drainer: after sample i's encode graph, the D2H of its payload
     std::mutex mu;
     std::condition_variable cv;
     std::deque<uint32_t> launched;
     uint32_t drained = 0;  // samples whose D2H is queued (or that were dropped)
+    uint32_t queued = 0;   // samples whose H2D is queued
+    uint32_t ran = 0;      // samples whose count graph (and run_done) is queued
+    std::string upload_err;
     bool stop = false;
     uint32_t overflow_at = n;  // the first sample whose planned count overflowed
     std::string drain_err;
     Consumer cons(c, sink);
+    cons.link = link;
     auto mark_drained = [&](uint32_t i) {
         {
             std::lock_guard<std::mutex> g(mu);
@@ -1569,11 +1592,17 @@ uint32_t batch_graph(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, 
                 std::copy(tab + 2 * ((size_t)P + 1), tab + 2 * ((size_t)P + 1) + 8 * (size_t)P, mm);
                 sl.enc_total = sm.bytes;
                 uint8_t *hb = static_cast<uint8_t *>(sl.enc_bytes.ensure(std::max<uint64_t>(sm.bytes, 1)));
-                if (prof) KITE_HIP(hipEventRecord(c->tl_events[i][4], c->d2h));
-                if (sm.bytes)
-                    KITE_HIP(hipMemcpyAsync(hb, c->graph_bytes[s].ptr, sm.bytes, hipMemcpyDeviceToHost, c->d2h));
-                if (prof) KITE_HIP(hipEventRecord(c->tl_events[i][5], c->d2h));
-                KITE_HIP(hipEventRecord(sl.d2h_done, c->d2h));
+                if (link) {
+                    // the encode graph's writes are visible: enc_done was
+                    // waited for above
+                    if (sm.bytes) sl.st.d2h_engine = link->d2h(s, hb, c->graph_bytes[s].ptr, sm.bytes);
+                } else {
+                    if (prof) KITE_HIP(hipEventRecord(c->tl_events[i][4], c->d2h));
+                    if (sm.bytes)
+                        KITE_HIP(hipMemcpyAsync(hb, c->graph_bytes[s].ptr, sm.bytes, hipMemcpyDeviceToHost, c->d2h));
+                    if (prof) KITE_HIP(hipEventRecord(c->tl_events[i][5], c->d2h));
+                    KITE_HIP(hipEventRecord(sl.d2h_done, c->d2h));
+                }
                 mark_drained(i);
                 taken = false;
                 cons.push(s);
@@ -1588,20 +1617,60 @@ uint32_t batch_graph(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, 
             }
         }
     });
+    // uploader: the H2D of sample j into buffer j % R once the count graph
+    // of sample j - R (which read that buffer) is queued, with a stream wait
+    // for its end.  One input copy is in flight at a time: with a single
+    // copy queued, the runtime keeps it on its preferred host-to-device
+    // engine instead of spreading copies over engines that also carry the
+    // D2H or run host traffic slower.
+    if (prof) KITE_HIP(hipEventRecord(c->tl_start, c->h2d));
+    std::thread uploader([&] {
+        try {
+            for (uint32_t j = 0; j < n; j++) {
+                const int b = (int)(j % (uint32_t)R);
+                if (j >= (uint32_t)R) {
+                    std::unique_lock<std::mutex> g(mu);
+                    cv.wait(g, [&] { return stop || ran > j - R; });
+                    if (stop) return;
+                }
+                if (j > 0) KITE_HIP(hipEventSynchronize(c->slots[(j - 1) % (uint32_t)R].in_ready));
+                if (j >= (uint32_t)R) KITE_HIP(hipStreamWaitEvent(c->h2d, c->slots[b].run_done, 0));
+                if (prof) KITE_HIP(hipEventRecord(c->tl_events[j][0], c->h2d));
+                if (smp[j].len)
+                    KITE_HIP(
+                        hipMemcpyAsync(c->batch_in[b].ptr, smp[j].fastq, smp[j].len, hipMemcpyHostToDevice, c->h2d));
+                if (prof) KITE_HIP(hipEventRecord(c->tl_events[j][1], c->h2d));
+                KITE_HIP(hipEventRecord(c->slots[b].in_ready, c->h2d));
+                {
+                    std::lock_guard<std::mutex> g(mu);
+                    queued = j + 1;
+                }
+                cv.notify_all();
+            }
+        } catch (const std::exception &e) {
+            {
+                std::lock_guard<std::mutex> g(mu);
+                if (upload_err.empty()) upload_err = e.what();
+            }
+            cv.notify_all();
+        }
+    });
     std::string failure;
     uint32_t launched_n = 0;
     try {
-        if (prof) KITE_HIP(hipEventRecord(c->tl_start, c->h2d));
-        for (uint32_t j = 0; j + 1 < (uint32_t)R && j < n; j++) h2d(j);
         for (uint32_t i = 0; i < n; i++) {
             {
-                std::lock_guard<std::mutex> g(mu);
+                std::unique_lock<std::mutex> g(mu);
                 if (i >= overflow_at || !cons.err().empty()) break;
+                cv.wait(g, [&] { return queued > i || !upload_err.empty(); });
+                if (!upload_err.empty()) throw kite::Error(upload_err);
             }
             const int s = (int)(i % (uint32_t)R);
             BatchSlot &sl = c->slots[s];
-            if (i + R - 1 < n) h2d(i + R - 1);
             KITE_HIP(hipStreamWaitEvent(c->stream, sl.in_ready, 0));
+            // the length word of buffer s: the count graph of sample i - R,
+            // earlier on this stream, has read it
+            c->count->dev_set_len(dlen + s, smp[i].len);
             if (prof) KITE_HIP(hipEventRecord(c->tl_events[i][2], c->stream));
             if (eager) {
                 apply_tuned(c, smp[i].len);
@@ -1611,15 +1680,20 @@ uint32_t batch_graph(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, 
                 KITE_HIP(hipGraphLaunch(exec[i].first, c->stream));
             }
             KITE_HIP(hipEventRecord(sl.run_done, c->stream));  // input buffer s is free again
+            {
+                std::lock_guard<std::mutex> g(mu);
+                ran = i + 1;
+            }
+            cv.notify_all();
             if (i >= (uint32_t)R) {
                 // the slot's payload buffer is read by the D2H of sample i - R:
-                // once the drainer has queued that copy, the stream waits for
-                // its end
+                // once the drainer has queued that copy, wait for its end
                 std::unique_lock<std::mutex> g(mu);
                 cv.wait(g, [&] { return drained > i - R; });
                 if (i >= overflow_at) break;
                 g.unlock();
-                KITE_HIP(hipStreamWaitEvent(c->stream, sl.d2h_done, 0));
+                if (link) link->wait(s);
+                else KITE_HIP(hipStreamWaitEvent(c->stream, sl.d2h_done, 0));
             }
             if (eager) c->count->dev_enqueue_encode(c->graph_enc, c->graph_bytes[s].as<uint8_t>(), bcap, summary(s));
             else KITE_HIP(hipGraphLaunch(exec[i].second, c->stream));
@@ -1641,9 +1715,11 @@ uint32_t batch_graph(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, 
     }
     cv.notify_all();
     drainer.join();
+    uploader.join();
     // (after a failure a slot may stay taken: nothing more is drained)
     if (failure.empty()) cons.drain();
     cons.stop();
+    if (link) link->quiesce();
     (void)hipStreamSynchronize(c->h2d);
     (void)hipStreamSynchronize(c->d2h);
     (void)hipStreamSynchronize(c->stream);

@@ -2486,6 +2486,13 @@ void CountEngine::dev_prepare(uint64_t cap_len, int k, uint32_t partitions, Enco
     encode_prepare_dev(c.n, P, enc);
 }
 
+__global__ void k_store_u64(uint64_t *p, uint64_t v) { *p = v; }
+
+void CountEngine::dev_set_len(uint64_t *d_len, uint64_t len) {
+    k_store_u64<<<1, 1, 0, stream_>>>(d_len, len);
+    KITE_CHECK_LAUNCH();
+}
+
 void CountEngine::dev_enqueue_count(const uint8_t *d_fq, const uint64_t *len, uint64_t max_len, int k, int canonical,
                                     uint32_t partitions) {
     const int W = k <= 31 ? 1 : (k <= 64 ? 2 : 4);

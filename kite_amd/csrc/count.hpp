@@ -166,6 +166,9 @@ class CountEngine {
     void dev_enqueue_count(const uint8_t *d_fq, const uint64_t *len, uint64_t max_len, int k, int canonical,
                            uint32_t partitions);
     void dev_enqueue_encode(EncodeBufs &enc, uint8_t *bytes, uint64_t bytes_cap, void *h_summary);
+    // Stores `len` into the device word `d_len` on the engine stream (one
+    // thread; no copy engine involved): the length a later count reads.
+    void dev_set_len(uint64_t *d_len, uint64_t len);
     const DevCaps &dev_caps() const { return dcap_; }
     // The settings the next count uses: the base settings with the KITE_*
     // environment overrides applied.

@@ -27,7 +27,7 @@ def test_library_exports_every_declared_symbol():
 def test_abi_version_and_error_string():
     from kite_amd import load_gpu
     lib = load_gpu()
-    assert lib.kite_gpu_abi_version() == 4
+    assert lib.kite_gpu_abi_version() == 5
     assert isinstance(lib.kite_gpu_last_error(), bytes)
 
 

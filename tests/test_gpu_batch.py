@@ -257,3 +257,31 @@ def test_batch_profiled_planned_launches(gpu_ctx):
         assert ks[name]["ms"] > 0 and ks[name]["launches"] >= len(arrs), name
     for p in pins:
         p.free()
+
+
+@pytest.mark.parametrize("ring", [2, 3])
+def test_batch_graph_d2h_on_chosen_engines(gpu_ctx, monkeypatch, ring):
+    """The graph batch queues each sample's payload D2H on an SDMA engine it
+    picks (one engine bit per non-empty sample in the stats); the chunks
+    equal those of the same batch with the engine left to the HIP runtime
+    (KITE_D2H_ENGINES=0), over repeated batches and ring depths 2 and 3
+    (the engine thread waits for the copy of sample i - R on the host)."""
+    ws, pins = _samples(0.003)
+    arrs = [p.array for p in pins] + [np.zeros(0, dtype=np.uint8)] + [p.array for p in pins[:3]]
+    monkeypatch.setenv("KITE_BATCH_RING", str(ring))
+    monkeypatch.setenv("KITE_D2H_ENGINES", "0")
+    ref = gpu_ctx.count_batch(arrs, 31, True, 256, mode=BATCH_PIPELINED, encoded=True)
+    monkeypatch.delenv("KITE_D2H_ENGINES")
+    for rep in range(2):
+        got = gpu_ctx.count_batch(arrs, 31, True, 256, mode=BATCH_PIPELINED, encoded=True)
+        for i, (g, r) in enumerate(zip(got, ref)):
+            assert g[5]["graph"] == 1 and r[5]["graph"] == 1, (rep, i)
+            assert r[5]["d2h_engine"] == 0, (rep, i)
+            e = g[5]["d2h_engine"]
+            if g[1]:
+                assert e != 0 and e & (e - 1) == 0, (rep, i, e)
+            assert g[0] == r[0] and g[1] == r[1], (rep, i)
+            for q in (2, 3, 4):
+                assert np.array_equal(g[q], r[q]), (rep, i, q)
+    for p in pins:
+        p.free()
