@@ -214,6 +214,31 @@ def autotune_cohort(ctx, arrs, names, k: int, canon: bool, P: int) -> dict:
     return out
 
 
+def kernel_passes(ctx, arrs, k: int, canon: bool, P: int, passes: int) -> dict:
+    """Kernel timing passes over the cohort (untimed): the launches the
+    batch's graphs hold, made one by one on the engine stream with HIP
+    events around each (a captured graph holds no timing) -- the same
+    kernels, grids and device-side sizes.  Per-kernel time, launches and
+    algorithmic bytes summed over the passes."""
+    from kite_amd import BATCH_PIPELINED
+
+    def nothing_enc(i, st, codec, data, boff, rows, mm):
+        return 0
+
+    kern: dict[str, dict] = {}
+    ctx.count_profile(True)
+    try:
+        for _ in range(passes):
+            ctx.count_batch(arrs, k, canon, P, mode=BATCH_PIPELINED, on_result=nothing_enc, encoded=True)
+            for name, v in ctx.count_kernel_stats().items():
+                a = kern.setdefault(name, {"ms": 0.0, "launches": 0, "bytes": 0})
+                for key in a:
+                    a[key] += v[key]
+    finally:
+        ctx.count_profile(False)
+    return kern
+
+
 def bench_batch24(ctx, pins, steps: int, warmup: int, plain_reps: int) -> tuple[dict, list, dict, int]:
     """The headline: the 24-sample cohort from pinned host memory through
     the pipelined batch (H2D of the FASTQ, count, DVLE chunk encode on the
@@ -292,19 +317,8 @@ def bench_batch24(ctx, pins, steps: int, warmup: int, plain_reps: int) -> tuple[
     log(f"batch24 timed {steps} steps: {wall / steps * 1e3:.1f} ms/step, {kmers * steps / wall / 1e9:.2f} G k-mers/s "
         f"(steps {min(walls) * 1e3:.1f}..{max(walls) * 1e3:.1f} ms; {graph_samples} of {len(arrs)} samples "
         f"as one graph launch each)")
-    # kernel timing passes (untimed): the launches the graphs hold, made one
-    # by one on the engine stream with HIP events around each (a captured
-    # graph holds no timing): the same kernels, grids and device-side sizes
-    kern: dict[str, dict] = {}
     kern_passes = 3
-    ctx.count_profile(True)
-    for i in range(kern_passes):
-        ctx.count_batch(arrs, k, canon, P, mode=BATCH_PIPELINED, on_result=nothing_enc, encoded=True)
-        for name, v in ctx.count_kernel_stats().items():
-            a = kern.setdefault(name, {"ms": 0.0, "launches": 0, "bytes": 0})
-            for key in a:
-                a[key] += v[key]
-    ctx.count_profile(False)
+    kern = kernel_passes(ctx, arrs, k, canon, P, kern_passes)
     # the classic pipeline for comparison: kernels launched one by one after
     # the host read-backs of each sample (KITE_BATCH_GRAPH=0)
     os.environ["KITE_BATCH_GRAPH"] = "0"
@@ -869,6 +883,10 @@ def main() -> int:
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one 1 GiB device-to-device copy (a known-size kernel that scales "
                          "FETCH_SIZE / WRITE_SIZE counter readings)")
+    ap.add_argument("--kernel-passes-only", action="store_true",
+                    help="for profiler runs: only the cohort's kernel timing passes (the graphs' launches made "
+                         "one by one between HIP events; no graph launch, no timed steps); prints their "
+                         "per-kernel table and roofline instead of the bench line")
     ap.add_argument("--no-analyze", action="store_true")
     ap.add_argument("--no-single", action="store_true", help="skip the HBM-resident single-sample leg")
     ap.add_argument("--no-dump", action="store_true", help="skip the kite dump leg")
@@ -914,11 +932,24 @@ def main() -> int:
         tune = apply_tuned(ctx, args.tuned, 31, 256)
     elif not args.no_autotune:
         tune = autotune_cohort(ctx, [p.array for p in pins], [w_.name for w_ in ws], 31, True, 256)
+    workload = "batch24-k31-canon-20mbp-snv2"
+    if args.kernel_passes_only:
+        # one untimed pass of the same launches first (buffers sized, first use)
+        kernel_passes(ctx, [p.array for p in pins], 31, True, 256, 1)
+        kern = kernel_passes(ctx, [p.array for p in pins], 31, True, 256, 3)
+        roof = roofline_of(kern, workload)
+        log(f"kernel passes only: {roof['kernel']} {roof['avg_launch_ms']:.4f} ms per launch over "
+            f"{roof['launches']} launches, {roof['achieved']:.0f} GB/s")
+        print(json.dumps({"kernel_passes_only": {"workload": workload, "passes": 3, "roofline": roof,
+                                                 "kernels": kernel_table(kern, 3)}}), flush=True)
+        for p in pins:
+            p.free()
+        ctx.close()
+        return 0
     batch, tables, kern, kmers = bench_batch24(ctx, pins, args.steps, args.warmup, args.plain_reps)
     wall = batch["wall_s"]
     value = kmers * args.steps / wall
     ms_per_step = wall / args.steps * 1e3
-    workload = "batch24-k31-canon-20mbp-snv2"
     roof = roofline_of(kern, workload)
     kernels = kernel_table(kern, batch["kernel_timing_passes"])
     log("batch24 kernels: " + json.dumps({n: round(v["ms_per_step"], 3) for n, v in kernels.items()}))

@@ -58,7 +58,8 @@ typedef struct kite_count_stats {
     uint32_t split_levels;    /* MSD split levels of the last batch */
     uint32_t skipped_chunks;  /* fed chunks dropped for not starting with '@' */
     uint32_t words;           /* key words (1, 2 or 4) */
-    uint32_t graph;           /* batch: 1 when the sample's device work ran as one captured HIP graph launch */
+    uint32_t graph;           /* batch: 1 when the sample's count and chunk encode ran as captured HIP graphs
+                                 (one launch each) */
     uint32_t d2h_engine;      /* batch: the SDMA engine (hsa_amd_sdma_engine_id_t bit) the library chose for the
                                  sample's payload D2H; 0 when the HIP runtime chose it */
     double device_ms;         /* device time of all batches (events) */
