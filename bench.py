@@ -144,7 +144,7 @@ def pcie_peak(ctx) -> dict:
     return out
 
 
-def settle_link(ctx, limit_s: float = 90.0) -> dict:
+def settle_link(ctx, limit_s: float = 60.0) -> dict:
     """Waits, before the timed passes, until device-to-host copies run at
     nearly the host-to-device rate (256 MiB page-locked copies, checked
     every half second, at most limit_s).  Right after a process frees many
