@@ -1545,6 +1545,7 @@ uint32_t batch_graph(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, 
         }
         cv.notify_all();
     };
+    if (prof) KITE_HIP(hipEventRecord(c->tl_start, c->h2d));
     std::thread drainer([&] {
         for (;;) {
             uint32_t i;
@@ -1623,8 +1624,7 @@ uint32_t batch_graph(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, 
     // copy queued, the runtime keeps it on its preferred host-to-device
     // engine instead of spreading copies over engines that also carry the
     // D2H or run host traffic slower.
-    if (prof) KITE_HIP(hipEventRecord(c->tl_start, c->h2d));
-    std::thread uploader([&] {
+    auto upload = [&] {
         try {
             for (uint32_t j = 0; j < n; j++) {
                 const int b = (int)(j % (uint32_t)R);
@@ -1654,7 +1654,19 @@ uint32_t batch_graph(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, 
             }
             cv.notify_all();
         }
-    });
+    };
+    std::thread uploader;
+    try {
+        uploader = std::thread(upload);
+    } catch (...) {
+        {
+            std::lock_guard<std::mutex> g(mu);
+            stop = true;
+        }
+        cv.notify_all();
+        drainer.join();
+        throw;
+    }
     std::string failure;
     uint32_t launched_n = 0;
     try {
