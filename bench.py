@@ -504,7 +504,7 @@ def merge_inputs(dirpath: Path, reads: int) -> list[Path]:
     return out
 
 
-def bench_merge(reads: int = 1_000_000) -> dict:
+def bench_merge(reads: int = 1_000_000, settle=None) -> dict:
     """`kite merge` of two k=63 samples (merge_inputs): the device engine
     (kite_gpu_merge_runs: the stored payloads to the device, run decode,
     header filter, merge-path union with u64 sums, u32 check and DVLE
@@ -513,7 +513,9 @@ def bench_merge(reads: int = 1_000_000) -> dict:
     host's worker threads), byte-identical outputs, best of two runs each.
     `merge_s`: the device engine's kite_gpu_merge_runs call (everything
     between reading the inputs' indexes and finalizing the output); the
-    host engine's unions alone."""
+    host engine's unions alone.  `settle`, when given, runs before every
+    timed device-engine run (the host link's recovery after the previous
+    process freed its device memory; see settle_link)."""
     import re
     import subprocess
     import tempfile
@@ -529,6 +531,8 @@ def bench_merge(reads: int = 1_000_000) -> dict:
             best = None
             for _ in range(2):
                 outp = tmp / f"m_{engine}.kite"
+                if settle and engine == "gpu":
+                    settle()
                 t = time.perf_counter()
                 r = subprocess.run([str(kite_bin), "merge", "--engine", engine, "-i", *map(str, ins), "-o", str(outp)],
                                    capture_output=True, text=True, timeout=600)
@@ -564,11 +568,14 @@ def bench_merge(reads: int = 1_000_000) -> dict:
         shutil.rmtree(tmp, ignore_errors=True)
 
 
-def bench_dump(w, fq) -> dict:
+def bench_dump(w, fq, settle=None) -> dict:
     """`kite dump` of the workload's .kite (written by `kite count` on the
     GPU): the device engine against the host engine (partition workers on
     this host's cores), wall time to /dev/null, and a hash of each engine's
-    full text (bit-exact check at full size)."""
+    full text (bit-exact check at full size).  `settle`, when given, runs
+    before every timed run of the CLI on the device: each CLI process frees
+    its device memory at exit, and device-to-host copies run at about half
+    rate for seconds after such a free (see settle_link)."""
     import re
     import subprocess
     import tempfile
@@ -585,6 +592,8 @@ def bench_dump(w, fq) -> dict:
         fq.tofile(fq_path)
         kp = tmp / "c.kite"
         # the drop-in CLI end to end: FASTQ file (page cache) -> .kite file
+        if settle:
+            settle()
         t0 = time.perf_counter()
         r = subprocess.run([str(kite_bin), "count", "-i", str(fq_path), "-o", str(kp), "-k", str(w.k),
                             "--canonical", "true" if w.canonical else "false", "--partitions", str(w.partitions)],
@@ -598,6 +607,8 @@ def bench_dump(w, fq) -> dict:
         # --mode compact: the same count, chunks zstd-compressed on the host
         # where the reference's policy keeps them
         kz = tmp / "z.kite"
+        if settle:
+            settle()
         t0 = time.perf_counter()
         r = subprocess.run([str(kite_bin), "count", "-i", str(fq_path), "-o", str(kz), "-k", str(w.k),
                             "--canonical", "true" if w.canonical else "false", "--partitions", str(w.partitions),
@@ -622,6 +633,8 @@ def bench_dump(w, fq) -> dict:
             cmd = [str(kite_bin), "dump", "-i", str(kp), "-t", str(threads)]
             best = None
             for _ in range(2):
+                if settle and engine == "gpu":
+                    settle()
                 t0 = time.perf_counter()
                 r = subprocess.run(cmd + ["-o", "/dev/null"], capture_output=True, text=True, env=env, timeout=600)
                 wall = time.perf_counter() - t0
@@ -660,7 +673,7 @@ def bench_dump(w, fq) -> dict:
                              if rows else None}
         out["speedup_wall"] = hst["wall_s"] / g["wall_s"]
         out["includes"] = ("wall: process start, .kite read, decode, merge, format, text to /dev/null; "
-                           "device: decode + merge + format kernels")
+                           "device: decode + merge + format kernels; each device run after the host link settled")
         return out
     finally:
         import shutil
@@ -989,10 +1002,10 @@ def main() -> int:
     if not args.no_dump:
         fq = w.fastq() if fq is None else fq
         single = single or {}
-        single["dump"] = bench_dump(w, fq)
+        single["dump"] = bench_dump(w, fq, settle=lambda: settle_link(ctx))
     merge = None
     if not args.no_merge:
-        merge = bench_merge(args.merge_reads)
+        merge = bench_merge(args.merge_reads, settle=lambda: settle_link(ctx))
     if not args.no_bgzf:
         fq = w.fastq() if fq is None else fq
         bgz = bench_bgzf(ctx, w, fq, threads)

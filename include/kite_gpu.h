@@ -371,7 +371,7 @@ typedef struct kite_dump_run {
 typedef struct kite_dump_stats {
     uint64_t runs, rows_in, rows, lines, bytes, groups;
     double decode_ms, merge_ms, format_ms;  /* device time per stage */
-    double copy_ms;  /* text phase: D2H pieces overlapped with the writer callback */
+    double copy_ms;  /* text phase, wall time: D2H pieces overlapped with the writer callback */
 } kite_dump_stats;
 
 /* Receives the text in order, in pieces; non-zero aborts the dump. */

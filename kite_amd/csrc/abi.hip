@@ -225,6 +225,19 @@ namespace {
 
 void bind(kite_gpu_ctx *c) { KITE_HIP(hipSetDevice(c->device)); }
 
+// The context's device-to-host copy link (SDMA engines picked by the
+// library), opened on first use; nullptr when KITE_D2H_ENGINES=0 or when no
+// engine qualifies (copies then take hipMemcpyAsync).
+kite::HostLink *host_link(kite_gpu_ctx *c) {
+    const char *e = getenv("KITE_D2H_ENGINES");
+    if (e && strcmp(e, "0") == 0) return nullptr;
+    if (!c->link_tried) {
+        c->link_tried = true;
+        c->link = std::make_unique<kite::HostLink>(c->device, MAX_RING);
+    }
+    return c->link && c->link->usable() ? c->link.get() : nullptr;
+}
+
 // Input size classes of the tuned settings: below 64 MiB, 512 MiB, 4 GiB, above.
 int size_class(uint64_t len) {
     return len < (64ull << 20) ? 0 : (len < (512ull << 20) ? 1 : (len < (4ull << 30) ? 2 : 3));
@@ -1424,13 +1437,7 @@ uint32_t batch_graph(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, 
     }
     const bool prof = c->timeline;
     // payload D2H on engines picked here (stage events need the d2h stream)
-    const char *eng_env = getenv("KITE_D2H_ENGINES");
-    const bool want_link = !prof && !(eng_env && strcmp(eng_env, "0") == 0);
-    if (want_link && !c->link_tried) {
-        c->link_tried = true;
-        c->link = std::make_unique<kite::HostLink>(c->device, MAX_RING);
-    }
-    kite::HostLink *link = want_link && c->link && c->link->usable() ? c->link.get() : nullptr;
+    kite::HostLink *link = prof ? nullptr : host_link(c);
     if (prof) {
         if (!c->tl_start) KITE_HIP(hipEventCreate(&c->tl_start));
         while (c->tl_events.size() < n) {
@@ -2158,41 +2165,50 @@ int kite_gpu_dump(kite_gpu_ctx *ctx, uint32_t k, uint32_t key_width, uint32_t pa
             st.decode_ms += t.decode_ms;
             st.merge_ms += t.merge_ms;
             st.format_ms += t.format_ms;
-            // text to the caller in pieces, the next piece copied while the
-            // caller takes the current one
+            // text to the caller in pieces through two page-locked buffers:
+            // pieces q and q + 1 are in flight while the caller takes piece
+            // q - 1, and piece q + 2 is queued once the caller returns piece
+            // q (its buffer), so the copy engine does not wait on the caller
             char *pin = static_cast<char *>(ctx->dump_text.ensure(2 * PIECE));
-            hipEvent_t done[2], t0, t1;
-            for (auto *e : {&done[0], &done[1], &t0, &t1}) KITE_HIP(hipEventCreate(e));
-            struct Ev {
-                hipEvent_t *e[4];
-                ~Ev() {
-                    for (auto *x : e) (void)hipEventDestroy(*x);
-                }
-            } evs{{&done[0], &done[1], &t0, &t1}};
             const char *text = ctx->dump->text();
             const uint64_t pieces = kite::ceil_div(bytes, PIECE);
-            KITE_HIP(hipEventRecord(t0, s));
+            kite::HostLink *link = host_link(ctx);
+            hipEvent_t done[2];
+            for (auto *e : {&done[0], &done[1]}) KITE_HIP(hipEventCreate(e));
+            struct Ev {
+                hipEvent_t *e[2];
+                kite::HostLink *link;
+                ~Ev() {
+                    if (link) link->quiesce();
+                    for (auto *x : e) (void)hipEventDestroy(*x);
+                }
+            } evs{{&done[0], &done[1]}, link};
+            // the text is complete before an SDMA copy reads it
+            if (link) KITE_HIP(hipStreamSynchronize(s));
+            const auto c0 = std::chrono::steady_clock::now();
             auto issue = [&](uint64_t q) {
                 const uint64_t a = q * PIECE, n = std::min<uint64_t>(PIECE, bytes - a);
-                KITE_HIP(hipMemcpyAsync(pin + (q & 1) * PIECE, text + a, n, hipMemcpyDeviceToHost, s));
-                KITE_HIP(hipEventRecord(done[q & 1], s));
+                if (link) {
+                    link->d2h((int)(q & 1), pin + (q & 1) * PIECE, text + a, n);
+                } else {
+                    KITE_HIP(hipMemcpyAsync(pin + (q & 1) * PIECE, text + a, n, hipMemcpyDeviceToHost, s));
+                    KITE_HIP(hipEventRecord(done[q & 1], s));
+                }
             };
-            if (pieces) issue(0);
+            for (uint64_t q = 0; q < std::min<uint64_t>(2, pieces); q++) issue(q);
             for (uint64_t q = 0; q < pieces; q++) {
-                KITE_HIP(hipEventSynchronize(done[q & 1]));
-                if (q + 1 < pieces) issue(q + 1);
+                if (link) link->wait((int)(q & 1));
+                else KITE_HIP(hipEventSynchronize(done[q & 1]));
                 const uint64_t a = q * PIECE, n = std::min<uint64_t>(PIECE, bytes - a);
                 if (fn(user, pin + (q & 1) * PIECE, n) != 0) {
                     (void)hipStreamSynchronize(s);
                     aborted = true;
                     return;
                 }
+                if (q + 2 < pieces) issue(q + 2);
             }
-            KITE_HIP(hipEventRecord(t1, s));
             KITE_HIP(hipStreamSynchronize(s));
-            float ms = 0;
-            KITE_HIP(hipEventElapsedTime(&ms, t0, t1));
-            st.copy_ms += ms;
+            st.copy_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c0).count();
             st.runs += nruns;
             st.rows_in += rows_in;
             st.rows += out_rows;

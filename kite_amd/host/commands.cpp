@@ -128,9 +128,22 @@ void dump_on_gpu(kite_gpu_ctx *g, const Reader &r, FILE *out) {
     std::vector<kite_dump_run> runs;
     std::vector<std::vector<uint8_t>> decoded;
     collect_runs(r, runs, decoded);
+    // pieces of tens of MB go straight to the descriptor: through stdio a
+    // part of each would first be copied into the stream's buffer
     auto write = [](void *user, const char *text, uint64_t len) -> int {
-        return fwrite(text, 1, len, static_cast<FILE *>(user)) == len ? 0 : 1;
+        const int fd = fileno(static_cast<FILE *>(user));
+        while (len) {
+            const ssize_t w = ::write(fd, text, len);
+            if (w < 0) {
+                if (errno == EINTR) continue;
+                return 1;
+            }
+            text += w;
+            len -= (uint64_t)w;
+        }
+        return 0;
     };
+    if (fflush(out) != 0) throw Error(std::string("write failed: ") + strerror(errno));
     kite_dump_stats st{};
     if (kite_gpu_dump(g, h.k, h.key_width, h.partitions, runs.data(), runs.size(), h.min_count.has_value(),
                       h.min_count.value_or(0), h.max_count.has_value(), h.max_count.value_or(0), write, out,
