@@ -31,7 +31,7 @@ HIP_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 GPU_SOURCES = ["scan.hip", "count.hip", "merge.hip", "corr.hip", "encode.hip", "dump.hip", "inflate.hip", "hostlink.hip",
-               "abi.hip"]
+               "devmem.hip", "abi.hip"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -520,6 +520,7 @@ void kite_gpu_close(kite_gpu_ctx *ctx) {
         if (ctx->h2d) (void)hipStreamDestroy(ctx->h2d);
         if (ctx->d2h) (void)hipStreamDestroy(ctx->d2h);
         (void)hipStreamDestroy(ctx->stream);
+        kite::free_retired();
     } catch (...) {
     }
     delete ctx;
@@ -2471,25 +2472,36 @@ int kite_gpu_merge_runs(kite_gpu_ctx *ctx, uint32_t key_width, uint32_t partitio
                 pieces.emplace_back(a, b);
                 a = b;
             }
+            // two pieces in flight on the library's D2H engine (the stream
+            // was synchronized above, so the chunks are written): piece
+            // q + 2 is queued once the caller has taken piece q
+            kite::HostLink *link = host_link(ctx);
             hipEvent_t done[2];
             for (auto &x : done) KITE_HIP(hipEventCreateWithFlags(&x, hipEventDisableTiming));
             struct Ev2 {
                 hipEvent_t *e;
+                kite::HostLink *link;
                 ~Ev2() {
+                    if (link) link->quiesce();
                     (void)hipEventDestroy(e[0]);
                     (void)hipEventDestroy(e[1]);
                 }
-            } ev2{done};
+            } ev2{done, link};
+            for (int b = 0; b < 2; b++) ctx->merge_out[b].ensure(PIECE);
             auto issue = [&](size_t q) {
                 const uint64_t a = boff[pieces[q].first], n = boff[pieces[q].second] - a;
                 uint8_t *pin = static_cast<uint8_t *>(ctx->merge_out[q & 1].ensure(std::max<uint64_t>(n, PIECE)));
-                if (n) KITE_HIP(hipMemcpyAsync(pin, ed.bytes + a, n, hipMemcpyDeviceToHost, s));
-                KITE_HIP(hipEventRecord(done[q & 1], s));
+                if (link) {
+                    if (n) link->d2h((int)(q & 1), pin, ed.bytes + a, n);
+                } else {
+                    if (n) KITE_HIP(hipMemcpyAsync(pin, ed.bytes + a, n, hipMemcpyDeviceToHost, s));
+                    KITE_HIP(hipEventRecord(done[q & 1], s));
+                }
             };
-            if (!pieces.empty()) issue(0);
+            for (size_t q = 0; q < std::min<size_t>(2, pieces.size()); q++) issue(q);
             for (size_t q = 0; q < pieces.size(); q++) {
-                KITE_HIP(hipEventSynchronize(done[q & 1]));
-                if (q + 1 < pieces.size()) issue(q + 1);
+                if (link) link->wait((int)(q & 1));
+                else KITE_HIP(hipEventSynchronize(done[q & 1]));
                 const uint8_t *pin = ctx->merge_out[q & 1].as<uint8_t>();
                 const uint64_t base = boff[pieces[q].first];
                 for (uint32_t p = pieces[q].first; p < pieces[q].second; p++) {
@@ -2502,6 +2514,7 @@ int kite_gpu_merge_runs(kite_gpu_ctx *ctx, uint32_t key_width, uint32_t partitio
                         return set_error(KITE_E_CALLBACK, "merge chunk callback aborted");
                     }
                 }
+                if (q + 2 < pieces.size()) issue(q + 2);
             }
             KITE_HIP(hipStreamSynchronize(s));
             st.out_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_out).count();

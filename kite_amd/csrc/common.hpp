@@ -26,6 +26,19 @@ struct Error : std::runtime_error {
 
 #define KITE_CHECK_LAUNCH() KITE_HIP(hipGetLastError())
 
+// Device memory a grown buffer gave up.  Freeing device memory makes the
+// driver wipe it on the same copy hardware the process's device-to-host
+// copies use: they run at about half rate while it does (about 30 ms per
+// GiB freed).  Allocations of RETIRE_MIN bytes or more that a growing
+// DevBuf replaces are therefore kept, still allocated, and freed together
+// by free_retired (a context's close), or at once past RETIRE_CAP bytes
+// kept.  Nothing waits for the device when a buffer is retired: queued
+// work may still read it, and free_retired waits for the device first.
+constexpr size_t RETIRE_MIN = 64ull << 20;
+constexpr size_t RETIRE_CAP = 32ull << 30;
+void retire_device(void *p, size_t bytes);
+void free_retired();
+
 // Grow-only device allocation.  Contents are not preserved on growth.
 struct DevBuf {
     void *ptr = nullptr;
@@ -42,6 +55,11 @@ struct DevBuf {
     void *ensure(size_t need) {
         if (need == 0) need = 256;
         if (need > bytes) {
+            if (ptr && bytes >= RETIRE_MIN) {
+                retire_device(ptr, bytes);
+                ptr = nullptr;
+                bytes = 0;
+            }
             // work queued on the device may still use the old allocation
             if (ptr) KITE_HIP(hipDeviceSynchronize());
             release();

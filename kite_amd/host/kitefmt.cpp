@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cerrno>
 #include <cstring>
+#include <functional>
 
 namespace kh {
 
@@ -84,6 +85,127 @@ Header Header::parse(const uint8_t *h) {
 }
 
 // ---- writer -----------------------------------------------------------------
+namespace {
+
+// Splits one large copy or file write over a few helper threads and the
+// caller (appends come from one thread, and one core copies only a few GB/s
+// out of page-locked memory).  Parts of at least PART bytes; smaller
+// requests run on the caller alone.
+class SplitPool {
+   public:
+    static constexpr size_t PART = 1u << 20;
+    static SplitPool &get() {
+        static SplitPool p;
+        return p;
+    }
+    // fn(part offset, part length) over [0, len) in parts, the caller
+    // taking its share; returns the first error message ("" when none)
+    template <class F>
+    std::string run(size_t len, F &&fn) {
+        const size_t parts = std::min<size_t>(threads_.size() + 1, len / PART);
+        if (parts <= 1) {
+            try {
+                fn(size_t(0), len);
+            } catch (const std::exception &e) {
+                return e.what();
+            }
+            return "";
+        }
+        std::lock_guard<std::mutex> one(caller_);  // one request at a time
+        const size_t step = ((len + parts - 1) / parts + 63) & ~size_t(63);
+        Task t;
+        t.fn = [&](size_t q) {
+            const size_t a = q * step;
+            if (a < len) fn(a, std::min(step, len - a));
+        };
+        t.parts = parts;
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            task_ = &t;
+            gen_++;
+        }
+        cv_.notify_all();
+        work(t);
+        // every part done and no helper still holding the task (it lives on
+        // this stack); no helper can take it once task_ is cleared here
+        std::unique_lock<std::mutex> g(mu_);
+        done_cv_.wait(g, [&] { return t.done == t.parts && t.holders == 0; });
+        task_ = nullptr;
+        return t.err;
+    }
+
+   private:
+    struct Task {
+        std::function<void(size_t)> fn;
+        size_t parts = 0, next = 0, done = 0, holders = 0;
+        std::string err;
+    };
+    SplitPool() {
+        const unsigned n = std::min(4u, std::max(1u, std::thread::hardware_concurrency()) - 1u);
+        for (unsigned i = 0; i < n; i++) threads_.emplace_back([this] { loop(); });
+    }
+    ~SplitPool() {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : threads_) t.join();
+    }
+    // takes parts of t until none is left (mu_ not held on entry)
+    void work(Task &t) {
+        for (;;) {
+            size_t q;
+            {
+                std::lock_guard<std::mutex> g(mu_);
+                if (t.next >= t.parts) return;
+                q = t.next++;
+            }
+            std::string e;
+            try {
+                t.fn(q);
+            } catch (const std::exception &x) {
+                e = x.what();
+            }
+            bool last;
+            {
+                std::lock_guard<std::mutex> g(mu_);
+                if (!e.empty() && t.err.empty()) t.err = e;
+                last = ++t.done == t.parts;
+            }
+            if (last) done_cv_.notify_all();
+        }
+    }
+    void loop() {
+        uint64_t seen = 0;
+        for (;;) {
+            Task *t;
+            {
+                std::unique_lock<std::mutex> g(mu_);
+                cv_.wait(g, [&] { return stop_ || (gen_ != seen && task_); });
+                if (stop_) return;
+                seen = gen_;
+                t = task_;
+                t->holders++;
+            }
+            work(*t);
+            {
+                std::lock_guard<std::mutex> g(mu_);
+                t->holders--;
+            }
+            done_cv_.notify_all();
+        }
+    }
+    std::vector<std::thread> threads_;
+    std::mutex mu_, caller_;
+    std::condition_variable cv_, done_cv_;
+    Task *task_ = nullptr;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+};
+
+}  // namespace
+
 int Writer::suggest_write_lanes(int workers) {
     int l = (workers + 7) / 8;
     return std::min(6, std::max(2, l));
@@ -181,8 +303,14 @@ void Writer::stop_writers() {
 
 uint64_t Writer::claim(const uint8_t *data, size_t len) {
     uint64_t off = next_offset_;
-    if (fd_ >= 0) write_at(data, len, off);  // the caller's buffer: written now
-    else if (len && fwrite(data, 1, len, f_) != len) throw Error("write failed");
+    if (fd_ >= 0) {
+        // the caller's buffer: written now, in parts at their own offsets
+        const std::string e =
+            SplitPool::get().run(len, [&](size_t a, size_t n) { write_at(data + a, n, off + a); });
+        if (!e.empty()) throw Error(e);
+    } else if (len && fwrite(data, 1, len, f_) != len) {
+        throw Error("write failed");
+    }
     next_offset_ += len;
     return off;
 }
@@ -203,7 +331,7 @@ void Writer::flush_partition(uint32_t p) {
         jobs_.push_back(Job{off, std::move(s.buf)});
         g.unlock();
         cv_.notify_one();
-        s.buf = std::vector<uint8_t>();
+        s.buf = Bytes();
     } else {
         off = claim(s.buf.data(), s.buf.size());
     }
@@ -259,7 +387,10 @@ void Writer::append_chunk(uint32_t p, uint32_t codec, uint64_t rows, uint64_t ra
         }
     }
     if (s.buf.capacity() < s.buf.size() + len) s.buf.reserve(std::max(s.buf.size() + len, std::min(S_, 2 * s.buf.capacity())));
-    s.buf.insert(s.buf.end(), data, data + len);
+    const size_t at = s.buf.size();
+    s.buf.resize(at + len);
+    uint8_t *dst = s.buf.data() + at;
+    (void)SplitPool::get().run(len, [&](size_t a, size_t n) { memcpy(dst + a, data + a, n); });
     s.pending.push_back(m);
     staged_ += len;
     if (s.buf.size() >= S_) flush_partition(p);

@@ -22,6 +22,8 @@
 #include <deque>
 #include <mutex>
 #include <thread>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 namespace kh {
@@ -76,6 +78,28 @@ struct ChunkMeta {
     uint64_t kmin[4] = {0, 0, 0, 0}, kmax[4] = {0, 0, 0, 0};  // (lo128, hi128)
 };
 
+// std::vector allocator whose resize leaves new elements uninitialised, so
+// that growing a byte buffer before a copy into it does not zero it first.
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = NoInitAlloc<U>;
+    };
+    NoInitAlloc() = default;
+    template <class U>
+    NoInitAlloc(const NoInitAlloc<U> &) noexcept {}
+    template <class U>
+    void construct(U *p) noexcept(std::is_nothrow_default_constructible<U>::value) {
+        ::new (static_cast<void *>(p)) U;
+    }
+    template <class U, class... A>
+    void construct(U *p, A &&...a) {
+        ::new (static_cast<void *>(p)) U(std::forward<A>(a)...);
+    }
+};
+using Bytes = std::vector<uint8_t, NoInitAlloc<uint8_t>>;
+
 // ConcurrentKiteWriter (src/count/writer.rs:71-495) restated for appends
 // from one thread: per-partition staging buffers of `staging_bytes`, chunks
 // at least that large written directly, excess staging drained largest
@@ -95,7 +119,7 @@ class Writer {
         ChunkMeta m;
     };
     struct Stage {
-        std::vector<uint8_t> buf;
+        Bytes buf;
         std::vector<ChunkMeta> pending, committed;
     };
     uint64_t claim(const uint8_t *data, size_t len);
@@ -105,7 +129,7 @@ class Writer {
     // offsets by `write_lanes` threads (the file layout does not change)
     struct Job {
         uint64_t off;
-        std::vector<uint8_t> buf;
+        Bytes buf;
     };
     void write_at(const uint8_t *data, size_t len, uint64_t off);
     void worker();
@@ -129,7 +153,7 @@ class Writer {
     std::string werr_;
     // written staging buffers kept for reuse (their pages stay mapped, so a
     // staged chunk is a plain copy rather than a copy plus page faults)
-    std::vector<std::vector<uint8_t>> spare_;
+    std::vector<Bytes> spare_;
     size_t spare_cap_ = 0;
 };
 

@@ -180,3 +180,26 @@ def test_cli_merge_cpu_engine_matches_oracle(tmp_path, k, canon, P, n_inputs):
     assert "engine=cpu" in r.stderr
     ol.kite_merge(paths, tmp_path / "o.kite", lanes=lanes)
     assert (tmp_path / "c.kite").read_bytes() == (tmp_path / "o.kite").read_bytes()
+
+
+@pytest.mark.parametrize("P", [2, 8])
+def test_cli_merge_large_chunks_match_oracle(tmp_path, P):
+    """Merged partitions of megabytes: chunks above the 16 MiB staging size
+    (P = 2) are written in parts by several threads at their own offsets,
+    chunks of a few MiB (P = 8) are copied into staging in parts; the file
+    is the oracle's byte for byte."""
+    paths = []
+    for i in range(2):
+        rng = random.Random(100 + i)
+        seq = "".join(rng.choice("ACGT") for _ in range(3_000_000))
+        recs = [f"@r{r}\n{seq[j:j + 150]}\n+\n{'I' * 150}\n" for r, j in enumerate(range(0, len(seq) - 150, 75))]
+        t = ol.count_fastq("".join(recs).encode(), 31, True, P)
+        p = tmp_path / f"in{i}.kite"
+        ol.write_table(p, t)
+        paths.append(p)
+    lanes = min(6, max(2, ((os.cpu_count() or 1) + 7) // 8))
+    kite("merge", "--engine", "cpu", "-i", *paths, "-o", tmp_path / "c.kite")
+    ol.kite_merge(paths, tmp_path / "o.kite", lanes=lanes)
+    got, exp = (tmp_path / "c.kite").read_bytes(), (tmp_path / "o.kite").read_bytes()
+    assert len(exp) > (20 << 20)
+    assert got == exp
