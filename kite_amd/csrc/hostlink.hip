@@ -109,17 +109,22 @@ uint32_t HostLink::d2h(int slot, void *dst, const void *src, size_t bytes) {
             break;
         }
     if (!e) e = last_ ? last_ : engines_[0];
-    last_ = e;
     hsa_signal_store_relaxed(d, 1);
-    const hsa_status_t st = hsa_amd_memory_async_copy_on_engine(dst, cpu_, src, gpu_, bytes, 0, nullptr, d,
-                                                                (hsa_amd_sdma_engine_id_t)e, false);
+    hsa_status_t st = hsa_amd_memory_async_copy_on_engine(dst, cpu_, src, gpu_, bytes, 0, nullptr, d,
+                                                          (hsa_amd_sdma_engine_id_t)e, false);
+    if (st != HSA_STATUS_SUCCESS) {
+        // the engine refused this copy (the runtime can withhold an engine):
+        // the runtime picks one, on the same completion signal
+        e = 0;
+        st = hsa_amd_memory_async_copy(dst, cpu_, src, gpu_, bytes, 0, nullptr, d);
+    }
     if (st != HSA_STATUS_SUCCESS) {
         hsa_signal_store_relaxed(d, 0);
         const char *msg = nullptr;
         (void)hsa_status_string(st, &msg);
-        throw Error(std::string("device-to-host copy on SDMA engine mask ") + std::to_string(e) +
-                    " failed to queue: " + (msg ? msg : std::to_string((int)st)));
+        throw Error(std::string("device-to-host copy failed to queue: ") + (msg ? msg : std::to_string((int)st)));
     }
+    if (e) last_ = e;
     return e;
 }
 

@@ -42,7 +42,8 @@ public:
 
     // Queues `bytes` from device memory `src` to page-locked host memory
     // `dst`; completion signal `slot` drops to 0 when the copy has ended.
-    // Returns the engine bit used.  Not thread safe: one caller at a time.
+    // Returns the engine bit used, 0 when the engine refused the copy and
+    // the runtime placed it.  Not thread safe: one caller at a time.
     uint32_t d2h(int slot, void *dst, const void *src, size_t bytes);
     // Blocks until the copy last queued on `slot` has ended (at once when
     // none is in flight); throws kite::Error when it reported an error.
