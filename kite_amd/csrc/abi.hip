@@ -1224,19 +1224,103 @@ struct Consumer {
     }
 };
 
+// The H2D side of a ring batch, on a thread of its own: the input of sample
+// j goes into buffer j % R on the copy stream h2d once the run of sample
+// j - R, which read that buffer, is queued (ran()), behind a stream wait for
+// that run's end (slot j % R's run_done), and once the H2D of sample j - 1
+// has ended; in_ready of the slot follows the copy.  One input copy is in
+// flight at a time: with a single copy queued, the runtime keeps it on its
+// preferred host-to-device engine instead of spreading queued copies over
+// engines that also carry the D2H or move host traffic slower.
+class Uploader {
+   public:
+    Uploader(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, int R, bool prof)
+        : c_(c), smp_(smp), n_(n), R_((uint32_t)R), prof_(prof) {}
+    ~Uploader() { stop(); }
+    void start() { th_ = std::thread([this] { loop(); }); }
+    // the run of sample i is queued and its slot's run_done recorded
+    void ran(uint32_t i) {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            ran_ = i + 1;
+        }
+        cv_.notify_all();
+    }
+    // blocks until the H2D of sample i is queued; throws the uploader's error
+    void wait_queued(uint32_t i) {
+        std::unique_lock<std::mutex> g(mu_);
+        cv_.wait(g, [&] { return queued_ > i || !err_.empty(); });
+        if (!err_.empty()) throw kite::Error(err_);
+    }
+    void stop() {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        if (th_.joinable()) th_.join();
+    }
+
+   private:
+    void loop() {
+        try {
+            for (uint32_t j = 0; j < n_; j++) {
+                const uint32_t b = j % R_;
+                {
+                    std::unique_lock<std::mutex> g(mu_);
+                    cv_.wait(g, [&] { return stop_ || j < R_ || ran_ > j - R_; });
+                    if (stop_) return;
+                }
+                if (j > 0) KITE_HIP(hipEventSynchronize(c_->slots[(j - 1) % R_].in_ready));
+                if (j >= R_) KITE_HIP(hipStreamWaitEvent(c_->h2d, c_->slots[b].run_done, 0));
+                if (prof_) KITE_HIP(hipEventRecord(c_->tl_events[j][0], c_->h2d));
+                if (smp_[j].len)
+                    KITE_HIP(hipMemcpyAsync(c_->batch_in[b].ptr, smp_[j].fastq, smp_[j].len, hipMemcpyHostToDevice,
+                                            c_->h2d));
+                if (prof_) KITE_HIP(hipEventRecord(c_->tl_events[j][1], c_->h2d));
+                KITE_HIP(hipEventRecord(c_->slots[b].in_ready, c_->h2d));
+                {
+                    std::lock_guard<std::mutex> g(mu_);
+                    queued_ = j + 1;
+                }
+                cv_.notify_all();
+            }
+        } catch (const std::exception &e) {
+            {
+                std::lock_guard<std::mutex> g(mu_);
+                if (err_.empty()) err_ = e.what();
+            }
+            cv_.notify_all();
+        }
+    }
+    kite_gpu_ctx *c_;
+    const kite_batch_sample *smp_;
+    uint32_t n_, R_;
+    bool prof_;
+    std::thread th_;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    uint32_t queued_ = 0, ran_ = 0;
+    bool stop_ = false;
+    std::string err_;
+};
+
 // Samples flow through a ring of R input buffers and R result slots (input
 // buffer, engine output slot, encode buffers and host buffers of slot i % R
-// belong to sample i):
-//   copy stream h2d:   H2D of sample j into buffer j % R, after the run of
-//                      sample j - R read that buffer;
-//   engine stream:     count of sample i once its H2D is done (raw rows:
+// belong to sample i), with the kernels launched one by one after the
+// count's host read-backs:
+//   uploader thread:   the H2D of sample j into buffer j % R (Uploader);
+//   engine stream:     the count of sample i once its H2D is done (raw rows:
 //                      also once the D2H of sample i - R has read the
-//                      engine's output slot), then the chunk encode;
-//   copy stream d2h:   D2H of sample i after its run / encode;
+//                      engine's output slot), then the chunk encode (once
+//                      the D2H of sample i - R has read the encode buffers);
+//   D2H of sample i after its run: on the context's HostLink from a drainer
+//                      thread (the engine thread then waits on the host for
+//                      the end of the D2H of sample i - R), or on the copy
+//                      stream d2h when stage events are on, when
+//                      KITE_D2H_ENGINES=0 or when no engine qualifies;
 //   consumer thread:   the callback of sample i after its D2H, which frees
 //                      the slot's host buffers for sample i + R.
-// The H2D queue runs R - 1 samples ahead of the engine, so the two copy
-// directions stay busy while the engine works.
 void batch_pipelined(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, const BatchSink &sink) {
     const int R = batch_ring();
     if (!c->h2d) KITE_HIP(hipStreamCreateWithFlags(&c->h2d, hipStreamNonBlocking));
@@ -1257,33 +1341,95 @@ void batch_pipelined(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, 
     }
     uint64_t maxlen = 0;
     for (uint32_t i = 0; i < n; i++) maxlen = std::max<uint64_t>(maxlen, smp[i].len);
-    // every input buffer is sized before any copy is queued into one
+    // every input buffer is sized before any copy is queued into one, and
+    // the count's key arrays for the longest input (a FASTQ record's
+    // sequence line is at most half its bytes: at most len / 2 k-mers)
     for (int b = 0; b < std::min<int>(R, (int)n); b++) c->batch_in[b].ensure(maxlen);
     const int W = c->words;
-
-    auto h2d = [&](uint32_t j) {
-        const int b = (int)(j % (uint32_t)R);
-        if (j >= (uint32_t)R) KITE_HIP(hipStreamWaitEvent(c->h2d, c->slots[b].run_done, 0));
-        if (prof) KITE_HIP(hipEventRecord(c->tl_events[j][0], c->h2d));
-        if (smp[j].len)
-            KITE_HIP(hipMemcpyAsync(c->batch_in[b].ptr, smp[j].fastq, smp[j].len, hipMemcpyHostToDevice, c->h2d));
-        if (prof) KITE_HIP(hipEventRecord(c->tl_events[j][1], c->h2d));
-        KITE_HIP(hipEventRecord(c->slots[b].in_ready, c->h2d));
-    };
+    c->count->reserve(maxlen / 2, W);
+    kite::HostLink *link = prof ? nullptr : host_link(c);
 
     Consumer cons(c, sink);
+    cons.link = link;
+    // drainer (HostLink copies): after sample i's run, the D2H of its results
+    struct Pending {
+        uint32_t i;
+        int s;
+        kite::CopyPiece piece[2];
+    };
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<Pending> todo;
+    uint32_t drained = 0;  // samples whose D2H is queued (or failed to queue)
+    bool stop = false;
+    std::string drain_err;
+    if (prof) KITE_HIP(hipEventRecord(c->tl_start, c->h2d));
+    Uploader up(c, smp, n, R, prof);
+    up.start();
+    auto drain = [&] {
+        for (;;) {
+            Pending t;
+            {
+                std::unique_lock<std::mutex> g(mu);
+                cv.wait(g, [&] { return stop || !todo.empty(); });
+                if (todo.empty()) return;
+                t = todo.front();
+                todo.pop_front();
+            }
+            bool ok = false;
+            try {
+                KITE_HIP(hipEventSynchronize(c->slots[t.s].run_done));
+                c->slots[t.s].st.d2h_engine = link->d2h_many(t.s, t.piece, 2);
+                ok = true;
+            } catch (const std::exception &e) {
+                std::lock_guard<std::mutex> g(mu);
+                if (drain_err.empty()) drain_err = e.what();
+            }
+            if (ok) cons.push(t.s);
+            else cons.release(t.s);
+            {
+                std::lock_guard<std::mutex> g(mu);
+                drained = t.i + 1;
+            }
+            cv.notify_all();
+        }
+    };
+    std::thread drainer;
+    if (link) drainer = std::thread(drain);
+    auto finish_drainer = [&] {
+        {
+            std::lock_guard<std::mutex> g(mu);
+            stop = true;
+        }
+        cv.notify_all();
+        if (drainer.joinable()) drainer.join();
+    };
+    // before slot s is written again: the D2H of sample i - R has read it
+    auto slot_read = [&](uint32_t i, int s) {
+        if (i < (uint32_t)R) return;
+        if (link) {
+            std::unique_lock<std::mutex> g(mu);
+            cv.wait(g, [&] { return drained > i - R; });
+            g.unlock();
+            link->wait(s);
+        } else {
+            KITE_HIP(hipStreamWaitEvent(c->stream, c->slots[s].d2h_done, 0));
+        }
+    };
+
     std::string failure;
     try {
-        if (prof) KITE_HIP(hipEventRecord(c->tl_start, c->h2d));
-        for (uint32_t j = 0; j + 1 < (uint32_t)R && j < n; j++) h2d(j);
         for (uint32_t i = 0; i < n; i++) {
+            {
+                std::lock_guard<std::mutex> g(mu);
+                if (!drain_err.empty()) break;
+            }
             const int s = (int)(i % (uint32_t)R);
             BatchSlot &sl = c->slots[s];
-            // input buffer (i - 1) % R was last read by the run of sample i - 1
-            if (i + R - 1 < n) h2d(i + R - 1);
+            up.wait_queued(i);
             KITE_HIP(hipStreamWaitEvent(c->stream, sl.in_ready, 0));
             // raw rows: the D2H of sample i - R reads the engine's output slot s
-            if (!sink.encoded() && i >= (uint32_t)R) KITE_HIP(hipStreamWaitEvent(c->stream, sl.d2h_done, 0));
+            if (!sink.encoded()) slot_read(i, s);
             if (prof) KITE_HIP(hipEventRecord(c->tl_events[i][2], c->stream));
             c->count->set_slot(s);
             c->count->set_timing(true);
@@ -1297,43 +1443,55 @@ void batch_pipelined(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, 
             batch_stats(c, smp[i], sl.st);
             sl.part_off = c->count->part_off();
             sl.sample = i;
+            Pending t{i, s, {}};
             if (sink.encoded()) {
-                // encode buffer s was last read by the D2H of sample i - R (ended, see above)
-                if (i >= (uint32_t)R) KITE_HIP(hipStreamWaitEvent(c->stream, sl.d2h_done, 0));
+                // encode buffer s was last read by the D2H of sample i - R
+                slot_read(i, s);
                 const kite::EncodeDevice ed = encode_slot(c, sl, c->enc_slot[s]);
                 KITE_HIP(hipEventRecord(sl.run_done, c->stream));
+                up.ran(i);
                 if (prof) KITE_HIP(hipEventRecord(c->tl_events[i][3], c->stream));
-                KITE_HIP(hipStreamWaitEvent(c->d2h, sl.run_done, 0));
-                if (prof) KITE_HIP(hipEventRecord(c->tl_events[i][4], c->d2h));
-                if (ed.total)
-                    KITE_HIP(hipMemcpyAsync(sl.enc_bytes.ptr, ed.bytes, ed.total, hipMemcpyDeviceToHost, c->d2h));
-                KITE_HIP(hipMemcpyAsync(sl.enc_minmax.ptr, ed.minmax, (size_t)c->partitions * 64,
-                                        hipMemcpyDeviceToHost, c->d2h));
+                t.piece[0] = kite::CopyPiece{sl.enc_bytes.ptr, ed.bytes, ed.total};
+                t.piece[1] = kite::CopyPiece{sl.enc_minmax.ptr, ed.minmax, (size_t)c->partitions * 64};
             } else {
                 KITE_HIP(hipEventRecord(sl.run_done, c->stream));
+                up.ran(i);
                 if (prof) KITE_HIP(hipEventRecord(c->tl_events[i][3], c->stream));
                 uint64_t *keys = static_cast<uint64_t *>(sl.keys.ensure(u * 8 * W));
                 uint32_t *counts = static_cast<uint32_t *>(sl.counts.ensure(u * 4));
+                t.piece[0] = kite::CopyPiece{keys, c->count->keys(), u * 8 * W};
+                t.piece[1] = kite::CopyPiece{counts, c->count->counts(), u * 4};
+            }
+            if (link) {
+                {
+                    std::lock_guard<std::mutex> g(mu);
+                    todo.push_back(t);
+                }
+                cv.notify_all();
+            } else {
                 KITE_HIP(hipStreamWaitEvent(c->d2h, sl.run_done, 0));
                 if (prof) KITE_HIP(hipEventRecord(c->tl_events[i][4], c->d2h));
-                if (u) {
-                    KITE_HIP(hipMemcpyAsync(keys, c->count->keys(), u * 8 * W, hipMemcpyDeviceToHost, c->d2h));
-                    KITE_HIP(hipMemcpyAsync(counts, c->count->counts(), u * 4, hipMemcpyDeviceToHost, c->d2h));
-                }
+                for (const kite::CopyPiece &q : t.piece)
+                    if (q.bytes) KITE_HIP(hipMemcpyAsync(q.dst, q.src, q.bytes, hipMemcpyDeviceToHost, c->d2h));
+                if (prof) KITE_HIP(hipEventRecord(c->tl_events[i][5], c->d2h));
+                KITE_HIP(hipEventRecord(sl.d2h_done, c->d2h));
+                cons.push(s);
             }
-            if (prof) KITE_HIP(hipEventRecord(c->tl_events[i][5], c->d2h));
-            KITE_HIP(hipEventRecord(sl.d2h_done, c->d2h));
-            cons.push(s);
         }
-        cons.drain();
+        finish_drainer();
+        if (drain_err.empty()) cons.drain();
     } catch (const std::exception &e) {
         failure = e.what();
     }
+    finish_drainer();
+    up.stop();
     cons.stop();
+    if (link) link->quiesce();
     (void)hipStreamSynchronize(c->h2d);
     (void)hipStreamSynchronize(c->d2h);
     (void)hipStreamSynchronize(c->stream);
     if (!failure.empty()) throw kite::Error(failure);
+    if (!drain_err.empty()) throw kite::Error(drain_err);
     std::string ce = cons.err();
     if (!ce.empty()) throw kite::Error(ce);
     if (prof) {
@@ -1538,9 +1696,6 @@ uint32_t batch_graph(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, 
     std::condition_variable cv;
     std::deque<uint32_t> launched;
     uint32_t drained = 0;  // samples whose D2H is queued (or that were dropped)
-    uint32_t queued = 0;   // samples whose H2D is queued
-    uint32_t ran = 0;      // samples whose count graph (and run_done) is queued
-    std::string upload_err;
     bool stop = false;
     uint32_t overflow_at = n;  // the first sample whose planned count overflowed
     std::string drain_err;
@@ -1554,6 +1709,8 @@ uint32_t batch_graph(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, 
         cv.notify_all();
     };
     if (prof) KITE_HIP(hipEventRecord(c->tl_start, c->h2d));
+    Uploader up(c, smp, n, R, prof);
+    up.start();
     std::thread drainer([&] {
         for (;;) {
             uint32_t i;
@@ -1626,65 +1783,15 @@ uint32_t batch_graph(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, 
             }
         }
     });
-    // uploader: the H2D of sample j into buffer j % R once the count graph
-    // of sample j - R (which read that buffer) is queued, with a stream wait
-    // for its end.  One input copy is in flight at a time: with a single
-    // copy queued, the runtime keeps it on its preferred host-to-device
-    // engine instead of spreading copies over engines that also carry the
-    // D2H or run host traffic slower.
-    auto upload = [&] {
-        try {
-            for (uint32_t j = 0; j < n; j++) {
-                const int b = (int)(j % (uint32_t)R);
-                if (j >= (uint32_t)R) {
-                    std::unique_lock<std::mutex> g(mu);
-                    cv.wait(g, [&] { return stop || ran > j - R; });
-                    if (stop) return;
-                }
-                if (j > 0) KITE_HIP(hipEventSynchronize(c->slots[(j - 1) % (uint32_t)R].in_ready));
-                if (j >= (uint32_t)R) KITE_HIP(hipStreamWaitEvent(c->h2d, c->slots[b].run_done, 0));
-                if (prof) KITE_HIP(hipEventRecord(c->tl_events[j][0], c->h2d));
-                if (smp[j].len)
-                    KITE_HIP(
-                        hipMemcpyAsync(c->batch_in[b].ptr, smp[j].fastq, smp[j].len, hipMemcpyHostToDevice, c->h2d));
-                if (prof) KITE_HIP(hipEventRecord(c->tl_events[j][1], c->h2d));
-                KITE_HIP(hipEventRecord(c->slots[b].in_ready, c->h2d));
-                {
-                    std::lock_guard<std::mutex> g(mu);
-                    queued = j + 1;
-                }
-                cv.notify_all();
-            }
-        } catch (const std::exception &e) {
-            {
-                std::lock_guard<std::mutex> g(mu);
-                if (upload_err.empty()) upload_err = e.what();
-            }
-            cv.notify_all();
-        }
-    };
-    std::thread uploader;
-    try {
-        uploader = std::thread(upload);
-    } catch (...) {
-        {
-            std::lock_guard<std::mutex> g(mu);
-            stop = true;
-        }
-        cv.notify_all();
-        drainer.join();
-        throw;
-    }
     std::string failure;
     uint32_t launched_n = 0;
     try {
         for (uint32_t i = 0; i < n; i++) {
             {
-                std::unique_lock<std::mutex> g(mu);
+                std::lock_guard<std::mutex> g(mu);
                 if (i >= overflow_at || !cons.err().empty()) break;
-                cv.wait(g, [&] { return queued > i || !upload_err.empty(); });
-                if (!upload_err.empty()) throw kite::Error(upload_err);
             }
+            up.wait_queued(i);
             const int s = (int)(i % (uint32_t)R);
             BatchSlot &sl = c->slots[s];
             KITE_HIP(hipStreamWaitEvent(c->stream, sl.in_ready, 0));
@@ -1700,11 +1807,7 @@ uint32_t batch_graph(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, 
                 KITE_HIP(hipGraphLaunch(exec[i].first, c->stream));
             }
             KITE_HIP(hipEventRecord(sl.run_done, c->stream));  // input buffer s is free again
-            {
-                std::lock_guard<std::mutex> g(mu);
-                ran = i + 1;
-            }
-            cv.notify_all();
+            up.ran(i);
             if (i >= (uint32_t)R) {
                 // the slot's payload buffer is read by the D2H of sample i - R:
                 // once the drainer has queued that copy, wait for its end
@@ -1735,7 +1838,7 @@ uint32_t batch_graph(kite_gpu_ctx *c, const kite_batch_sample *smp, uint32_t n, 
     }
     cv.notify_all();
     drainer.join();
-    uploader.join();
+    up.stop();
     // (after a failure a slot may stay taken: nothing more is drained)
     if (failure.empty()) cons.drain();
     cons.stop();

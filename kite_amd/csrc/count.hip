@@ -2035,6 +2035,13 @@ CountTunables CountEngine::resolved_tunables() const {
     return t;
 }
 
+void CountEngine::reserve(uint64_t max_kmers, int words) {
+    const uint64_t n = std::max<uint64_t>(max_kmers, 1);
+    a_.ensure(n * 8 * words);
+    b_.ensure(n * 8 * words);
+    counts_tmp_.ensure(n * 4);
+}
+
 void CountEngine::run(const uint8_t *d_fq, uint64_t len, int k, int canonical, uint32_t partitions) {
     if (k < 1 || k > 128) throw Error("invalid k-mer length: k must be between 1 and 128");
     if (partitions == 0 || (partitions & (partitions - 1)))

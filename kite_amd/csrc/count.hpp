@@ -111,6 +111,10 @@ class CountEngine {
     // the next run: keys() (n_unique * words u64), counts() (u32), and
     // part_off() (host copy, partitions + 1 entries).
     void run(const uint8_t *d_fq, uint64_t len, int k, int canonical, uint32_t partitions);
+    // Sizes the key arrays run() uses for up to max_kmers k-mers of `words`
+    // words now, so that a batch of growing inputs does not grow them (and
+    // free the smaller ones) sample by sample.
+    void reserve(uint64_t max_kmers, int words);
 
     uint64_t n_kmers() const { return n_kmers_; }
     uint64_t n_unique() const { return n_unique_; }

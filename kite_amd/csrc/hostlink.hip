@@ -96,8 +96,17 @@ HostLink::~HostLink() {
 }
 
 uint32_t HostLink::d2h(int slot, void *dst, const void *src, size_t bytes) {
+    const CopyPiece p{dst, src, bytes};
+    return d2h_many(slot, &p, 1);
+}
+
+uint32_t HostLink::d2h_many(int slot, const CopyPiece *pieces, int n) {
     if (!usable()) throw Error("HostLink::d2h: no SDMA engine for device-to-host copies");
     hsa_signal_t d = done_.at((size_t)slot);
+    int live = 0;
+    for (int i = 0; i < n; i++) live += pieces[i].bytes ? 1 : 0;
+    hsa_signal_store_relaxed(d, live);
+    if (!live) return 0;
     // an engine of the list with nothing queued, else the one this link
     // used last (its queue holds only this link's own copies)
     uint32_t free_mask = 0;
@@ -109,23 +118,30 @@ uint32_t HostLink::d2h(int slot, void *dst, const void *src, size_t bytes) {
             break;
         }
     if (!e) e = last_ ? last_ : engines_[0];
-    hsa_signal_store_relaxed(d, 1);
-    hsa_status_t st = hsa_amd_memory_async_copy_on_engine(dst, cpu_, src, gpu_, bytes, 0, nullptr, d,
-                                                          (hsa_amd_sdma_engine_id_t)e, false);
-    if (st != HSA_STATUS_SUCCESS) {
-        // the engine refused this copy (the runtime can withhold an engine):
-        // the runtime picks one, on the same completion signal
-        e = 0;
-        st = hsa_amd_memory_async_copy(dst, cpu_, src, gpu_, bytes, 0, nullptr, d);
+    uint32_t used = e;
+    for (int i = 0; i < n; i++) {
+        const CopyPiece &p = pieces[i];
+        if (!p.bytes) continue;
+        hsa_status_t st = hsa_amd_memory_async_copy_on_engine(p.dst, cpu_, p.src, gpu_, p.bytes, 0, nullptr, d,
+                                                              (hsa_amd_sdma_engine_id_t)e, false);
+        if (st != HSA_STATUS_SUCCESS) {
+            // the engine refused this copy (the runtime can withhold an
+            // engine): the runtime picks one, on the same completion signal
+            used = 0;
+            st = hsa_amd_memory_async_copy(p.dst, cpu_, p.src, gpu_, p.bytes, 0, nullptr, d);
+        }
+        if (st != HSA_STATUS_SUCCESS) {
+            // the pieces not queued will not count the signal down
+            hsa_signal_subtract_relaxed(d, live);
+            const char *msg = nullptr;
+            (void)hsa_status_string(st, &msg);
+            throw Error(std::string("device-to-host copy failed to queue: ") +
+                        (msg ? msg : std::to_string((int)st)));
+        }
+        live--;
     }
-    if (st != HSA_STATUS_SUCCESS) {
-        hsa_signal_store_relaxed(d, 0);
-        const char *msg = nullptr;
-        (void)hsa_status_string(st, &msg);
-        throw Error(std::string("device-to-host copy failed to queue: ") + (msg ? msg : std::to_string((int)st)));
-    }
-    if (e) last_ = e;
-    return e;
+    last_ = e;
+    return used;
 }
 
 void HostLink::wait(int slot) const {

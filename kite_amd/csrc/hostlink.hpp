@@ -25,6 +25,13 @@
 
 namespace kite {
 
+// One device-to-host copy of a HostLink request.
+struct CopyPiece {
+    void *dst;
+    const void *src;
+    size_t bytes;
+};
+
 class HostLink {
 public:
     // The SDMA engines of HIP device `device` (its HSA agent is found by PCI
@@ -45,6 +52,9 @@ public:
     // Returns the engine bit used, 0 when the engine refused the copy and
     // the runtime placed it.  Not thread safe: one caller at a time.
     uint32_t d2h(int slot, void *dst, const void *src, size_t bytes);
+    // The same for several pieces on one engine and one completion signal,
+    // which drops to 0 once all of them (empty ones skipped) have ended.
+    uint32_t d2h_many(int slot, const CopyPiece *pieces, int n);
     // Blocks until the copy last queued on `slot` has ended (at once when
     // none is in flight); throws kite::Error when it reported an error.
     void wait(int slot) const;
