@@ -31,11 +31,12 @@ struct Error : std::runtime_error {
 // copies use: they run at about half rate while it does (about 30 ms per
 // GiB freed).  Allocations of RETIRE_MIN bytes or more that a growing
 // DevBuf replaces are therefore kept, still allocated, and freed together
-// by free_retired (a context's close), or at once past RETIRE_CAP bytes
-// kept.  Nothing waits for the device when a buffer is retired: queued
-// work may still read it, and free_retired waits for the device first.
+// by free_retired: at a context's close, when an allocation fails for
+// lack of memory (then retried), or at once past RETIRE_CAP bytes kept.
+// Nothing waits for the device when a buffer is retired: queued work may
+// still read it, and free_retired waits for the device first.
 constexpr size_t RETIRE_MIN = 64ull << 20;
-constexpr size_t RETIRE_CAP = 32ull << 30;
+constexpr size_t RETIRE_CAP = 128ull << 30;
 void retire_device(void *p, size_t bytes);
 void free_retired();
 
@@ -64,7 +65,17 @@ struct DevBuf {
             if (ptr) KITE_HIP(hipDeviceSynchronize());
             release();
             size_t want = need + need / 8;  // slack so small growth does not reallocate
-            KITE_HIP(hipMalloc(&ptr, want));
+            hipError_t e = hipMalloc(&ptr, want);
+            if (e == hipErrorOutOfMemory) {
+                // retired buffers hold memory that no queued work needs
+                (void)hipGetLastError();
+                free_retired();
+                e = hipMalloc(&ptr, want);
+            }
+            if (e != hipSuccess) {
+                ptr = nullptr;
+                throw Error("hipMalloc of " + std::to_string(want) + " bytes failed: " + hipGetErrorString(e));
+            }
             bytes = want;
         }
         return ptr;
