@@ -994,6 +994,11 @@ def main() -> int:
     for p in pins:
         p.free()
     del pins
+    # the batch legs' device buffers go before the single-sample and CLI
+    # legs: a `kite` process sizes its device batches from the device
+    # memory free when it opens
+    ctx.close()
+    ctx = GpuContext(0)
 
     single = fq = None
     w = WORKLOADS[args.single_workload]
