@@ -238,6 +238,72 @@ kite::HostLink *host_link(kite_gpu_ctx *c) {
     return c->link && c->link->usable() ? c->link.get() : nullptr;
 }
 
+// Hands the chunks of partitions [0, np) to fn in partition order (fn sees
+// partition p0 + p): payload bytes on the device at d_bytes + boff[p], the
+// byte and row offsets and key ranges on the host.  The device work that
+// wrote the payloads has finished.  They leave through the context's two
+// page-locked piece buffers in pieces of whole partitions up to 64 MiB (a
+// larger partition is a piece of its own), two pieces in flight on the
+// library's D2H engine while fn takes the one before; piece q + 2 is queued
+// once fn has taken piece q.  Returns -1, or the partition (p0 + p) whose
+// callback returned non-zero.
+int64_t emit_device_chunks(kite_gpu_ctx *ctx, const uint8_t *d_bytes, const uint64_t *boff, const uint64_t *roff,
+                           const uint64_t *mm, uint32_t np, uint32_t p0, uint32_t codec, kite_append_chunk_fn fn,
+                           void *user) {
+    hipStream_t s = ctx->stream;
+    constexpr uint64_t PIECE = 64ull << 20;
+    std::vector<std::pair<uint32_t, uint32_t>> pieces;
+    for (uint32_t a = 0; a < np;) {
+        uint32_t b = a + 1;
+        while (b < np && boff[b + 1] - boff[a] <= PIECE) b++;
+        pieces.emplace_back(a, b);
+        a = b;
+    }
+    kite::HostLink *link = host_link(ctx);
+    hipEvent_t done[2];
+    for (auto &x : done) KITE_HIP(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+    struct Ev2 {
+        hipEvent_t *e;
+        kite::HostLink *link;
+        ~Ev2() {
+            if (link) link->quiesce();
+            (void)hipEventDestroy(e[0]);
+            (void)hipEventDestroy(e[1]);
+        }
+    } ev2{done, link};
+    for (int b = 0; b < 2; b++) ctx->merge_out[b].ensure(PIECE);
+    auto issue = [&](size_t q) {
+        const uint64_t a = boff[pieces[q].first], n = boff[pieces[q].second] - a;
+        uint8_t *pin = static_cast<uint8_t *>(ctx->merge_out[q & 1].ensure(std::max<uint64_t>(n, PIECE)));
+        if (link) {
+            if (n) link->d2h((int)(q & 1), pin, d_bytes + a, n);
+        } else {
+            if (n) KITE_HIP(hipMemcpyAsync(pin, d_bytes + a, n, hipMemcpyDeviceToHost, s));
+            KITE_HIP(hipEventRecord(done[q & 1], s));
+        }
+    };
+    for (size_t q = 0; q < std::min<size_t>(2, pieces.size()); q++) issue(q);
+    for (size_t q = 0; q < pieces.size(); q++) {
+        if (link) link->wait((int)(q & 1));
+        else KITE_HIP(hipEventSynchronize(done[q & 1]));
+        const uint8_t *pin = ctx->merge_out[q & 1].as<uint8_t>();
+        const uint64_t base = boff[pieces[q].first];
+        for (uint32_t p = pieces[q].first; p < pieces[q].second; p++) {
+            const uint64_t nr = roff[p + 1] - roff[p];
+            if (!nr) continue;
+            const uint64_t len = boff[p + 1] - boff[p];
+            if (fn(user, p0 + p, codec, nr, len, &mm[(size_t)p * 8], &mm[(size_t)p * 8 + 4], pin + (boff[p] - base),
+                   (size_t)len) != 0) {
+                (void)hipStreamSynchronize(s);
+                return (int64_t)(p0 + p);
+            }
+        }
+        if (q + 2 < pieces.size()) issue(q + 2);
+    }
+    KITE_HIP(hipStreamSynchronize(s));
+    return -1;
+}
+
 // Input size classes of the tuned settings: below 64 MiB, 512 MiB, 4 GiB, above.
 int size_class(uint64_t len) {
     return len < (64ull << 20) ? 0 : (len < (512ull << 20) ? 1 : (len < (4ull << 30) ? 2 : 3));
@@ -721,22 +787,35 @@ int kite_gpu_count_emit(kite_gpu_ctx *ctx, kite_append_chunk_fn fn, void *user) 
     return guarded([&]() -> int {
         bind(ctx);
         const uint32_t P = ctx->partitions;
-        kite::EncodedRuns last;
-        if (ctx->device_result) encode_device_result(ctx, last);
-        std::vector<const kite::EncodedRuns *> runs;
-        for (const kite::EncodedRuns &r : ctx->host_runs) runs.push_back(&r);
-        if (ctx->device_result) runs.push_back(&last);
-        for (const kite::EncodedRuns *r : runs) {
+        // earlier device batches, encoded into host memory when the next began
+        for (const kite::EncodedRuns &r : ctx->host_runs) {
             for (uint32_t p = 0; p < P; p++) {
-                const uint64_t rows = r->rows_off[p + 1] - r->rows_off[p];
+                const uint64_t rows = r.rows_off[p + 1] - r.rows_off[p];
                 if (!rows) continue;
-                const uint64_t b0 = r->byte_off[p], len = r->byte_off[p + 1] - b0;
-                int rc = fn(user, p, r->codec, rows, len, &r->minmax[(size_t)p * 8], &r->minmax[(size_t)p * 8 + 4],
-                            r->bytes.data() + b0, len);
+                const uint64_t b0 = r.byte_off[p], len = r.byte_off[p + 1] - b0;
+                int rc = fn(user, p, r.codec, rows, len, &r.minmax[(size_t)p * 8], &r.minmax[(size_t)p * 8 + 4],
+                            r.bytes.data() + b0, len);
                 if (rc)
                     return set_error(KITE_E_CALLBACK,
                                      "append callback failed for partition " + std::to_string(p));
             }
+        }
+        // the last batch: encoded on the device, its chunks streamed out
+        if (ctx->device_result) {
+            const int W = ctx->words;
+            const std::vector<uint64_t> &roff = ctx->count->part_off();
+            uint64_t *hb = static_cast<uint64_t *>(ctx->enc_bufs.host_byte_off.ensure(((size_t)P + 1) * 8));
+            const kite::EncodeDevice ed =
+                kite::encode_on_device(ctx->count->keys(), ctx->count->counts(), ctx->count->d_part_off(),
+                                       roff.empty() ? 0 : roff[P], P, W, ctx->stream, ctx->enc_bufs, hb);
+            const std::vector<uint64_t> boff(hb, hb + P + 1);
+            std::vector<uint64_t> mm((size_t)P * 8);
+            KITE_HIP(hipMemcpyAsync(mm.data(), ed.minmax, (size_t)P * 64, hipMemcpyDeviceToHost, ctx->stream));
+            KITE_HIP(hipStreamSynchronize(ctx->stream));
+            const int64_t bad = emit_device_chunks(ctx, ed.bytes, boff.data(), roff.data(), mm.data(), P, 0,
+                                                   W == 4 ? KITE_CODEC_RAW : KITE_CODEC_DVLE, fn, user);
+            if (bad >= 0)
+                return set_error(KITE_E_CALLBACK, "append callback failed for partition " + std::to_string(bad));
         }
         return KITE_OK;
     });
@@ -2563,63 +2642,9 @@ int kite_gpu_merge_runs(kite_gpu_ctx *ctx, uint32_t key_width, uint32_t partitio
             st.rows += u.rows;
             st.bytes += ed.total;
 
-            // chunks out in pieces of whole partitions (a partition larger
-            // than a piece is a piece of its own), the next piece copied
-            // while fn takes the current one
             const auto t_out = std::chrono::steady_clock::now();
-            constexpr uint64_t PIECE = 64ull << 20;
-            std::vector<std::pair<uint32_t, uint32_t>> pieces;
-            for (uint32_t a = 0; a < np;) {
-                uint32_t b = a + 1;
-                while (b < np && boff[b + 1] - boff[a] <= PIECE) b++;
-                pieces.emplace_back(a, b);
-                a = b;
-            }
-            // two pieces in flight on the library's D2H engine (the stream
-            // was synchronized above, so the chunks are written): piece
-            // q + 2 is queued once the caller has taken piece q
-            kite::HostLink *link = host_link(ctx);
-            hipEvent_t done[2];
-            for (auto &x : done) KITE_HIP(hipEventCreateWithFlags(&x, hipEventDisableTiming));
-            struct Ev2 {
-                hipEvent_t *e;
-                kite::HostLink *link;
-                ~Ev2() {
-                    if (link) link->quiesce();
-                    (void)hipEventDestroy(e[0]);
-                    (void)hipEventDestroy(e[1]);
-                }
-            } ev2{done, link};
-            for (int b = 0; b < 2; b++) ctx->merge_out[b].ensure(PIECE);
-            auto issue = [&](size_t q) {
-                const uint64_t a = boff[pieces[q].first], n = boff[pieces[q].second] - a;
-                uint8_t *pin = static_cast<uint8_t *>(ctx->merge_out[q & 1].ensure(std::max<uint64_t>(n, PIECE)));
-                if (link) {
-                    if (n) link->d2h((int)(q & 1), pin, ed.bytes + a, n);
-                } else {
-                    if (n) KITE_HIP(hipMemcpyAsync(pin, ed.bytes + a, n, hipMemcpyDeviceToHost, s));
-                    KITE_HIP(hipEventRecord(done[q & 1], s));
-                }
-            };
-            for (size_t q = 0; q < std::min<size_t>(2, pieces.size()); q++) issue(q);
-            for (size_t q = 0; q < pieces.size(); q++) {
-                if (link) link->wait((int)(q & 1));
-                else KITE_HIP(hipEventSynchronize(done[q & 1]));
-                const uint8_t *pin = ctx->merge_out[q & 1].as<uint8_t>();
-                const uint64_t base = boff[pieces[q].first];
-                for (uint32_t p = pieces[q].first; p < pieces[q].second; p++) {
-                    const uint64_t nr = roff[p + 1] - roff[p];
-                    if (!nr) continue;
-                    const uint64_t len = boff[p + 1] - boff[p];
-                    if (fn(user, p0 + p, codec, nr, len, &mm[(size_t)p * 8], &mm[(size_t)p * 8 + 4],
-                           pin + (boff[p] - base), (size_t)len) != 0) {
-                        (void)hipStreamSynchronize(s);
-                        return set_error(KITE_E_CALLBACK, "merge chunk callback aborted");
-                    }
-                }
-                if (q + 2 < pieces.size()) issue(q + 2);
-            }
-            KITE_HIP(hipStreamSynchronize(s));
+            if (emit_device_chunks(ctx, ed.bytes, boff.data(), roff.data(), mm.data(), np, p0, codec, fn, user) >= 0)
+                return set_error(KITE_E_CALLBACK, "merge chunk callback aborted");
             st.out_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_out).count();
             st.groups++;
             p0 = p1;
