@@ -153,12 +153,17 @@ size_t split_4lines(const uint8_t *d, size_t n) {
 // PlainChunkedReader / GzipChunkedReader::next_chunk (decompress.rs:36-82, 104-138)
 class Chunker {
    public:
-    Chunker(Source &s, size_t chunk) : src_(s), chunk_(chunk) { buf_.resize(chunk); }
-    bool next(std::vector<uint8_t> &out) {
+    Chunker(Source &s, size_t chunk) : src_(s), chunk_(chunk) {}
+    // the input is read straight into `out`, behind the partial record the
+    // previous chunk left over (no zeroing: the buffer type leaves new bytes
+    // uninitialised)
+    bool next(Bytes &out) {
         out.assign(carry_.begin(), carry_.end());
         carry_.clear();
-        size_t got = src_.read(buf_.data(), chunk_);
-        out.insert(out.end(), buf_.begin(), buf_.begin() + got);
+        const size_t have = out.size();
+        out.resize(have + chunk_);
+        const size_t got = src_.read(out.data() + have, chunk_);
+        out.resize(have + got);
         if (out.empty()) return false;
         if (got < chunk_) return true;
         size_t sp = split_4lines(out.data(), out.size());
@@ -171,7 +176,7 @@ class Chunker {
    private:
     Source &src_;
     size_t chunk_;
-    std::vector<uint8_t> buf_, carry_;
+    std::vector<uint8_t> carry_;
 };
 
 struct Plan {
@@ -193,7 +198,7 @@ uint64_t ceil_pow2(uint64_t v) {
 void sample_input(const std::string &path, Comp comp, Plan &pl) {
     Source s(path, comp);
     Chunker ch(s, 8 * MIB);
-    std::vector<uint8_t> c;
+    Bytes c;
     uint64_t reads = 0, bases = 0, decomp = 0;
     while (reads < 200000 && decomp < 32 * MIB) {
         if (!ch.next(c)) break;
@@ -585,7 +590,7 @@ int run_count(int argc, char **argv) {
         // the next chunk is read on a second thread while this one is fed
         Source src(input, comp);
         Chunker ch(src, pl.chunk);
-        std::vector<uint8_t> c, nx;
+        Bytes c, nx;
         bool more = ch.next(c);
         while (more) {
             bool more_n = false;
