@@ -61,8 +61,10 @@ struct Bits {
     __device__ void fill(int n) {
         if (cnt >= n) return;
         if (end - p >= 8) {
+            // (the aligned address by pointer arithmetic on p, not from an
+            // integer, so that the compiler still knows p's memory space)
             const uintptr_t a = reinterpret_cast<uintptr_t>(p);
-            const uint32_t *w = reinterpret_cast<const uint32_t *>(a & ~static_cast<uintptr_t>(3));
+            const uint32_t *w = reinterpret_cast<const uint32_t *>(p - (a & 3));
             const uint64_t two = (uint64_t)w[0] | ((uint64_t)w[1] << 32);
             buf |= ((two >> (8 * (a & 3))) & 0xFFFFFFFFull) << cnt;  // cnt < 32: no bit lost
             cnt += 32;
@@ -135,32 +137,71 @@ __device__ int decode(Bits &b, const uint16_t *count, const uint16_t *symbol) {
 // first LB / DB code bits, and the token queue.
 constexpr int LB = 10;    // literal/length table bits
 constexpr int DB = 8;     // distance table bits
-constexpr int NTOK = 1024;
+#ifndef KITE_INFLATE_NTOK
+#define KITE_INFLATE_NTOK 256
+#endif
+#ifndef KITE_INFLATE_WAVES
+#define KITE_INFLATE_WAVES 6
+#endif
+constexpr int NTOK = KITE_INFLATE_NTOK;
+// Tokens: a literal is TOK_LIT | (count << 24) | its 1 or 2 bytes (first in
+// bits 0-7); a match is (length << 16) | distance - 1 (bit 31 clear).
 constexpr uint32_t TOK_LIT = 0x80000000u;
+// Literal/length table entries: bits 0-8 the first symbol, 9-12 the bits the
+// entry consumes (0: the code is longer than LB bits), bit 13 set when a
+// second literal's code follows within the LB bits, bits 16-23 that literal.
+constexpr uint32_t PAIR = 1u << 13;
 
 struct Member {
-    uint16_t lut_l[1 << LB], lut_d[1 << DB];  // (code length << 9) | symbol, 0 = longer code
+    uint32_t lut_l[1 << LB];
+    uint16_t lut_d[1 << DB];  // (code length << 9) | symbol, 0 = longer code
     uint16_t lcount[16], lsym[288];
     uint16_t dcount[16], dsym[32];
     uint16_t offs[16];
     uint8_t lengths[320];
-    uint32_t tok[NTOK];  // literal: TOK_LIT | byte; match: (length << 16) | distance - 1
-    uint32_t ntok, done, err;
+    uint32_t tok[NTOK];
 };
 
 // Direct table of the codes up to `bits` long: entry [reversed code + k <<
-// length] for every k, so that the next `bits` stream bits index it.
-__device__ void fill_lut(uint16_t *lut, int bits, const uint16_t *count, const uint16_t *symbol) {
-    for (int i = 0; i < (1 << bits); i++) lut[i] = 0;
+// length] for every k, so that the next `bits` stream bits index it.  The
+// wave runs it together: the code walk is the same on every lane, the
+// entries of one code are split over the lanes.
+template <class E>
+__device__ void fill_lut(E *lut, int bits, const uint16_t *count, const uint16_t *symbol, uint32_t lane) {
+    for (uint32_t i = lane; i < (1u << bits); i += 64) lut[i] = 0;
     int code = 0, idx = 0;
     for (int len = 1; len <= bits; len++) {
         for (int q = 0; q < count[len]; q++, idx++, code++) {
             const uint32_t r = __builtin_bitreverse32((uint32_t)code) >> (32 - len);
-            const uint16_t e = (uint16_t)((len << 9) | symbol[idx]);
-            for (uint32_t k = r; k < (1u << bits); k += 1u << len) lut[k] = e;
+            const E e = (E)((len << 9) | symbol[idx]);
+            for (uint32_t k = r + (lane << len); k < (1u << bits); k += 64u << len) lut[k] = e;
         }
         code <<= 1;
     }
+}
+
+// Literal pairs in the literal/length table: where an entry's code is a
+// literal shorter than LB bits and the remaining bits of the index hold a
+// whole second literal code, the entry yields both.
+__device__ void pair_lut(uint32_t *lut, uint32_t lane) {
+    constexpr int PER = (1 << LB) / 64;
+    uint32_t v[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const uint32_t i = lane + 64u * (uint32_t)q;
+        const uint32_t e1 = lut[i];
+        const uint32_t l1 = (e1 >> 9) & 15u, s1 = e1 & 511u;
+        v[q] = e1;
+        if (l1 && l1 < (uint32_t)LB && s1 < 256) {
+            const uint32_t e2 = lut[i >> l1];
+            const uint32_t l2 = (e2 >> 9) & 15u, s2 = e2 & 511u;
+            if (l2 && l1 + l2 <= (uint32_t)LB && s2 < 256) v[q] = s1 | ((l1 + l2) << 9) | PAIR | (s2 << 16);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; q++) lut[lane + 64u * (uint32_t)q] = v[q];
+    __syncthreads();
 }
 
 __device__ __forceinline__ int decode_fast(Bits &b, const uint16_t *lut, int bits, const uint16_t *count,
@@ -176,8 +217,10 @@ __device__ __forceinline__ int decode_fast(Bits &b, const uint16_t *lut, int bit
     return decode(b, count, symbol);
 }
 
-// Block header: the tables of a fixed or dynamic block (lane 0).
-__device__ uint32_t read_tables(Bits &b, Member &t, uint32_t type) {
+// Block header: the tables of a fixed or dynamic block (run by every lane
+// on the same bits: every lane stores the same values, and the direct
+// tables are filled by the lanes together).
+__device__ uint32_t read_tables(Bits &b, Member &t, uint32_t type, uint32_t lane) {
     if (type == 1) {
         // fixed codes (RFC 1951 3.2.6)
         for (int s = 0; s < 288; s++) t.lengths[s] = s < 144 ? 8 : (s < 256 ? 9 : (s < 280 ? 7 : 8));
@@ -218,93 +261,73 @@ __device__ uint32_t read_tables(Bits &b, Member &t, uint32_t type) {
         r = construct(t.dcount, t.dsym, t.offs, t.lengths + nlen, ndist);
         if (r < 0 || (r > 0 && ndist - t.dcount[0] != 1)) return E_STREAM;
     }
-    fill_lut(t.lut_l, LB, t.lcount, t.lsym);
-    fill_lut(t.lut_d, DB, t.dcount, t.dsym);
+    fill_lut(t.lut_l, LB, t.lcount, t.lsym, lane);
+    fill_lut(t.lut_d, DB, t.dcount, t.dsym, lane);
+    __syncthreads();
+    pair_lut(t.lut_l, lane);
     return 0;
 }
 
-// One workgroup (one wave) per member.  Lane 0 decodes up to NTOK tokens
-// (literals and length/distance pairs) into LDS; then the wave writes the
+// One workgroup (one wave) per member.  The wave decodes up to NTOK tokens
+// (literals and length/distance pairs) into LDS; then it writes the
 // literals in parallel and copies each match with all lanes, in order: a
 // copy whose source overlaps its destination (distance below the length)
 // goes in rounds that double the copied span, each round reading only
 // bytes written before it.
-__global__ __launch_bounds__(64, 4) void k_inflate(const uint8_t *__restrict__ comp, const InflateBlock *__restrict__ blocks,
+__global__ __launch_bounds__(64, KITE_INFLATE_WAVES) void k_inflate(const uint8_t *__restrict__ comp, const InflateBlock *__restrict__ blocks,
                                                 uint32_t nb, uint8_t *__restrict__ out, uint32_t *__restrict__ err) {
     __shared__ Member t;
     const uint32_t lane = threadIdx.x;
     const InflateBlock bk = blocks[blockIdx.x];
     uint8_t *o = out + bk.dst;
     const uint32_t osz = bk.isize;
-    // lane 0's decoder state
+    // The decoder: every lane runs it on the same bits, so its state and
+    // branches are wave-uniform (scalar registers and scalar branches, no
+    // execution-mask changes per symbol).  Tokens are queued 64 at a time
+    // in one vector register (token i in lane i % 64) and stored to the LDS
+    // queue by the whole wave.
     Bits b{comp + bk.src, comp + bk.src + bk.clen, 0, 0, 0};
     uint32_t mode = 0;         // 0: between blocks, 1: stored, 2: Huffman codes
-    bool last = false;
+    bool last = false, done = false;
     uint32_t stored_left = 0;  // bytes left in a stored block
     uint32_t produced = 0;     // output bytes of the tokens decoded so far
     uint32_t pos = 0;          // output bytes written
-    if (lane == 0) {
-        t.done = 0;
-        t.err = 0;
-    }
-    __syncthreads();
-    while (!t.done) {
-        if (lane == 0) {
-            uint32_t n = 0, e = 0;
-            while (n < NTOK) {
-                if (mode == 0) {
-                    if (last) {
-                        t.done = 1;
-                        break;
+    uint32_t e = 0;
+    while (!done) {
+        uint32_t n = 0, qv = 0;
+        auto put = [&](uint32_t tok) {
+            qv = lane == (n & 63u) ? tok : qv;
+            if ((n & 63u) == 63u) t.tok[(n & ~63u) + lane] = qv;
+            n++;
+        };
+        while (n < NTOK) {
+            if (mode == 2) {
+                // a symbol of a Huffman block: through the direct table
+                // (one or two literals, or a length code), else the
+                // canonical walk for a code longer than LB bits
+                b.fill(15);
+                const uint32_t ent = t.lut_l[b.buf & ((1u << LB) - 1)];
+                const int tl = (int)((ent >> 9) & 15u);
+                int sym;
+                if (tl) {
+                    b.buf >>= tl;
+                    b.cnt -= tl;
+                    sym = (int)(ent & 511u);
+                    if (ent & PAIR) {
+                        put(TOK_LIT | (2u << 24) | (uint32_t)sym | ((ent >> 8) & 0xFF00u));
+                        produced += 2;
+                        continue;
                     }
-                    last = b.get(1) != 0;
-                    const uint32_t type = b.get(2);
-                    if (type == 0) {
-                        b.get(b.cnt & 7);
-                        const uint32_t len = b.get(16), nlen = b.get(16);
-                        if (len != (~nlen & 0xFFFFu) || b.overrun()) {
-                            e = E_STREAM;
-                            break;
-                        }
-                        stored_left = len;
-                        mode = len ? 1u : 0u;
-                    } else if (type == 1 || type == 2) {
-                        e = read_tables(b, t, type);
-                        if (e) break;
-                        mode = 2;
-                    } else {
+                } else {
+                    sym = decode(b, t.lcount, t.lsym);
+                    if (sym < 0) {
                         e = E_STREAM;
                         break;
                     }
-                    continue;
-                }
-                if (mode == 1) {
-                    if (produced >= osz) {
-                        e = E_SIZE;
-                        break;
-                    }
-                    t.tok[n++] = TOK_LIT | b.get(8);
-                    produced++;
-                    if (--stored_left == 0) {
-                        mode = 0;
-                        if (b.overrun()) {
-                            e = E_STREAM;
-                            break;
-                        }
-                    }
-                    continue;
-                }
-                int sym = decode_fast(b, t.lut_l, LB, t.lcount, t.lsym);
-                if (sym < 0) {
-                    e = E_STREAM;
-                    break;
                 }
                 if (sym < 256) {
-                    if (produced >= osz) {
-                        e = E_SIZE;
-                        break;
-                    }
-                    t.tok[n++] = TOK_LIT | (uint32_t)sym;
+                    // (the output size is checked once the batch is decoded)
+                    put(TOK_LIT | (1u << 24) | (uint32_t)sym);
                     produced++;
                     continue;
                 }
@@ -340,26 +363,62 @@ __global__ __launch_bounds__(64, 4) void k_inflate(const uint8_t *__restrict__ c
                     e = E_SIZE;
                     break;
                 }
-                t.tok[n++] = (len << 16) | (dist - 1);
+                put((len << 16) | (dist - 1));
                 produced += len;
+                continue;
             }
-            if (e) {
-                t.err = e;
-                t.done = 1;
+            if (mode == 0) {
+                if (last) {
+                    done = true;
+                    break;
+                }
+                last = b.get(1) != 0;
+                const uint32_t type = b.get(2);
+                if (type == 0) {
+                    b.get(b.cnt & 7);
+                    const uint32_t len = b.get(16), nlen = b.get(16);
+                    if (len != (~nlen & 0xFFFFu) || b.overrun()) {
+                        e = E_STREAM;
+                        break;
+                    }
+                    stored_left = len;
+                    mode = len ? 1u : 0u;
+                } else if (type == 1 || type == 2) {
+                    e = read_tables(b, t, type, lane);
+                    if (e) break;
+                    mode = 2;
+                } else {
+                    e = E_STREAM;
+                    break;
+                }
+                continue;
             }
-            t.ntok = n;
+            // mode 1: a byte of a stored block
+            put(TOK_LIT | (1u << 24) | b.get(8));
+            produced++;
+            if (--stored_left == 0) {
+                mode = 0;
+                if (b.overrun()) {
+                    e = E_STREAM;
+                    break;
+                }
+            }
         }
+        // no token is written past the member's output size
+        if (!e && produced > osz) e = E_SIZE;
+        // the partial group (slots from n on are never read)
+        if (n & 63u) t.tok[(n & ~63u) + lane] = qv;
+        if (e) done = true;
         __syncthreads();
-        if (t.err) break;
-        const uint32_t n = t.ntok;
-        // output offsets of the tokens: 16 consecutive tokens per lane
+        if (e) break;
+        // output offsets of the tokens: NTOK / 64 consecutive tokens per lane
         constexpr int TPL = NTOK / 64;
         uint32_t lens[TPL], sum = 0;
 #pragma unroll
         for (int q = 0; q < TPL; q++) {
             const uint32_t i = lane * TPL + q;
             const uint32_t tk = i < n ? t.tok[i] : TOK_LIT;
-            lens[q] = i < n ? ((tk & TOK_LIT) ? 1u : (tk >> 16)) : 0u;
+            lens[q] = i < n ? ((tk & TOK_LIT) ? (tk >> 24) & 3u : (tk >> 16)) : 0u;
             sum += lens[q];
         }
         const uint32_t incl = wave_incl_scan_u32(sum);
@@ -371,28 +430,42 @@ __global__ __launch_bounds__(64, 4) void k_inflate(const uint8_t *__restrict__ c
             const uint32_t i = lane * TPL + q;
             if (i < n) {
                 const uint32_t tk = t.tok[i];
-                if (tk & TOK_LIT) o[off] = (uint8_t)tk;
+                if (tk & TOK_LIT) {
+                    o[off] = (uint8_t)tk;
+                    if (lens[q] == 2) o[off + 1] = (uint8_t)(tk >> 8);
+                }
             }
             off += lens[q];
         }
         __threadfence_block();
-        // matches, in order
+        // matches, in order.  A copy reads bytes of earlier copies only
+        // after their stores have completed: `dirty` is the lowest output
+        // byte stored since the last wait (every later store lies in
+        // [dirty, at)), and a match whose source ends at or below it (bytes
+        // of earlier batches, this batch's literals, copies before the last
+        // wait) copies without waiting.
         uint32_t at = pos;
+        uint32_t dirty = 0xFFFFFFFFu;
         for (uint32_t i = 0; i < n; i++) {
             const uint32_t tk = t.tok[i];
             if (tk & TOK_LIT) {
-                at++;
+                at += (tk >> 24) & 3u;
                 continue;
             }
             const uint32_t len = tk >> 16, dist = (tk & 0xFFFFu) + 1;
             uint32_t c = dist < len ? dist : len;
+            if (at - dist + c > dirty) {
+                __threadfence_block();
+                dirty = 0xFFFFFFFFu;
+            }
             for (uint32_t j = lane; j < c; j += 64) o[at + j] = o[at - dist + j];
-            __threadfence_block();
+            if (dirty > at) dirty = at;
             while (c < len) {
                 // the bytes so far repeat with period c (a multiple of dist)
+                __threadfence_block();
                 const uint32_t m = (len - c) < c ? (len - c) : c;
                 for (uint32_t j = lane; j < m; j += 64) o[at + c + j] = o[at + j];
-                __threadfence_block();
+                dirty = at + c;
                 c += m;
             }
             at += len;
@@ -401,7 +474,6 @@ __global__ __launch_bounds__(64, 4) void k_inflate(const uint8_t *__restrict__ c
         __syncthreads();
     }
     if (lane == 0) {
-        uint32_t e = t.err;
         if (!e && pos != osz) e = E_SIZE;
         if (e) atomicOr(err, e);
     }
