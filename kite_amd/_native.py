@@ -238,10 +238,11 @@ class PinnedArray:
 
     _release = free
 
-    def __del__(self):  # pragma: no cover
+    def __del__(self, _finalizing=sys.is_finalizing):  # pragma: no cover
         # no HIP call while the interpreter finalizes: the exit hook has
-        # already released what was still live
-        if sys.is_finalizing():
+        # already released what was still live (the check is bound at
+        # definition: module globals are gone late in shutdown)
+        if _finalizing():
             return
         try:
             self.free()
@@ -293,8 +294,9 @@ class GpuContext:
     def __exit__(self, *exc):
         self.close()
 
-    def __del__(self):  # pragma: no cover
-        if sys.is_finalizing():
+    def __del__(self, _finalizing=sys.is_finalizing):  # pragma: no cover
+        # (bound at definition: module globals are gone late in shutdown)
+        if _finalizing():
             return
         try:
             self.close()

@@ -48,10 +48,12 @@ __constant__ uint8_t c_clorder[19] = {16, 17, 18, 0, 8, 7, 9, 6, 10, 5, 11, 4, 1
 
 enum : uint32_t { E_STREAM = 1, E_SIZE = 2 };
 
-// LSB-first bit reader over [p, end); reads past the end supply zero bytes
-// and are counted, so that consuming them is detected.
+// LSB-first bit reader over the `len` bytes at `base`; reads past the end
+// supply zero bytes and are counted, so that consuming them is detected.
+// (32-bit offsets: their compares are single scalar instructions.)
 struct Bits {
-    const uint8_t *p, *end;
+    const uint8_t *base;
+    uint32_t off, len;
     uint64_t buf;
     int cnt;
     uint32_t pad;
@@ -60,20 +62,22 @@ struct Bits {
     // bytes instead of per byte), then byte by byte.
     __device__ void fill(int n) {
         if (cnt >= n) return;
-        if (end - p >= 8) {
-            // (the aligned address by pointer arithmetic on p, not from an
-            // integer, so that the compiler still knows p's memory space)
-            const uintptr_t a = reinterpret_cast<uintptr_t>(p);
-            const uint32_t *w = reinterpret_cast<const uint32_t *>(p - (a & 3));
+        if (len - off >= 8) {
+            // (the aligned address by pointer arithmetic on base, not from
+            // an integer, so that the compiler still knows its memory space;
+            // in 64 bits: at off = 0 the aligned word starts before base)
+            const uint32_t mis = ((uint32_t)reinterpret_cast<uintptr_t>(base) + off) & 3u;
+            const uint8_t *q = base + off;
+            const uint32_t *w = reinterpret_cast<const uint32_t *>(q - mis);
             const uint64_t two = (uint64_t)w[0] | ((uint64_t)w[1] << 32);
-            buf |= ((two >> (8 * (a & 3))) & 0xFFFFFFFFull) << cnt;  // cnt < 32: no bit lost
+            buf |= ((two >> (8 * mis)) & 0xFFFFFFFFull) << cnt;  // cnt < 32: no bit lost
             cnt += 32;
-            p += 4;
+            off += 4;
             return;
         }
         while (cnt < n) {
             uint64_t v = 0;
-            if (p < end) v = *p++;
+            if (off < len) v = base[off++];
             else pad++;
             buf |= v << cnt;
             cnt += 8;
@@ -286,7 +290,7 @@ __global__ __launch_bounds__(64, KITE_INFLATE_WAVES) void k_inflate(const uint8_
     // execution-mask changes per symbol).  Tokens are queued 64 at a time
     // in one vector register (token i in lane i % 64) and stored to the LDS
     // queue by the whole wave.
-    Bits b{comp + bk.src, comp + bk.src + bk.clen, 0, 0, 0};
+    Bits b{comp + bk.src, 0, bk.clen, 0, 0, 0};
     uint32_t mode = 0;         // 0: between blocks, 1: stored, 2: Huffman codes
     bool last = false, done = false;
     uint32_t stored_left = 0;  // bytes left in a stored block
@@ -302,69 +306,73 @@ __global__ __launch_bounds__(64, KITE_INFLATE_WAVES) void k_inflate(const uint8_
         };
         while (n < NTOK) {
             if (mode == 2) {
-                // a symbol of a Huffman block: through the direct table
-                // (one or two literals, or a length code), else the
-                // canonical walk for a code longer than LB bits
-                b.fill(15);
-                const uint32_t ent = t.lut_l[b.buf & ((1u << LB) - 1)];
-                const int tl = (int)((ent >> 9) & 15u);
-                int sym;
-                if (tl) {
-                    b.buf >>= tl;
-                    b.cnt -= tl;
-                    sym = (int)(ent & 511u);
-                    if (ent & PAIR) {
-                        put(TOK_LIT | (2u << 24) | (uint32_t)sym | ((ent >> 8) & 0xFF00u));
-                        produced += 2;
+                // the symbols of a Huffman block, until it ends or the
+                // queue is full: through the direct table (one or two
+                // literals, or a length code), else the canonical walk for
+                // a code longer than LB bits
+                for (;;) {
+                    b.fill(15);
+                    const uint32_t ent = t.lut_l[b.buf & ((1u << LB) - 1)];
+                    const int tl = (int)((ent >> 9) & 15u);
+                    int sym;
+                    if (tl) {
+                        b.buf >>= tl;
+                        b.cnt -= tl;
+                        sym = (int)(ent & 511u);
+                        if (ent & PAIR) {
+                            put(TOK_LIT | (2u << 24) | (uint32_t)sym | ((ent >> 8) & 0xFF00u));
+                            produced += 2;
+                            if (n == NTOK) break;
+                            continue;
+                        }
+                    } else {
+                        sym = decode(b, t.lcount, t.lsym);
+                        if (sym < 0) {
+                            e = E_STREAM;
+                            break;
+                        }
+                    }
+                    if (sym < 256) {
+                        // (the output size is checked once the batch is decoded)
+                        put(TOK_LIT | (1u << 24) | (uint32_t)sym);
+                        produced++;
+                        if (n == NTOK) break;
                         continue;
                     }
-                } else {
-                    sym = decode(b, t.lcount, t.lsym);
-                    if (sym < 0) {
+                    if (sym == 256) {
+                        mode = 0;
+                        if (b.overrun()) e = E_STREAM;
+                        break;
+                    }
+                    sym -= 257;
+                    if (sym >= 29) {
                         e = E_STREAM;
                         break;
                     }
-                }
-                if (sym < 256) {
-                    // (the output size is checked once the batch is decoded)
-                    put(TOK_LIT | (1u << 24) | (uint32_t)sym);
-                    produced++;
-                    continue;
-                }
-                if (sym == 256) {
-                    mode = 0;
-                    if (b.overrun()) {
+                    int lx;
+                    const uint32_t lb = len_base((uint32_t)sym, &lx);
+                    const uint32_t len = lb + b.get(lx);
+                    const int ds = decode_fast(b, t.lut_d, DB, t.dcount, t.dsym);
+                    if (ds < 0 || ds >= 30) {
                         e = E_STREAM;
                         break;
                     }
-                    continue;
+                    int dx;
+                    const uint32_t db = dist_base((uint32_t)ds, &dx);
+                    const uint32_t dist = db + b.get(dx);
+                    if (dist > produced) {
+                        e = E_STREAM;
+                        break;
+                    }
+                    if (produced + len > osz) {
+                        e = E_SIZE;
+                        break;
+                    }
+                    put((len << 16) | (dist - 1));
+                    produced += len;
+                    if (n == NTOK) break;
                 }
-                sym -= 257;
-                if (sym >= 29) {
-                    e = E_STREAM;
-                    break;
-                }
-                int lx;
-                const uint32_t lb = len_base((uint32_t)sym, &lx);
-                const uint32_t len = lb + b.get(lx);
-                const int ds = decode_fast(b, t.lut_d, DB, t.dcount, t.dsym);
-                if (ds < 0 || ds >= 30) {
-                    e = E_STREAM;
-                    break;
-                }
-                int dx;
-                const uint32_t db = dist_base((uint32_t)ds, &dx);
-                const uint32_t dist = db + b.get(dx);
-                if (dist > produced) {
-                    e = E_STREAM;
-                    break;
-                }
-                if (produced + len > osz) {
-                    e = E_SIZE;
-                    break;
-                }
-                put((len << 16) | (dist - 1));
-                produced += len;
+                if (e) break;
                 continue;
             }
             if (mode == 0) {
