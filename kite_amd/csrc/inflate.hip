@@ -421,12 +421,13 @@ __global__ __launch_bounds__(64, KITE_INFLATE_WAVES) void k_inflate(const uint8_
         if (e) break;
         // output offsets of the tokens: NTOK / 64 consecutive tokens per lane
         constexpr int TPL = NTOK / 64;
-        uint32_t lens[TPL], sum = 0;
+        uint32_t tks[TPL], lens[TPL], offs[TPL], sum = 0, mbits = 0;
 #pragma unroll
         for (int q = 0; q < TPL; q++) {
             const uint32_t i = lane * TPL + q;
-            const uint32_t tk = i < n ? t.tok[i] : TOK_LIT;
-            lens[q] = i < n ? ((tk & TOK_LIT) ? (tk >> 24) & 3u : (tk >> 16)) : 0u;
+            tks[q] = i < n ? t.tok[i] : TOK_LIT;
+            lens[q] = i < n ? ((tks[q] & TOK_LIT) ? (tks[q] >> 24) & 3u : (tks[q] >> 16)) : 0u;
+            if (i < n && !(tks[q] & TOK_LIT)) mbits |= 1u << q;
             sum += lens[q];
         }
         const uint32_t incl = wave_incl_scan_u32(sum);
@@ -435,48 +436,50 @@ __global__ __launch_bounds__(64, KITE_INFLATE_WAVES) void k_inflate(const uint8_
         // literals
 #pragma unroll
         for (int q = 0; q < TPL; q++) {
-            const uint32_t i = lane * TPL + q;
-            if (i < n) {
-                const uint32_t tk = t.tok[i];
-                if (tk & TOK_LIT) {
-                    o[off] = (uint8_t)tk;
-                    if (lens[q] == 2) o[off + 1] = (uint8_t)(tk >> 8);
-                }
+            offs[q] = off;
+            if (lens[q] && (tks[q] & TOK_LIT)) {
+                o[off] = (uint8_t)tks[q];
+                if (lens[q] == 2) o[off + 1] = (uint8_t)(tks[q] >> 8);
             }
             off += lens[q];
         }
         __threadfence_block();
-        // matches, in order.  A copy reads bytes of earlier copies only
-        // after their stores have completed: `dirty` is the lowest output
-        // byte stored since the last wait (every later store lies in
-        // [dirty, at)), and a match whose source ends at or below it (bytes
-        // of earlier batches, this batch's literals, copies before the last
-        // wait) copies without waiting.
-        uint32_t at = pos;
+        // matches, in order: the lanes holding one in ascending order, each
+        // lane's in token order, read from its registers.  A copy reads
+        // bytes of earlier copies only after their stores have completed:
+        // `dirty` is the lowest output byte stored since the last wait
+        // (every later store lies above it, below the current match), and a
+        // match whose source ends at or below it (bytes of earlier batches,
+        // this batch's literals, copies before the last wait) copies
+        // without waiting.
         uint32_t dirty = 0xFFFFFFFFu;
-        for (uint32_t i = 0; i < n; i++) {
-            const uint32_t tk = t.tok[i];
-            if (tk & TOK_LIT) {
-                at += (tk >> 24) & 3u;
-                continue;
+        uint64_t lanes = __ballot(mbits != 0);
+        while (lanes) {
+            const int L = __builtin_ctzll(lanes);
+            lanes &= lanes - 1;
+            const uint32_t lm = (uint32_t)__builtin_amdgcn_readlane((int)mbits, L);
+#pragma unroll
+            for (int q = 0; q < TPL; q++) {
+                if (!((lm >> q) & 1u)) continue;
+                const uint32_t tk = (uint32_t)__builtin_amdgcn_readlane((int)tks[q], L);
+                const uint32_t at = (uint32_t)__builtin_amdgcn_readlane((int)offs[q], L);
+                const uint32_t len = tk >> 16, dist = (tk & 0xFFFFu) + 1;
+                uint32_t c = dist < len ? dist : len;
+                if (at - dist + c > dirty) {
+                    __threadfence_block();
+                    dirty = 0xFFFFFFFFu;
+                }
+                for (uint32_t j = lane; j < c; j += 64) o[at + j] = o[at - dist + j];
+                if (dirty > at) dirty = at;
+                while (c < len) {
+                    // the bytes so far repeat with period c (a multiple of dist)
+                    __threadfence_block();
+                    const uint32_t m = (len - c) < c ? (len - c) : c;
+                    for (uint32_t j = lane; j < m; j += 64) o[at + c + j] = o[at + j];
+                    dirty = at + c;
+                    c += m;
+                }
             }
-            const uint32_t len = tk >> 16, dist = (tk & 0xFFFFu) + 1;
-            uint32_t c = dist < len ? dist : len;
-            if (at - dist + c > dirty) {
-                __threadfence_block();
-                dirty = 0xFFFFFFFFu;
-            }
-            for (uint32_t j = lane; j < c; j += 64) o[at + j] = o[at - dist + j];
-            if (dirty > at) dirty = at;
-            while (c < len) {
-                // the bytes so far repeat with period c (a multiple of dist)
-                __threadfence_block();
-                const uint32_t m = (len - c) < c ? (len - c) : c;
-                for (uint32_t j = lane; j < m; j += 64) o[at + c + j] = o[at + j];
-                dirty = at + c;
-                c += m;
-            }
-            at += len;
         }
         pos += total;
         __syncthreads();
