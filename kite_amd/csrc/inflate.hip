@@ -151,10 +151,11 @@ constexpr int NTOK = KITE_INFLATE_NTOK;
 // Tokens: a literal is TOK_LIT | (count << 24) | its 1 or 2 bytes (first in
 // bits 0-7); a match is (length << 16) | distance - 1 (bit 31 clear).
 constexpr uint32_t TOK_LIT = 0x80000000u;
-// Literal/length table entries: bits 0-8 the first symbol, 9-12 the bits the
-// entry consumes (0: the code is longer than LB bits), bit 13 set when a
-// second literal's code follows within the LB bits, bits 16-23 that literal.
-constexpr uint32_t PAIR = 1u << 13;
+// Literal/length table entries, bits 26-29 the bits the entry consumes (0:
+// the code is longer than LB bits): for one or two literals the token itself
+// with those bits added, else the symbol (256..285) in bits 0-8.
+constexpr int ENT_LEN = 26;
+constexpr uint32_t ENT_TOK = ~(15u << ENT_LEN);
 
 struct Member {
     uint32_t lut_l[1 << LB];
@@ -184,10 +185,11 @@ __device__ void fill_lut(E *lut, int bits, const uint16_t *count, const uint16_t
     }
 }
 
-// Literal pairs in the literal/length table: where an entry's code is a
-// literal shorter than LB bits and the remaining bits of the index hold a
-// whole second literal code, the entry yields both.
-__device__ void pair_lut(uint32_t *lut, uint32_t lane) {
+// The literal/length table from fill_lut's (length << 9) | symbol entries
+// to the decoder's: literal entries become tokens, and where an entry's code
+// is a literal shorter than LB bits and the remaining bits of the index hold
+// a whole second literal code, the entry is the token of both.
+__device__ void literal_lut(uint32_t *lut, uint32_t lane) {
     constexpr int PER = (1 << LB) / 64;
     uint32_t v[PER];
 #pragma unroll
@@ -195,11 +197,12 @@ __device__ void pair_lut(uint32_t *lut, uint32_t lane) {
         const uint32_t i = lane + 64u * (uint32_t)q;
         const uint32_t e1 = lut[i];
         const uint32_t l1 = (e1 >> 9) & 15u, s1 = e1 & 511u;
-        v[q] = e1;
+        v[q] = !l1 ? 0u : s1 < 256 ? (TOK_LIT | (1u << 24) | s1 | (l1 << ENT_LEN)) : (s1 | (l1 << ENT_LEN));
         if (l1 && l1 < (uint32_t)LB && s1 < 256) {
             const uint32_t e2 = lut[i >> l1];
             const uint32_t l2 = (e2 >> 9) & 15u, s2 = e2 & 511u;
-            if (l2 && l1 + l2 <= (uint32_t)LB && s2 < 256) v[q] = s1 | ((l1 + l2) << 9) | PAIR | (s2 << 16);
+            if (l2 && l1 + l2 <= (uint32_t)LB && s2 < 256)
+                v[q] = TOK_LIT | (2u << 24) | s1 | (s2 << 8) | ((l1 + l2) << ENT_LEN);
         }
     }
     __syncthreads();
@@ -268,7 +271,7 @@ __device__ uint32_t read_tables(Bits &b, Member &t, uint32_t type, uint32_t lane
     fill_lut(t.lut_l, LB, t.lcount, t.lsym, lane);
     fill_lut(t.lut_d, DB, t.dcount, t.dsym, lane);
     __syncthreads();
-    pair_lut(t.lut_l, lane);
+    literal_lut(t.lut_l, lane);
     return 0;
 }
 
@@ -313,31 +316,34 @@ __global__ __launch_bounds__(64, KITE_INFLATE_WAVES) void k_inflate(const uint8_
                 for (;;) {
                     b.fill(15);
                     const uint32_t ent = t.lut_l[b.buf & ((1u << LB) - 1)];
-                    const int tl = (int)((ent >> 9) & 15u);
+                    const int tl = (int)(ent >> ENT_LEN) & 15;
+                    if (ent & TOK_LIT) {
+                        // one or two literals (the output size is checked
+                        // once the batch is decoded)
+                        b.buf >>= tl;
+                        b.cnt -= tl;
+                        put(ent & ENT_TOK);
+                        produced += (ent >> 24) & 3u;
+                        if (n == NTOK) break;
+                        continue;
+                    }
                     int sym;
                     if (tl) {
                         b.buf >>= tl;
                         b.cnt -= tl;
                         sym = (int)(ent & 511u);
-                        if (ent & PAIR) {
-                            put(TOK_LIT | (2u << 24) | (uint32_t)sym | ((ent >> 8) & 0xFF00u));
-                            produced += 2;
-                            if (n == NTOK) break;
-                            continue;
-                        }
                     } else {
                         sym = decode(b, t.lcount, t.lsym);
                         if (sym < 0) {
                             e = E_STREAM;
                             break;
                         }
-                    }
-                    if (sym < 256) {
-                        // (the output size is checked once the batch is decoded)
-                        put(TOK_LIT | (1u << 24) | (uint32_t)sym);
-                        produced++;
-                        if (n == NTOK) break;
-                        continue;
+                        if (sym < 256) {
+                            put(TOK_LIT | (1u << 24) | (uint32_t)sym);
+                            produced++;
+                            if (n == NTOK) break;
+                            continue;
+                        }
                     }
                     if (sym == 256) {
                         mode = 0;
