@@ -874,6 +874,18 @@ def single_only(ctx, args) -> int:
     return 0
 
 
+def bgzf_only(ctx, args) -> int:
+    """--bgzf-only: the BGZF inflate leg alone on the single-sample workload,
+    printed as the line (for profiling commands; not the headline)."""
+    from kite_amd.workloads import WORKLOADS
+    w = WORKLOADS[args.single_workload]
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    bgz = bench_bgzf(ctx, w, w.fastq(), threads)
+    print(json.dumps({"bgzf_only": {"workload": w.name, **bgz}}), flush=True)
+    ctx.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -904,6 +916,7 @@ def main() -> int:
     ap.add_argument("--no-single", action="store_true", help="skip the HBM-resident single-sample leg")
     ap.add_argument("--no-dump", action="store_true", help="skip the kite dump leg")
     ap.add_argument("--no-bgzf", action="store_true", help="skip the BGZF inflate leg")
+    ap.add_argument("--bgzf-only", action="store_true", help="run the BGZF inflate leg alone (profiling)")
     ap.add_argument("--no-merge", action="store_true", help="skip the kite merge leg")
     ap.add_argument("--merge-reads", type=int, default=1_000_000, help="reads of each k=63 merge input")
     ap.add_argument("--merge-inputs", default="",
@@ -938,6 +951,8 @@ def main() -> int:
         calibrate_copy()
     if args.no_batch:
         return single_only(ctx, args)
+    if args.bgzf_only:
+        return bgzf_only(ctx, args)
     pcie = pcie_peak(ctx)
     ws, pins = make_batch24(args.batch_scale)
     tune = None
