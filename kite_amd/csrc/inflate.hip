@@ -83,6 +83,13 @@ struct Bits {
             cnt += 8;
         }
     }
+    // n bits that are already in the buffer (n <= cnt)
+    __device__ uint32_t take(int n) {
+        const uint32_t r = (uint32_t)(buf & ((1ull << n) - 1));
+        buf >>= n;
+        cnt -= n;
+        return r;
+    }
     __device__ uint32_t get(int n) {
         if (n == 0) return 0;
         fill(n);
@@ -153,9 +160,12 @@ constexpr int NTOK = KITE_INFLATE_NTOK;
 constexpr uint32_t TOK_LIT = 0x80000000u;
 // Literal/length table entries, bits 26-29 the bits the entry consumes (0:
 // the code is longer than LB bits): for one or two literals the token itself
-// with those bits added, else the symbol (256..285) in bits 0-8.
+// with those bits added; for a length code its base length in bits 0-8 and
+// extra bits in 9-11; ENT_EOB for the end of block, ENT_BAD for symbols 286
+// and 287.
 constexpr int ENT_LEN = 26;
 constexpr uint32_t ENT_TOK = ~(15u << ENT_LEN);
+constexpr uint32_t ENT_EOB = 1u << 12, ENT_BAD = 1u << 13;
 
 struct Member {
     uint32_t lut_l[1 << LB];
@@ -197,7 +207,19 @@ __device__ void literal_lut(uint32_t *lut, uint32_t lane) {
         const uint32_t i = lane + 64u * (uint32_t)q;
         const uint32_t e1 = lut[i];
         const uint32_t l1 = (e1 >> 9) & 15u, s1 = e1 & 511u;
-        v[q] = !l1 ? 0u : s1 < 256 ? (TOK_LIT | (1u << 24) | s1 | (l1 << ENT_LEN)) : (s1 | (l1 << ENT_LEN));
+        if (!l1) {
+            v[q] = 0;
+        } else if (s1 < 256) {
+            v[q] = TOK_LIT | (1u << 24) | s1 | (l1 << ENT_LEN);
+        } else if (s1 == 256) {
+            v[q] = ENT_EOB | (l1 << ENT_LEN);
+        } else if (s1 >= 286) {
+            v[q] = ENT_BAD | (l1 << ENT_LEN);
+        } else {
+            int lx;
+            const uint32_t lb = len_base(s1 - 257, &lx);
+            v[q] = lb | ((uint32_t)lx << 9) | (l1 << ENT_LEN);
+        }
         if (l1 && l1 < (uint32_t)LB && s1 < 256) {
             const uint32_t e2 = lut[i >> l1];
             const uint32_t l2 = (e2 >> 9) & 15u, s2 = e2 & 511u;
@@ -209,19 +231,6 @@ __device__ void literal_lut(uint32_t *lut, uint32_t lane) {
 #pragma unroll
     for (int q = 0; q < PER; q++) lut[lane + 64u * (uint32_t)q] = v[q];
     __syncthreads();
-}
-
-__device__ __forceinline__ int decode_fast(Bits &b, const uint16_t *lut, int bits, const uint16_t *count,
-                                           const uint16_t *symbol) {
-    b.fill(15);
-    const uint16_t e = lut[b.buf & ((1u << bits) - 1)];
-    if (e >> 9) {
-        const int len = e >> 9;
-        b.buf >>= len;
-        b.cnt -= len;
-        return e & 511;
-    }
-    return decode(b, count, symbol);
 }
 
 // Block header: the tables of a fixed or dynamic block (run by every lane
@@ -327,13 +336,21 @@ __global__ __launch_bounds__(64, KITE_INFLATE_WAVES) void k_inflate(const uint8_
                         if (n == NTOK) break;
                         continue;
                     }
-                    int sym;
+                    uint32_t lb;
+                    int lx;
                     if (tl) {
                         b.buf >>= tl;
                         b.cnt -= tl;
-                        sym = (int)(ent & 511u);
+                        if (ent & (ENT_EOB | ENT_BAD)) {
+                            if (ent & ENT_BAD) e = E_STREAM;
+                            mode = 0;
+                            if (b.overrun()) e = E_STREAM;
+                            break;
+                        }
+                        lb = ent & 511u;
+                        lx = (int)((ent >> 9) & 7u);
                     } else {
-                        sym = decode(b, t.lcount, t.lsym);
+                        int sym = decode(b, t.lcount, t.lsym);
                         if (sym < 0) {
                             e = E_STREAM;
                             break;
@@ -344,21 +361,32 @@ __global__ __launch_bounds__(64, KITE_INFLATE_WAVES) void k_inflate(const uint8_
                             if (n == NTOK) break;
                             continue;
                         }
+                        if (sym == 256) {
+                            mode = 0;
+                            if (b.overrun()) e = E_STREAM;
+                            break;
+                        }
+                        sym -= 257;
+                        if (sym >= 29) {
+                            e = E_STREAM;
+                            break;
+                        }
+                        lb = len_base((uint32_t)sym, &lx);
                     }
-                    if (sym == 256) {
-                        mode = 0;
-                        if (b.overrun()) e = E_STREAM;
-                        break;
+                    // the length's extra bits (<= 5) and the distance code
+                    // (<= 15) from one refill
+                    b.fill(32);
+                    const uint32_t len = lb + b.take(lx);
+                    int ds;
+                    {
+                        const uint16_t de = t.lut_d[b.buf & ((1u << DB) - 1)];
+                        if (de >> 9) {
+                            b.take(de >> 9);
+                            ds = de & 511;
+                        } else {
+                            ds = decode(b, t.dcount, t.dsym);
+                        }
                     }
-                    sym -= 257;
-                    if (sym >= 29) {
-                        e = E_STREAM;
-                        break;
-                    }
-                    int lx;
-                    const uint32_t lb = len_base((uint32_t)sym, &lx);
-                    const uint32_t len = lb + b.get(lx);
-                    const int ds = decode_fast(b, t.lut_d, DB, t.dcount, t.dsym);
                     if (ds < 0 || ds >= 30) {
                         e = E_STREAM;
                         break;
