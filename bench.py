@@ -953,6 +953,8 @@ def main() -> int:
         return single_only(ctx, args)
     if args.bgzf_only:
         return bgzf_only(ctx, args)
+    # (an earlier process's device memory frees slow D2H copies for a while)
+    settle_link(ctx)
     pcie = pcie_peak(ctx)
     ws, pins = make_batch24(args.batch_scale)
     tune = None
