@@ -148,6 +148,9 @@ __device__ int decode(Bits &b, const uint16_t *count, const uint16_t *symbol) {
 // first LB / DB code bits, and the token queue.
 constexpr int LB = 10;    // literal/length table bits
 constexpr int DB = 8;     // distance table bits
+// Token queue entries and waves per SIMD (build-time settings): at 256
+// tokens a member holds 6.7 KB of LDS, so six waves per SIMD fit in the
+// CU's 160 KB, and the kernel fits their 80 VGPRs.
 #ifndef KITE_INFLATE_NTOK
 #define KITE_INFLATE_NTOK 256
 #endif
@@ -155,6 +158,7 @@ constexpr int DB = 8;     // distance table bits
 #define KITE_INFLATE_WAVES 6
 #endif
 constexpr int NTOK = KITE_INFLATE_NTOK;
+static_assert(NTOK % 64 == 0 && NTOK >= 64, "the queue is written 64 tokens at a time");
 // Tokens: a literal is TOK_LIT | (count << 24) | its 1 or 2 bytes (first in
 // bits 0-7); a match is (length << 16) | distance - 1 (bit 31 clear).
 constexpr uint32_t TOK_LIT = 0x80000000u;
