@@ -149,8 +149,8 @@ __device__ int decode(Bits &b, const uint16_t *count, const uint16_t *symbol) {
 constexpr int LB = 10;    // literal/length table bits
 constexpr int DB = 8;     // distance table bits
 // Token queue entries and waves per SIMD (build-time settings): at 256
-// tokens a member holds 6.7 KB of LDS, so six waves per SIMD fit in the
-// CU's 160 KB, and the kernel fits their 80 VGPRs.
+// tokens a member holds 7.2 KB of LDS, 22 members in the CU's 160 KB; the
+// kernel fits six waves per SIMD's 80 VGPRs.
 #ifndef KITE_INFLATE_NTOK
 #define KITE_INFLATE_NTOK 256
 #endif
@@ -170,10 +170,13 @@ constexpr uint32_t TOK_LIT = 0x80000000u;
 constexpr int ENT_LEN = 26;
 constexpr uint32_t ENT_TOK = ~(15u << ENT_LEN);
 constexpr uint32_t ENT_EOB = 1u << 12, ENT_BAD = 1u << 13;
+// Distance table entries: the base distance in bits 0-14, its extra bits in
+// 15-18, the code's length in 19-22; DIST_BAD for codes 30 and 31.
+constexpr uint32_t DIST_BAD = 1u << 23;
 
 struct Member {
     uint32_t lut_l[1 << LB];
-    uint16_t lut_d[1 << DB];  // (code length << 9) | symbol, 0 = longer code
+    uint32_t lut_d[1 << DB];  // distance entries (distance_lut), 0 = longer code
     uint16_t lcount[16], lsym[288];
     uint16_t dcount[16], dsym[32];
     uint16_t offs[16];
@@ -237,6 +240,25 @@ __device__ void literal_lut(uint32_t *lut, uint32_t lane) {
     __syncthreads();
 }
 
+// The distance table from fill_lut's (length << 9) | symbol entries to the
+// decoder's: base distance and extra bits resolved per entry.
+__device__ void distance_lut(uint32_t *lut, uint32_t lane) {
+#pragma unroll
+    for (int q = 0; q < (1 << DB) / 64; q++) {
+        const uint32_t i = lane + 64u * (uint32_t)q;
+        const uint32_t e1 = lut[i];
+        const uint32_t l1 = (e1 >> 9) & 15u, d = e1 & 511u;
+        if (!l1) continue;
+        if (d >= 30) {
+            lut[i] = DIST_BAD | (l1 << 19);
+        } else {
+            int dx;
+            const uint32_t db = dist_base(d, &dx);
+            lut[i] = db | ((uint32_t)dx << 15) | (l1 << 19);
+        }
+    }
+}
+
 // Block header: the tables of a fixed or dynamic block (run by every lane
 // on the same bits: every lane stores the same values, and the direct
 // tables are filled by the lanes together).
@@ -284,7 +306,8 @@ __device__ uint32_t read_tables(Bits &b, Member &t, uint32_t type, uint32_t lane
     fill_lut(t.lut_l, LB, t.lcount, t.lsym, lane);
     fill_lut(t.lut_d, DB, t.dcount, t.dsym, lane);
     __syncthreads();
-    literal_lut(t.lut_l, lane);
+    distance_lut(t.lut_d, lane);  // (each lane its own entries)
+    literal_lut(t.lut_l, lane);   // (ends with a barrier)
     return 0;
 }
 
@@ -381,23 +404,27 @@ __global__ __launch_bounds__(64, KITE_INFLATE_WAVES) void k_inflate(const uint8_
                     // (<= 15) from one refill
                     b.fill(32);
                     const uint32_t len = lb + b.take(lx);
-                    int ds;
-                    {
-                        const uint16_t de = t.lut_d[b.buf & ((1u << DB) - 1)];
-                        if (de >> 9) {
-                            b.take(de >> 9);
-                            ds = de & 511;
-                        } else {
-                            ds = decode(b, t.dcount, t.dsym);
+                    uint32_t dist;
+                    const uint32_t de = t.lut_d[b.buf & ((1u << DB) - 1)];
+                    if (de) {
+                        // code <= DB bits: at least 19 bits are left for the
+                        // extra bits (<= 13)
+                        b.take((int)((de >> 19) & 15u));
+                        if (de & DIST_BAD) {
+                            e = E_STREAM;
+                            break;
                         }
+                        dist = (de & 0x7FFFu) + b.take((int)((de >> 15) & 15u));
+                    } else {
+                        const int ds = decode(b, t.dcount, t.dsym);
+                        if (ds < 0 || ds >= 30) {
+                            e = E_STREAM;
+                            break;
+                        }
+                        int dx;
+                        const uint32_t db = dist_base((uint32_t)ds, &dx);
+                        dist = db + b.get(dx);
                     }
-                    if (ds < 0 || ds >= 30) {
-                        e = E_STREAM;
-                        break;
-                    }
-                    int dx;
-                    const uint32_t db = dist_base((uint32_t)ds, &dx);
-                    const uint32_t dist = db + b.get(dx);
                     if (dist > produced) {
                         e = E_STREAM;
                         break;
