@@ -92,6 +92,77 @@ def test_inflate_refuses_malformed(gpu_ctx):
         gpu_ctx.inflate_bgzf(bytes(cut))
 
 
+class _Bits:
+    """LSB-first DEFLATE bit writer (RFC 1951 3.1.1): fields LSB first,
+    Huffman codes most significant bit first."""
+
+    def __init__(self):
+        self.v, self.n = 0, 0
+
+    def put(self, value, nbits):
+        self.v |= value << self.n
+        self.n += nbits
+
+    def huff(self, code, nbits):
+        self.put(int(format(code, f"0{nbits}b")[::-1], 2), nbits)
+
+    def bytes(self):
+        return self.v.to_bytes((self.n + 7) // 8, "little")
+
+
+def _fixed_stream(symbols):
+    """One final fixed-Huffman block (RFC 1951 3.2.6): ("lit", byte),
+    ("len", symbol 257..287), ("dist", code 0..31), ("eob",); no extra bits
+    (the lengths and distances used here have none)."""
+    w = _Bits()
+    w.put(1, 1)
+    w.put(1, 2)
+    for s in symbols:
+        if s[0] == "dist":
+            w.huff(s[1], 5)
+            continue
+        v = {"lit": lambda: s[1], "len": lambda: s[1], "eob": lambda: 256}[s[0]]()
+        if v < 144:
+            w.huff(0x30 + v, 8)
+        elif v < 256:
+            w.huff(0x190 + v - 144, 9)
+        elif v < 280:
+            w.huff(v - 256, 7)
+        else:
+            w.huff(0xC0 + v - 280, 8)
+    return w.bytes()
+
+
+def _raw_member(deflate, isize):
+    import struct
+    bsize = 12 + 6 + len(deflate) + 8
+    head = struct.pack("<BBBBIBBH", 0x1F, 0x8B, 8, 4, 0, 0, 0xFF, 6) + b"BC" + struct.pack("<HH", 2, bsize - 1)
+    return head + deflate + struct.pack("<II", 0, isize)
+
+
+def test_inflate_refuses_invalid_codes(gpu_ctx):
+    """Symbols a fixed-Huffman block can encode but RFC 1951 forbids:
+    literal/length 286-287, distance codes 30-31, and a distance beyond the
+    bytes produced.  Each crafted stream is pinned by zlib first (the valid
+    one inflates to ABABA, the others raise)."""
+    ok = _fixed_stream([("lit", 65), ("lit", 66), ("len", 257), ("dist", 1), ("eob",)])
+    assert zlib.decompress(ok, -15) == b"ABABA"
+    out, _ = gpu_ctx.inflate_bgzf(_raw_member(ok, 5))
+    assert out == b"ABABA"
+    bad = {
+        "literal/length 286": _fixed_stream([("lit", 65), ("len", 286), ("eob",)]),
+        "literal/length 287": _fixed_stream([("lit", 65), ("len", 287), ("eob",)]),
+        "distance code 30": _fixed_stream([("lit", 65), ("lit", 66), ("len", 257), ("dist", 30), ("eob",)]),
+        "distance code 31": _fixed_stream([("lit", 65), ("lit", 66), ("len", 257), ("dist", 31), ("eob",)]),
+        "distance too far": _fixed_stream([("lit", 65), ("len", 257), ("dist", 1), ("eob",)]),
+    }
+    for name, raw in bad.items():
+        with pytest.raises(zlib.error):
+            zlib.decompress(raw, -15)
+        with pytest.raises(KiteError, match="deflate"):
+            gpu_ctx.inflate_bgzf(_raw_member(raw, 5))
+
+
 @pytest.mark.parametrize("k,canon,P", [(21, True, 64), (31, False, 16), (45, True, 8), (77, False, 4)])
 def test_count_bgzf_equals_plain(k, canon, P):
     """A tiny device budget makes many batches, each cut after its last
