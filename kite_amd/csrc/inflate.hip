@@ -354,8 +354,7 @@ __global__ __launch_bounds__(64, KITE_INFLATE_WAVES) void k_inflate(const uint8_
                     const uint32_t ent = t.lut_l[b.buf & ((1u << LB) - 1)];
                     const int tl = (int)(ent >> ENT_LEN) & 15;
                     if (ent & TOK_LIT) {
-                        // one or two literals (the output size is checked
-                        // once the batch is decoded)
+                        // one or two literals
                         b.buf >>= tl;
                         b.cnt -= tl;
                         put(ent & ENT_TOK);
@@ -429,10 +428,6 @@ __global__ __launch_bounds__(64, KITE_INFLATE_WAVES) void k_inflate(const uint8_
                         e = E_STREAM;
                         break;
                     }
-                    if (produced + len > osz) {
-                        e = E_SIZE;
-                        break;
-                    }
                     put((len << 16) | (dist - 1));
                     produced += len;
                     if (n == NTOK) break;
@@ -477,7 +472,8 @@ __global__ __launch_bounds__(64, KITE_INFLATE_WAVES) void k_inflate(const uint8_
                 }
             }
         }
-        // no token is written past the member's output size
+        // no token is written past the member's output size (checked here
+        // once for the batch's literals and matches)
         if (!e && produced > osz) e = E_SIZE;
         // the partial group (slots from n on are never read)
         if (n & 63u) t.tok[(n & ~63u) + lane] = qv;
